@@ -1,0 +1,84 @@
+// wos_build.h -- internal interface between the C ABI (wos_capi.hip), the
+// scene builder (wos_build.hip) and the solver kernels (wos_kernels.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <string.h>
+
+#include "wos_device.h"
+
+namespace wos {
+
+// Records a formatted message for wos_last_error() (thread-local).
+void set_error(const char *fmt, ...);
+
+struct BuildInputs {
+    int dim, n_vertices, n_prims;
+    const float *vtx;  // device, owned by the scene
+    const int *idx;    // device, owned by the scene
+};
+
+struct BuildOutputs {
+    float4 *nodes;          // (n_prims - 1) * (dim == 2 ? 3 : 4) float4
+    float4 *leaves;         // n_prims * (dim == 2 ? 2 : 3) float4
+    float *normals;         // n_vertices * dim
+    int *sorted_prim;       // n_prims
+    uint32_t *sorted_codes; // n_prims
+    int *children;          // 2 * (n_prims - 1), nullable
+    float amin[3], amax[3];
+    int max_depth;
+    uint32_t error_bits;    // 1: degenerate primitive, 2: index out of range
+};
+
+int build_scene_gpu(BuildInputs &in, hipStream_t stream, BuildOutputs &out);
+
+// ---- solver launches (wos_kernels.hip) -------------------------------------
+struct Workspace;   // per-device scratch owned by the C ABI layer
+
+struct WalkSettings {
+    int nwalks, nsteps;
+    float epsilon;
+    int double_sided, prevent_fd_artifacts;
+};
+
+struct WalkOutputs {
+    float *walk_value;   // P*W, nullable
+    int *rec_prim;       // P*W, nullable
+    float *rec_t;        // P*W*(dim-1), nullable
+    float *basis;        // K*P*W (coefficient-major), nullable
+    float *mean;         // P, nullable
+    double *mean64;      // P, nullable
+    double *var64;       // P, nullable
+};
+
+// Grows (never shrinks) the traversal spill and the launch counters.
+int2 *spill_buffer(const SceneView &s, long lanes, long *stride, hipStream_t stream);
+unsigned long long *launch_counter(hipStream_t stream);
+
+int launch_closest_index(const SceneView &s, const float *p, int n, int *out_idx, float *out_d2, hipStream_t st);
+int launch_sdf(const SceneView &s, const float *p, int n, float *out, hipStream_t st);
+int launch_closest_point(const SceneView &s, const float *p, int n, uint8_t *valid, float *op, float *on,
+                         float *ot, int *oprim, float *ocontrib, hipStream_t st);
+int launch_lib(const SceneView &s, const uint8_t *valid, const float *p, const float *nrm, int n, float eps,
+               float *out, hipStream_t st);
+int launch_dirichlet(const SceneView &s, const uint8_t *valid, const int *prim, const float *t, int n,
+                     float *out, hipStream_t st);
+int launch_single_walk(const SceneView &s, const WalkSettings &w, const float *p, int n, uint64_t *state,
+                       uint64_t *inc, uint8_t *valid, float *op, float *on, float *ot, int *oprim,
+                       float *ocontrib, hipStream_t st);
+int launch_walk_reduce(const SceneView &s, const WalkSettings &w, const float *points, int n_points,
+                       int32_t seed, int chunk, const WalkOutputs &o, hipStream_t st);
+int launch_adjoint_values(const SceneView &s, int n_points, int nwalks, const int *rec_prim, const float *rec_t,
+                          const float *basis, const float *grad_out, float *grad_values, float *grad_coeffs,
+                          hipStream_t st);
+
+struct GradSettings {
+    int nwalks, nsteps;
+    float epsilon, epsilon2, clamping;
+    int double_sided, control_variates;
+};
+int launch_shape_gradient(const SceneView &s, const GradSettings &g, const float *points, int n_points,
+                          int32_t seed, const float *grad_out, float *grad_vertices, float *sample_grad_u,
+                          hipStream_t st);
+
+}  // namespace wos

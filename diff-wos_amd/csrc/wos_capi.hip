@@ -1,0 +1,387 @@
+// wos_capi.hip -- the C ABI declared in include/wos_amd.h: scene ownership,
+// argument checks, workspace, and dispatch to the kernels.
+#include <stdarg.h>
+#include <stdio.h>
+
+#include <algorithm>
+#include <mutex>
+
+#include "../../include/wos_amd.h"
+#include "wos_build.h"
+
+struct wos_scene {
+    int dim = 0, n_vertices = 0, n_prims = 0, use_bvh = 0;
+    int source_type = 0, n_src = 0;
+    int device = 0;
+    float *vtx = nullptr, *val = nullptr, *nrm = nullptr, *src = nullptr;
+    int *idx = nullptr, *sorted_prim = nullptr, *children = nullptr;
+    uint32_t *sorted_codes = nullptr;
+    float4 *nodes = nullptr, *leaves = nullptr;
+    float amin[3] = {0, 0, 0}, amax[3] = {0, 0, 0};
+    float tol = 0.0f;
+    int max_depth = 0;
+};
+
+namespace wos {
+
+static thread_local char g_err[512] = "";
+
+void set_error(const char *fmt, ...) {
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof(g_err), fmt, ap);
+    va_end(ap);
+}
+
+// ---- workspace: traversal spill and queue counters (per process, device 0..7)
+namespace {
+struct DeviceWorkspace {
+    int2 *spill = nullptr;
+    size_t spill_bytes = 0;
+    unsigned long long *counters = nullptr;
+    int next_counter = 0;
+};
+constexpr int kMaxDevices = 16;
+constexpr int kCounters = 64;
+DeviceWorkspace g_ws[kMaxDevices];
+std::mutex g_ws_lock;
+}  // namespace
+
+int2 *spill_buffer(const SceneView &s, long lanes, long *stride, hipStream_t stream) {
+    *stride = lanes;
+    long deep = (long)s.max_depth + 2 - kLdsStack;
+    if (!s.use_bvh || deep <= 0) return nullptr;
+    size_t need = (size_t)deep * (size_t)lanes * sizeof(int2);
+    int dev = 0;
+    (void)hipGetDevice(&dev);
+    std::lock_guard<std::mutex> g(g_ws_lock);
+    DeviceWorkspace &w = g_ws[dev % kMaxDevices];
+    if (w.spill_bytes < need) {
+        // the old buffer may still be read by queued kernels
+        (void)hipDeviceSynchronize();
+        if (w.spill) (void)hipFree(w.spill);
+        w.spill = nullptr;
+        w.spill_bytes = 0;
+        if (hipMalloc((void **)&w.spill, need) != hipSuccess) {
+            set_error("cannot allocate %zu bytes of traversal spill", need);
+            return nullptr;
+        }
+        w.spill_bytes = need;
+    }
+    (void)stream;
+    return w.spill;
+}
+
+unsigned long long *launch_counter(hipStream_t stream) {
+    (void)stream;
+    int dev = 0;
+    (void)hipGetDevice(&dev);
+    std::lock_guard<std::mutex> g(g_ws_lock);
+    DeviceWorkspace &w = g_ws[dev % kMaxDevices];
+    if (!w.counters) {
+        if (hipMalloc((void **)&w.counters, kCounters * sizeof(unsigned long long)) != hipSuccess) {
+            set_error("cannot allocate queue counters");
+            return nullptr;
+        }
+    }
+    unsigned long long *c = w.counters + w.next_counter;
+    w.next_counter = (w.next_counter + 1) % kCounters;
+    return c;
+}
+
+static SceneView view_of(const wos_scene *s) {
+    SceneView v;
+    v.dim = s->dim;
+    v.n_vertices = s->n_vertices;
+    v.n_prims = s->n_prims;
+    v.use_bvh = s->use_bvh;
+    v.vtx = s->vtx;
+    v.idx = s->idx;
+    v.val = s->val;
+    v.nrm = s->nrm;
+    for (int k = 0; k < 3; ++k) { v.amin[k] = s->amin[k]; v.amax[k] = s->amax[k]; }
+    v.source_type = s->source_type;
+    v.n_src = s->n_src;
+    v.src = s->src;
+    v.nodes = s->nodes;
+    v.leaves = s->leaves;
+    v.tol = s->tol;
+    v.max_depth = s->max_depth;
+    return v;
+}
+
+}  // namespace wos
+
+using namespace wos;
+
+#define CHK(x)                                                       \
+    do {                                                             \
+        hipError_t e_ = (x);                                         \
+        if (e_ != hipSuccess) {                                      \
+            set_error("%s: %s", #x, hipGetErrorString(e_));          \
+            return -1;                                               \
+        }                                                            \
+    } while (0)
+
+#define REQUIRE(cond, ...)            \
+    do {                              \
+        if (!(cond)) {                \
+            set_error(__VA_ARGS__);   \
+            return -1;                \
+        }                             \
+    } while (0)
+
+static hipStream_t S(void *stream) { return reinterpret_cast<hipStream_t>(stream); }
+
+static void free_scene(wos_scene *s) {
+    if (!s) return;
+    (void)hipFree(s->vtx);
+    (void)hipFree(s->val);
+    (void)hipFree(s->nrm);
+    (void)hipFree(s->src);
+    (void)hipFree(s->idx);
+    (void)hipFree(s->sorted_prim);
+    (void)hipFree(s->children);
+    (void)hipFree(s->sorted_codes);
+    (void)hipFree(s->nodes);
+    (void)hipFree(s->leaves);
+    delete s;
+}
+
+static int check_source(int source_type, const float *params, int n) {
+    REQUIRE(source_type >= 0 && source_type <= 3, "source_type must be 0..3 (got %d)", source_type);
+    if (source_type == 1 || source_type == 2)
+        REQUIRE(params && n >= 5, "source_type %d needs 5 parameters (got %d)", source_type, n);
+    if (source_type == 3)
+        REQUIRE(params && n >= 4 && n % 4 == 0, "Fourier source needs 4 parameters per term (got %d)", n);
+    return 0;
+}
+
+extern "C" {
+
+const char *wos_last_error(void) { return g_err; }
+
+int wos_scene_create(int dim, const float *vertices, int n_vertices, const int *indices, int n_prims,
+                     const float *values, int use_bvh, int source_type, const float *source_params,
+                     int n_source_params, void *stream, wos_scene **out) {
+    g_err[0] = 0;
+    REQUIRE(out, "out is NULL");
+    *out = nullptr;
+    REQUIRE(dim == 2 || dim == 3, "dim must be 2 or 3 (got %d)", dim);
+    REQUIRE(n_vertices > 0 && n_prims > 0, "empty scene (%d vertices, %d primitives)", n_vertices, n_prims);
+    REQUIRE(vertices && indices && values, "vertices, indices and values are required");
+    if (check_source(source_type, source_params, n_source_params)) return -1;
+    hipStream_t st = S(stream);
+    wos_scene *s = new wos_scene();
+    s->dim = dim;
+    s->n_vertices = n_vertices;
+    s->n_prims = n_prims;
+    s->use_bvh = use_bvh ? 1 : 0;
+    s->source_type = source_type;
+    s->n_src = source_type ? n_source_params : 0;
+    (void)hipGetDevice(&s->device);
+    int nn = n_prims > 1 ? n_prims - 1 : 1;
+    size_t vb = sizeof(float) * (size_t)n_vertices * dim;
+#define ACHK(x)                \
+    if ((x) != hipSuccess) {   \
+        set_error("%s failed", #x); \
+        free_scene(s);         \
+        return -1;             \
+    }
+    ACHK(hipMalloc((void **)&s->vtx, vb));
+    ACHK(hipMalloc((void **)&s->nrm, vb));
+    ACHK(hipMalloc((void **)&s->val, sizeof(float) * (size_t)n_vertices));
+    ACHK(hipMalloc((void **)&s->idx, sizeof(int) * (size_t)n_prims * dim));
+    ACHK(hipMalloc((void **)&s->sorted_prim, sizeof(int) * (size_t)n_prims));
+    ACHK(hipMalloc((void **)&s->sorted_codes, sizeof(uint32_t) * (size_t)n_prims));
+    ACHK(hipMalloc((void **)&s->children, sizeof(int) * 2 * (size_t)nn));
+    ACHK(hipMalloc((void **)&s->nodes, sizeof(float4) * (dim == 2 ? 3 : 4) * (size_t)nn));
+    ACHK(hipMalloc((void **)&s->leaves, sizeof(float4) * (dim == 2 ? 2 : 3) * (size_t)n_prims));
+    ACHK(hipMalloc((void **)&s->src, sizeof(float) * (size_t)std::max(1, s->n_src)));
+    ACHK(hipMemcpyAsync(s->vtx, vertices, vb, hipMemcpyDefault, st));
+    ACHK(hipMemcpyAsync(s->val, values, sizeof(float) * (size_t)n_vertices, hipMemcpyDefault, st));
+    ACHK(hipMemcpyAsync(s->idx, indices, sizeof(int) * (size_t)n_prims * dim, hipMemcpyDefault, st));
+    if (s->n_src) ACHK(hipMemcpyAsync(s->src, source_params, sizeof(float) * (size_t)s->n_src, hipMemcpyDefault, st));
+#undef ACHK
+    BuildInputs in{dim, n_vertices, n_prims, s->vtx, s->idx};
+    BuildOutputs bo;
+    bo.nodes = s->nodes;
+    bo.leaves = s->leaves;
+    bo.normals = s->nrm;
+    bo.sorted_prim = s->sorted_prim;
+    bo.sorted_codes = s->sorted_codes;
+    bo.children = s->children;
+    if (build_scene_gpu(in, st, bo) != 0) {
+        free_scene(s);
+        return -1;
+    }
+    if (bo.error_bits & 2u) {
+        free_scene(s);
+        set_error("primitive index out of range [0, %d)", n_vertices);
+        return -1;
+    }
+    if (bo.error_bits & 1u) {
+        free_scene(s);
+        set_error("degenerate primitive (zero length or zero area)");
+        return -1;
+    }
+    float m = 1.0f;
+    for (int k = 0; k < 3; ++k) {
+        s->amin[k] = bo.amin[k];
+        s->amax[k] = bo.amax[k];
+        m = std::max(m, std::max(fabsf(bo.amin[k]), fabsf(bo.amax[k])));
+    }
+    s->tol = 1e-6f * m;
+    s->max_depth = bo.max_depth;
+    *out = s;
+    return 0;
+}
+
+int wos_scene_destroy(wos_scene *scene) {
+    if (!scene) return 0;
+    (void)hipDeviceSynchronize();
+    free_scene(scene);
+    return 0;
+}
+
+int wos_scene_set_values(wos_scene *s, const float *values, void *stream) {
+    REQUIRE(s && values, "scene and values are required");
+    CHK(hipMemcpyAsync(s->val, values, sizeof(float) * (size_t)s->n_vertices, hipMemcpyDefault, S(stream)));
+    return 0;
+}
+
+int wos_scene_set_source(wos_scene *s, int source_type, const float *params, int n, void *stream) {
+    REQUIRE(s, "scene is NULL");
+    if (check_source(source_type, params, n)) return -1;
+    int need = source_type ? n : 0;
+    if (need > s->n_src) {
+        CHK(hipStreamSynchronize(S(stream)));
+        (void)hipFree(s->src);
+        s->src = nullptr;
+        CHK(hipMalloc((void **)&s->src, sizeof(float) * (size_t)need));
+    }
+    if (need) CHK(hipMemcpyAsync(s->src, params, sizeof(float) * (size_t)need, hipMemcpyDefault, S(stream)));
+    s->source_type = source_type;
+    s->n_src = need;
+    return 0;
+}
+
+int wos_scene_set_use_bvh(wos_scene *s, int use_bvh) {
+    REQUIRE(s, "scene is NULL");
+    s->use_bvh = use_bvh ? 1 : 0;
+    return 0;
+}
+
+int wos_scene_get_normals(const wos_scene *s, float *out, void *stream) {
+    REQUIRE(s && out, "scene and out are required");
+    CHK(hipMemcpyAsync(out, s->nrm, sizeof(float) * (size_t)s->n_vertices * s->dim, hipMemcpyDefault, S(stream)));
+    return 0;
+}
+
+int wos_scene_get_info(const wos_scene *s, float *amin, float *amax, int *dim, int *nv, int *np, int *depth) {
+    REQUIRE(s, "scene is NULL");
+    for (int k = 0; k < 3; ++k) {
+        if (amin) amin[k] = s->amin[k];
+        if (amax) amax[k] = s->amax[k];
+    }
+    if (dim) *dim = s->dim;
+    if (nv) *nv = s->n_vertices;
+    if (np) *np = s->n_prims;
+    if (depth) *depth = s->max_depth;
+    return 0;
+}
+
+int wos_scene_get_bvh(const wos_scene *s, int *sorted_prim, uint32_t *codes, int *children, void *stream) {
+    REQUIRE(s, "scene is NULL");
+    hipStream_t st = S(stream);
+    if (sorted_prim) CHK(hipMemcpyAsync(sorted_prim, s->sorted_prim, sizeof(int) * (size_t)s->n_prims, hipMemcpyDefault, st));
+    if (codes) CHK(hipMemcpyAsync(codes, s->sorted_codes, sizeof(uint32_t) * (size_t)s->n_prims, hipMemcpyDefault, st));
+    if (children && s->n_prims > 1)
+        CHK(hipMemcpyAsync(children, s->children, sizeof(int) * 2 * (size_t)(s->n_prims - 1), hipMemcpyDefault, st));
+    return 0;
+}
+
+int wos_closest_point_index(const wos_scene *s, const float *p, int n, int *out, float *out_d2, void *stream) {
+    REQUIRE(s && (n == 0 || (p && out)), "scene, p and out are required");
+    REQUIRE(n >= 0, "negative count");
+    return launch_closest_index(view_of(s), p, n, out, out_d2, S(stream));
+}
+
+int wos_sdf(const wos_scene *s, const float *p, int n, float *out, void *stream) {
+    REQUIRE(s && (n == 0 || (p && out)), "scene, p and out are required");
+    REQUIRE(n >= 0, "negative count");
+    return launch_sdf(view_of(s), p, n, out, S(stream));
+}
+
+int wos_closest_point(const wos_scene *s, const float *p, int n, const wos_record *o, void *stream) {
+    REQUIRE(s && o && (n == 0 || p), "scene, p and out are required");
+    REQUIRE(n >= 0, "negative count");
+    return launch_closest_point(view_of(s), p, n, o->valid, o->p, o->n, o->t, o->prim_id, o->contrib, S(stream));
+}
+
+int wos_largest_inscribed_ball(const wos_scene *s, const wos_record *its, int n, float epsilon, float *out,
+                               void *stream) {
+    REQUIRE(s && its && out && (n == 0 || (its->p && its->n)), "scene, its.p, its.n and out are required");
+    REQUIRE(n >= 0, "negative count");
+    return launch_lib(view_of(s), its->valid, its->p, its->n, n, epsilon, out, S(stream));
+}
+
+int wos_dirichlet(const wos_scene *s, const wos_record *rec, int n, float *out, void *stream) {
+    REQUIRE(s && rec && out && (n == 0 || (rec->prim_id && rec->t)), "scene, rec.prim_id, rec.t and out are required");
+    REQUIRE(n >= 0, "negative count");
+    return launch_dirichlet(view_of(s), rec->valid, rec->prim_id, rec->t, n, out, S(stream));
+}
+
+static int check_walk(const wos_walk_params *w) {
+    REQUIRE(w, "walk params are NULL");
+    REQUIRE(w->nwalks >= 1, "nwalks must be >= 1 (got %d)", w->nwalks);
+    REQUIRE(w->nsteps >= 0, "nsteps must be >= 0 (got %d)", w->nsteps);
+    REQUIRE(w->epsilon >= 0.0f, "epsilon must be >= 0");
+    return 0;
+}
+
+int wos_single_walk(const wos_scene *s, const wos_walk_params *w, const float *p, int n, uint64_t *state,
+                    uint64_t *inc, const wos_record *o, void *stream) {
+    REQUIRE(s && o && (n == 0 || (p && state && inc)), "scene, p, sampler state/inc and out are required");
+    REQUIRE(n >= 0, "negative count");
+    if (check_walk(w)) return -1;
+    WalkSettings ws{w->nwalks, w->nsteps, w->epsilon, w->double_sided, w->prevent_fd_artifacts};
+    return launch_single_walk(view_of(s), ws, p, n, state, inc, o->valid, o->p, o->n, o->t, o->prim_id, o->contrib,
+                              S(stream));
+}
+
+int wos_walk_reduce(const wos_scene *s, const wos_walk_params *w, const float *points, int n_points, int seed,
+                    int chunk, float *mean, double *mean64, double *var64, float *walk_value, int *rec_prim,
+                    float *rec_t, float *basis, void *stream) {
+    REQUIRE(s && (n_points == 0 || points), "scene and points are required");
+    REQUIRE(n_points >= 0, "negative count");
+    if (check_walk(w)) return -1;
+    REQUIRE((long long)n_points * w->nwalks < (1LL << 40), "too many walks");
+    REQUIRE(!(rec_prim && !rec_t) && !(rec_t && !rec_prim), "rec_prim and rec_t go together");
+    if (basis) REQUIRE(s->source_type == 3, "basis output needs a Fourier source (source_type 3)");
+    WalkSettings ws{w->nwalks, w->nsteps, w->epsilon, w->double_sided, w->prevent_fd_artifacts};
+    WalkOutputs o{walk_value, rec_prim, rec_t, basis, mean, mean64, var64};
+    return launch_walk_reduce(view_of(s), ws, points, n_points, (int32_t)seed, chunk, o, S(stream));
+}
+
+int wos_adjoint_values(const wos_scene *s, int n_points, int nwalks, const int *rec_prim, const float *rec_t,
+                       const float *basis, const float *grad_out, float *grad_values, float *grad_coeffs,
+                       void *stream) {
+    REQUIRE(s && (n_points == 0 || (rec_prim && rec_t && grad_out)), "scene, records and grad_out are required");
+    REQUIRE(n_points >= 0 && nwalks >= 1, "bad sizes");
+    if (basis) REQUIRE(s->source_type == 3, "basis input needs a Fourier source (source_type 3)");
+    return launch_adjoint_values(view_of(s), n_points, nwalks, rec_prim, rec_t, basis, grad_out, grad_values,
+                                 grad_coeffs, S(stream));
+}
+
+int wos_shape_gradient(const wos_scene *s, const wos_grad_params *g, const float *points, int n_points, int seed,
+                       const float *grad_out, float *grad_vertices, float *sample_grad_u, void *stream) {
+    REQUIRE(s && g && (n_points == 0 || points), "scene, params and points are required");
+    REQUIRE(n_points >= 0 && g->nwalks >= 1 && g->nsteps >= 0, "bad sizes");
+    REQUIRE(!grad_vertices || grad_out, "grad_vertices needs grad_out");
+    GradSettings gs{g->nwalks, g->nsteps, g->epsilon, g->epsilon2, g->clamping, g->double_sided, g->control_variates};
+    return launch_shape_gradient(view_of(s), gs, points, n_points, (int32_t)seed, grad_out, grad_vertices,
+                                 sample_grad_u, S(stream));
+}
+
+}  // extern "C"

@@ -1,0 +1,585 @@
+// wos_device.h -- device-side building blocks of the MI355X walk-on-spheres
+// path: counter-keyed PCG32 streams, elementary functions, closest-point
+// geometry, the LBVH closest-primitive traversal and the scene view that the
+// kernels receive by value.
+//
+// Floating point follows one canonical operation order (explicit fmaf, the
+// library is built with -ffp-contract=off, IEEE division and sqrt), so a
+// closest-primitive index is the same whether it comes from the brute-force
+// scan or the BVH, and agrees with a CPU evaluation of the same expressions.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace wos {
+
+constexpr float kPi = 3.14159265358979323846f;
+constexpr float kInv2Pi = 0.15915494309189533577f;
+constexpr int kBlock = 256;          // threads per workgroup (4 wave64)
+constexpr int kLdsStack = 16;        // traversal stack entries kept in LDS per lane
+
+// ---------------------------------------------------------------------------
+// random numbers
+// ---------------------------------------------------------------------------
+
+// PCG32 XSH-RR (include/sampler2.cuh:30-37).
+struct Pcg32 {
+    uint64_t state, inc;
+    __device__ __forceinline__ uint32_t next_u32() {
+        uint64_t old = state;
+        state = old * 0x5851f42d4c957f2dULL + inc;
+        uint32_t xs = (uint32_t)(((old >> 18u) ^ old) >> 27u);
+        uint32_t rot = (uint32_t)(old >> 59u);
+        return (xs >> rot) | (xs << ((0u - rot) & 31u));
+    }
+    // include/sampler2.cuh:20-28
+    __device__ __forceinline__ void seed(uint64_t index, uint64_t initstate, uint64_t initseq) {
+        state = 0;
+        inc = ((initseq + index) << 1) | 1u;
+        next_u32();
+        state += initstate;
+        next_u32();
+    }
+    // include/sampler2.cuh:39-44
+    __device__ __forceinline__ float next_1d() {
+        return __uint_as_float((next_u32() >> 9) | 0x3f800000u) - 1.0f;
+    }
+    // Dr.Jit next_float64: low word drawn first
+    __device__ __forceinline__ double next_f64() {
+        uint64_t lo = next_u32();
+        uint64_t hi = next_u32();
+        uint64_t v = ((lo | (hi << 32)) >> 12) | 0x3ff0000000000000ULL;
+        return __longlong_as_double((long long)v) - 1.0;
+    }
+};
+
+// wos/utils.py:250-256 on int32 lanes (wrapping add/shl, arithmetic shr).
+__device__ __forceinline__ void tea(int32_t v0_, int32_t v1_, int32_t &o0, int32_t &o1) {
+    uint32_t v0 = (uint32_t)v0_, v1 = (uint32_t)v1_, sum = 0;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        sum += 0x9e3779b9u;
+        v0 += ((v1 << 4) + 0xa341316cu) ^ (v1 + sum) ^ ((uint32_t)((int32_t)v1 >> 5) + 0xc8013ea4u);
+        v1 += ((v0 << 4) + 0xad90777du) ^ (v0 + sum) ^ ((uint32_t)((int32_t)v0 >> 5) + 0x7e95761eu);
+    }
+    o0 = (int32_t)v0;
+    o1 = (int32_t)v1;
+}
+
+// Stream of walk k = i*W + w of point i under Solver.walk_reduce
+// (wos/solver.py:54-60): TEA(seed, i) seeds a PCG32 whose sequence index is k.
+__device__ __forceinline__ Pcg32 walk_stream(int32_t seed, int64_t point, int64_t k) {
+    int32_t v0, v1;
+    tea(seed, (int32_t)point, v0, v1);
+    Pcg32 r;
+    r.seed((uint64_t)k, (uint64_t)(int64_t)v0, (uint64_t)(int64_t)v1);
+    return r;
+}
+
+// ---------------------------------------------------------------------------
+// elementary functions: Taylor polynomials after exact / Cody-Waite reduction
+// ---------------------------------------------------------------------------
+
+__device__ __forceinline__ void sincos_poly(float x, float &s, float &c) {
+    float x2 = x * x;
+    float ps = fmaf(x2, -2.50521083854417187751e-08f, 2.75573192239858906526e-06f);
+    ps = fmaf(x2, ps, -1.98412698412698412698e-04f);
+    ps = fmaf(x2, ps, 8.33333333333333333333e-03f);
+    ps = fmaf(x2, ps, -1.66666666666666666667e-01f);
+    s = fmaf(x * x2, ps, x);
+    float pc = fmaf(x2, 2.08767569878680989792e-09f, -2.75573192239858906526e-07f);
+    pc = fmaf(x2, pc, 2.48015873015873015873e-05f);
+    pc = fmaf(x2, pc, -1.38888888888888888889e-03f);
+    pc = fmaf(x2, pc, 4.16666666666666666667e-02f);
+    pc = fmaf(x2, pc, -0.5f);
+    c = fmaf(x2, pc, 1.0f);
+}
+
+__device__ __forceinline__ void apply_quadrant(int q, float s0, float c0, float &s, float &c) {
+    q &= 3;
+    float a = (q & 1) ? c0 : s0;
+    float b = (q & 1) ? s0 : c0;
+    s = (q & 2) ? -a : a;
+    c = ((q + 1) & 2) ? -b : b;
+}
+
+// sin and cos of 2*pi*u; the quadrant split of 4u is exact.
+__device__ __forceinline__ void sincos_turns(float u, float &s, float &c) {
+    float r = u * 4.0f;
+    float q = rintf(r);
+    float f = r - q;
+    float x = f * 1.57079632679489661923f;
+    float s0, c0;
+    sincos_poly(x, s0, c0);
+    apply_quadrant((int)q, s0, c0, s, c);
+}
+
+__device__ __forceinline__ float sin_cw(float x) {
+    float k = rintf(x * 0.636619772367581343076f);
+    float r = fmaf(-k, 1.57079637050628662109375f, x);
+    r = fmaf(-k, -4.37113900018624283e-08f, r);
+    r = fmaf(-k, -1.7151245100059209e-15f, r);
+    float s0, c0, s, c;
+    sincos_poly(r, s0, c0);
+    apply_quadrant((int)k, s0, c0, s, c);
+    return s;
+}
+
+__device__ __forceinline__ float log_poly(float x) {
+    uint32_t b = __float_as_uint(x);
+    int e = (int)((b >> 23) & 0xff) - 127;
+    float m = __uint_as_float((b & 0x007fffffu) | 0x3f800000u);
+    if (m > 1.41421356237309504880f) { m = m * 0.5f; e += 1; }
+    float s = (m - 1.0f) / (m + 1.0f);
+    float s2 = s * s;
+    float p = fmaf(s2, 0.0769230769230769230769f, 0.0909090909090909090909f);
+    p = fmaf(s2, p, 0.111111111111111111111f);
+    p = fmaf(s2, p, 0.142857142857142857143f);
+    p = fmaf(s2, p, 0.2f);
+    p = fmaf(s2, p, 0.333333333333333333333f);
+    float lm = fmaf(2.0f * s, s2 * p, 2.0f * s);
+    float fe = (float)e;
+    return fmaf(fe, 0.693147182464599609375f, fmaf(fe, -1.90465429995776e-09f, lm));
+}
+
+__device__ __forceinline__ float exp_poly(float x) {
+    if (x < -87.0f) return 0.0f;
+    if (x > 88.0f) return __builtin_inff();
+    float k = rintf(x * 1.44269504088896340736f);
+    float r = fmaf(-k, 0.693147182464599609375f, x);
+    r = fmaf(-k, -1.90465429995776e-09f, r);
+    float p = fmaf(r, 2.75573192239858906526e-06f, 2.48015873015873015873e-05f);
+    p = fmaf(r, p, 1.98412698412698412698e-04f);
+    p = fmaf(r, p, 1.38888888888888888889e-03f);
+    p = fmaf(r, p, 8.33333333333333333333e-03f);
+    p = fmaf(r, p, 4.16666666666666666667e-02f);
+    p = fmaf(r, p, 1.66666666666666666667e-01f);
+    p = fmaf(r, p, 0.5f);
+    p = fmaf(r, p, 1.0f);
+    p = fmaf(r, p, 1.0f);
+    int ki = (int)k;
+    int k1 = ki / 2, k2 = ki - k1;
+    float f1 = __uint_as_float((uint32_t)(k1 + 127) << 23);
+    float f2 = __uint_as_float((uint32_t)(k2 + 127) << 23);
+    return (p * f1) * f2;
+}
+
+// ---------------------------------------------------------------------------
+// geometry
+// ---------------------------------------------------------------------------
+
+__device__ __forceinline__ float dot2(float ax, float ay, float bx, float by) { return fmaf(ax, bx, ay * by); }
+__device__ __forceinline__ float dot3(const float *a, const float *b) {
+    return fmaf(a[0], b[0], fmaf(a[1], b[1], a[2] * b[2]));
+}
+__device__ __forceinline__ float clamp01(float h) { return fminf(fmaxf(h, 0.0f), 1.0f); }
+__device__ __forceinline__ float sgn(float x) { return x > 0.0f ? 1.0f : -1.0f; }  // include/fwd.cuh:122-128
+__device__ __forceinline__ void cross3(const float *a, const float *b, float *o) {
+    o[0] = fmaf(a[1], b[2], -(a[2] * b[1]));
+    o[1] = fmaf(a[2], b[0], -(a[0] * b[2]));
+    o[2] = fmaf(a[0], b[1], -(a[1] * b[0]));
+}
+
+// Squared distance to segment a + h*ba, h = clamp((p-a).ba * inv, 0, 1)
+// (include/util.cuh:108-114).
+__device__ __forceinline__ float seg_d2(float px, float py, float ax, float ay, float bax, float bay,
+                                        float inv, float &h, float &dx, float &dy) {
+    float pax = px - ax, pay = py - ay;
+    h = clamp01(dot2(pax, pay, bax, bay) * inv);
+    dx = fmaf(-bax, h, pax);
+    dy = fmaf(-bay, h, pay);
+    return dot2(dx, dy, dx, dy);
+}
+
+// Ericson closest point on a triangle (include/util.cuh:22-106).  Returns the
+// squared distance; bary = (weight of a, weight of b); q = closest point.
+__device__ __forceinline__ float tri_closest(const float *pa, const float *pb, const float *pc,
+                                             const float *x, float *q, float *bary) {
+    float ab[3] = {pb[0] - pa[0], pb[1] - pa[1], pb[2] - pa[2]};
+    float ac[3] = {pc[0] - pa[0], pc[1] - pa[1], pc[2] - pa[2]};
+    float ax[3] = {x[0] - pa[0], x[1] - pa[1], x[2] - pa[2]};
+    float d1 = dot3(ab, ax), d2 = dot3(ac, ax);
+    float bx[3] = {x[0] - pb[0], x[1] - pb[1], x[2] - pb[2]};
+    float d3 = dot3(ab, bx), d4 = dot3(ac, bx);
+    float cx[3] = {x[0] - pc[0], x[1] - pc[1], x[2] - pc[2]};
+    float d5 = dot3(ab, cx), d6 = dot3(ac, cx);
+    float vc = fmaf(d1, d4, -(d3 * d2));
+    float vb = fmaf(d5, d2, -(d1 * d6));
+    float va = fmaf(d3, d6, -(d5 * d4));
+    float e43 = d4 - d3, e56 = d5 - d6;
+    if (d1 <= 0.0f && d2 <= 0.0f) {
+        bary[0] = 1.0f; bary[1] = 0.0f;
+        q[0] = pa[0]; q[1] = pa[1]; q[2] = pa[2];
+    } else if (d3 >= 0.0f && d4 <= d3) {
+        bary[0] = 0.0f; bary[1] = 1.0f;
+        q[0] = pb[0]; q[1] = pb[1]; q[2] = pb[2];
+    } else if (d6 >= 0.0f && d5 <= d6) {
+        bary[0] = 0.0f; bary[1] = 0.0f;
+        q[0] = pc[0]; q[1] = pc[1]; q[2] = pc[2];
+    } else if (vc <= 0.0f && d1 >= 0.0f && d3 <= 0.0f) {
+        float v = d1 / (d1 - d3);
+        bary[0] = 1.0f - v; bary[1] = v;
+        for (int k = 0; k < 3; ++k) q[k] = fmaf(ab[k], v, pa[k]);
+    } else if (vb <= 0.0f && d2 >= 0.0f && d6 <= 0.0f) {
+        float w = d2 / (d2 - d6);
+        bary[0] = 1.0f - w; bary[1] = 0.0f;
+        for (int k = 0; k < 3; ++k) q[k] = fmaf(ac[k], w, pa[k]);
+    } else if (va <= 0.0f && e43 >= 0.0f && e56 >= 0.0f) {
+        float w = e43 / (e43 + e56);
+        bary[0] = 0.0f; bary[1] = 1.0f - w;
+        for (int k = 0; k < 3; ++k) q[k] = fmaf(pc[k] - pb[k], w, pb[k]);
+    } else {
+        float denom = 1.0f / (va + vb + vc);
+        float v = vb * denom, w = vc * denom;
+        bary[0] = 1.0f - v - w; bary[1] = v;
+        for (int k = 0; k < 3; ++k) q[k] = fmaf(ac[k], w, fmaf(ab[k], v, pa[k]));
+    }
+    float dd[3] = {x[0] - q[0], x[1] - q[1], x[2] - q[2]};
+    return dot3(dd, dd);
+}
+
+// ---------------------------------------------------------------------------
+// scene view (passed by value to every kernel)
+// ---------------------------------------------------------------------------
+//
+// BVH node layout (binary LBVH, children's boxes stored in the parent):
+//   2D node: 3 x float4  {Lminx, Lminy, Lmaxx, Lmaxy} {R...} {left, right, -, -}
+//   3D node: 4 x float4  {Lmin.xyz, Lmax.x} {Lmax.yz, Rmin.xy} {Rmin.z, Rmax.xyz} {left, right, -, -}
+//   child >= 0: internal node index; child < 0: leaf ~j (j = Morton order)
+//   2D leaf: 2 x float4  {a.xy, (b-a).xy} {1/|b-a|^2, prim, -, -}
+//   3D leaf: 3 x float4  {a.xyz, b.x} {b.yz, c.xy} {c.z, prim, -, -}
+struct SceneView {
+    int dim, n_vertices, n_prims, use_bvh;
+    const float *vtx;      // n_vertices * dim
+    const int *idx;        // n_prims * dim
+    const float *val;      // n_vertices
+    const float *nrm;      // n_vertices * 2 (2D vertex normals)
+    float amin[3], amax[3];
+    int source_type, n_src;
+    const float *src;      // source parameters (device)
+    const float4 *nodes;   // n_prims - 1 internal nodes
+    const float4 *leaves;  // n_prims leaves
+    float tol;             // absolute pruning margin (scale-relative)
+    int max_depth;
+};
+
+// Per-lane traversal stack: the first kLdsStack entries live in LDS (lane
+// stride kBlock), deeper entries spill to a global buffer (lane stride
+// spill_stride).  Depth never exceeds the tree depth, which sizes the spill.
+struct TravStack {
+    int *lds_node;
+    float *lds_d2;
+    int2 *spill;          // may be null when the tree fits the LDS part
+    long spill_stride;
+};
+
+// Box distance^2 from p to [lo, hi].
+__device__ __forceinline__ float box_d2_2(float px, float py, float4 b) {
+    float dx = fmaxf(fmaxf(b.x - px, px - b.z), 0.0f);
+    float dy = fmaxf(fmaxf(b.y - py, py - b.w), 0.0f);
+    return fmaf(dx, dx, dy * dy);
+}
+__device__ __forceinline__ float box_d2_3(const float *p, float lx, float ly, float lz, float hx, float hy,
+                                          float hz) {
+    float dx = fmaxf(fmaxf(lx - p[0], p[0] - hx), 0.0f);
+    float dy = fmaxf(fmaxf(ly - p[1], p[1] - hy), 0.0f);
+    float dz = fmaxf(fmaxf(lz - p[2], p[2] - hz), 0.0f);
+    return fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+}
+
+// (d2, prim) lexicographic minimum: equal squared distances keep the
+// smallest primitive index, which is what the in-order brute-force scan with
+// a strict '<' returns (include/scene.cuh:102-128).
+__device__ __forceinline__ bool better(float d2, int id, float best_d2, int best_id) {
+    return d2 < best_d2 || (d2 == best_d2 && id < best_id);
+}
+
+// Prune threshold for boxes: (best + tol)^2, which absorbs the rounding gap
+// between a box distance and the distance of a primitive on its face.
+__device__ __forceinline__ float prune_threshold(float best_d2, float tol) {
+    float d = sqrtf(best_d2) + tol;
+    return d * d;
+}
+
+template <int D>
+__device__ __forceinline__ float leaf_d2(const SceneView &s, int j, const float *p, int &prim) {
+    if constexpr (D == 2) {
+        float4 l0 = s.leaves[2 * j], l1 = s.leaves[2 * j + 1];
+        prim = __float_as_int(l1.y);
+        float h, dx, dy;
+        return seg_d2(p[0], p[1], l0.x, l0.y, l0.z, l0.w, l1.x, h, dx, dy);
+    } else {
+        float4 l0 = s.leaves[3 * j], l1 = s.leaves[3 * j + 1], l2 = s.leaves[3 * j + 2];
+        prim = __float_as_int(l2.y);
+        float a[3] = {l0.x, l0.y, l0.z}, b[3] = {l0.w, l1.x, l1.y}, c[3] = {l1.z, l1.w, l2.x};
+        float q[3], bary[2];
+        return tri_closest(a, b, c, p, q, bary);
+    }
+}
+
+// Closest primitive by brute force in index order (include/scene.cuh:102-128).
+template <int D>
+__device__ __forceinline__ int closest_brute(const SceneView &s, const float *p, float &best_d2) {
+    float best = __builtin_inff();
+    int id = -1;
+    for (int f = 0; f < s.n_prims; ++f) {
+        float d2;
+        if constexpr (D == 2) {
+            int i0 = s.idx[2 * f], i1 = s.idx[2 * f + 1];
+            float ax = s.vtx[2 * i0], ay = s.vtx[2 * i0 + 1];
+            float bax = s.vtx[2 * i1] - ax, bay = s.vtx[2 * i1 + 1] - ay;
+            float inv = 1.0f / dot2(bax, bay, bax, bay);
+            float h, dx, dy;
+            d2 = seg_d2(p[0], p[1], ax, ay, bax, bay, inv, h, dx, dy);
+        } else {
+            int i0 = s.idx[3 * f], i1 = s.idx[3 * f + 1], i2 = s.idx[3 * f + 2];
+            float a[3] = {s.vtx[3 * i0], s.vtx[3 * i0 + 1], s.vtx[3 * i0 + 2]};
+            float b[3] = {s.vtx[3 * i1], s.vtx[3 * i1 + 1], s.vtx[3 * i1 + 2]};
+            float c[3] = {s.vtx[3 * i2], s.vtx[3 * i2 + 1], s.vtx[3 * i2 + 2]};
+            float q[3], bary[2];
+            d2 = tri_closest(a, b, c, p, q, bary);
+        }
+        if (d2 < best) { best = d2; id = f; }
+    }
+    best_d2 = best;
+    return id;
+}
+
+// Nearest-first LBVH traversal.  bound_d2: squared distance within which the
+// closest primitive is known to lie (infinity when unknown).  Returns the
+// primitive index (-1 if none lies within the bound) and its squared distance.
+template <int D>
+__device__ int closest_bvh(const SceneView &s, const float *p, float bound_d2, float &best_d2,
+                           const TravStack &st, int lane_slot, long spill_lane) {
+    float best = __builtin_inff();
+    int best_id = 0x7fffffff;
+    float thr = bound_d2;
+    if (s.n_prims == 1) {
+        int prim;
+        float d2 = leaf_d2<D>(s, 0, p, prim);
+        best_d2 = d2;
+        return prim;
+    }
+    int node = 0, sp = 0;
+    for (;;) {
+        float dL, dR;
+        int cL, cR;
+        if constexpr (D == 2) {
+            float4 b0 = s.nodes[3 * node], b1 = s.nodes[3 * node + 1], c = s.nodes[3 * node + 2];
+            dL = box_d2_2(p[0], p[1], b0);
+            dR = box_d2_2(p[0], p[1], b1);
+            cL = __float_as_int(c.x);
+            cR = __float_as_int(c.y);
+        } else {
+            float4 n0 = s.nodes[4 * node], n1 = s.nodes[4 * node + 1], n2 = s.nodes[4 * node + 2],
+                   c = s.nodes[4 * node + 3];
+            dL = box_d2_3(p, n0.x, n0.y, n0.z, n0.w, n1.x, n1.y);
+            dR = box_d2_3(p, n1.z, n1.w, n2.x, n2.y, n2.z, n2.w);
+            cL = __float_as_int(c.x);
+            cR = __float_as_int(c.y);
+        }
+        bool goL = false, goR = false;
+        if (dL <= thr) {
+            if (cL < 0) {
+                int prim;
+                float d2 = leaf_d2<D>(s, ~cL, p, prim);
+                if (better(d2, prim, best, best_id)) {
+                    best = d2; best_id = prim;
+                    thr = fminf(thr, prune_threshold(best, s.tol));
+                }
+            } else goL = true;
+        }
+        if (dR <= thr) {
+            if (cR < 0) {
+                int prim;
+                float d2 = leaf_d2<D>(s, ~cR, p, prim);
+                if (better(d2, prim, best, best_id)) {
+                    best = d2; best_id = prim;
+                    thr = fminf(thr, prune_threshold(best, s.tol));
+                }
+            } else goR = true;
+        }
+        goL = goL && dL <= thr;
+        if (goL && goR) {
+            bool leftNear = dL <= dR;
+            int nearN = leftNear ? cL : cR, farN = leftNear ? cR : cL;
+            float farD = leftNear ? dR : dL;
+            if (sp < kLdsStack) {
+                st.lds_node[sp * kBlock + lane_slot] = farN;
+                st.lds_d2[sp * kBlock + lane_slot] = farD;
+            } else {
+                st.spill[(long)(sp - kLdsStack) * st.spill_stride + spill_lane] = make_int2(farN, __float_as_int(farD));
+            }
+            ++sp;
+            node = nearN;
+            continue;
+        }
+        if (goL) { node = cL; continue; }
+        if (goR) { node = cR; continue; }
+        // pop until an entry survives the current threshold
+        bool found = false;
+        while (sp > 0) {
+            --sp;
+            int n;
+            float dd;
+            if (sp < kLdsStack) {
+                n = st.lds_node[sp * kBlock + lane_slot];
+                dd = st.lds_d2[sp * kBlock + lane_slot];
+            } else {
+                int2 e = st.spill[(long)(sp - kLdsStack) * st.spill_stride + spill_lane];
+                n = e.x;
+                dd = __int_as_float(e.y);
+            }
+            if (dd <= thr) { node = n; found = true; break; }
+        }
+        if (!found) break;
+    }
+    best_d2 = best;
+    return best_id == 0x7fffffff ? -1 : best_id;
+}
+
+// Closest primitive with an optional distance bound; falls back to an
+// unbounded search when nothing lies inside the bound.
+template <int D>
+__device__ __forceinline__ int closest_prim(const SceneView &s, const float *p, float bound_d2, float &best_d2,
+                                            const TravStack &st, int lane_slot, long spill_lane) {
+    if (!s.use_bvh) return closest_brute<D>(s, p, best_d2);
+    int id = closest_bvh<D>(s, p, bound_d2, best_d2, st, lane_slot, spill_lane);
+    if (id < 0 && bound_d2 < __builtin_inff())
+        id = closest_bvh<D>(s, p, __builtin_inff(), best_d2, st, lane_slot, spill_lane);
+    return id;
+}
+
+// ---------------------------------------------------------------------------
+// scene queries on a known closest primitive
+// ---------------------------------------------------------------------------
+
+struct Rec {
+    int valid;
+    float p[3], n[3];
+    float t;      // 2D segment parameter
+    float uv[2];  // 3D barycentrics of (b, c)
+    float d;
+    int prim;
+};
+
+// include/scene.cuh:148-178 (3D: only the sign of n.(p-x) is used, so the
+// normal is not normalised).
+template <int D>
+__device__ __forceinline__ float signed_dist(const SceneView &s, int f, const float *p) {
+    if constexpr (D == 2) {
+        int i0 = s.idx[2 * f], i1 = s.idx[2 * f + 1];
+        float ax = s.vtx[2 * i0], ay = s.vtx[2 * i0 + 1];
+        float bax = s.vtx[2 * i1] - ax, bay = s.vtx[2 * i1 + 1] - ay;
+        float inv = 1.0f / dot2(bax, bay, bax, bay);
+        float h, dx, dy;
+        float d2 = seg_d2(p[0], p[1], ax, ay, bax, bay, inv, h, dx, dy);
+        float nax = s.nrm[2 * i0], nay = s.nrm[2 * i0 + 1];
+        float nx = fmaf(h, s.nrm[2 * i1] - nax, nax), ny = fmaf(h, s.nrm[2 * i1 + 1] - nay, nay);
+        return sqrtf(d2) * sgn(dot2(nx, ny, dx, dy));
+    } else {
+        int i0 = s.idx[3 * f], i1 = s.idx[3 * f + 1], i2 = s.idx[3 * f + 2];
+        float a[3] = {s.vtx[3 * i0], s.vtx[3 * i0 + 1], s.vtx[3 * i0 + 2]};
+        float b[3] = {s.vtx[3 * i1], s.vtx[3 * i1 + 1], s.vtx[3 * i1 + 2]};
+        float c[3] = {s.vtx[3 * i2], s.vtx[3 * i2 + 1], s.vtx[3 * i2 + 2]};
+        float q[3], bary[2];
+        float d2 = tri_closest(a, b, c, p, q, bary);
+        float ab[3] = {b[0] - a[0], b[1] - a[1], b[2] - a[2]};
+        float ac[3] = {c[0] - a[0], c[1] - a[1], c[2] - a[2]};
+        float n[3];
+        cross3(ab, ac, n);
+        float px[3] = {p[0] - q[0], p[1] - q[1], p[2] - q[2]};
+        return sqrtf(d2) * sgn(dot3(n, px));
+    }
+}
+
+// include/scene.cuh:180-223
+template <int D>
+__device__ __forceinline__ void closest_record(const SceneView &s, int f, const float *p, Rec &r) {
+    r.valid = 1;
+    r.prim = f;
+    if constexpr (D == 2) {
+        int i0 = s.idx[2 * f], i1 = s.idx[2 * f + 1];
+        float ax = s.vtx[2 * i0], ay = s.vtx[2 * i0 + 1];
+        float bax = s.vtx[2 * i1] - ax, bay = s.vtx[2 * i1 + 1] - ay;
+        float inv = 1.0f / dot2(bax, bay, bax, bay);
+        float h, dx, dy;
+        float d2 = seg_d2(p[0], p[1], ax, ay, bax, bay, inv, h, dx, dy);
+        float pax = p[0] - ax, pay = p[1] - ay;
+        float l = sqrtf(dot2(bay, -bax, bay, -bax));
+        float nx = bay / l, ny = -bax / l;
+        float sg = sgn(dot2(nx, ny, pax, pay));
+        r.p[0] = fmaf(h, bax, ax);
+        r.p[1] = fmaf(h, bay, ay);
+        r.p[2] = 0.0f;
+        r.n[0] = sg * nx;
+        r.n[1] = sg * ny;
+        r.n[2] = 0.0f;
+        r.t = h;
+        r.uv[0] = r.uv[1] = 0.0f;
+        r.d = sqrtf(d2);
+    } else {
+        int i0 = s.idx[3 * f], i1 = s.idx[3 * f + 1], i2 = s.idx[3 * f + 2];
+        float a[3] = {s.vtx[3 * i0], s.vtx[3 * i0 + 1], s.vtx[3 * i0 + 2]};
+        float b[3] = {s.vtx[3 * i1], s.vtx[3 * i1 + 1], s.vtx[3 * i1 + 2]};
+        float c[3] = {s.vtx[3 * i2], s.vtx[3 * i2 + 1], s.vtx[3 * i2 + 2]};
+        float q[3], bary[2];
+        float d2 = tri_closest(a, b, c, p, q, bary);
+        float ab[3] = {b[0] - a[0], b[1] - a[1], b[2] - a[2]};
+        float ac[3] = {c[0] - a[0], c[1] - a[1], c[2] - a[2]};
+        float n[3];
+        cross3(ab, ac, n);
+        float l = sqrtf(dot3(n, n));
+        n[0] = n[0] / l; n[1] = n[1] / l; n[2] = n[2] / l;
+        float pa[3] = {p[0] - a[0], p[1] - a[1], p[2] - a[2]};
+        float sg = sgn(dot3(n, pa));
+        for (int k = 0; k < 3; ++k) { r.p[k] = q[k]; r.n[k] = sg * n[k]; }
+        r.uv[0] = bary[1];
+        r.uv[1] = 1.0f - bary[0] - bary[1];
+        r.t = 0.0f;
+        r.d = sqrtf(d2);
+    }
+}
+
+// include/scene.cuh:251-264 (3D: b*u + c*v + a*(1-u-v), wos/utils.py:13-14)
+template <int D>
+__device__ __forceinline__ float dirichlet_at(const SceneView &s, int f, float t, float u, float v) {
+    if constexpr (D == 2) {
+        float a = s.val[s.idx[2 * f]], b = s.val[s.idx[2 * f + 1]];
+        return fmaf(t, b - a, a);
+    } else {
+        float a = s.val[s.idx[3 * f]], b = s.val[s.idx[3 * f + 1]], c = s.val[s.idx[3 * f + 2]];
+        return fmaf(b, u, fmaf(c, v, a * (1.0f - u - v)));
+    }
+}
+
+// include/scene.cuh:278-301, plus type 3: sum_k c_k sin(ax_k x + ay_k y + ph_k)
+__device__ __forceinline__ float source_at(const SceneView &s, float x, float y) {
+    const float *q = s.src;
+    if (s.source_type == 1) {
+        float ex = x - q[0], ey = y - q[1];
+        float dx = ex * ex / (2.0f * q[2] * q[2]);
+        float dy = ey * ey / (2.0f * q[3] * q[3]);
+        return q[4] * exp_poly(-dx - dy);
+    } else if (s.source_type == 2) {
+        float phi = fmaf(q[0], x, fmaf(q[1], y, q[2]));
+        return fmaf(q[3], sin_cw(phi), q[4]);
+    } else if (s.source_type == 3) {
+        float f = 0.0f;
+        int K = s.n_src / 4;
+        for (int k = 0; k < K; ++k) {
+            float phi = fmaf(q[4 * k], x, fmaf(q[4 * k + 1], y, q[4 * k + 2]));
+            f = fmaf(q[4 * k + 3], sin_cw(phi), f);
+        }
+        return f;
+    }
+    return 0.0f;
+}
+
+// wave helpers ---------------------------------------------------------------
+__device__ __forceinline__ int lane_id() { return __lane_id(); }
+__device__ __forceinline__ int rank_in_mask(uint64_t m) {
+    return (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+}
+
+}  // namespace wos

@@ -1,0 +1,880 @@
+// wos_kernels.hip -- walk-on-spheres kernels for gfx950.
+//
+//   k_closest_index / k_sdf / k_closest_point / k_lib / k_dirichlet
+//       Scene<N>::closestPointPreliminary / sdf / closestPoint /
+//       largestInscribedBall / dirichlet (include/scene.cuh:102-264, 497-704)
+//   k_single_walk      WoS<N>::singleWalk (include/wos.cuh:374-533): one lane
+//                      per walk, sampler state read and written back
+//   k_walk_persistent  Solver.walk_reduce (wos/solver.py:49-65) over
+//                      WoSDevice::walk (include/wos.cuh:97-145): persistent
+//                      waves that refill finished lanes from a chunked queue
+//   k_reduce           per-point FP64 mean / variance of the walk values
+//   k_adjoint_values   reverse mode of the walk value w.r.t. vertex values and
+//                      Fourier source coefficients, from stored exit records
+//   k_shape_gradient   OursCUDA / Ours3D (wos/wos_grad.py:442-463,
+//                      wos/wos_grad_3d.py:295-315) transposed onto vertices
+#include "wos_build.h"
+
+namespace wos {
+
+constexpr int kMaxBasis = 16;   // Fourier source terms carried per lane
+constexpr int kGridCap = 4096;  // workgroups of a grid-stride launch
+
+#define KCHK()                                                        \
+    do {                                                              \
+        hipError_t e_ = hipGetLastError();                            \
+        if (e_ != hipSuccess) {                                       \
+            set_error("kernel launch: %s", hipGetErrorString(e_));    \
+            return -1;                                                \
+        }                                                             \
+    } while (0)
+
+static int grid_of(long n, int cap = kGridCap) {
+    long g = (n + kBlock - 1) / kBlock;
+    if (g < 1) g = 1;
+    if (g > cap) g = cap;
+    return (int)g;
+}
+
+#define DECLARE_STACK(spill_, stride_)                                          \
+    __shared__ int s_stack_node[kLdsStack * kBlock];                            \
+    __shared__ float s_stack_d2[kLdsStack * kBlock];                            \
+    TravStack st{s_stack_node, s_stack_d2, (spill_), (stride_)};                \
+    const int slot = threadIdx.x;                                               \
+    const long lane_g = (long)blockIdx.x * kBlock + threadIdx.x;
+
+__device__ __forceinline__ float bound_d2(const SceneView &s, float r) {
+    float b = fmaf(r, 1.00001f, 2.0f * s.tol);
+    return b * b;
+}
+
+template <int D>
+__device__ __forceinline__ void load_point(const float *p, long i, float *q) {
+    q[0] = p[D * i];
+    q[1] = p[D * i + 1];
+    q[2] = (D == 3) ? p[D * i + 2] : 0.0f;
+}
+
+// ---------------------------------------------------------------------------
+// point queries
+// ---------------------------------------------------------------------------
+template <int D>
+__global__ __launch_bounds__(kBlock) void k_closest_index(SceneView s, const float *p, int n, int *out,
+                                                          float *out_d2, int2 *spill, long stride) {
+    DECLARE_STACK(spill, stride);
+    for (long i = lane_g; i < n; i += (long)gridDim.x * kBlock) {
+        float q[3];
+        load_point<D>(p, i, q);
+        float d2;
+        int id = closest_prim<D>(s, q, __builtin_inff(), d2, st, slot, lane_g);
+        out[i] = id;
+        if (out_d2) out_d2[i] = d2;
+    }
+}
+
+template <int D>
+__global__ __launch_bounds__(kBlock) void k_sdf(SceneView s, const float *p, int n, float *out, int2 *spill,
+                                                long stride) {
+    DECLARE_STACK(spill, stride);
+    for (long i = lane_g; i < n; i += (long)gridDim.x * kBlock) {
+        float q[3];
+        load_point<D>(p, i, q);
+        float d2;
+        int id = closest_prim<D>(s, q, __builtin_inff(), d2, st, slot, lane_g);
+        out[i] = id < 0 ? __builtin_inff() : signed_dist<D>(s, id, q);
+    }
+}
+
+template <int D>
+__device__ __forceinline__ void store_record(const Rec &r, long i, uint8_t *valid, float *op, float *on, float *ot,
+                                             int *oprim, float *ocontrib, float contrib) {
+    if (valid) valid[i] = (uint8_t)r.valid;
+    if (op)
+        for (int k = 0; k < D; ++k) op[D * i + k] = r.valid ? r.p[k] : 0.0f;
+    if (on)
+        for (int k = 0; k < D; ++k) on[D * i + k] = r.valid ? r.n[k] : 0.0f;
+    if (ot) {
+        if (D == 2) ot[i] = r.valid ? r.t : 0.0f;
+        else { ot[2 * i] = r.valid ? r.uv[0] : 0.0f; ot[2 * i + 1] = r.valid ? r.uv[1] : 0.0f; }
+    }
+    if (oprim) oprim[i] = r.valid ? r.prim : -1;
+    if (ocontrib) ocontrib[i] = contrib;
+}
+
+template <int D>
+__global__ __launch_bounds__(kBlock) void k_closest_point(SceneView s, const float *p, int n, uint8_t *valid,
+                                                          float *op, float *on, float *ot, int *oprim,
+                                                          float *ocontrib, int2 *spill, long stride) {
+    DECLARE_STACK(spill, stride);
+    for (long i = lane_g; i < n; i += (long)gridDim.x * kBlock) {
+        float q[3];
+        load_point<D>(p, i, q);
+        float d2;
+        int id = closest_prim<D>(s, q, __builtin_inff(), d2, st, slot, lane_g);
+        Rec r;
+        r.valid = 0;
+        if (id >= 0) closest_record<D>(s, id, q, r);
+        store_record<D>(r, i, valid, op, on, ot, oprim, ocontrib, 0.0f);
+    }
+}
+
+// include/scene.cuh:225-239; the sdf at p + n m lies within m (+ rounding) of
+// the boundary point p, which bounds the search.
+template <int D>
+__device__ float lib_one(const SceneView &s, const float *p, const float *nrm, float eps, const TravStack &st,
+                         int slot, long lane_g) {
+    float b = fminf(fabsf(s.amax[0] - s.amin[0]), fabsf(s.amax[1] - s.amin[1]));
+    if (D == 3) b = fminf(b, fabsf(s.amax[2] - s.amin[2]));
+    b = b / 2.0f;
+    float a = eps;
+    for (int it = 0; it < 10; ++it) {
+        float m = (a + b) / 2.0f;
+        float q[3] = {fmaf(nrm[0], m, p[0]), fmaf(nrm[1], m, p[1]), D == 3 ? fmaf(nrm[2], m, p[2]) : 0.0f};
+        float d2;
+        int id = closest_prim<D>(s, q, bound_d2(s, m), d2, st, slot, lane_g);
+        float d = fabsf(signed_dist<D>(s, id, q));
+        if (fabsf(d - m) < 1e-5f) a = m; else b = m;
+    }
+    return a;
+}
+
+template <int D>
+__global__ __launch_bounds__(kBlock) void k_lib(SceneView s, const uint8_t *valid, const float *p, const float *nrm,
+                                                int n, float eps, float *out, int2 *spill, long stride) {
+    DECLARE_STACK(spill, stride);
+    for (long i = lane_g; i < n; i += (long)gridDim.x * kBlock) {
+        if (valid && !valid[i]) { out[i] = 0.0f; continue; }
+        float q[3], nn[3];
+        load_point<D>(p, i, q);
+        load_point<D>(nrm, i, nn);
+        out[i] = lib_one<D>(s, q, nn, eps, st, slot, lane_g);
+    }
+}
+
+template <int D>
+__global__ void k_dirichlet(SceneView s, const uint8_t *valid, const int *prim, const float *t, int n, float *out) {
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+        int f = prim[i];
+        bool ok = (!valid || valid[i]) && f >= 0 && f < s.n_prims;
+        float v = 0.0f;
+        if (ok) v = (D == 2) ? dirichlet_at<2>(s, f, t[i], 0.0f, 0.0f) : dirichlet_at<3>(s, f, 0.0f, t[2 * i], t[2 * i + 1]);
+        out[i] = v;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// one walk on one lane (include/wos.cuh:97-145)
+// ---------------------------------------------------------------------------
+template <int D>
+__device__ __forceinline__ void source_step(const SceneView &s, const float *p, float d, Pcg32 &rng,
+                                            float &contrib) {
+    // include/wos.cuh:115-139: uniform point in the ball, G_2D / pdf
+    float u0 = rng.next_1d(), u1 = rng.next_1d();
+    float sn, cs;
+    sincos_turns(u0, sn, cs);
+    float r = sqrtf(u1);
+    float dyx = (r * cs) * d, dyy = (r * sn) * d;
+    float area = kPi * (d * d);
+    float pdf = 1.0f / area;
+    float rr = fmaxf(sqrtf(dot2(dyx, dyy, dyx, dyy)), 1e-4f);
+    float G = log_poly(d / rr) * kInv2Pi;
+    if (pdf > 1e-5f) contrib = fmaf(source_at(s, p[0] + dyx, p[1] + dyy) * G, area, contrib);
+}
+
+template <int D>
+__device__ __forceinline__ void move_point(float *p, float d, Pcg32 &rng) {
+    if constexpr (D == 2) {
+        float u = rng.next_1d();
+        float sn, cs;
+        sincos_turns(u, sn, cs);
+        p[0] = fmaf(cs, d, p[0]);
+        p[1] = fmaf(sn, d, p[1]);
+    } else {
+        float u0 = rng.next_1d(), u1 = rng.next_1d();
+        float z = 1.0f - 2.0f * u0;
+        float r = sqrtf(fmaxf(0.0f, 1.0f - z * z));
+        float sn, cs;
+        sincos_turns(u1, sn, cs);
+        p[0] = fmaf(r * cs, d, p[0]);
+        p[1] = fmaf(r * sn, d, p[1]);
+        p[2] = fmaf(z, d, p[2]);
+    }
+}
+
+// init_prim >= 0: the closest primitive of p0 is already known (init_d2).
+template <int D>
+__device__ void walk_one(const SceneView &s, const WalkSettings &w, const float *p0, Pcg32 &rng, int init_prim,
+                         float init_d2, const TravStack &st, int slot, long lane_g, Rec &rec, float &contrib) {
+    float p[3] = {p0[0], p0[1], D == 3 ? p0[2] : 0.0f};
+    rec.valid = 0;
+    rec.prim = -1;
+    contrib = 0.0f;
+    float d2;
+    int prim = init_prim;
+    if (prim < 0) prim = closest_prim<D>(s, p, __builtin_inff(), d2, st, slot, lane_g);
+    else d2 = init_d2;
+    if (prim < 0) return;
+    float sd = signed_dist<D>(s, prim, p);
+    if (sd >= 0.0f && !w.double_sided) return;
+    float d = fabsf(sd);
+    if (w.prevent_fd_artifacts) d *= 0.5f;
+    const bool src = (D == 2) && s.source_type != 0;
+    for (int i = 0; i < w.nsteps; ++i) {
+        if (d < w.epsilon) {
+            closest_record<D>(s, prim, p, rec);
+            return;
+        }
+        if (src) source_step<D>(s, p, d, rng, contrib);
+        move_point<D>(p, d, rng);
+        prim = closest_prim<D>(s, p, bound_d2(s, 2.0f * d), d2, st, slot, lane_g);
+        d = sqrtf(d2);
+    }
+}
+
+template <int D>
+__global__ __launch_bounds__(kBlock) void k_single_walk(SceneView s, WalkSettings w, const float *p, int n,
+                                                        uint64_t *state, uint64_t *inc, uint8_t *valid, float *op,
+                                                        float *on, float *ot, int *oprim, float *ocontrib,
+                                                        int2 *spill, long stride) {
+    DECLARE_STACK(spill, stride);
+    for (long i = lane_g; i < n; i += (long)gridDim.x * kBlock) {
+        float q[3];
+        load_point<D>(p, i, q);
+        Pcg32 rng;
+        rng.state = state[i];
+        rng.inc = inc[i];
+        Rec r;
+        float contrib;
+        walk_one<D>(s, w, q, rng, -1, 0.0f, st, slot, lane_g, r, contrib);
+        store_record<D>(r, i, valid, op, on, ot, oprim, ocontrib, r.valid ? contrib : 0.0f);
+        state[i] = rng.state;
+        inc[i] = rng.inc;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// persistent walk kernel
+// ---------------------------------------------------------------------------
+struct PersistArgs {
+    const float *points;
+    int n_points;
+    WalkSettings w;
+    int32_t seed;
+    long long total;                 // n_points * nwalks
+    unsigned long long *counter;     // zero before launch
+    int chunk;                       // walks claimed per queue visit
+    float *walk_value;
+    int *rec_prim;
+    float *rec_t;
+    float *basis;
+    int K;
+    int2 *spill;
+    long spill_stride;
+};
+
+enum : int { kIdle = 0, kInit = 1, kStep = 2 };
+
+// Each wave keeps 64 walks in flight.  Lanes whose walk ends take the next
+// walk index of the wave's chunk (ballot + mbcnt), and the wave claims a new
+// chunk from the global counter when its chunk runs dry, so divergent walk
+// lengths do not idle the SIMD.  Every walk re-derives its random stream from
+// (seed, point, walk index), so results do not depend on the schedule.
+template <int D, bool SRC, bool BASIS>
+__global__ __launch_bounds__(kBlock) void k_walk_persistent(SceneView s, PersistArgs a) {
+    DECLARE_STACK(a.spill, a.spill_stride);
+    const int lane = __lane_id();
+    long long k = -1;
+    int mode = kIdle;
+    float p[3] = {0.0f, 0.0f, 0.0f};
+    float d = 0.0f, contrib = 0.0f;
+    int step = 0, prim = -1;
+    Pcg32 rng;
+    rng.state = rng.inc = 0;
+    float bas[BASIS ? kMaxBasis : 1];
+#pragma unroll
+    for (int q = 0; q < (BASIS ? kMaxBasis : 1); ++q) bas[q] = 0.0f;
+    long long c_next = 0, c_end = 0;
+    bool exhausted = false;
+    const int W = a.w.nwalks;
+
+    for (;;) {
+        // 1. lanes whose walk has ended write their result
+        if (mode == kStep) {
+            bool done_valid = d < a.w.epsilon && step < a.w.nsteps;
+            bool done_invalid = !done_valid && step >= a.w.nsteps;
+            if (done_valid || done_invalid) {
+                float v = 0.0f, t0 = 0.0f, t1 = 0.0f;
+                int pr = -1;
+                if (done_valid) {
+                    Rec r;
+                    closest_record<D>(s, prim, p, r);
+                    v = dirichlet_at<D>(s, prim, r.t, r.uv[0], r.uv[1]) + contrib;
+                    pr = prim;
+                    t0 = (D == 2) ? r.t : r.uv[0];
+                    t1 = r.uv[1];
+                }
+                if (a.walk_value) a.walk_value[k] = v;
+                if (a.rec_prim) a.rec_prim[k] = pr;
+                if (a.rec_t) {
+                    if (D == 2) a.rec_t[k] = t0;
+                    else { a.rec_t[2 * k] = t0; a.rec_t[2 * k + 1] = t1; }
+                }
+                if constexpr (BASIS) {
+#pragma unroll
+                    for (int q = 0; q < kMaxBasis; ++q)
+                        if (q < a.K) a.basis[(long long)q * a.total + k] = done_valid ? bas[q] : 0.0f;
+                }
+                mode = kIdle;
+                k = -1;
+            }
+        }
+        // 2. refill idle lanes from the wave's chunk
+        uint64_t idle = __ballot(mode == kIdle);
+        if (idle != 0 && !exhausted) {
+            int need = __popcll(idle);
+            long long avail = c_end - c_next;
+            long long base2 = 0, end2 = 0;
+            bool fresh = false;
+            if (avail < need) {
+                unsigned long long b = 0;
+                if (lane == 0) b = atomicAdd(a.counter, (unsigned long long)a.chunk);
+                uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)b);
+                uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(b >> 32));
+                long long bb = (long long)(((uint64_t)hi << 32) | lo);
+                if (bb >= a.total) {
+                    exhausted = true;
+                } else {
+                    fresh = true;
+                    base2 = bb;
+                    end2 = bb + a.chunk < a.total ? bb + a.chunk : a.total;
+                }
+            }
+            if (mode == kIdle) {
+                int r = rank_in_mask(idle);
+                long long kk = -1;
+                if (r < avail) kk = c_next + r;
+                else if (fresh && (r - avail) < end2 - base2) kk = base2 + (r - avail);
+                if (kk >= 0) {
+                    k = kk;
+                    long long i = kk / W;
+                    rng = walk_stream(a.seed, i, kk);
+                    load_point<D>(a.points, i, p);
+                    contrib = 0.0f;
+#pragma unroll
+                    for (int q = 0; q < (BASIS ? kMaxBasis : 1); ++q) bas[q] = 0.0f;
+                    step = 0;
+                    mode = kInit;
+                }
+            }
+            if (avail >= need) {
+                c_next += need;
+            } else if (fresh) {
+                long long used2 = need - avail;
+                c_next = base2 + used2 < end2 ? base2 + used2 : end2;
+                c_end = end2;
+            } else {
+                c_next = c_end;
+            }
+        }
+        if (exhausted && __ballot(mode != kIdle) == 0) break;
+
+        // 3. stepping lanes sample the source term and jump to the sphere
+        float bd2 = __builtin_inff();
+        if (mode == kStep) {
+            if (SRC) {
+                if constexpr (BASIS) {
+                    float u0 = rng.next_1d(), u1 = rng.next_1d();
+                    float sn, cs;
+                    sincos_turns(u0, sn, cs);
+                    float r = sqrtf(u1);
+                    float dyx = (r * cs) * d, dyy = (r * sn) * d;
+                    float area = kPi * (d * d);
+                    float pdf = 1.0f / area;
+                    float rr = fmaxf(sqrtf(dot2(dyx, dyy, dyx, dyy)), 1e-4f);
+                    float G = log_poly(d / rr) * kInv2Pi;
+                    if (pdf > 1e-5f) {
+                        float yx = p[0] + dyx, yy = p[1] + dyy;
+                        float f = 0.0f;
+                        float GA = 0.0f;
+#pragma unroll
+                        for (int q = 0; q < kMaxBasis; ++q) {
+                            if (q < a.K) {
+                                const float *c = s.src + 4 * q;
+                                float b = sin_cw(fmaf(c[0], yx, fmaf(c[1], yy, c[2])));
+                                f = fmaf(c[3], b, f);
+                                bas[q] = fmaf(b * G, area, bas[q]);
+                            }
+                        }
+                        GA = f * G;
+                        contrib = fmaf(GA, area, contrib);
+                    }
+                } else {
+                    source_step<D>(s, p, d, rng, contrib);
+                }
+            }
+            move_point<D>(p, d, rng);
+            ++step;
+            bd2 = bound_d2(s, 2.0f * d);
+        }
+        // 4. one closest-primitive query per active lane
+        float d2 = 0.0f;
+        if (mode != kIdle) prim = closest_prim<D>(s, p, bd2, d2, st, slot, lane_g);
+        // 5. new walks: inside test (include/wos.cuh:100-106)
+        if (mode == kInit) {
+            float sd = prim >= 0 ? signed_dist<D>(s, prim, p) : __builtin_inff();
+            if (prim < 0 || (sd >= 0.0f && !a.w.double_sided)) {
+                if (a.walk_value) a.walk_value[k] = 0.0f;
+                if (a.rec_prim) a.rec_prim[k] = -1;
+                if (a.rec_t) {
+                    if (D == 2) a.rec_t[k] = 0.0f;
+                    else { a.rec_t[2 * k] = 0.0f; a.rec_t[2 * k + 1] = 0.0f; }
+                }
+                if constexpr (BASIS) {
+#pragma unroll
+                    for (int q = 0; q < kMaxBasis; ++q)
+                        if (q < a.K) a.basis[(long long)q * a.total + k] = 0.0f;
+                }
+                mode = kIdle;
+                k = -1;
+            } else {
+                d = fabsf(sd);
+                if (a.w.prevent_fd_artifacts) d *= 0.5f;
+                mode = kStep;
+            }
+        } else if (mode == kStep) {
+            d = sqrtf(d2);
+        }
+    }
+}
+
+// One wave per point: FP64 sum and sum of squares of its walk values.
+__global__ __launch_bounds__(kBlock) void k_reduce(const float *vals, int n_points, int W, float *mean,
+                                                   double *mean64, double *var64) {
+    const int lane = __lane_id();
+    long wave = ((long)blockIdx.x * kBlock + threadIdx.x) >> 6;
+    long nwaves = ((long)gridDim.x * kBlock) >> 6;
+    for (long i = wave; i < n_points; i += nwaves) {
+        double sm = 0.0, s2 = 0.0;
+        for (int w = lane; w < W; w += 64) {
+            double v = vals[i * W + w];
+            sm += v;
+            s2 += v * v;
+        }
+        for (int off = 32; off > 0; off >>= 1) {
+            sm += __shfl_xor(sm, off);
+            s2 += __shfl_xor(s2, off);
+        }
+        if (lane == 0) {
+            double m = sm / W;
+            if (mean) mean[i] = (float)m;
+            if (mean64) mean64[i] = m;
+            if (var64) var64[i] = W > 1 ? (s2 - sm * m) / (W - 1) : 0.0;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// adjoint of the walk value w.r.t. vertex values and Fourier coefficients
+// ---------------------------------------------------------------------------
+constexpr int kLdsValues = 12288;   // vertex-value histogram kept in LDS up to this size
+
+template <int D, bool LDSHIST>
+__global__ __launch_bounds__(kBlock) void k_adjoint_values(SceneView s, int n_points, int W, const int *rec_prim,
+                                                           const float *rec_t, const float *basis, int K,
+                                                           const float *grad_out, float *grad_values,
+                                                           float *grad_coeffs) {
+    __shared__ float h[LDSHIST ? kLdsValues : 1];
+    __shared__ float cacc[kMaxBasis];
+    if (LDSHIST)
+        for (int v = threadIdx.x; v < s.n_vertices; v += kBlock) h[v] = 0.0f;
+    if (threadIdx.x < kMaxBasis) cacc[threadIdx.x] = 0.0f;
+    __syncthreads();
+    long long total = (long long)n_points * W;
+    float cl[kMaxBasis];
+#pragma unroll
+    for (int q = 0; q < kMaxBasis; ++q) cl[q] = 0.0f;
+    for (long long k = (long long)blockIdx.x * kBlock + threadIdx.x; k < total; k += (long long)gridDim.x * kBlock) {
+        int f = rec_prim[k];
+        if (f < 0) continue;
+        float g = grad_out[k / W] / (float)W;
+        if (grad_values) {
+            if constexpr (D == 2) {
+                float t = rec_t[k];
+                int i0 = s.idx[2 * f], i1 = s.idx[2 * f + 1];
+                float g0 = g * (1.0f - t), g1 = g * t;
+                if (LDSHIST) { atomicAdd(&h[i0], g0); atomicAdd(&h[i1], g1); }
+                else { atomicAdd(&grad_values[i0], g0); atomicAdd(&grad_values[i1], g1); }
+            } else {
+                float u = rec_t[2 * k], v = rec_t[2 * k + 1];
+                int i0 = s.idx[3 * f], i1 = s.idx[3 * f + 1], i2 = s.idx[3 * f + 2];
+                float g0 = g * (1.0f - u - v), g1 = g * u, g2 = g * v;
+                if (LDSHIST) { atomicAdd(&h[i0], g0); atomicAdd(&h[i1], g1); atomicAdd(&h[i2], g2); }
+                else { atomicAdd(&grad_values[i0], g0); atomicAdd(&grad_values[i1], g1); atomicAdd(&grad_values[i2], g2); }
+            }
+        }
+        if (basis) {
+#pragma unroll
+            for (int q = 0; q < kMaxBasis; ++q)
+                if (q < K) cl[q] = fmaf(g, basis[(long long)q * total + k], cl[q]);
+        }
+    }
+    if (basis && grad_coeffs) {
+#pragma unroll
+        for (int q = 0; q < kMaxBasis; ++q) {
+            if (q < K) {
+                float v = cl[q];
+                for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+                if (__lane_id() == 0) atomicAdd(&cacc[q], v);
+            }
+        }
+    }
+    __syncthreads();
+    if (LDSHIST && grad_values)
+        for (int v = threadIdx.x; v < s.n_vertices; v += kBlock)
+            if (h[v] != 0.0f) atomicAdd(&grad_values[v], h[v]);
+    if (basis && grad_coeffs && threadIdx.x < K) atomicAdd(&grad_coeffs[threadIdx.x], cacc[threadIdx.x]);
+}
+
+// ---------------------------------------------------------------------------
+// shape gradient (OursCUDA / Ours3D), one lane per (point, walk) sample
+// ---------------------------------------------------------------------------
+template <int D>
+__device__ float secondary_u(const SceneView &s, const WalkSettings &w2, const float *y, Pcg32 &rng,
+                             float R, const TravStack &st, int slot, long lane_g) {
+    float d2;
+    int id = closest_prim<D>(s, y, bound_d2(s, 2.0f * R), d2, st, slot, lane_g);
+    float sd = signed_dist<D>(s, id, y);
+    if (fabsf(sd) < w2.epsilon) {
+        // wos/wos.py:221-223; the walk would stop at once and draw nothing
+        Rec c;
+        closest_record<D>(s, id, y, c);
+        return dirichlet_at<D>(s, id, c.t, c.uv[0], c.uv[1]);
+    }
+    Rec r;
+    float contrib;
+    walk_one<D>(s, w2, y, rng, id, d2, st, slot, lane_g, r, contrib);
+    if (!r.valid) return 0.0f;
+    return dirichlet_at<D>(s, r.prim, r.t, r.uv[0], r.uv[1]) + contrib;
+}
+
+template <int D>
+__global__ __launch_bounds__(kBlock) void k_shape_gradient(SceneView s, GradSettings g, const float *points,
+                                                           int n_points, int32_t seed, const float *grad_out,
+                                                           float *grad_vertices, float *sample_grad_u,
+                                                           int2 *spill, long stride) {
+    DECLARE_STACK(spill, stride);
+    const int W = g.nwalks;
+    long long total = (long long)n_points * W;
+    WalkSettings w1{g.nwalks, g.nsteps, g.epsilon, g.double_sided, 0};
+    WalkSettings w2{g.nwalks, g.nsteps, g.epsilon2, g.double_sided, 0};
+    for (long long k = lane_g; k < total; k += (long long)gridDim.x * kBlock) {
+        long long i = k / W;
+        Pcg32 rng = walk_stream(seed, i, k);
+        float p[3];
+        load_point<D>(points, i, p);
+        float d2;
+        int id0 = closest_prim<D>(s, p, __builtin_inff(), d2, st, slot, lane_g);
+        float gu[3] = {0.0f, 0.0f, 0.0f};
+        bool ok = false;
+        Rec rec;
+        rec.valid = 0;
+        if (id0 >= 0) {
+            float sd0 = signed_dist<D>(s, id0, p);
+            bool inside = g.double_sided || sd0 < 0.0f;
+            float contrib;
+            walk_one<D>(s, w1, p, rng, id0, d2, st, slot, lane_g, rec, contrib);
+            if (rec.valid) {
+                ok = inside;
+                float n[3] = {-rec.n[0], -rec.n[1], D == 3 ? -rec.n[2] : 0.0f};
+                float u_ref = dirichlet_at<D>(s, rec.prim, rec.t, rec.uv[0], rec.uv[1]);
+                float R = lib_one<D>(s, rec.p, rec.n, 1e-3f, st, slot, lane_g);
+                float c[3];
+                for (int q = 0; q < 3; ++q) c[q] = fmaf(-n[q], R, rec.p[q]);
+                float grad = 0.0f;
+                if constexpr (D == 2) {
+                    float t = (float)rng.next_f64();
+                    float lo = g.clamping * kInv2Pi;
+                    t = fminf(fmaxf(t, lo), 1.0f - lo);
+                    float sn, cs;
+                    sincos_turns(t, sn, cs);
+                    float pd[2] = {-n[1], n[0]};
+                    float P = 1.0f / (cs - 1.0f);
+                    for (int side = 0; side < 2; ++side) {
+                        float ss = side ? -sn : sn;
+                        float y[3] = {fmaf(R, fmaf(n[0], cs, pd[0] * ss), c[0]),
+                                      fmaf(R, fmaf(n[1], cs, pd[1] * ss), c[1]), 0.0f};
+                        float u = secondary_u<D>(s, w2, y, rng, R, st, slot, lane_g);
+                        float num = g.control_variates ? (u - u_ref) : u;
+                        grad = grad + P * num / R;
+                    }
+                } else {
+                    double z64 = 1.0 - 2.0 * rng.next_f64();
+                    double ph64 = rng.next_f64();
+                    float ct = fminf(fmaxf((float)z64, -1.0f), 1.0f - g.clamping);
+                    float up[3] = {0.0f, 0.0f, 1.0f};
+                    if (!(fabsf(n[2]) < 0.9f)) { up[0] = 1.0f; up[2] = 0.0f; }
+                    float pd[3];
+                    cross3(n, up, pd);
+                    float l = sqrtf(dot3(pd, pd));
+                    pd[0] = pd[0] / l; pd[1] = pd[1] / l; pd[2] = pd[2] / l;
+                    float qd[3];
+                    cross3(n, pd, qd);
+                    float r = sqrtf(fmaxf(0.0f, 1.0f - ct * ct));
+                    float sn, cs;
+                    sincos_turns((float)ph64, sn, cs);
+                    float om = 1.0f - ct;
+                    float P = -1.0f / (1.41421356237309504880f * (om * sqrtf(om)));
+                    for (int side = 0; side < 2; ++side) {
+                        float ss = side ? -sn : sn, cc = side ? -cs : cs;
+                        float y[3];
+                        for (int q = 0; q < 3; ++q) y[q] = fmaf(R, fmaf(ct, n[q], r * fmaf(cc, pd[q], ss * qd[q])), c[q]);
+                        float u = secondary_u<D>(s, w2, y, rng, R, st, slot, lane_g);
+                        grad = grad + P * (u - u_ref) / R;
+                    }
+                }
+                grad = grad / 2.0f;
+                for (int q = 0; q < D; ++q) gu[q] = grad * n[q];
+                // tangential derivative (wos/scene.py:451-460, wos/wos3d.py:137-156)
+                if constexpr (D == 2) {
+                    int i0 = s.idx[2 * rec.prim], i1 = s.idx[2 * rec.prim + 1];
+                    float ex = s.vtx[2 * i1] - s.vtx[2 * i0], ey = s.vtx[2 * i1 + 1] - s.vtx[2 * i0 + 1];
+                    float len = sqrtf(dot2(ex, ey, ex, ey));
+                    float dv = (s.val[i1] - s.val[i0]) / len;
+                    gu[0] += dv * (ex / len);
+                    gu[1] += dv * (ey / len);
+                } else {
+                    int i0 = s.idx[3 * rec.prim], i1 = s.idx[3 * rec.prim + 1], i2 = s.idx[3 * rec.prim + 2];
+                    float a[3], ab[3], ac[3];
+                    for (int q = 0; q < 3; ++q) {
+                        a[q] = s.vtx[3 * i0 + q];
+                        ab[q] = s.vtx[3 * i1 + q] - a[q];
+                        ac[q] = s.vtx[3 * i2 + q] - a[q];
+                    }
+                    float lab = sqrtf(dot3(ab, ab));
+                    float t[3] = {ab[0] / lab, ab[1] / lab, ab[2] / lab};
+                    float fn[3];
+                    cross3(ab, ac, fn);
+                    float lf = sqrtf(dot3(fn, fn));
+                    fn[0] = fn[0] / lf; fn[1] = fn[1] / lf; fn[2] = fn[2] / lf;
+                    float nn[3];
+                    cross3(fn, t, nn);
+                    float ln = sqrtf(dot3(nn, nn));
+                    nn[0] = nn[0] / ln; nn[1] = nn[1] / ln; nn[2] = nn[2] / ln;
+                    float va = s.val[i0], vb = s.val[i1], vc = s.val[i2];
+                    float gt = (vb - va) / lab;
+                    float gn = ((vc - va) - gt * dot3(ac, t)) / dot3(ac, nn);
+                    for (int q = 0; q < 3; ++q) gu[q] += fmaf(gt, t[q], gn * nn[q]);
+                }
+            }
+        }
+        if (sample_grad_u)
+            for (int q = 0; q < D; ++q) sample_grad_u[k * D + q] = ok ? gu[q] : 0.0f;
+        if (ok && grad_vertices && grad_out) {
+            float gi = grad_out[i] / (float)W;
+            if constexpr (D == 2) {
+                int i0 = s.idx[2 * rec.prim], i1 = s.idx[2 * rec.prim + 1];
+                float t = rec.t;
+                for (int q = 0; q < 2; ++q) {
+                    atomicAdd(&grad_vertices[2 * i0 + q], -gi * (1.0f - t) * gu[q]);
+                    atomicAdd(&grad_vertices[2 * i1 + q], -gi * t * gu[q]);
+                }
+            } else {
+                int i0 = s.idx[3 * rec.prim], i1 = s.idx[3 * rec.prim + 1], i2 = s.idx[3 * rec.prim + 2];
+                float u = rec.uv[0], v = rec.uv[1], wa = 1.0f - u - v;
+                for (int q = 0; q < 3; ++q) {
+                    atomicAdd(&grad_vertices[3 * i0 + q], -gi * wa * gu[q]);
+                    atomicAdd(&grad_vertices[3 * i1 + q], -gi * u * gu[q]);
+                    atomicAdd(&grad_vertices[3 * i2 + q], -gi * v * gu[q]);
+                }
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// host launchers
+// ---------------------------------------------------------------------------
+#define DISPATCH_D(s, ...)                      \
+    if ((s).dim == 2) { __VA_ARGS__(2) }        \
+    else { __VA_ARGS__(3) }
+
+static int occupancy_grid() {
+    static int cus = 0;
+    if (!cus) {
+        int dev = 0;
+        (void)hipGetDevice(&dev);
+        (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+        if (cus <= 0) cus = 256;
+    }
+    return cus * 4;   // four 256-lane workgroups per CU (32 KB LDS stack each)
+}
+
+int launch_closest_index(const SceneView &s, const float *p, int n, int *out, float *out_d2, hipStream_t st) {
+    if (n <= 0) return 0;
+    int g = grid_of(n, occupancy_grid());
+    long stride;
+    int2 *spill = spill_buffer(s, (long)g * kBlock, &stride, st);
+    if (s.dim == 2) k_closest_index<2><<<g, kBlock, 0, st>>>(s, p, n, out, out_d2, spill, stride);
+    else k_closest_index<3><<<g, kBlock, 0, st>>>(s, p, n, out, out_d2, spill, stride);
+    KCHK();
+    return 0;
+}
+
+int launch_sdf(const SceneView &s, const float *p, int n, float *out, hipStream_t st) {
+    if (n <= 0) return 0;
+    int g = grid_of(n, occupancy_grid());
+    long stride;
+    int2 *spill = spill_buffer(s, (long)g * kBlock, &stride, st);
+    if (s.dim == 2) k_sdf<2><<<g, kBlock, 0, st>>>(s, p, n, out, spill, stride);
+    else k_sdf<3><<<g, kBlock, 0, st>>>(s, p, n, out, spill, stride);
+    KCHK();
+    return 0;
+}
+
+int launch_closest_point(const SceneView &s, const float *p, int n, uint8_t *valid, float *op, float *on, float *ot,
+                         int *oprim, float *ocontrib, hipStream_t st) {
+    if (n <= 0) return 0;
+    int g = grid_of(n, occupancy_grid());
+    long stride;
+    int2 *spill = spill_buffer(s, (long)g * kBlock, &stride, st);
+    if (s.dim == 2)
+        k_closest_point<2><<<g, kBlock, 0, st>>>(s, p, n, valid, op, on, ot, oprim, ocontrib, spill, stride);
+    else
+        k_closest_point<3><<<g, kBlock, 0, st>>>(s, p, n, valid, op, on, ot, oprim, ocontrib, spill, stride);
+    KCHK();
+    return 0;
+}
+
+int launch_lib(const SceneView &s, const uint8_t *valid, const float *p, const float *nrm, int n, float eps,
+               float *out, hipStream_t st) {
+    if (n <= 0) return 0;
+    int g = grid_of(n, occupancy_grid());
+    long stride;
+    int2 *spill = spill_buffer(s, (long)g * kBlock, &stride, st);
+    if (s.dim == 2) k_lib<2><<<g, kBlock, 0, st>>>(s, valid, p, nrm, n, eps, out, spill, stride);
+    else k_lib<3><<<g, kBlock, 0, st>>>(s, valid, p, nrm, n, eps, out, spill, stride);
+    KCHK();
+    return 0;
+}
+
+int launch_dirichlet(const SceneView &s, const uint8_t *valid, const int *prim, const float *t, int n, float *out,
+                     hipStream_t st) {
+    if (n <= 0) return 0;
+    if (s.dim == 2) k_dirichlet<2><<<grid_of(n), kBlock, 0, st>>>(s, valid, prim, t, n, out);
+    else k_dirichlet<3><<<grid_of(n), kBlock, 0, st>>>(s, valid, prim, t, n, out);
+    KCHK();
+    return 0;
+}
+
+int launch_single_walk(const SceneView &s, const WalkSettings &w, const float *p, int n, uint64_t *state,
+                       uint64_t *inc, uint8_t *valid, float *op, float *on, float *ot, int *oprim, float *ocontrib,
+                       hipStream_t st) {
+    if (n <= 0) return 0;
+    int g = grid_of(n, occupancy_grid());
+    long stride;
+    int2 *spill = spill_buffer(s, (long)g * kBlock, &stride, st);
+    if (s.dim == 2)
+        k_single_walk<2><<<g, kBlock, 0, st>>>(s, w, p, n, state, inc, valid, op, on, ot, oprim, ocontrib, spill,
+                                               stride);
+    else
+        k_single_walk<3><<<g, kBlock, 0, st>>>(s, w, p, n, state, inc, valid, op, on, ot, oprim, ocontrib, spill,
+                                               stride);
+    KCHK();
+    return 0;
+}
+
+__global__ void k_zero_u64(unsigned long long *p) { *p = 0ull; }
+
+int launch_walk_reduce(const SceneView &s, const WalkSettings &w, const float *points, int n_points, int32_t seed,
+                       int chunk, const WalkOutputs &o, hipStream_t st) {
+    if (n_points <= 0) return 0;
+    long long total = (long long)n_points * w.nwalks;
+    int K = (s.source_type == 3) ? s.n_src / 4 : 0;
+    if (o.basis && K > kMaxBasis) {
+        set_error("Fourier source has %d terms; at most %d are supported", K, kMaxBasis);
+        return -1;
+    }
+    PersistArgs a;
+    a.points = points;
+    a.n_points = n_points;
+    a.w = w;
+    a.seed = seed;
+    a.total = total;
+    a.counter = launch_counter(st);
+    a.chunk = chunk > 0 ? chunk : 256;
+    a.walk_value = o.walk_value;
+    a.rec_prim = o.rec_prim;
+    a.rec_t = o.rec_t;
+    a.basis = o.basis;
+    a.K = K;
+    long waves_needed = (long)((total + 63) / 64);
+    int g = (int)((waves_needed + 3) / 4);
+    int cap = occupancy_grid();
+    if (g > cap) g = cap;
+    if (g < 1) g = 1;
+    a.spill = spill_buffer(s, (long)g * kBlock, &a.spill_stride, st);
+    if (!a.counter) return -1;
+    k_zero_u64<<<1, 1, 0, st>>>(a.counter);
+    bool src = s.dim == 2 && s.source_type != 0;
+    bool bas = src && o.basis != nullptr && K > 0;
+    if (s.dim == 2) {
+        if (bas) k_walk_persistent<2, true, true><<<g, kBlock, 0, st>>>(s, a);
+        else if (src) k_walk_persistent<2, true, false><<<g, kBlock, 0, st>>>(s, a);
+        else k_walk_persistent<2, false, false><<<g, kBlock, 0, st>>>(s, a);
+    } else {
+        k_walk_persistent<3, false, false><<<g, kBlock, 0, st>>>(s, a);
+    }
+    KCHK();
+    if (o.mean || o.mean64 || o.var64) {
+        if (!o.walk_value) {
+            set_error("per-point statistics need the per-walk value buffer");
+            return -1;
+        }
+        long waves = n_points;
+        int gr = (int)((waves * 64 + kBlock - 1) / kBlock);
+        if (gr > 8192) gr = 8192;
+        k_reduce<<<gr, kBlock, 0, st>>>(o.walk_value, n_points, w.nwalks, o.mean, o.mean64, o.var64);
+        KCHK();
+    }
+    return 0;
+}
+
+int launch_adjoint_values(const SceneView &s, int n_points, int nwalks, const int *rec_prim, const float *rec_t,
+                          const float *basis, const float *grad_out, float *grad_values, float *grad_coeffs,
+                          hipStream_t st) {
+    if (n_points <= 0) return 0;
+    int K = (s.source_type == 3) ? s.n_src / 4 : 0;
+    if (basis && K > kMaxBasis) {
+        set_error("Fourier source has %d terms; at most %d are supported", K, kMaxBasis);
+        return -1;
+    }
+    long long total = (long long)n_points * nwalks;
+    int g = grid_of((long)((total + 15) / 16), 2048);
+    bool lds = s.n_vertices <= kLdsValues;
+    if (s.dim == 2) {
+        if (lds) k_adjoint_values<2, true><<<g, kBlock, 0, st>>>(s, n_points, nwalks, rec_prim, rec_t, basis, K, grad_out, grad_values, grad_coeffs);
+        else k_adjoint_values<2, false><<<g, kBlock, 0, st>>>(s, n_points, nwalks, rec_prim, rec_t, basis, K, grad_out, grad_values, grad_coeffs);
+    } else {
+        if (lds) k_adjoint_values<3, true><<<g, kBlock, 0, st>>>(s, n_points, nwalks, rec_prim, rec_t, basis, K, grad_out, grad_values, grad_coeffs);
+        else k_adjoint_values<3, false><<<g, kBlock, 0, st>>>(s, n_points, nwalks, rec_prim, rec_t, basis, K, grad_out, grad_values, grad_coeffs);
+    }
+    KCHK();
+    return 0;
+}
+
+int launch_shape_gradient(const SceneView &s, const GradSettings &g, const float *points, int n_points, int32_t seed,
+                          const float *grad_out, float *grad_vertices, float *sample_grad_u, hipStream_t st) {
+    if (n_points <= 0) return 0;
+    long long total = (long long)n_points * g.nwalks;
+    int gr = grid_of((long)total, occupancy_grid());
+    long stride;
+    int2 *spill = spill_buffer(s, (long)gr * kBlock, &stride, st);
+    if (s.dim == 2)
+        k_shape_gradient<2><<<gr, kBlock, 0, st>>>(s, g, points, n_points, seed, grad_out, grad_vertices, sample_grad_u, spill, stride);
+    else
+        k_shape_gradient<3><<<gr, kBlock, 0, st>>>(s, g, points, n_points, seed, grad_out, grad_vertices, sample_grad_u, spill, stride);
+    KCHK();
+    return 0;
+}
+
+}  // namespace wos

@@ -1,0 +1,155 @@
+/*
+ * wos_amd.h -- C ABI of the MI355X walk-on-spheres library (libwos_amd.so).
+ *
+ * The entry points are the operations the reference's only FFI, the pybind11
+ * module wos_ext (src/python/scene.cu, src/python/wos.cu), exposes for the hot
+ * path, plus the fused estimator entry points the Python layer needs.  Every
+ * array argument is a device pointer (or any pointer HIP can address) that the
+ * caller owns; the library copies scene geometry into buffers the scene
+ * handle owns.  `stream` is a hipStream_t (NULL = default stream).  Every
+ * function returns 0 on success and -1 on error; wos_last_error() then holds a
+ * message.  Calls are asynchronous on `stream` unless stated otherwise.
+ */
+#ifndef WOS_AMD_H
+#define WOS_AMD_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct wos_scene wos_scene;
+
+/* WoS(nwalks, nsteps=32, epsilon=1e-3, double_sided=False,
+ *     prevent_fd_artifacts=False, use_IS_for_greens=True)
+ * -- src/python/wos.cu:9-12, fields include/wos.cuh:147-152.
+ * use_IS_for_greens is accepted and ignored, as in the CUDA walk
+ * (include/wos.cuh:121-134 is compiled out). */
+typedef struct {
+    int   nwalks;
+    int   nsteps;
+    float epsilon;
+    int   double_sided;
+    int   prevent_fd_artifacts;
+    int   use_IS_for_greens;
+} wos_walk_params;
+
+/* OursCUDA / Ours3D settings (wos/wos_grad.py:425-440, wos/wos_grad_3d.py:283-293) */
+typedef struct {
+    int   nwalks;
+    int   nsteps;
+    float epsilon;          /* primary walk */
+    float epsilon2;         /* secondary walks of the normal derivative */
+    float clamping;         /* angle clamp of the normal-derivative kernel */
+    int   double_sided;
+    int   control_variates; /* 2D only; 3D always uses the control variate */
+} wos_grad_params;
+
+/* A batch of closest-point records, dr::ClosestPointRecord<2/3>
+ * (include/scene.cuh:18-58).  Any member may be NULL when not wanted.
+ * p, n: n*dim floats (row-major); t: n floats (2D) or uv: 2n floats (3D). */
+typedef struct {
+    uint8_t *valid;
+    float   *p;
+    float   *n;
+    float   *t;
+    int     *prim_id;
+    float   *contrib;
+} wos_record;
+
+/* Message of the last failed call on this thread ("" if none). */
+const char *wos_last_error(void);
+
+/* Scene(vertices, indices, values, use_bvh=False[, source_type, source_params])
+ * -- src/python/scene.cu:29-33 and :69-71 (Scene3D), include/scene.cuh:330-342.
+ * dim = 2 (segments, indices n_prims*2) or 3 (triangles, n_prims*3).
+ * source_type: 0 none, 1 Gaussian (5 params), 2 sinusoid (5 params)
+ * (include/scene.cuh:270-301), 3 Fourier sum with n_source_params = 4K:
+ * (ax, ay, phase, coeff) per term, f = sum coeff sin(ax x + ay y + phase).
+ * Builds vertex normals, the AABB and the LBVH on the device (always, as the
+ * reference does, include/scene.cuh:437-452) and synchronises `stream`.
+ * Rejects indices out of range and zero-length / zero-area primitives. */
+int wos_scene_create(int dim, const float *vertices, int n_vertices, const int *indices, int n_prims,
+                     const float *values, int use_bvh, int source_type, const float *source_params,
+                     int n_source_params, void *stream, wos_scene **out);
+/* Scene destructor (the reference frees its thrust buffers with the object). */
+int wos_scene_destroy(wos_scene *scene);
+/* Scene.values / Scene.source_params setters (def_readwrite in
+ * src/python/scene.cu:38, :47): replace without rebuilding the BVH. */
+int wos_scene_set_values(wos_scene *scene, const float *values, void *stream);
+int wos_scene_set_source(wos_scene *scene, int source_type, const float *source_params, int n_source_params,
+                         void *stream);
+/* Scene.use_bvh (src/python/scene.cu:40) */
+int wos_scene_set_use_bvh(wos_scene *scene, int use_bvh);
+/* Read-only fields: Scene.normals (device copy, n_vertices*dim floats),
+ * Scene.aabb (host floats min[3], max[3]), tree depth, dimension. */
+int wos_scene_get_normals(const wos_scene *scene, float *out, void *stream);
+int wos_scene_get_info(const wos_scene *scene, float *aabb_min3, float *aabb_max3, int *dim, int *n_vertices,
+                       int *n_prims, int *max_depth);
+/* Copies the LBVH topology (Morton-sorted primitive ids, n_prims; Morton codes,
+ * n_prims; children of the n_prims-1 internal nodes, leaf j encoded as ~j) --
+ * the structure lbvh::bvh builds at include/scene.cuh:452. */
+int wos_scene_get_bvh(const wos_scene *scene, int *sorted_prim, uint32_t *sorted_codes, int *children,
+                      void *stream);
+
+/* Scene.closest_point_preliminary(p) -> Int (include/scene.cuh:497-537).
+ * Ties: equal squared distance keeps the smallest primitive index.
+ * out_d2 (nullable): squared distance. */
+int wos_closest_point_index(const wos_scene *scene, const float *p, int n, int *out, float *out_d2, void *stream);
+/* SceneDevice::sdf (include/scene.cuh:148-178) over a batch. */
+int wos_sdf(const wos_scene *scene, const float *p, int n, float *out, void *stream);
+/* Scene.closest_point(p) -> ClosestPointRecord (include/scene.cuh:539-657) */
+int wos_closest_point(const wos_scene *scene, const float *p, int n, const wos_record *out, void *stream);
+/* Scene.largest_inscribed_ball(its, epsilon=1e-3) (include/scene.cuh:659-704);
+ * reads its.valid (nullable = all valid), its.p, its.n. */
+int wos_largest_inscribed_ball(const wos_scene *scene, const wos_record *its, int n, float epsilon, float *out,
+                               void *stream);
+/* Scene.dirichlet(cRec) (include/scene.cuh:455-468); reads valid (nullable),
+ * prim_id, t (2D) / uv (3D). */
+int wos_dirichlet(const wos_scene *scene, const wos_record *rec, int n, float *out, void *stream);
+
+/* WoS.single_walk(p, scene, sampler) -> ClosestPointRecord
+ * (src/python/wos.cu:14, include/wos.cuh:374-533).  state/inc: the PCG32
+ * sampler of each walk, read and written back (include/wos.cuh:438-439). */
+int wos_single_walk(const wos_scene *scene, const wos_walk_params *params, const float *p, int n,
+                    uint64_t *state, uint64_t *inc, const wos_record *out, void *stream);
+
+/* Solver.walk_reduce(p, scene, seed) (wos/solver.py:49-65) over the CUDA walk
+ * value (wos/wos.py:171-179, wos/wos_with_source.py:257-265): W = nwalks
+ * walks per point, walk k = i*W + w keyed by TEA(seed, i) and k.  One
+ * persistent launch; outputs (all nullable except as noted):
+ *   mean     n_points floats       (needs walk_value)
+ *   mean64   n_points doubles      per-point FP64 mean  (needs walk_value)
+ *   var64    n_points doubles      per-point FP64 sample variance (needs walk_value)
+ *   walk_value  n_points*W floats  per-walk value
+ *   rec_prim    n_points*W ints    exit primitive (-1 when the walk is invalid)
+ *   rec_t       n_points*W floats (2D t) or 2*n_points*W (3D uv)
+ *   basis       K*n_points*W floats, d value / d coeff_k of a Fourier source
+ * chunk: walks a wave claims per queue visit (<= 0: default). */
+int wos_walk_reduce(const wos_scene *scene, const wos_walk_params *params, const float *points, int n_points,
+                    int seed, int chunk, float *mean, double *mean64, double *var64, float *walk_value,
+                    int *rec_prim, float *rec_t, float *basis, void *stream);
+
+/* Reverse mode of walk_reduce with respect to the vertex values and the
+ * Fourier coefficients: the gradient Dr.Jit's AD takes through
+ * Scene.dirichlet's gathers (include/scene.cuh:455-468) and through the
+ * contrib sum (include/wos.cuh:136-138), computed from the exit records of a
+ * walk_reduce call with the same settings.  grad_out: dL/d mean (n_points).
+ * Adds into grad_values (n_vertices, nullable) and grad_coeffs (K, nullable). */
+int wos_adjoint_values(const wos_scene *scene, int n_points, int nwalks, const int *rec_prim, const float *rec_t,
+                       const float *basis, const float *grad_out, float *grad_values, float *grad_coeffs,
+                       void *stream);
+
+/* Shape gradient: CustomSolver backward (wos/solver.py:109-114) with
+ * bwd_solver = OursCUDA (wos/wos_grad.py:442-463) in 2D or Ours3D
+ * (wos/wos_grad_3d.py:295-315) in 3D, walks recomputed from (seed, point,
+ * walk).  Adds dL/dvertices into grad_vertices (n_vertices*dim, nullable).
+ * sample_grad_u (nullable): per-sample spatial gradient estimate,
+ * n_points*nwalks*dim floats (0 for samples that do not contribute). */
+int wos_shape_gradient(const wos_scene *scene, const wos_grad_params *params, const float *points, int n_points,
+                       int seed, const float *grad_out, float *grad_vertices, float *sample_grad_u, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
