@@ -107,6 +107,7 @@ static SceneView view_of(const wos_scene *s) {
     v.leaves = s->leaves;
     v.tol = s->tol;
     v.max_depth = s->max_depth;
+    v.stats = nullptr;
     return v;
 }
 
@@ -362,6 +363,25 @@ int wos_walk_reduce(const wos_scene *s, const wos_walk_params *w, const float *p
     WalkSettings ws{w->nwalks, w->nsteps, w->epsilon, w->double_sided, w->prevent_fd_artifacts};
     WalkOutputs o{walk_value, rec_prim, rec_t, basis, mean, mean64, var64};
     return launch_walk_reduce(view_of(s), ws, points, n_points, (int32_t)seed, chunk, o, S(stream));
+}
+
+int wos_walk_reduce_counts(const wos_scene *s, const wos_walk_params *w, const float *points, int n_points,
+                           int seed, int with_basis, float *walk_value, unsigned long long *counts, void *stream) {
+    REQUIRE(s && counts && walk_value && (n_points == 0 || points), "scene, points, walk_value and counts are required");
+    if (check_walk(w)) return -1;
+    SceneView v = view_of(s);
+    v.stats = counts;
+    WalkSettings ws{w->nwalks, w->nsteps, w->epsilon, w->double_sided, w->prevent_fd_artifacts};
+    WalkOutputs o{walk_value, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+    float *basis = nullptr;
+    if (with_basis && s->source_type == 3) {
+        size_t nb = (size_t)(s->n_src / 4) * (size_t)n_points * (size_t)w->nwalks * sizeof(float);
+        CHK(hipMallocAsync((void **)&basis, nb, S(stream)));
+        o.basis = basis;
+    }
+    int rc = launch_walk_reduce(v, ws, points, n_points, (int32_t)seed, 0, o, S(stream));
+    if (basis) CHK(hipFreeAsync(basis, S(stream)));
+    return rc;
 }
 
 int wos_adjoint_values(const wos_scene *s, int n_points, int nwalks, const int *rec_prim, const float *rec_t,

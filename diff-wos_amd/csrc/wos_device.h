@@ -261,6 +261,7 @@ struct SceneView {
     const float4 *leaves;  // n_prims leaves
     float tol;             // absolute pruning margin (scale-relative)
     int max_depth;
+    unsigned long long *stats;  // null, or {box tests, primitive tests, queries, walk steps}
 };
 
 // Per-lane traversal stack: the first kLdsStack entries live in LDS (lane
@@ -341,6 +342,10 @@ __device__ __forceinline__ int closest_brute(const SceneView &s, const float *p,
         }
         if (d2 < best) { best = d2; id = f; }
     }
+    if (s.stats) {
+        atomicAdd(&s.stats[1], (unsigned long long)s.n_prims);
+        atomicAdd(&s.stats[2], 1ull);
+    }
     best_d2 = best;
     return id;
 }
@@ -357,12 +362,18 @@ __device__ int closest_bvh(const SceneView &s, const float *p, float bound_d2, f
     if (s.n_prims == 1) {
         int prim;
         float d2 = leaf_d2<D>(s, 0, p, prim);
+        if (s.stats) {
+            atomicAdd(&s.stats[1], 1ull);
+            atomicAdd(&s.stats[2], 1ull);
+        }
         best_d2 = d2;
         return prim;
     }
     int node = 0, sp = 0;
+    unsigned nbox = 0, nprim = 0;
     for (;;) {
         float dL, dR;
+        nbox += 2;
         int cL, cR;
         if constexpr (D == 2) {
             float4 b0 = s.nodes[3 * node], b1 = s.nodes[3 * node + 1], c = s.nodes[3 * node + 2];
@@ -382,6 +393,7 @@ __device__ int closest_bvh(const SceneView &s, const float *p, float bound_d2, f
         if (dL <= thr) {
             if (cL < 0) {
                 int prim;
+                ++nprim;
                 float d2 = leaf_d2<D>(s, ~cL, p, prim);
                 if (better(d2, prim, best, best_id)) {
                     best = d2; best_id = prim;
@@ -392,6 +404,7 @@ __device__ int closest_bvh(const SceneView &s, const float *p, float bound_d2, f
         if (dR <= thr) {
             if (cR < 0) {
                 int prim;
+                ++nprim;
                 float d2 = leaf_d2<D>(s, ~cR, p, prim);
                 if (better(d2, prim, best, best_id)) {
                     best = d2; best_id = prim;
@@ -433,6 +446,11 @@ __device__ int closest_bvh(const SceneView &s, const float *p, float bound_d2, f
             if (dd <= thr) { node = n; found = true; break; }
         }
         if (!found) break;
+    }
+    if (s.stats) {
+        atomicAdd(&s.stats[0], (unsigned long long)nbox);
+        atomicAdd(&s.stats[1], (unsigned long long)nprim);
+        atomicAdd(&s.stats[2], 1ull);
     }
     best_d2 = best;
     return best_id == 0x7fffffff ? -1 : best_id;

@@ -226,6 +226,7 @@ __device__ void walk_one(const SceneView &s, const WalkSettings &w, const float 
         }
         if (src) source_step<D>(s, p, d, rng, contrib);
         move_point<D>(p, d, rng);
+        if (s.stats) atomicAdd(&s.stats[3], 1ull);
         prim = closest_prim<D>(s, p, bound_d2(s, 2.0f * d), d2, st, slot, lane_g);
         d = sqrtf(d2);
     }
@@ -414,6 +415,7 @@ __global__ __launch_bounds__(kBlock) void k_walk_persistent(SceneView s, Persist
             }
             move_point<D>(p, d, rng);
             ++step;
+            if (s.stats) atomicAdd(&s.stats[3], 1ull);
             bd2 = bound_d2(s, 2.0f * d);
         }
         // 4. one closest-primitive query per active lane

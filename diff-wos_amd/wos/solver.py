@@ -1,0 +1,216 @@
+"""Solver base and CustomSolver (wos/solver.py:1-125) on torch autograd.
+
+``Solver.walk(p, scene, seed)`` returns the per-point mean of ``nwalks``
+walks (wos/solver.py:49-65) as a float32 tensor.  Walk k = i*W + w of point
+i draws from the PCG32 stream keyed by TEA(seed, i) and k, so a second call
+with the same seed retraces the same paths -- the property the reference's
+CustomSolver backward relies on (wos/solver.py:109-114).
+
+Gradients: a solver whose value depends on the scene parameters (the walk
+value solvers) returns a tensor attached to ``scene.values`` and, for a
+Fourier source, ``scene.source_params``; the adjoint reuses the exit records
+stored by the forward launch.  Shape-gradient solvers (OursCUDA, Ours3D)
+return zeros attached to ``scene.vertices``, as the reference's integrators
+return zero-valued, gradient-carrying results.
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass
+from enum import Enum
+
+import torch
+
+import wos_ext
+from wos.fwd import PCG32
+from wos.utils import sample_tea_32
+
+
+class ControlVarianceType(Enum):
+    NoControlVariate = 0
+    RunningControlVariate = 1
+    BoundaryControlVariate = 2
+    BoundaryAndRunningControlVariate = 3
+
+
+def _requires(t):
+    return isinstance(t, torch.Tensor) and t.requires_grad
+
+
+def _source_tensor(scene):
+    sp = getattr(scene, "source_params", None)
+    return sp if isinstance(sp, torch.Tensor) else None
+
+
+class _ValueWalk(torch.autograd.Function):
+    """mean_i = 1/W sum_w value(walk i,w); reverse mode through the exit
+    records (values) and the per-walk source basis sums (coefficients)."""
+
+    @staticmethod
+    def forward(ctx, cwos, nwalks, points, scene, seed, values, source_params):
+        need_v = ctx.needs_input_grad[5]
+        need_c = ctx.needs_input_grad[6] and scene.source_type == 3
+        out = cwos.walk_reduce(points, scene, seed, nwalks=nwalks, records=need_v or need_c, basis=need_c)
+        ctx.scene, ctx.nwalks, ctx.P = scene, nwalks, out["mean"].shape[0]
+        ctx.need_v, ctx.need_c = need_v, need_c
+        ctx.sp_shape = None if source_params is None else source_params.shape
+        ctx.out = {k: out[k] for k in ("rec_prim", "rec_t", "basis") if k in out}
+        return out["mean"]
+
+    @staticmethod
+    def backward(ctx, g):
+        o = ctx.out
+        gv = gc = None
+        if ctx.need_v or ctx.need_c:
+            gv, gc = wos_ext.adjoint_values(ctx.scene, ctx.P, ctx.nwalks, o["rec_prim"], o["rec_t"], g.contiguous(),
+                                            basis=o.get("basis"), want_values=ctx.need_v, want_coeffs=ctx.need_c)
+        gsp = None
+        if ctx.need_c:
+            gsp = torch.zeros(ctx.sp_shape, dtype=torch.float32, device=g.device).reshape(-1, 4)
+            gsp[:, 3] = gc
+            gsp = gsp.reshape(ctx.sp_shape)
+        return None, None, None, None, None, gv, gsp
+
+
+class _ShapeWalk(torch.autograd.Function):
+    """Zero-valued output whose gradient w.r.t. the vertices is the
+    OursCUDA / Ours3D shape-gradient estimate, walks recomputed from the key."""
+
+    @staticmethod
+    def forward(ctx, solver, points, scene, seed, vertices):
+        ctx.solver, ctx.points, ctx.scene, ctx.seed = solver, points.detach(), scene, seed
+        return torch.zeros(points.shape[0], dtype=torch.float32, device=points.device)
+
+    @staticmethod
+    def backward(ctx, g):
+        s = ctx.solver
+        gv, _ = wos_ext.shape_gradient(ctx.scene, ctx.points, ctx.seed, g.contiguous(), nwalks=s.nwalks,
+                                       nsteps=s.nsteps, epsilon=s.epsilon, epsilon2=s.epsilon2,
+                                       clamping=s.clamping, double_sided=s.double_sided,
+                                       control_variates=getattr(s, "control_variates", True))
+        return None, None, None, None, gv
+
+
+@dataclass
+class Solver:
+    epsilon: float = 0.005   # controls condition to stop recursion
+    nwalks: int = 20         # number of samples per point queried
+    nsteps: int = 4          # maximum depth of a sampled path
+    double_sided: bool = False
+    control_variance: ControlVarianceType = ControlVarianceType.NoControlVariate
+    antithetic: bool = False
+
+    # ---- wos/solver.py:41-65 ------------------------------------------------
+    def solve(self, p, scene, seed=0):
+        return self.walk_reduce(p, scene, seed)
+
+    def walk(self, p, scene, seed=0):
+        return self.walk_reduce(p, scene, seed)
+
+    def walk_reduce(self, p, scene, seed=0):
+        return self._walk(p, scene, seed, self.nwalks)
+
+    def walk_detector(self, scene, detector, seed=0):
+        """wos/solver.py:67-84 (pixel jitter drawn per walk before walking).
+        The jitter streams are the reference's TEA(seed, k) keying; walks run
+        on the jittered points with the per-point mean over nwalks."""
+        p = detector.make_points()
+        n = p.shape[0] * self.nwalks
+        idx = torch.arange(n, dtype=torch.int64, device=p.device)
+        v0, v1 = sample_tea_32(seed, idx)
+        sampler = PCG32(n, v0.to(torch.int64), v1.to(torch.int64))
+        jp = detector.make_jittered_points(sampler, self.nwalks)
+        vals = self._walk(jp, scene, seed, 1)
+        return vals.reshape(-1, self.nwalks).mean(dim=1)
+
+    def single_walk(self, p, scene, sampler):
+        raise NotImplementedError
+
+    # subclasses provide _walk
+    def _walk(self, p, scene, seed, nwalks):
+        raise NotImplementedError
+
+
+class _CustomWalk(torch.autograd.Function):
+    """dr.custom(_CustomSolver) (wos/solver.py:90-114): detached forward
+    value of fwd_solver, gradient of bwd_solver under the same seed."""
+
+    @staticmethod
+    def forward(ctx, fwd, bwd, points, scene, seed, vertices, values, source_params):
+        with torch.no_grad():
+            val = fwd._walk(points, scene, seed, fwd.nwalks)
+        ctx.bwd, ctx.points, ctx.scene, ctx.seed = bwd, points.detach(), scene, seed
+        ctx.sp = source_params
+        return val.detach()
+
+    @staticmethod
+    def backward(ctx, g):
+        bwd, scene = ctx.bwd, ctx.scene
+        with torch.enable_grad():
+            vt = scene.vertices.detach().requires_grad_(ctx.needs_input_grad[5])
+            vv = scene.values.detach().requires_grad_(ctx.needs_input_grad[6])
+            sp = ctx.sp.detach().requires_grad_(ctx.needs_input_grad[7]) if ctx.sp is not None else None
+            out = bwd._walk_with(ctx.points, scene, ctx.seed, bwd.nwalks, vt, vv, sp)
+            ins = [x for x in (vt, vv, sp) if x is not None and x.requires_grad]
+            if not ins or not out.requires_grad:
+                return None, None, None, None, None, None, None, None
+            grads = torch.autograd.grad(out, ins, g, allow_unused=True)
+        it = iter(grads)
+        gvt = next(it) if vt.requires_grad else None
+        gvv = next(it) if vv.requires_grad else None
+        gsp = next(it) if (sp is not None and sp.requires_grad) else None
+        return None, None, None, None, None, gvt, gvv, gsp
+
+
+@dataclass
+class CustomSolver:
+    """wos/solver.py:117-125."""
+    fwd_solver: Solver = None
+    bwd_solver: Solver = None
+
+    def walk(self, p, scene, seed=0):
+        return _CustomWalk.apply(self.fwd_solver, self.bwd_solver, p, scene, seed, scene.vertices, scene.values,
+                                 _source_tensor(scene))
+
+
+class ValueSolverMixin:
+    """Solvers whose walk value is g(exit) (+ source contribution)."""
+
+    def _cwos(self):
+        raise NotImplementedError
+
+    def _walk(self, p, scene, seed, nwalks):
+        return self._walk_with(p, scene, seed, nwalks, scene.vertices, scene.values, _source_tensor(scene))
+
+    def _walk_with(self, p, scene, seed, nwalks, vertices, values, source_params):
+        need = _requires(values) or _requires(source_params)
+        if not (torch.is_grad_enabled() and need):
+            return self._cwos().walk_reduce(p, scene, seed, nwalks=nwalks)["mean"]
+        return _ValueWalk.apply(self._cwos(), nwalks, p, scene, seed, values, source_params)
+
+    def walk_stats(self, p, scene, seed=0):
+        """Per-point FP64 mean and variance of the walk values."""
+        o = self._cwos().walk_reduce(p, scene, seed, nwalks=self.nwalks, stats=True)
+        return o["mean64"], o["var64"]
+
+
+class ShapeSolverMixin:
+    """Solvers that return zero with the shape derivative as gradient."""
+
+    def _walk(self, p, scene, seed, nwalks):
+        return self._walk_with(p, scene, seed, nwalks, scene.vertices, scene.values, _source_tensor(scene))
+
+    def _walk_with(self, p, scene, seed, nwalks, vertices, values, source_params):
+        if nwalks != self.nwalks:
+            raise ValueError("shape-gradient solvers walk nwalks per point")
+        if not (torch.is_grad_enabled() and _requires(vertices)):
+            return torch.zeros(p.shape[0], dtype=torch.float32, device=p.device)
+        return _ShapeWalk.apply(self, p, scene, seed, vertices)
+
+    def sample_grad_u(self, p, scene, seed=0):
+        """Per-sample spatial gradient estimates (n_points*nwalks, dim)."""
+        _, sg = wos_ext.shape_gradient(scene, p, seed, None, nwalks=self.nwalks, nsteps=self.nsteps,
+                                       epsilon=self.epsilon, epsilon2=self.epsilon2, clamping=self.clamping,
+                                       double_sided=self.double_sided,
+                                       control_variates=getattr(self, "control_variates", True),
+                                       sample_grad_u=True)
+        return sg

@@ -83,6 +83,7 @@ _sig("wos_largest_inscribed_ball", _vp, ctypes.POINTER(_Record), _i, _f, _vp, _v
 _sig("wos_dirichlet", _vp, ctypes.POINTER(_Record), _i, _vp, _vp)
 _sig("wos_single_walk", _vp, ctypes.POINTER(_WalkParams), _vp, _i, _vp, _vp, ctypes.POINTER(_Record), _vp)
 _sig("wos_walk_reduce", _vp, ctypes.POINTER(_WalkParams), _vp, _i, _i, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp)
+_sig("wos_walk_reduce_counts", _vp, ctypes.POINTER(_WalkParams), _vp, _i, _i, _i, _vp, _vp, _vp)
 _sig("wos_adjoint_values", _vp, _i, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp)
 _sig("wos_shape_gradient", _vp, ctypes.POINTER(_GradParams), _vp, _i, _i, _vp, _vp, _vp, _vp)
 
@@ -90,7 +91,7 @@ EXPORTED = ("wos_last_error", "wos_scene_create", "wos_scene_destroy", "wos_scen
             "wos_scene_set_source", "wos_scene_set_use_bvh", "wos_scene_get_normals", "wos_scene_get_info",
             "wos_scene_get_bvh", "wos_closest_point_index", "wos_sdf", "wos_closest_point",
             "wos_largest_inscribed_ball", "wos_dirichlet", "wos_single_walk", "wos_walk_reduce",
-            "wos_adjoint_values", "wos_shape_gradient")
+            "wos_walk_reduce_counts", "wos_adjoint_values", "wos_shape_gradient")
 
 
 def _check(rc):
@@ -481,6 +482,20 @@ class WoS:
                                    _p(g("mean64")), _p(g("var64")), _p(g("walk_value")), _p(g("rec_prim")),
                                    _p(g("rec_t")), _p(g("basis")), _stream()))
         return out
+
+    def walk_counts(self, p, scene, seed=0, *, nwalks=None, basis=True):
+        """Instrumented walk_reduce: totals of AABB tests, primitive tests,
+        closest-primitive queries and walk steps of one launch."""
+        q = _dev(p, torch.float32, scene.dim)
+        P = q.shape[0]
+        W = self.nwalks if nwalks is None else int(nwalks)
+        wv = torch.empty(P * W, dtype=torch.float32, device=q.device)
+        cnt = torch.zeros(4, dtype=torch.int64, device=q.device)
+        wp = self._params(W)
+        _check(lib.wos_walk_reduce_counts(scene.handle, ctypes.byref(wp), _p(q), P, int(seed), int(bool(basis)),
+                                          _p(wv), _p(cnt), _stream()))
+        c = [int(x) for x in cnt.cpu()]
+        return dict(box_tests=c[0], prim_tests=c[1], queries=c[2], steps=c[3], walks=P * W)
 
     def solve(self, p, scene, seed=0):
         """WoS::solve (include/wos.cuh:320-364): mean of nwalks walks per point.

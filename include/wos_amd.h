@@ -130,6 +130,13 @@ int wos_walk_reduce(const wos_scene *scene, const wos_walk_params *params, const
                     int seed, int chunk, float *mean, double *mean64, double *var64, float *walk_value,
                     int *rec_prim, float *rec_t, float *basis, void *stream);
 
+/* Instrumented walk_reduce for measurement: same launch as wos_walk_reduce
+ * (with the Fourier basis outputs when with_basis), and adds into counts
+ * (4 device uint64, zeroed by the caller): AABB tests, primitive tests,
+ * closest-primitive queries, walk steps.  Slower than the plain launch. */
+int wos_walk_reduce_counts(const wos_scene *scene, const wos_walk_params *params, const float *points, int n_points,
+                           int seed, int with_basis, float *walk_value, unsigned long long *counts, void *stream);
+
 /* Reverse mode of walk_reduce with respect to the vertex values and the
  * Fourier coefficients: the gradient Dr.Jit's AD takes through
  * Scene.dirichlet's gathers (include/scene.cuh:455-468) and through the

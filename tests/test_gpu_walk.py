@@ -1,0 +1,146 @@
+"""GPU parity of the walks: single walks from explicit PCG32 states against
+the oracle (per walk), walk_reduce per point within 3 sigma of the oracle,
+RMSE against the exact harmonic solutions, and the value / coefficient
+adjoints against the oracle and central finite differences."""
+import numpy as np
+import pytest
+import torch
+
+import oracle as O
+from wos import configs as C
+
+pytestmark = pytest.mark.gpu
+
+
+def _t(a):
+    return torch.from_numpy(np.ascontiguousarray(a)).cuda()
+
+
+def _z(m1, v1, m2, v2, W):
+    se = np.sqrt((v1 + v2) / W) + 1e-9
+    return np.abs(m1 - m2) / se
+
+
+@pytest.mark.parametrize("source", [False, True])
+def test_single_walk_matches_oracle_per_walk(source):
+    import wos_ext
+    from wos.fwd import PCG32
+    c = C.config2(res=64, twin=True)
+    kw = dict(source_type=3, source_params=_t(c["source_params"])) if source else {}
+    sc = wos_ext.Scene(_t(c["vertices"]), _t(c["indices"]), _t(c["values"]), use_bvh=True, **kw)
+    ref = O.Scene(c["vertices"], c["indices"], c["values"], source_type=3 if source else 0,
+                  source_params=c["source_params"] if source else None)
+    p = np.repeat(c["points"][::3], 4, axis=0)
+    n = p.shape[0]
+    smp = PCG32(n, 12345, 678)
+    st0, inc0 = smp.state.cpu().numpy().view(np.uint64).copy(), smp.inc.cpu().numpy().view(np.uint64).copy()
+    w = wos_ext.WoS(nwalks=1, nsteps=128, epsilon=1e-3)
+    rec = w.single_walk(_t(p), sc, smp)
+    r = ref.single_walk(p, st0, inc0, nsteps=128, epsilon=1e-3)
+    same = (rec.prim_id.cpu().numpy() == r["prim"])
+    assert same.mean() > 0.995
+    np.testing.assert_array_equal(smp.state.cpu().numpy().view(np.uint64)[same], r["state"][same])
+    np.testing.assert_allclose(rec.p.cpu().numpy()[same], r["p"][same], atol=1e-5)
+    if source:
+        np.testing.assert_allclose(rec.contrib.cpu().numpy()[same], r["contrib"][same], rtol=1e-3, atol=1e-4)
+
+
+def test_pcg32_torch_matches_oracle():
+    from wos.fwd import PCG32
+    s = PCG32(5, 42, 54)
+    got = [s.next_uint32().cpu().numpy() for _ in range(4)]
+    for k in range(5):
+        ref = O.pcg32_stream(k, 42, 54, 4)
+        np.testing.assert_array_equal(np.array([g[k] for g in got], dtype=np.uint32), ref)
+
+
+def test_walk_reduce_config1_parity_and_rmse():
+    import wos_ext
+    c = C.config1()
+    sc = wos_ext.Scene(_t(c["vertices"]), _t(c["indices"]), _t(c["values"]), use_bvh=True)
+    w = wos_ext.WoS(nwalks=64, nsteps=256, epsilon=1e-3)
+    out = w.walk_reduce(_t(c["points"]), sc, seed=0, stats=True)
+    m, v = out["mean64"].cpu().numpy(), out["var64"].cpu().numpy()
+    rmse = np.sqrt(np.mean((m - c["exact"]) ** 2))
+    # RMSE of a 64-walk estimate of a function of size ~1: Monte Carlo bound
+    assert rmse < 1.2 * np.sqrt(np.mean(v / 64)) + 1e-3, rmse
+    sub = slice(None, None, 37)
+    ref = O.Scene(c["vertices"], c["indices"], c["values"]).walk_reduce(c["points"][sub], 64, 256, 1e-3, seed=0)
+    z = _z(m[sub], v[sub], ref["mean"], ref["var"], 64)
+    assert np.mean(z > 3) < 0.01, np.mean(z > 3)
+    # identical streams: most points agree to rounding
+    assert np.mean(np.abs(m[sub] - ref["mean"]) < 1e-4) > 0.9
+
+
+def test_walk_reduce_bvh_equals_bruteforce():
+    import wos_ext
+    c = C.config2(res=48)
+    sc = wos_ext.Scene(_t(c["vertices"]), _t(c["indices"]), _t(c["values"]), use_bvh=True)
+    w = wos_ext.WoS(nwalks=16, nsteps=256, epsilon=1e-4)
+    a = w.walk_reduce(_t(c["points"]), sc, seed=3, records=True)
+    sc.use_bvh = False
+    b = w.walk_reduce(_t(c["points"]), sc, seed=3, records=True)
+    np.testing.assert_array_equal(a["walk_value"].cpu().numpy(), b["walk_value"].cpu().numpy())
+    np.testing.assert_array_equal(a["rec_prim"].cpu().numpy(), b["rec_prim"].cpu().numpy())
+
+
+def test_config2_twin_poisson_parity():
+    import wos_ext
+    c = C.config2(res=96, twin=True)
+    sc = wos_ext.Scene(_t(c["vertices"]), _t(c["indices"]), _t(c["values"]), use_bvh=True, source_type=3,
+                       source_params=_t(c["source_params"]))
+    w = wos_ext.WoS(nwalks=128, nsteps=256, epsilon=1e-4)
+    out = w.walk_reduce(_t(c["points"]), sc, seed=0, stats=True)
+    m, v = out["mean64"].cpu().numpy(), out["var64"].cpu().numpy()
+    err = m - c["exact"]
+    assert np.sqrt(np.mean(err ** 2)) < 1.2 * np.sqrt(np.mean(v / 128)) + 2e-3
+    ref = O.Scene(c["vertices"], c["indices"], c["values"], 3, c["source_params"])
+    sub = slice(None, None, 23)
+    r = ref.walk_reduce(c["points"][sub], 128, 256, 1e-4, seed=0)
+    z = _z(m[sub], v[sub], r["mean"], r["var"], 128)
+    assert np.mean(z > 3) < 0.01
+
+
+def test_walk_reduce_3d_parity():
+    import wos_ext
+    v, f, noise = C.displaced_sphere(3)
+    vals = C.g_harmonic3(v).astype(np.float32)
+    p = C.points_in_star(2000, noise)
+    sc = wos_ext.Scene3D(_t(v), _t(f), _t(vals), use_bvh=True)
+    w = wos_ext.WoS3D(nwalks=32, nsteps=256, epsilon=1e-3)
+    out = w.walk_reduce(_t(p), sc, seed=0, stats=True)
+    m, var = out["mean64"].cpu().numpy(), out["var64"].cpu().numpy()
+    r = O.Scene(v, f, vals).walk_reduce(p[::5], 32, 256, 1e-3, seed=0)
+    z = _z(m[::5], var[::5], r["mean"], r["var"], 32)
+    assert np.mean(z > 3) < 0.01
+
+
+def test_adjoint_values_and_coeffs():
+    import wos_ext
+    c = C.config2(res=40)
+    sc = wos_ext.Scene(_t(c["vertices"]), _t(c["indices"]), _t(c["values"]), use_bvh=True, source_type=3,
+                       source_params=_t(c["source_params"]))
+    W = 32
+    w = wos_ext.WoS(nwalks=W, nsteps=256, epsilon=1e-4)
+    P = c["points"].shape[0]
+    out = w.walk_reduce(_t(c["points"]), sc, seed=5, records=True, basis=True)
+    g = np.random.default_rng(0).normal(size=P).astype(np.float32)
+    gv, gc = wos_ext.adjoint_values(sc, P, W, out["rec_prim"], out["rec_t"], _t(g), basis=out["basis"],
+                                    want_values=True, want_coeffs=True)
+    ref = O.Scene(c["vertices"], c["indices"], c["values"], 3, c["source_params"])
+    gv_ref = ref.adjoint_values(P, W, out["rec_prim"].cpu().numpy(), out["rec_t"].cpu().numpy(), g)
+    np.testing.assert_allclose(gv.cpu().numpy(), gv_ref, rtol=1e-4, atol=1e-6)
+    # coefficients: d mean / d c_k by central differences under common random numbers (exactly linear)
+    K = c["source_params"].shape[0]
+    for k in (0, 2, 5):
+        h = 1e-2
+        sp = c["source_params"].copy()
+        sp[k, 3] += h
+        sc.set_source(3, _t(sp))
+        up = w.walk_reduce(_t(c["points"]), sc, seed=5)["mean"].cpu().numpy().astype(np.float64)
+        sp[k, 3] -= 2 * h
+        sc.set_source(3, _t(sp))
+        dn = w.walk_reduce(_t(c["points"]), sc, seed=5)["mean"].cpu().numpy().astype(np.float64)
+        fd = float(g.astype(np.float64) @ ((up - dn) / (2 * h)))
+        assert abs(fd - float(gc[k])) <= 2e-3 * (abs(fd) + 1.0), (k, fd, float(gc[k]))
+    assert gc.shape[0] == K
