@@ -86,6 +86,51 @@ def plain_step(torch, wos_ext, scene_bf, cwos, pts, target, seed, W):
     return u, gv
 
 
+def primal_workload(args, torch, wos_ext, C):
+    """Primal-only Laplace workloads: walks/s, closest-point queries/s, RMSE
+    against the exact harmonic solution, LBVH build time."""
+    c = C.config1() if args.workload == "circle" else C.config3()
+    dim = c["vertices"].shape[1]
+    dv = torch.device("cuda", 0)
+    t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dv)
+    V, F, G, pts = t(c["vertices"]), t(c["indices"]), t(c["values"]), t(c["points"])
+    Scene, WoS = (wos_ext.Scene, wos_ext.WoS) if dim == 2 else (wos_ext.Scene3D, wos_ext.WoS3D)
+    Scene(V, F, G, use_bvh=True)   # warm the build kernels
+    torch.cuda.synchronize()
+    tb = time.perf_counter()
+    scene = Scene(V, F, G, use_bvh=True)
+    build_ms = (time.perf_counter() - tb) * 1e3
+    P, W = pts.shape[0], c["nwalks"]
+    cw = WoS(nwalks=W, nsteps=c["nsteps"], epsilon=c["epsilon"])
+    for i in range(args.warmup):
+        cw.walk_reduce(pts, scene, seed=i)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        out = cw.walk_reduce(pts, scene, seed=100 + i, stats=(i == args.steps - 1))
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / args.steps
+    rmse = float(np.sqrt(np.mean((out["mean64"].cpu().numpy() - c["exact"]) ** 2)))
+    # closest-point queries/s (LBVH traversal alone)
+    scene.closest_point_preliminary(pts)
+    torch.cuda.synchronize()
+    tq = time.perf_counter()
+    for _ in range(3):
+        scene.closest_point_preliminary(pts)
+    torch.cuda.synchronize()
+    qps = 3 * P / (time.perf_counter() - tq)
+    cnt = cw.walk_counts(pts, scene, seed=100, basis=False)
+    log(f"{args.workload}: {ms:.2f} ms/solve, {P * W / ms * 1e3:.3e} walks/s, rmse {rmse:.4f}, "
+        f"build {build_ms:.2f} ms, {qps:.3e} queries/s, counts {cnt}")
+    print(json.dumps({"metric": "walks/s (primal)", "value": P * W / ms * 1e3, "unit": "walks/s", "n_gpus": 1,
+                      "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True,
+                      "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+                      "config": {"workload": args.workload, "points": P, "walks_per_point": W,
+                                 "primitives": int(c["indices"].shape[0])},
+                      "rmse_exact": rmse, "lbvh_build_ms": build_ms, "closest_point_queries_per_s": qps,
+                      "counts": cnt}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -94,6 +139,12 @@ def main():
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="run only a 1 GiB device-to-device copy (3 times), the reference kernel for "
+                         "scaling FETCH_SIZE / WRITE_SIZE counters")
+    ap.add_argument("--workload", choices=["clover", "circle", "sphere"], default="clover",
+                    help="clover: the headline primal+adjoint workload; circle / sphere: primal-only "
+                         "Laplace workloads (1024-segment circle, 81,920-triangle displaced icosphere)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU; multi-GPU runs are not part of it",
@@ -105,6 +156,16 @@ def main():
     from wos import configs as C
 
     torch.cuda.set_device(0)
+    if args.calibrate_copy:
+        a = torch.empty(1 << 28, dtype=torch.float32, device="cuda").fill_(1.0)
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"calibration": "copy", "bytes_read": a.numel() * 4, "bytes_written": b.numel() * 4}))
+        return
+    if args.workload != "clover":
+        return primal_workload(args, torch, wos_ext, C)
     c, d = build_inputs(torch, C)
     P, W = c["points"].shape[0], c["nwalks"]
     log(f"workload: {P} points x {W} walks, 4096 segments, eps 1e-4")

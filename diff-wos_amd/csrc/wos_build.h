@@ -27,6 +27,7 @@ struct BuildOutputs {
     int *children;          // 2 * (n_prims - 1), nullable
     float amin[3], amax[3];
     int max_depth;
+    int top_counts[17];     // internal nodes with depth < k, k = 0..16 (depth-major order)
     uint32_t error_bits;    // 1: degenerate primitive, 2: index out of range
 };
 
@@ -50,6 +51,14 @@ struct WalkOutputs {
     double *mean64;      // P, nullable
     double *var64;       // P, nullable
 };
+
+// Launch tuning shared by all walk launches (set through wos_set_tuning).
+struct Tuning {
+    int chunk = 256;        // walks a wave claims per queue visit
+    int wgs_per_cu = 8;     // persistent workgroups per CU
+    int top_levels = 6;     // BVH levels kept in LDS
+};
+extern Tuning g_tuning;
 
 // Grows (never shrinks) the traversal spill and the launch counters.
 int2 *spill_buffer(const SceneView &s, long lanes, long *stride, hipStream_t stream);

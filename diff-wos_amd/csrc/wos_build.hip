@@ -18,6 +18,10 @@ __device__ __forceinline__ float o2f(uint32_t o) {
     return __uint_as_float(b);
 }
 
+__global__ void k_iota(int *p, int n) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = i;
+}
+
 __global__ void k_fill_u32(uint32_t *p, uint32_t v, long n) {
     for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) p[i] = v;
 }
@@ -375,6 +379,37 @@ __global__ void k_depth(int n, const int *par_int, const int *par_leaf, uint32_t
     }
 }
 
+// Depth of every internal node (root 0) and a histogram of depths.
+__global__ void k_internal_depth(int n_int, const int *par_int, uint32_t *depth, uint32_t *hist) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n_int; i += gridDim.x * blockDim.x) {
+        uint32_t dep = 0;
+        int x = par_int[i];
+        while (x >= 0) { ++dep; x = par_int[x]; }
+        depth[i] = dep;
+        atomicAdd(&hist[dep < 63 ? dep : 63], 1u);
+    }
+}
+
+// pos[old node] = new node (the stable depth-sorted order).
+__global__ void k_scatter_pos(const int *sorted_old, int n_int, int *pos) {
+    for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < n_int; j += gridDim.x * blockDim.x)
+        pos[sorted_old[j]] = j;
+}
+
+// Copies every node to its depth-major slot, renumbering internal children.
+__global__ void k_reorder_nodes(const float4 *src, const int *pos, int n_int, int D, float4 *dst) {
+    const int W = (D == 2) ? 3 : 4;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n_int; i += gridDim.x * blockDim.x) {
+        int j = pos[i];
+        for (int k = 0; k < W - 1; ++k) dst[W * j + k] = src[W * i + k];
+        float4 c = src[W * i + W - 1];
+        int l = __float_as_int(c.x), r = __float_as_int(c.y);
+        if (l >= 0) l = pos[l];
+        if (r >= 0) r = pos[r];
+        dst[W * j + W - 1] = make_float4(__int_as_float(l), __int_as_float(r), 0.0f, 0.0f);
+    }
+}
+
 // 2D vertex normals: sum of the unit normals (dy, -dx) of adjacent segments
 // (include/scene.cuh:369-380).  Two contributions per vertex sum exactly in
 // either order.
@@ -457,9 +492,14 @@ int build_scene_gpu(BuildInputs &in, hipStream_t stream, BuildOutputs &out) {
     uint32_t *flags = nullptr;
     int nblocks = (n + kRsTile - 1) / kRsTile;
     int rc = -1;
-    uint32_t hmeta[14];
+    uint32_t hmeta[14 + 64];
+    uint32_t *dhist = nullptr, *ndepth = nullptr;
+    int *dvals[2] = {nullptr, nullptr}, *pos = nullptr;
+    uint32_t *dkeys = nullptr;
+    float4 *tmp_nodes = nullptr;
 
-    BCHK(hipMallocAsync((void **)&meta, 16 * sizeof(uint32_t), stream));
+    BCHK(hipMallocAsync((void **)&meta, (14 + 64) * sizeof(uint32_t), stream));
+    dhist = meta + 14;
     BCHK(hipMallocAsync((void **)&cent, sizeof(float) * 3 * (size_t)n, stream));
     for (int b = 0; b < 2; ++b) {
         BCHK(hipMallocAsync((void **)&keys[b], sizeof(uint32_t) * (size_t)n, stream));
@@ -498,6 +538,28 @@ int build_scene_gpu(BuildInputs &in, hipStream_t stream, BuildOutputs &out) {
         k_refit<<<grid_for(n), kBlock, 0, stream>>>(n, D, children, par_int, par_leaf, in.vtx, in.idx, vals[0],
                                                   out.nodes, flags);
         k_depth<<<grid_for(n), kBlock, 0, stream>>>(n, par_int, par_leaf, meta + 13);
+        // depth-major node order: the top levels become a contiguous prefix
+        // (root stays 0), which the traversal can keep in LDS
+        const int ni = n - 1;
+        const int nb = (ni + kRsTile - 1) / kRsTile;
+        const int W = (D == 2) ? 3 : 4;
+        BCHK(hipMallocAsync((void **)&ndepth, sizeof(uint32_t) * (size_t)ni, stream));
+        BCHK(hipMallocAsync((void **)&dkeys, sizeof(uint32_t) * (size_t)ni, stream));
+        for (int b = 0; b < 2; ++b) BCHK(hipMallocAsync((void **)&dvals[b], sizeof(int) * (size_t)ni, stream));
+        BCHK(hipMallocAsync((void **)&pos, sizeof(int) * (size_t)ni, stream));
+        BCHK(hipMallocAsync((void **)&tmp_nodes, sizeof(float4) * W * (size_t)ni, stream));
+        k_fill_u32<<<1, 64, 0, stream>>>(dhist, 0u, 64);
+        k_internal_depth<<<grid_for(ni), kBlock, 0, stream>>>(ni, par_int, ndepth, dhist);
+        k_iota<<<grid_for(ni), kBlock, 0, stream>>>(dvals[0], ni);
+        uint32_t *hist2 = nullptr;
+        BCHK(hipMallocAsync((void **)&hist2, sizeof(uint32_t) * 256 * (size_t)nb, stream));
+        k_rs_hist<<<nb, kBlock, 0, stream>>>(ndepth, ni, 0, nb, hist2);
+        k_rs_scan<<<1, 1024, 0, stream>>>(hist2, 256L * nb);
+        k_rs_scatter<<<nb, kBlock, 0, stream>>>(ndepth, dvals[0], ni, 0, nb, hist2, dkeys, dvals[1]);
+        k_scatter_pos<<<grid_for(ni), kBlock, 0, stream>>>(dvals[1], ni, pos);
+        k_reorder_nodes<<<grid_for(ni), kBlock, 0, stream>>>(out.nodes, pos, ni, D, tmp_nodes);
+        BCHK(hipMemcpyAsync(out.nodes, tmp_nodes, sizeof(float4) * W * (size_t)ni, hipMemcpyDeviceToDevice, stream));
+        (void)hipFreeAsync(hist2, stream);
         if (out.children) BCHK(hipMemcpyAsync(out.children, children, sizeof(int) * 2 * (size_t)(n - 1),
                                               hipMemcpyDeviceToDevice, stream));
     }
@@ -521,6 +583,14 @@ int build_scene_gpu(BuildInputs &in, hipStream_t stream, BuildOutputs &out) {
         if (k >= D) out.amin[k] = out.amax[k] = 0.0f;
     }
     out.error_bits = hmeta[12];
+    {
+        uint32_t acc = 0;
+        out.top_counts[0] = 0;
+        for (int k = 0; k < 16; ++k) {
+            acc += (n > 1) ? hmeta[14 + k] : 0u;
+            out.top_counts[k + 1] = (int)acc;
+        }
+    }
     out.max_depth = (int)hmeta[13];
     rc = 0;
     (void)hipFreeAsync(meta, stream);
@@ -534,6 +604,12 @@ int build_scene_gpu(BuildInputs &in, hipStream_t stream, BuildOutputs &out) {
     (void)hipFreeAsync(par_int, stream);
     (void)hipFreeAsync(par_leaf, stream);
     (void)hipFreeAsync(flags, stream);
+    if (ndepth) (void)hipFreeAsync(ndepth, stream);
+    if (dkeys) (void)hipFreeAsync(dkeys, stream);
+    for (int b = 0; b < 2; ++b)
+        if (dvals[b]) (void)hipFreeAsync(dvals[b], stream);
+    if (pos) (void)hipFreeAsync(pos, stream);
+    if (tmp_nodes) (void)hipFreeAsync(tmp_nodes, stream);
     return rc;
 }
 

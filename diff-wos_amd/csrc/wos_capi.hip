@@ -20,6 +20,7 @@ struct wos_scene {
     float amin[3] = {0, 0, 0}, amax[3] = {0, 0, 0};
     float tol = 0.0f;
     int max_depth = 0;
+    int top_counts[17] = {0};
 };
 
 namespace wos {
@@ -108,6 +109,11 @@ static SceneView view_of(const wos_scene *s) {
     v.tol = s->tol;
     v.max_depth = s->max_depth;
     v.stats = nullptr;
+    int lv = std::min(std::max(g_tuning.top_levels, 0), 16);
+    int nt = s->n_prims > 1 ? s->top_counts[lv] : 0;
+    const int node_bytes = (s->dim == 2 ? 3 : 4) * (int)sizeof(float4);
+    while (nt > 0 && (size_t)nt * node_bytes > 32768) nt = s->top_counts[--lv];
+    v.n_top = nt;
     return v;
 }
 
@@ -234,7 +240,25 @@ int wos_scene_create(int dim, const float *vertices, int n_vertices, const int *
     }
     s->tol = 1e-6f * m;
     s->max_depth = bo.max_depth;
+    for (int k = 0; k < 17; ++k) s->top_counts[k] = bo.top_counts[k];
     *out = s;
+    return 0;
+}
+
+int wos_set_tuning(int chunk, int wgs_per_cu, int top_levels) {
+    REQUIRE(chunk >= 64 && chunk <= (1 << 20), "chunk must be in [64, 2^20] (got %d)", chunk);
+    REQUIRE(wgs_per_cu >= 1 && wgs_per_cu <= 32, "wgs_per_cu must be in [1, 32] (got %d)", wgs_per_cu);
+    REQUIRE(top_levels >= 0 && top_levels <= 16, "top_levels must be in [0, 16] (got %d)", top_levels);
+    g_tuning.chunk = chunk;
+    g_tuning.wgs_per_cu = wgs_per_cu;
+    g_tuning.top_levels = top_levels;
+    return 0;
+}
+
+int wos_get_tuning(int *chunk, int *wgs_per_cu, int *top_levels) {
+    if (chunk) *chunk = g_tuning.chunk;
+    if (wgs_per_cu) *wgs_per_cu = g_tuning.wgs_per_cu;
+    if (top_levels) *top_levels = g_tuning.top_levels;
     return 0;
 }
 

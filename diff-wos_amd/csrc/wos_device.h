@@ -16,7 +16,7 @@ namespace wos {
 constexpr float kPi = 3.14159265358979323846f;
 constexpr float kInv2Pi = 0.15915494309189533577f;
 constexpr int kBlock = 256;          // threads per workgroup (4 wave64)
-constexpr int kLdsStack = 16;        // traversal stack entries kept in LDS per lane
+constexpr int kLdsStack = 8;         // traversal stack entries kept in LDS per lane
 
 // ---------------------------------------------------------------------------
 // random numbers
@@ -262,12 +262,14 @@ struct SceneView {
     float tol;             // absolute pruning margin (scale-relative)
     int max_depth;
     unsigned long long *stats;  // null, or {box tests, primitive tests, queries, walk steps}
+    int n_top;             // leading nodes (depth-major order) the kernel keeps in LDS
 };
 
 // Per-lane traversal stack: the first kLdsStack entries live in LDS (lane
 // stride kBlock), deeper entries spill to a global buffer (lane stride
 // spill_stride).  Depth never exceeds the tree depth, which sizes the spill.
 struct TravStack {
+    const float4 *top;    // LDS copy of nodes [0, n_top)
     int *lds_node;
     float *lds_d2;
     int2 *spill;          // may be null when the tree fits the LDS part
@@ -376,14 +378,15 @@ __device__ int closest_bvh(const SceneView &s, const float *p, float bound_d2, f
         nbox += 2;
         int cL, cR;
         if constexpr (D == 2) {
-            float4 b0 = s.nodes[3 * node], b1 = s.nodes[3 * node + 1], c = s.nodes[3 * node + 2];
+            const float4 *nb = (node < s.n_top) ? st.top + 3 * node : s.nodes + 3 * node;
+            float4 b0 = nb[0], b1 = nb[1], c = nb[2];
             dL = box_d2_2(p[0], p[1], b0);
             dR = box_d2_2(p[0], p[1], b1);
             cL = __float_as_int(c.x);
             cR = __float_as_int(c.y);
         } else {
-            float4 n0 = s.nodes[4 * node], n1 = s.nodes[4 * node + 1], n2 = s.nodes[4 * node + 2],
-                   c = s.nodes[4 * node + 3];
+            const float4 *nb = (node < s.n_top) ? st.top + 4 * node : s.nodes + 4 * node;
+            float4 n0 = nb[0], n1 = nb[1], n2 = nb[2], c = nb[3];
             dL = box_d2_3(p, n0.x, n0.y, n0.z, n0.w, n1.x, n1.y);
             dR = box_d2_3(p, n1.z, n1.w, n2.x, n2.y, n2.z, n2.w);
             cL = __float_as_int(c.x);

@@ -13,6 +13,8 @@
 //                      Fourier source coefficients, from stored exit records
 //   k_shape_gradient   OursCUDA / Ours3D (wos/wos_grad.py:442-463,
 //                      wos/wos_grad_3d.py:295-315) transposed onto vertices
+#include <algorithm>
+
 #include "wos_build.h"
 
 namespace wos {
@@ -36,10 +38,18 @@ static int grid_of(long n, int cap = kGridCap) {
     return (int)g;
 }
 
+// Per-workgroup traversal state: the LDS stack and, when s.n_top > 0, an LDS
+// copy of the first s.n_top nodes (dynamic shared memory).
 #define DECLARE_STACK(spill_, stride_)                                          \
     __shared__ int s_stack_node[kLdsStack * kBlock];                            \
     __shared__ float s_stack_d2[kLdsStack * kBlock];                            \
-    TravStack st{s_stack_node, s_stack_d2, (spill_), (stride_)};                \
+    extern __shared__ float4 s_top_nodes[];                                     \
+    if (s.n_top > 0 && s.use_bvh) {                                             \
+        const int nw_ = s.n_top * (s.dim == 2 ? 3 : 4);                         \
+        for (int i_ = threadIdx.x; i_ < nw_; i_ += kBlock) s_top_nodes[i_] = s.nodes[i_]; \
+        __syncthreads();                                                        \
+    }                                                                           \
+    TravStack st{s_top_nodes, s_stack_node, s_stack_d2, (spill_), (stride_)};   \
     const int slot = threadIdx.x;                                               \
     const long lane_g = (long)blockIdx.x * kBlock + threadIdx.x;
 
@@ -264,6 +274,8 @@ struct PersistArgs {
     long long total;                 // n_points * nwalks
     unsigned long long *counter;     // zero before launch
     int chunk;                       // walks claimed per queue visit
+    const int *init_prim;            // per point: closest primitive of the start point
+    const float *init_sd;            // per point: signed distance of the start point
     float *walk_value;
     int *rec_prim;
     float *rec_t;
@@ -273,7 +285,49 @@ struct PersistArgs {
     long spill_stride;
 };
 
-enum : int { kIdle = 0, kInit = 1, kStep = 2 };
+// All walks of a point start at the same place, so the (unbounded) start
+// query runs once per point here instead of once per walk.
+template <int D>
+__global__ __launch_bounds__(kBlock) void k_point_init(SceneView s, const float *p, int n, int *prim, float *sd,
+                                                       int2 *spill, long stride) {
+    DECLARE_STACK(spill, stride);
+    for (long i = lane_g; i < n; i += (long)gridDim.x * kBlock) {
+        float q[3];
+        load_point<D>(p, i, q);
+        float d2;
+        int id = closest_prim<D>(s, q, __builtin_inff(), d2, st, slot, lane_g);
+        prim[i] = id;
+        sd[i] = id < 0 ? __builtin_inff() : signed_dist<D>(s, id, q);
+    }
+}
+
+enum : int { kIdle = 0, kStep = 2 };
+
+template <int D, bool BASIS>
+__device__ __forceinline__ void write_walk(const SceneView &s, const PersistArgs &a, long long k, bool valid,
+                                           int prim, const float *p, float contrib, const float *bas) {
+    float v = 0.0f, t0 = 0.0f, t1 = 0.0f;
+    int pr = -1;
+    if (valid) {
+        Rec r;
+        closest_record<D>(s, prim, p, r);
+        v = dirichlet_at<D>(s, prim, r.t, r.uv[0], r.uv[1]) + contrib;
+        pr = prim;
+        t0 = (D == 2) ? r.t : r.uv[0];
+        t1 = r.uv[1];
+    }
+    if (a.walk_value) a.walk_value[k] = v;
+    if (a.rec_prim) a.rec_prim[k] = pr;
+    if (a.rec_t) {
+        if (D == 2) a.rec_t[k] = t0;
+        else { a.rec_t[2 * k] = t0; a.rec_t[2 * k + 1] = t1; }
+    }
+    if constexpr (BASIS) {
+#pragma unroll
+        for (int q = 0; q < kMaxBasis; ++q)
+            if (q < a.K) a.basis[(long long)q * a.total + k] = valid ? bas[q] : 0.0f;
+    }
+}
 
 // Each wave keeps 64 walks in flight.  Lanes whose walk ends take the next
 // walk index of the wave's chunk (ballot + mbcnt), and the wave claims a new
@@ -299,32 +353,13 @@ __global__ __launch_bounds__(kBlock) void k_walk_persistent(SceneView s, Persist
     const int W = a.w.nwalks;
 
     for (;;) {
-        // 1. lanes whose walk has ended write their result
+        // 1. walks that reached the boundary or ran out of steps
+        //    (include/wos.cuh:107-113, 144)
         if (mode == kStep) {
             bool done_valid = d < a.w.epsilon && step < a.w.nsteps;
             bool done_invalid = !done_valid && step >= a.w.nsteps;
             if (done_valid || done_invalid) {
-                float v = 0.0f, t0 = 0.0f, t1 = 0.0f;
-                int pr = -1;
-                if (done_valid) {
-                    Rec r;
-                    closest_record<D>(s, prim, p, r);
-                    v = dirichlet_at<D>(s, prim, r.t, r.uv[0], r.uv[1]) + contrib;
-                    pr = prim;
-                    t0 = (D == 2) ? r.t : r.uv[0];
-                    t1 = r.uv[1];
-                }
-                if (a.walk_value) a.walk_value[k] = v;
-                if (a.rec_prim) a.rec_prim[k] = pr;
-                if (a.rec_t) {
-                    if (D == 2) a.rec_t[k] = t0;
-                    else { a.rec_t[2 * k] = t0; a.rec_t[2 * k + 1] = t1; }
-                }
-                if constexpr (BASIS) {
-#pragma unroll
-                    for (int q = 0; q < kMaxBasis; ++q)
-                        if (q < a.K) a.basis[(long long)q * a.total + k] = done_valid ? bas[q] : 0.0f;
-                }
+                write_walk<D, BASIS>(s, a, k, done_valid, prim, p, contrib, bas);
                 mode = kIdle;
                 k = -1;
             }
@@ -356,15 +391,29 @@ __global__ __launch_bounds__(kBlock) void k_walk_persistent(SceneView s, Persist
                 if (r < avail) kk = c_next + r;
                 else if (fresh && (r - avail) < end2 - base2) kk = base2 + (r - avail);
                 if (kk >= 0) {
-                    k = kk;
                     long long i = kk / W;
-                    rng = walk_stream(a.seed, i, kk);
+                    int ip = a.init_prim[i];
+                    float sd = a.init_sd[i];
                     load_point<D>(a.points, i, p);
                     contrib = 0.0f;
 #pragma unroll
                     for (int q = 0; q < (BASIS ? kMaxBasis : 1); ++q) bas[q] = 0.0f;
-                    step = 0;
-                    mode = kInit;
+                    // include/wos.cuh:100-106, then the first loop test
+                    bool outside = ip < 0 || (sd >= 0.0f && !a.w.double_sided);
+                    float d0 = fabsf(sd);
+                    if (a.w.prevent_fd_artifacts) d0 *= 0.5f;
+                    if (outside || a.w.nsteps <= 0) {
+                        write_walk<D, BASIS>(s, a, kk, false, ip, p, 0.0f, bas);
+                    } else if (d0 < a.w.epsilon) {
+                        write_walk<D, BASIS>(s, a, kk, true, ip, p, 0.0f, bas);
+                    } else {
+                        k = kk;
+                        rng = walk_stream(a.seed, i, kk);
+                        prim = ip;
+                        d = d0;
+                        step = 0;
+                        mode = kStep;
+                    }
                 }
             }
             if (avail >= need) {
@@ -379,8 +428,8 @@ __global__ __launch_bounds__(kBlock) void k_walk_persistent(SceneView s, Persist
         }
         if (exhausted && __ballot(mode != kIdle) == 0) break;
 
-        // 3. stepping lanes sample the source term and jump to the sphere
-        float bd2 = __builtin_inff();
+        // 3. stepping lanes sample the source term, jump to the sphere and
+        //    query the closest primitive within 2d of the new point
         if (mode == kStep) {
             if (SRC) {
                 if constexpr (BASIS) {
@@ -396,7 +445,6 @@ __global__ __launch_bounds__(kBlock) void k_walk_persistent(SceneView s, Persist
                     if (pdf > 1e-5f) {
                         float yx = p[0] + dyx, yy = p[1] + dyy;
                         float f = 0.0f;
-                        float GA = 0.0f;
 #pragma unroll
                         for (int q = 0; q < kMaxBasis; ++q) {
                             if (q < a.K) {
@@ -406,8 +454,7 @@ __global__ __launch_bounds__(kBlock) void k_walk_persistent(SceneView s, Persist
                                 bas[q] = fmaf(b * G, area, bas[q]);
                             }
                         }
-                        GA = f * G;
-                        contrib = fmaf(GA, area, contrib);
+                        contrib = fmaf(f * G, area, contrib);
                     }
                 } else {
                     source_step<D>(s, p, d, rng, contrib);
@@ -416,34 +463,8 @@ __global__ __launch_bounds__(kBlock) void k_walk_persistent(SceneView s, Persist
             move_point<D>(p, d, rng);
             ++step;
             if (s.stats) atomicAdd(&s.stats[3], 1ull);
-            bd2 = bound_d2(s, 2.0f * d);
-        }
-        // 4. one closest-primitive query per active lane
-        float d2 = 0.0f;
-        if (mode != kIdle) prim = closest_prim<D>(s, p, bd2, d2, st, slot, lane_g);
-        // 5. new walks: inside test (include/wos.cuh:100-106)
-        if (mode == kInit) {
-            float sd = prim >= 0 ? signed_dist<D>(s, prim, p) : __builtin_inff();
-            if (prim < 0 || (sd >= 0.0f && !a.w.double_sided)) {
-                if (a.walk_value) a.walk_value[k] = 0.0f;
-                if (a.rec_prim) a.rec_prim[k] = -1;
-                if (a.rec_t) {
-                    if (D == 2) a.rec_t[k] = 0.0f;
-                    else { a.rec_t[2 * k] = 0.0f; a.rec_t[2 * k + 1] = 0.0f; }
-                }
-                if constexpr (BASIS) {
-#pragma unroll
-                    for (int q = 0; q < kMaxBasis; ++q)
-                        if (q < a.K) a.basis[(long long)q * a.total + k] = 0.0f;
-                }
-                mode = kIdle;
-                k = -1;
-            } else {
-                d = fabsf(sd);
-                if (a.w.prevent_fd_artifacts) d *= 0.5f;
-                mode = kStep;
-            }
-        } else if (mode == kStep) {
+            float d2;
+            prim = closest_prim<D>(s, p, bound_d2(s, 2.0f * d), d2, st, slot, lane_g);
             d = sqrtf(d2);
         }
     }
@@ -700,6 +721,12 @@ __global__ __launch_bounds__(kBlock) void k_shape_gradient(SceneView s, GradSett
     if ((s).dim == 2) { __VA_ARGS__(2) }        \
     else { __VA_ARGS__(3) }
 
+Tuning g_tuning;
+
+static size_t top_bytes(const SceneView &s) {
+    return s.use_bvh ? (size_t)s.n_top * sizeof(float4) * (s.dim == 2 ? 3 : 4) : 0;
+}
+
 static int occupancy_grid() {
     static int cus = 0;
     if (!cus) {
@@ -708,7 +735,7 @@ static int occupancy_grid() {
         (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
         if (cus <= 0) cus = 256;
     }
-    return cus * 4;   // four 256-lane workgroups per CU (32 KB LDS stack each)
+    return cus * (g_tuning.wgs_per_cu > 0 ? g_tuning.wgs_per_cu : 8);
 }
 
 int launch_closest_index(const SceneView &s, const float *p, int n, int *out, float *out_d2, hipStream_t st) {
@@ -716,8 +743,8 @@ int launch_closest_index(const SceneView &s, const float *p, int n, int *out, fl
     int g = grid_of(n, occupancy_grid());
     long stride;
     int2 *spill = spill_buffer(s, (long)g * kBlock, &stride, st);
-    if (s.dim == 2) k_closest_index<2><<<g, kBlock, 0, st>>>(s, p, n, out, out_d2, spill, stride);
-    else k_closest_index<3><<<g, kBlock, 0, st>>>(s, p, n, out, out_d2, spill, stride);
+    if (s.dim == 2) k_closest_index<2><<<g, kBlock, top_bytes(s), st>>>(s, p, n, out, out_d2, spill, stride);
+    else k_closest_index<3><<<g, kBlock, top_bytes(s), st>>>(s, p, n, out, out_d2, spill, stride);
     KCHK();
     return 0;
 }
@@ -727,8 +754,8 @@ int launch_sdf(const SceneView &s, const float *p, int n, float *out, hipStream_
     int g = grid_of(n, occupancy_grid());
     long stride;
     int2 *spill = spill_buffer(s, (long)g * kBlock, &stride, st);
-    if (s.dim == 2) k_sdf<2><<<g, kBlock, 0, st>>>(s, p, n, out, spill, stride);
-    else k_sdf<3><<<g, kBlock, 0, st>>>(s, p, n, out, spill, stride);
+    if (s.dim == 2) k_sdf<2><<<g, kBlock, top_bytes(s), st>>>(s, p, n, out, spill, stride);
+    else k_sdf<3><<<g, kBlock, top_bytes(s), st>>>(s, p, n, out, spill, stride);
     KCHK();
     return 0;
 }
@@ -740,9 +767,9 @@ int launch_closest_point(const SceneView &s, const float *p, int n, uint8_t *val
     long stride;
     int2 *spill = spill_buffer(s, (long)g * kBlock, &stride, st);
     if (s.dim == 2)
-        k_closest_point<2><<<g, kBlock, 0, st>>>(s, p, n, valid, op, on, ot, oprim, ocontrib, spill, stride);
+        k_closest_point<2><<<g, kBlock, top_bytes(s), st>>>(s, p, n, valid, op, on, ot, oprim, ocontrib, spill, stride);
     else
-        k_closest_point<3><<<g, kBlock, 0, st>>>(s, p, n, valid, op, on, ot, oprim, ocontrib, spill, stride);
+        k_closest_point<3><<<g, kBlock, top_bytes(s), st>>>(s, p, n, valid, op, on, ot, oprim, ocontrib, spill, stride);
     KCHK();
     return 0;
 }
@@ -753,8 +780,8 @@ int launch_lib(const SceneView &s, const uint8_t *valid, const float *p, const f
     int g = grid_of(n, occupancy_grid());
     long stride;
     int2 *spill = spill_buffer(s, (long)g * kBlock, &stride, st);
-    if (s.dim == 2) k_lib<2><<<g, kBlock, 0, st>>>(s, valid, p, nrm, n, eps, out, spill, stride);
-    else k_lib<3><<<g, kBlock, 0, st>>>(s, valid, p, nrm, n, eps, out, spill, stride);
+    if (s.dim == 2) k_lib<2><<<g, kBlock, top_bytes(s), st>>>(s, valid, p, nrm, n, eps, out, spill, stride);
+    else k_lib<3><<<g, kBlock, top_bytes(s), st>>>(s, valid, p, nrm, n, eps, out, spill, stride);
     KCHK();
     return 0;
 }
@@ -776,10 +803,10 @@ int launch_single_walk(const SceneView &s, const WalkSettings &w, const float *p
     long stride;
     int2 *spill = spill_buffer(s, (long)g * kBlock, &stride, st);
     if (s.dim == 2)
-        k_single_walk<2><<<g, kBlock, 0, st>>>(s, w, p, n, state, inc, valid, op, on, ot, oprim, ocontrib, spill,
+        k_single_walk<2><<<g, kBlock, top_bytes(s), st>>>(s, w, p, n, state, inc, valid, op, on, ot, oprim, ocontrib, spill,
                                                stride);
     else
-        k_single_walk<3><<<g, kBlock, 0, st>>>(s, w, p, n, state, inc, valid, op, on, ot, oprim, ocontrib, spill,
+        k_single_walk<3><<<g, kBlock, top_bytes(s), st>>>(s, w, p, n, state, inc, valid, op, on, ot, oprim, ocontrib, spill,
                                                stride);
     KCHK();
     return 0;
@@ -803,7 +830,8 @@ int launch_walk_reduce(const SceneView &s, const WalkSettings &w, const float *p
     a.seed = seed;
     a.total = total;
     a.counter = launch_counter(st);
-    a.chunk = chunk > 0 ? chunk : 256;
+    if (!a.counter) return -1;
+    a.chunk = chunk > 0 ? chunk : (g_tuning.chunk > 0 ? g_tuning.chunk : 256);
     a.walk_value = o.walk_value;
     a.rec_prim = o.rec_prim;
     a.rec_t = o.rec_t;
@@ -814,18 +842,33 @@ int launch_walk_reduce(const SceneView &s, const WalkSettings &w, const float *p
     int cap = occupancy_grid();
     if (g > cap) g = cap;
     if (g < 1) g = 1;
-    a.spill = spill_buffer(s, (long)g * kBlock, &a.spill_stride, st);
-    if (!a.counter) return -1;
+    int gi = grid_of(n_points, cap);
+    long lanes = (long)std::max(g, gi) * kBlock;
+    a.spill = spill_buffer(s, lanes, &a.spill_stride, st);
+    int *iprim = nullptr;
+    float *isd = nullptr;
+    if (hipMallocAsync((void **)&iprim, sizeof(int) * (size_t)n_points, st) != hipSuccess ||
+        hipMallocAsync((void **)&isd, sizeof(float) * (size_t)n_points, st) != hipSuccess) {
+        set_error("cannot allocate %d start records", n_points);
+        return -1;
+    }
+    a.init_prim = iprim;
+    a.init_sd = isd;
+    size_t tb = top_bytes(s);
+    if (s.dim == 2) k_point_init<2><<<gi, kBlock, tb, st>>>(s, points, n_points, iprim, isd, a.spill, a.spill_stride);
+    else k_point_init<3><<<gi, kBlock, tb, st>>>(s, points, n_points, iprim, isd, a.spill, a.spill_stride);
     k_zero_u64<<<1, 1, 0, st>>>(a.counter);
     bool src = s.dim == 2 && s.source_type != 0;
     bool bas = src && o.basis != nullptr && K > 0;
     if (s.dim == 2) {
-        if (bas) k_walk_persistent<2, true, true><<<g, kBlock, 0, st>>>(s, a);
-        else if (src) k_walk_persistent<2, true, false><<<g, kBlock, 0, st>>>(s, a);
-        else k_walk_persistent<2, false, false><<<g, kBlock, 0, st>>>(s, a);
+        if (bas) k_walk_persistent<2, true, true><<<g, kBlock, tb, st>>>(s, a);
+        else if (src) k_walk_persistent<2, true, false><<<g, kBlock, tb, st>>>(s, a);
+        else k_walk_persistent<2, false, false><<<g, kBlock, tb, st>>>(s, a);
     } else {
-        k_walk_persistent<3, false, false><<<g, kBlock, 0, st>>>(s, a);
+        k_walk_persistent<3, false, false><<<g, kBlock, tb, st>>>(s, a);
     }
+    (void)hipFreeAsync(iprim, st);
+    (void)hipFreeAsync(isd, st);
     KCHK();
     if (o.mean || o.mean64 || o.var64) {
         if (!o.walk_value) {
@@ -872,9 +915,9 @@ int launch_shape_gradient(const SceneView &s, const GradSettings &g, const float
     long stride;
     int2 *spill = spill_buffer(s, (long)gr * kBlock, &stride, st);
     if (s.dim == 2)
-        k_shape_gradient<2><<<gr, kBlock, 0, st>>>(s, g, points, n_points, seed, grad_out, grad_vertices, sample_grad_u, spill, stride);
+        k_shape_gradient<2><<<gr, kBlock, top_bytes(s), st>>>(s, g, points, n_points, seed, grad_out, grad_vertices, sample_grad_u, spill, stride);
     else
-        k_shape_gradient<3><<<gr, kBlock, 0, st>>>(s, g, points, n_points, seed, grad_out, grad_vertices, sample_grad_u, spill, stride);
+        k_shape_gradient<3><<<gr, kBlock, top_bytes(s), st>>>(s, g, points, n_points, seed, grad_out, grad_vertices, sample_grad_u, spill, stride);
     KCHK();
     return 0;
 }

@@ -90,8 +90,8 @@ class PCG32:
     """Dr.Jit PCG32 (the struct include/sampler2.cuh:11-56 restates):
     ``PCG32(size, initstate, initseq)`` seeds element k with sequence index k."""
 
-    def __init__(self, size=1, initstate=PCG32_DEFAULT_STATE, initseq=PCG32_DEFAULT_STREAM):
-        dv = device()
+    def __init__(self, size=1, initstate=PCG32_DEFAULT_STATE, initseq=PCG32_DEFAULT_STREAM, dev=None):
+        dv = device() if dev is None else torch.device(dev)
         idx = torch.arange(int(size), dtype=torch.int64, device=dv)
         st = initstate if isinstance(initstate, torch.Tensor) else torch.full((int(size),), _u64(initstate),
                                                                                dtype=torch.int64, device=dv)

@@ -84,6 +84,8 @@ _sig("wos_dirichlet", _vp, ctypes.POINTER(_Record), _i, _vp, _vp)
 _sig("wos_single_walk", _vp, ctypes.POINTER(_WalkParams), _vp, _i, _vp, _vp, ctypes.POINTER(_Record), _vp)
 _sig("wos_walk_reduce", _vp, ctypes.POINTER(_WalkParams), _vp, _i, _i, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp)
 _sig("wos_walk_reduce_counts", _vp, ctypes.POINTER(_WalkParams), _vp, _i, _i, _i, _vp, _vp, _vp)
+_sig("wos_set_tuning", _i, _i, _i)
+_sig("wos_get_tuning", _vp, _vp, _vp)
 _sig("wos_adjoint_values", _vp, _i, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp)
 _sig("wos_shape_gradient", _vp, ctypes.POINTER(_GradParams), _vp, _i, _i, _vp, _vp, _vp, _vp)
 
@@ -91,7 +93,21 @@ EXPORTED = ("wos_last_error", "wos_scene_create", "wos_scene_destroy", "wos_scen
             "wos_scene_set_source", "wos_scene_set_use_bvh", "wos_scene_get_normals", "wos_scene_get_info",
             "wos_scene_get_bvh", "wos_closest_point_index", "wos_sdf", "wos_closest_point",
             "wos_largest_inscribed_ball", "wos_dirichlet", "wos_single_walk", "wos_walk_reduce",
-            "wos_walk_reduce_counts", "wos_adjoint_values", "wos_shape_gradient")
+            "wos_walk_reduce_counts", "wos_adjoint_values", "wos_shape_gradient", "wos_set_tuning",
+            "wos_get_tuning")
+
+
+def set_tuning(chunk=None, wgs_per_cu=None, top_levels=None):
+    """Process-wide launch tuning (walks per queue claim, workgroups per CU,
+    BVH levels in LDS); unspecified values keep their current setting."""
+    c, w, t = get_tuning()
+    _check(lib.wos_set_tuning(int(chunk or c), int(wgs_per_cu or w), int(t if top_levels is None else top_levels)))
+
+
+def get_tuning():
+    c, w, t = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+    _check(lib.wos_get_tuning(ctypes.byref(c), ctypes.byref(w), ctypes.byref(t)))
+    return c.value, w.value, t.value
 
 
 def _check(rc):

@@ -72,6 +72,12 @@ const char *wos_last_error(void);
 int wos_scene_create(int dim, const float *vertices, int n_vertices, const int *indices, int n_prims,
                      const float *values, int use_bvh, int source_type, const float *source_params,
                      int n_source_params, void *stream, wos_scene **out);
+/* Launch tuning of every walk launch (no counterpart in the reference, whose
+ * thrust launches are untuned): walks a wave claims per queue visit (chunk),
+ * persistent workgroups per CU, and BVH levels kept in LDS.  Process-wide;
+ * chosen by the autotuner in the Python package. */
+int wos_set_tuning(int chunk, int wgs_per_cu, int top_levels);
+int wos_get_tuning(int *chunk, int *wgs_per_cu, int *top_levels);
 /* Scene destructor (the reference frees its thrust buffers with the object). */
 int wos_scene_destroy(wos_scene *scene);
 /* Scene.values / Scene.source_params setters (def_readwrite in

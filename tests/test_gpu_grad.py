@@ -1,0 +1,100 @@
+"""GPU parity of the gradients: the shape-gradient samples against the oracle,
+torch autograd through the drop-in solvers, and a directional finite
+difference of the shape derivative."""
+import numpy as np
+import pytest
+import torch
+
+import oracle as O
+from wos import configs as C
+
+pytestmark = pytest.mark.gpu
+
+
+def _t(a):
+    return torch.from_numpy(np.ascontiguousarray(a)).cuda()
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_shape_gradient_samples_match_oracle(dim):
+    import wos_ext
+    rng = np.random.default_rng(0)
+    if dim == 2:
+        v, ix = C.star_polyline(1024)
+        vals = C.g_saddle(v).astype(np.float32)
+        p = rng.uniform(-0.5, 0.5, size=(200, 2)).astype(np.float32)
+        sc = wos_ext.Scene(_t(v), _t(ix), _t(vals), use_bvh=True)
+    else:
+        v, ix, noise = C.displaced_sphere(2)
+        vals = C.g_harmonic3(v).astype(np.float32)
+        p = C.points_in_star(100, noise)
+        sc = wos_ext.Scene3D(_t(v), _t(ix), _t(vals), use_bvh=True)
+    W = 8
+    kw = dict(nwalks=W, nsteps=128, epsilon=1e-3, epsilon2=1e-3, clamping=0.1)
+    g = rng.normal(size=p.shape[0]).astype(np.float32)
+    gv, sg = wos_ext.shape_gradient(sc, _t(p), 3, _t(g), sample_grad_u=True, **kw)
+    ref = O.Scene(v, ix, vals)
+    gv_ref, sg_ref = ref.shape_gradient(p, W, 128, 1e-3, 1e-3, 0.1, seed=3, grad_out=g, sample_grad_u=True)
+    sg = sg.cpu().numpy()
+    close = np.all(np.abs(sg - sg_ref) <= 1e-3 * (1 + np.abs(sg_ref)), axis=1)
+    assert close.mean() > 0.97, close.mean()
+    gv = gv.cpu().numpy()
+    scale = np.abs(gv_ref).max()
+    assert np.abs(gv - gv_ref).max() < 0.1 * scale
+
+
+def test_autograd_values_and_source_coefficients():
+    from wos.scene import Polyline
+    from wos.wos_with_source import WoSWithSourceCUDA
+    import wos_ext
+    c = C.config2(res=40)
+    vals = _t(c["values"]).requires_grad_(True)
+    sp = _t(c["source_params"]).requires_grad_(True)
+    scene = Polyline(_t(c["vertices"]), _t(c["indices"]), vals, source_type=3, source_params=sp)
+    solver = WoSWithSourceCUDA(nwalks=16, nsteps=256, epsilon=1e-4)
+    pts = _t(c["points"])
+    u = solver.walk(pts, scene, seed=4)
+    loss = (u ** 2).mean()
+    loss.backward()
+    out = solver.cwos.walk_reduce(pts, scene, seed=4, records=True, basis=True)
+    np.testing.assert_array_equal(out["mean"].cpu().numpy(), u.detach().cpu().numpy())
+    g = 2 * u.detach() / u.shape[0]
+    gv, gc = wos_ext.adjoint_values(scene, pts.shape[0], 16, out["rec_prim"], out["rec_t"], g, basis=out["basis"],
+                                    want_values=True, want_coeffs=True)
+    np.testing.assert_allclose(vals.grad.cpu().numpy(), gv.cpu().numpy(), rtol=1e-5, atol=1e-8)
+    np.testing.assert_allclose(sp.grad[:, 3].cpu().numpy(), gc.cpu().numpy(), rtol=1e-5, atol=1e-8)
+    assert (sp.grad[:, :3] == 0).all()
+
+
+def test_custom_solver_shape_gradient_directional_fd():
+    """d L / d s for a uniform scaling s of the boundary: adjoint (OursCUDA)
+    against a central difference of the primal under common seeds."""
+    from wos.scene import Polyline
+    from wos.solver import CustomSolver
+    from wos.wos import WoSCUDA
+    from wos.wos_grad import OursCUDA
+    v, ix = C.star_polyline(1024)
+    vals = _t(C.g_saddle(v).astype(np.float32))
+    rng = np.random.default_rng(1)
+    r, t = 0.55 * np.sqrt(rng.random(2048)), 2 * np.pi * rng.random(2048)
+    pts = _t(np.stack([r * np.cos(t), r * np.sin(t)], 1).astype(np.float32))   # inside for all scalings used
+    target = torch.zeros(pts.shape[0], device="cuda")
+    base = _t(v)
+
+    def loss_at(s, solver, grad=False):
+        sv = torch.tensor(s, device="cuda", requires_grad=grad)
+        scene = Polyline(base * sv, _t(ix), vals)
+        u = solver.walk(pts, scene, seed=11)
+        L = ((u - target) ** 2).mean()
+        if grad:
+            L.backward()
+            return L.item(), sv.grad.item()
+        return L.item()
+
+    fwd = WoSCUDA(nwalks=256, nsteps=256, epsilon=1e-4)
+    bwd = OursCUDA(nwalks=256, nsteps=256, epsilon=1e-4, epsilon2=1e-4, clamping=0.1)
+    _, dL = loss_at(1.0, CustomSolver(fwd, bwd), grad=True)
+    h = 2e-2
+    fd = (loss_at(1 + h, fwd) - loss_at(1 - h, fwd)) / (2 * h)
+    assert np.isfinite(dL)
+    assert abs(dL - fd) < 0.1 * abs(fd) + 1e-3, (dL, fd)
