@@ -1,0 +1,448 @@
+// wos_batch.hip -- many scenes of uneven size in one persistent launch.
+//
+// The reference solves one scene per call (Solver.walk_reduce,
+// wos/solver.py:49-65).  A batch concatenates the walk index spaces of all
+// 2D items and of all 3D items; one persistent launch per dimension walks
+// them, each lane resolving its walk to (item, point, walk) by a binary
+// search over the item offsets kept in LDS.  Every item keeps its own scene,
+// walk settings, seed and outputs, and its walks draw exactly the streams of
+// a single-scene walk_reduce with the same seed, so batched and per-scene
+// results are identical.  All buffers are owned by the batch handle, so a
+// run performs no allocation and can be captured into a graph.
+#include <algorithm>
+#include <vector>
+
+#include "../../include/wos_amd.h"
+#include "wos_build.h"
+
+namespace wos {
+
+constexpr int kMaxBatchItems = 1024;
+constexpr int kMaxBatchBasis = 16;
+
+struct BatchItem {
+    SceneView v;
+    WalkSettings w;
+    const float *points;
+    int n_points;
+    int32_t seed;
+    int K;                      // Fourier terms written to basis (0: none)
+    long long walk_begin;       // offset in the dimension group's walk space
+    long long point_begin;      // offset in the dimension group's point space
+    float *walk_value, *rec_t, *basis, *mean;
+    int *rec_prim;
+    double *mean64, *var64;
+};
+
+__device__ __forceinline__ int find_item(const long long *begins, int n, long long x) {
+    int lo = 0, hi = n - 1;
+    while (lo < hi) {
+        int mid = (lo + hi + 1) >> 1;
+        if (begins[mid] <= x) lo = mid; else hi = mid - 1;
+    }
+    return lo;
+}
+
+template <int D>
+__global__ __launch_bounds__(kBlock) void k_batch_init(const BatchItem *items, int n_items, long long n_total_pts,
+                                                       int *iprim, float *isd, int2 *spill, long stride) {
+    __shared__ long long s_begin[kMaxBatchItems];
+    for (int i = threadIdx.x; i < n_items; i += kBlock) s_begin[i] = items[i].point_begin;
+    __syncthreads();
+    __shared__ int s_stack_node[kLdsStack * kBlock];
+    __shared__ float s_stack_d2[kLdsStack * kBlock];
+    TravStack st{nullptr, s_stack_node, s_stack_d2, spill, stride};
+    const long lane_g = (long)blockIdx.x * kBlock + threadIdx.x;
+    for (long long g = lane_g; g < n_total_pts; g += (long long)gridDim.x * kBlock) {
+        int it = find_item(s_begin, n_items, g);
+        const SceneView s = items[it].v;
+        long long i = g - s_begin[it];
+        float q[3];
+        load_point<D>(items[it].points, i, q);
+        float d2;
+        int id = closest_prim<D>(s, q, __builtin_inff(), d2, st, threadIdx.x, lane_g);
+        iprim[g] = id;
+        isd[g] = id < 0 ? __builtin_inff() : signed_dist<D>(s, id, q);
+    }
+}
+
+template <int D>
+__device__ __forceinline__ void batch_write(const BatchItem &b, long long k, bool valid, int prim, const float *p,
+                                            float contrib, const float *bas) {
+    float v = 0.0f, t0 = 0.0f, t1 = 0.0f;
+    int pr = -1;
+    if (valid) {
+        Rec r;
+        closest_record<D>(b.v, prim, p, r);
+        v = dirichlet_at<D>(b.v, prim, r.t, r.uv[0], r.uv[1]) + contrib;
+        pr = prim;
+        t0 = (D == 2) ? r.t : r.uv[0];
+        t1 = r.uv[1];
+    }
+    if (b.walk_value) b.walk_value[k] = v;
+    if (b.rec_prim) b.rec_prim[k] = pr;
+    if (b.rec_t) {
+        if (D == 2) b.rec_t[k] = t0;
+        else { b.rec_t[2 * k] = t0; b.rec_t[2 * k + 1] = t1; }
+    }
+    if (b.basis) {
+        long long tot = (long long)b.n_points * b.w.nwalks;
+#pragma unroll
+        for (int q = 0; q < kMaxBatchBasis; ++q)
+            if (q < b.K) b.basis[(long long)q * tot + k] = valid ? bas[q] : 0.0f;
+    }
+}
+
+template <int D>
+__global__ __launch_bounds__(kBlock) void k_batch_walk(const BatchItem *items, int n_items, long long total,
+                                                       unsigned long long *counter, int chunk, const int *iprim,
+                                                       const float *isd, int2 *spill, long stride) {
+    __shared__ long long s_begin[kMaxBatchItems];
+    for (int i = threadIdx.x; i < n_items; i += kBlock) s_begin[i] = items[i].walk_begin;
+    __syncthreads();
+    __shared__ int s_stack_node[kLdsStack * kBlock];
+    __shared__ float s_stack_d2[kLdsStack * kBlock];
+    TravStack st{nullptr, s_stack_node, s_stack_d2, spill, stride};
+    const long lane_g = (long)blockIdx.x * kBlock + threadIdx.x;
+    const int lane = __lane_id();
+
+    int item = 0;
+    long long k = -1;      // walk index within the item
+    bool active = false;
+    float p[3] = {0.0f, 0.0f, 0.0f};
+    float d = 0.0f, contrib = 0.0f;
+    int step = 0, prim = -1;
+    Pcg32 rng;
+    rng.state = rng.inc = 0;
+    float bas[kMaxBatchBasis];
+#pragma unroll
+    for (int q = 0; q < kMaxBatchBasis; ++q) bas[q] = 0.0f;
+    long long c_next = 0, c_end = 0;
+    bool exhausted = false;
+
+    for (;;) {
+        if (active) {
+            const BatchItem &b = items[item];
+            bool done_valid = d < b.w.epsilon && step < b.w.nsteps;
+            bool done_invalid = !done_valid && step >= b.w.nsteps;
+            if (done_valid || done_invalid) {
+                batch_write<D>(b, k, done_valid, prim, p, contrib, bas);
+                active = false;
+            }
+        }
+        uint64_t idle = __ballot(!active);
+        if (idle != 0 && !exhausted) {
+            int need = __popcll(idle);
+            long long avail = c_end - c_next;
+            long long base2 = 0, end2 = 0;
+            bool fresh = false;
+            if (avail < need) {
+                unsigned long long bq = 0;
+                if (lane == 0) bq = atomicAdd(counter, (unsigned long long)chunk);
+                uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)bq);
+                uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(bq >> 32));
+                long long bb = (long long)(((uint64_t)hi << 32) | lo);
+                if (bb >= total) {
+                    exhausted = true;
+                } else {
+                    fresh = true;
+                    base2 = bb;
+                    end2 = bb + chunk < total ? bb + chunk : total;
+                }
+            }
+            if (!active) {
+                int r = rank_in_mask(idle);
+                long long gk = -1;
+                if (r < avail) gk = c_next + r;
+                else if (fresh && (r - avail) < end2 - base2) gk = base2 + (r - avail);
+                if (gk >= 0) {
+                    item = find_item(s_begin, n_items, gk);
+                    const BatchItem &b = items[item];
+                    long long kk = gk - s_begin[item];
+                    long long i = kk / b.w.nwalks;
+                    long long gp = b.point_begin + i;
+                    int ip = iprim[gp];
+                    float sd = isd[gp];
+                    load_point<D>(b.points, i, p);
+                    contrib = 0.0f;
+#pragma unroll
+                    for (int q = 0; q < kMaxBatchBasis; ++q) bas[q] = 0.0f;
+                    bool outside = ip < 0 || (sd >= 0.0f && !b.w.double_sided);
+                    float d0 = fabsf(sd);
+                    if (b.w.prevent_fd_artifacts) d0 *= 0.5f;
+                    if (outside || b.w.nsteps <= 0) {
+                        batch_write<D>(b, kk, false, ip, p, 0.0f, bas);
+                    } else if (d0 < b.w.epsilon) {
+                        batch_write<D>(b, kk, true, ip, p, 0.0f, bas);
+                    } else {
+                        k = kk;
+                        rng = walk_stream(b.seed, i, kk);
+                        prim = ip;
+                        d = d0;
+                        step = 0;
+                        active = true;
+                    }
+                }
+            }
+            if (avail >= need) {
+                c_next += need;
+            } else if (fresh) {
+                long long used2 = need - avail;
+                c_next = base2 + used2 < end2 ? base2 + used2 : end2;
+                c_end = end2;
+            } else {
+                c_next = c_end;
+            }
+        }
+        if (exhausted && __ballot(active) == 0) break;
+        if (active) {
+            const SceneView s = items[item].v;
+            if (D == 2 && s.source_type != 0) {
+                const int K = items[item].K;
+                float u0 = rng.next_1d(), u1 = rng.next_1d();
+                float sn, cs;
+                sincos_turns(u0, sn, cs);
+                float r = sqrtf(u1);
+                float dyx = (r * cs) * d, dyy = (r * sn) * d;
+                float area = kPi * (d * d);
+                float pdf = 1.0f / area;
+                float rr = fmaxf(sqrtf(dot2(dyx, dyy, dyx, dyy)), 1e-4f);
+                float G = log_poly(d / rr) * kInv2Pi;
+                if (pdf > 1e-5f) {
+                    float yx = p[0] + dyx, yy = p[1] + dyy;
+                    if (K > 0) {
+                        float f = 0.0f;
+#pragma unroll
+                        for (int q = 0; q < kMaxBatchBasis; ++q) {
+                            if (q < K) {
+                                const float *c = s.src + 4 * q;
+                                float bq = sin_cw(fmaf(c[0], yx, fmaf(c[1], yy, c[2])));
+                                f = fmaf(c[3], bq, f);
+                                bas[q] = fmaf(bq * G, area, bas[q]);
+                            }
+                        }
+                        contrib = fmaf(f * G, area, contrib);
+                    } else {
+                        contrib = fmaf(source_at(s, yx, yy) * G, area, contrib);
+                    }
+                }
+            }
+            move_point<D>(p, d, rng);
+            ++step;
+            float d2;
+            prim = closest_prim<D>(s, p, bound_d2(s, 2.0f * d), d2, st, threadIdx.x, lane_g);
+            d = sqrtf(d2);
+        }
+    }
+}
+
+// One wave per point of the group: FP64 mean / variance into each item's outputs.
+__global__ __launch_bounds__(kBlock) void k_batch_reduce(const BatchItem *items, int n_items, long long n_pts) {
+    __shared__ long long s_begin[kMaxBatchItems];
+    for (int i = threadIdx.x; i < n_items; i += kBlock) s_begin[i] = items[i].point_begin;
+    __syncthreads();
+    const int lane = __lane_id();
+    long long wave = ((long long)blockIdx.x * kBlock + threadIdx.x) >> 6;
+    long long nwaves = ((long long)gridDim.x * kBlock) >> 6;
+    for (long long g = wave; g < n_pts; g += nwaves) {
+        int it = find_item(s_begin, n_items, g);
+        const BatchItem &b = items[it];
+        long long i = g - s_begin[it];
+        const int W = b.w.nwalks;
+        double sm = 0.0, s2 = 0.0;
+        for (int w = lane; w < W; w += 64) {
+            double v = b.walk_value[i * W + w];
+            sm += v;
+            s2 += v * v;
+        }
+        for (int off = 32; off > 0; off >>= 1) {
+            sm += __shfl_xor(sm, off);
+            s2 += __shfl_xor(s2, off);
+        }
+        if (lane == 0) {
+            double m = sm / W;
+            if (b.mean) b.mean[i] = (float)m;
+            if (b.mean64) b.mean64[i] = m;
+            if (b.var64) b.var64[i] = W > 1 ? (s2 - sm * m) / (W - 1) : 0.0;
+        }
+    }
+}
+
+__global__ void k_batch_zero(unsigned long long *c, int n) {
+    if (threadIdx.x < n) c[threadIdx.x] = 0ull;
+}
+
+struct BatchGroup {
+    BatchItem *items = nullptr;   // device copy
+    int n_items = 0;
+    long long walks = 0, points = 0;
+    int *iprim = nullptr;
+    float *isd = nullptr;
+    int max_depth = 0;
+};
+
+}  // namespace wos
+
+using namespace wos;
+
+struct wos_batch {
+    BatchGroup g[2];          // [0] 2D items, [1] 3D items
+    unsigned long long *counters = nullptr;
+    int2 *spill = nullptr;    // traversal spill owned by the batch
+    long spill_lanes = 0;
+    int device = 0;
+};
+
+// defined in wos_capi.hip
+namespace wos {
+SceneView scene_view(const wos_scene *s);
+}
+
+#define BREQ(cond, ...)              \
+    do {                             \
+        if (!(cond)) {               \
+            set_error(__VA_ARGS__);  \
+            return -1;               \
+        }                            \
+    } while (0)
+
+static void free_batch(wos_batch *b) {
+    if (!b) return;
+    for (auto &g : b->g) {
+        (void)hipFree(g.items);
+        (void)hipFree(g.iprim);
+        (void)hipFree(g.isd);
+    }
+    (void)hipFree(b->counters);
+    (void)hipFree(b->spill);
+    delete b;
+}
+
+static int occupancy_grid_b() {
+    static int cus = 0;
+    if (!cus) {
+        int dev = 0;
+        (void)hipGetDevice(&dev);
+        (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+        if (cus <= 0) cus = 256;
+    }
+    return cus * (g_tuning.wgs_per_cu > 0 ? g_tuning.wgs_per_cu : 8);
+}
+
+extern "C" int wos_batch_create(const wos_batch_item *items, int n_items, wos_batch **out) {
+    BREQ(out && items && n_items > 0, "items and out are required");
+    *out = nullptr;
+    std::vector<BatchItem> grp[2];
+    for (int j = 0; j < n_items; ++j) {
+        const wos_batch_item &it = items[j];
+        BREQ(it.scene, "item %d has no scene", j);
+        BREQ(it.params.nwalks >= 1 && it.params.nsteps >= 0, "item %d: bad walk settings", j);
+        BREQ(it.n_points >= 0 && (it.n_points == 0 || it.points), "item %d: bad points", j);
+        BREQ(it.walk_value, "item %d: walk_value is required", j);
+        SceneView v = scene_view(it.scene);
+        v.n_top = 0;
+        BatchItem b;
+        b.v = v;
+        b.w = WalkSettings{it.params.nwalks, it.params.nsteps, it.params.epsilon, it.params.double_sided,
+                           it.params.prevent_fd_artifacts};
+        b.points = it.points;
+        b.n_points = it.n_points;
+        b.seed = it.seed;
+        b.K = 0;
+        if (it.basis) {
+            BREQ(v.dim == 2 && v.source_type == 3, "item %d: basis needs a 2D Fourier source", j);
+            b.K = v.n_src / 4;
+            BREQ(b.K <= kMaxBatchBasis, "item %d: at most %d Fourier terms", j, kMaxBatchBasis);
+        }
+        b.walk_value = it.walk_value;
+        b.rec_prim = it.rec_prim;
+        b.rec_t = it.rec_t;
+        b.basis = it.basis;
+        b.mean = it.mean;
+        b.mean64 = it.mean64;
+        b.var64 = it.var64;
+        grp[v.dim == 2 ? 0 : 1].push_back(b);
+    }
+    wos_batch *B = new wos_batch();
+    (void)hipGetDevice(&B->device);
+    for (int d = 0; d < 2; ++d) {
+        auto &v = grp[d];
+        BREQ((int)v.size() <= kMaxBatchItems, "at most %d items per dimension", kMaxBatchItems);
+        long long wb = 0, pb = 0;
+        int depth = 0;
+        for (auto &b : v) {
+            b.walk_begin = wb;
+            b.point_begin = pb;
+            wb += (long long)b.n_points * b.w.nwalks;
+            pb += b.n_points;
+            depth = std::max(depth, b.v.use_bvh ? b.v.max_depth : 0);
+        }
+        BatchGroup &g = B->g[d];
+        g.n_items = (int)v.size();
+        g.walks = wb;
+        g.points = pb;
+        g.max_depth = depth;
+        if (!g.n_items) continue;
+        if (hipMalloc((void **)&g.items, sizeof(BatchItem) * v.size()) != hipSuccess ||
+            hipMalloc((void **)&g.iprim, sizeof(int) * (size_t)std::max(pb, 1LL)) != hipSuccess ||
+            hipMalloc((void **)&g.isd, sizeof(float) * (size_t)std::max(pb, 1LL)) != hipSuccess ||
+            hipMemcpy(g.items, v.data(), sizeof(BatchItem) * v.size(), hipMemcpyHostToDevice) != hipSuccess) {
+            free_batch(B);
+            set_error("cannot allocate the batch tables");
+            return -1;
+        }
+    }
+    if (hipMalloc((void **)&B->counters, 2 * sizeof(unsigned long long)) != hipSuccess) {
+        free_batch(B);
+        set_error("cannot allocate the batch counters");
+        return -1;
+    }
+    int depth = std::max(B->g[0].max_depth, B->g[1].max_depth);
+    long deep = (long)depth + 2 - kLdsStack;
+    B->spill_lanes = (long)occupancy_grid_b() * kBlock;
+    if (deep > 0 && hipMalloc((void **)&B->spill, sizeof(int2) * (size_t)deep * (size_t)B->spill_lanes) != hipSuccess) {
+        free_batch(B);
+        set_error("cannot allocate the batch traversal spill");
+        return -1;
+    }
+    *out = B;
+    return 0;
+}
+
+extern "C" int wos_batch_destroy(wos_batch *b) {
+    (void)hipDeviceSynchronize();
+    free_batch(b);
+    return 0;
+}
+
+extern "C" int wos_batch_run(wos_batch *B, void *stream) {
+    BREQ(B, "batch is NULL");
+    hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+    const int cap = (int)(B->spill_lanes / kBlock);
+    k_batch_zero<<<1, 64, 0, st>>>(B->counters, 2);
+    for (int d = 0; d < 2; ++d) {
+        BatchGroup &g = B->g[d];
+        if (!g.n_items || !g.points) continue;
+        long stride = B->spill_lanes;
+        int2 *spill = B->spill;
+        long gpts = (g.points + kBlock - 1) / kBlock;
+        int gi = (int)std::min<long>(std::max<long>(gpts, 1), cap);
+        long gw = (g.walks / 64 + 4) / 4;
+        int gwk = (int)std::min<long>(std::max<long>(gw, 1), cap);
+        int chunk = g_tuning.chunk > 0 ? g_tuning.chunk : 256;
+        if (d == 0) {
+            k_batch_init<2><<<gi, kBlock, 0, st>>>(g.items, g.n_items, g.points, g.iprim, g.isd, spill, stride);
+            k_batch_walk<2><<<gwk, kBlock, 0, st>>>(g.items, g.n_items, g.walks, B->counters + d, chunk, g.iprim,
+                                                    g.isd, spill, stride);
+        } else {
+            k_batch_init<3><<<gi, kBlock, 0, st>>>(g.items, g.n_items, g.points, g.iprim, g.isd, spill, stride);
+            k_batch_walk<3><<<gwk, kBlock, 0, st>>>(g.items, g.n_items, g.walks, B->counters + d, chunk, g.iprim,
+                                                    g.isd, spill, stride);
+        }
+        long gr = std::min<long>((g.points * 64 + kBlock - 1) / kBlock, 8192);
+        k_batch_reduce<<<(int)std::max<long>(gr, 1), kBlock, 0, st>>>(g.items, g.n_items, g.points);
+    }
+    hipError_t e = hipGetLastError();
+    BREQ(e == hipSuccess, "batch launch: %s", hipGetErrorString(e));
+    return 0;
+}

@@ -300,7 +300,8 @@ __device__ __forceinline__ bool better(float d2, int id, float best_d2, int best
 // Prune threshold for boxes: (best + tol)^2, which absorbs the rounding gap
 // between a box distance and the distance of a primitive on its face.
 __device__ __forceinline__ float prune_threshold(float best_d2, float tol) {
-    float d = sqrtf(best_d2) + tol;
+    // hardware square root (about 1 ulp) inflated by 2^-20: still an upper bound
+    float d = fmaf(__builtin_amdgcn_sqrtf(best_d2), 1.00000095367431640625f, tol);
     return d * d;
 }
 
@@ -459,6 +460,129 @@ __device__ int closest_bvh(const SceneView &s, const float *p, float bound_d2, f
     return best_id == 0x7fffffff ? -1 : best_id;
 }
 
+// Resumable traversal: the walk kernel advances every lane's query by a few
+// node visits per round, so lanes with short and long queries stay busy.
+struct Query {
+    int node, sp;
+    float best, thr;   // best squared distance so far, box pruning threshold
+    int best_id;
+};
+
+// Starts a query whose answer is known to lie within bound_d2, seeded with a
+// candidate primitive (seed_id, seed_d2) -- e.g. the previous step's closest
+// primitive -- which tightens pruning from the root.
+__device__ __forceinline__ void query_begin(Query &q, float bound_d2, int seed_id, float seed_d2, float tol) {
+    q.node = 0;
+    q.sp = 0;
+    q.best = seed_d2;
+    q.best_id = seed_id;
+    q.thr = fminf(bound_d2, prune_threshold(seed_d2, tol));
+}
+
+// One node visit; returns true when the query is complete.  Same pruning and
+// (d2, index) minimum as closest_bvh.
+template <int D>
+__device__ __forceinline__ bool query_step(const SceneView &s, const float *p, Query &q, const TravStack &st,
+                                           int lane_slot, long spill_lane, float tol) {
+    const int node = q.node;
+    float dL, dR;
+    int cL, cR;
+    if constexpr (D == 2) {
+        const float4 *nb = (node < s.n_top) ? st.top + 3 * node : s.nodes + 3 * node;
+        float4 b0 = nb[0], b1 = nb[1], c = nb[2];
+        dL = box_d2_2(p[0], p[1], b0);
+        dR = box_d2_2(p[0], p[1], b1);
+        cL = __float_as_int(c.x);
+        cR = __float_as_int(c.y);
+    } else {
+        const float4 *nb = (node < s.n_top) ? st.top + 4 * node : s.nodes + 4 * node;
+        float4 n0 = nb[0], n1 = nb[1], n2 = nb[2], c = nb[3];
+        dL = box_d2_3(p, n0.x, n0.y, n0.z, n0.w, n1.x, n1.y);
+        dR = box_d2_3(p, n1.z, n1.w, n2.x, n2.y, n2.z, n2.w);
+        cL = __float_as_int(c.x);
+        cR = __float_as_int(c.y);
+    }
+    if (s.stats) atomicAdd(&s.stats[0], 2ull);
+    bool goL = false, goR = false;
+    if (dL <= q.thr) {
+        if (cL < 0) {
+            int prim;
+            float d2 = leaf_d2<D>(s, ~cL, p, prim);
+            if (s.stats) atomicAdd(&s.stats[1], 1ull);
+            if (better(d2, prim, q.best, q.best_id)) {
+                q.best = d2;
+                q.best_id = prim;
+                q.thr = fminf(q.thr, prune_threshold(d2, tol));
+            }
+        } else goL = true;
+    }
+    if (dR <= q.thr) {
+        if (cR < 0) {
+            int prim;
+            float d2 = leaf_d2<D>(s, ~cR, p, prim);
+            if (s.stats) atomicAdd(&s.stats[1], 1ull);
+            if (better(d2, prim, q.best, q.best_id)) {
+                q.best = d2;
+                q.best_id = prim;
+                q.thr = fminf(q.thr, prune_threshold(d2, tol));
+            }
+        } else goR = true;
+    }
+    goL = goL && dL <= q.thr;
+    if (goL && goR) {
+        bool leftNear = dL <= dR;
+        int nearN = leftNear ? cL : cR, farN = leftNear ? cR : cL;
+        float farD = leftNear ? dR : dL;
+        if (q.sp < kLdsStack) {
+            st.lds_node[q.sp * kBlock + lane_slot] = farN;
+            st.lds_d2[q.sp * kBlock + lane_slot] = farD;
+        } else {
+            st.spill[(long)(q.sp - kLdsStack) * st.spill_stride + spill_lane] = make_int2(farN, __float_as_int(farD));
+        }
+        ++q.sp;
+        q.node = nearN;
+        return false;
+    }
+    if (goL) { q.node = cL; return false; }
+    if (goR) { q.node = cR; return false; }
+    while (q.sp > 0) {
+        --q.sp;
+        int n;
+        float dd;
+        if (q.sp < kLdsStack) {
+            n = st.lds_node[q.sp * kBlock + lane_slot];
+            dd = st.lds_d2[q.sp * kBlock + lane_slot];
+        } else {
+            int2 e = st.spill[(long)(q.sp - kLdsStack) * st.spill_stride + spill_lane];
+            n = e.x;
+            dd = __int_as_float(e.y);
+        }
+        if (dd <= q.thr) { q.node = n; return false; }
+    }
+    return true;
+}
+
+// Squared distance from p to primitive f computed from the scene arrays
+// (identical to the leaf test of the same primitive).
+template <int D>
+__device__ __forceinline__ float prim_d2(const SceneView &s, int f, const float *p) {
+    if constexpr (D == 2) {
+        int i0 = s.idx[2 * f], i1 = s.idx[2 * f + 1];
+        float ax = s.vtx[2 * i0], ay = s.vtx[2 * i0 + 1];
+        float bax = s.vtx[2 * i1] - ax, bay = s.vtx[2 * i1 + 1] - ay;
+        float inv = 1.0f / dot2(bax, bay, bax, bay);
+        float h, dx, dy;
+        return seg_d2(p[0], p[1], ax, ay, bax, bay, inv, h, dx, dy);
+    } else {
+        int i0 = s.idx[3 * f], i1 = s.idx[3 * f + 1], i2 = s.idx[3 * f + 2];
+        float a[3] = {s.vtx[3 * i0], s.vtx[3 * i0 + 1], s.vtx[3 * i0 + 2]};
+        float b[3] = {s.vtx[3 * i1], s.vtx[3 * i1 + 1], s.vtx[3 * i1 + 2]};
+        float c[3] = {s.vtx[3 * i2], s.vtx[3 * i2 + 1], s.vtx[3 * i2 + 2]};
+        float q[3], bary[2];
+        return tri_closest(a, b, c, p, q, bary);
+    }
+}
+
 // Closest primitive with an optional distance bound; falls back to an
 // unbounded search when nothing lies inside the bound.
 template <int D>
@@ -595,6 +719,57 @@ __device__ __forceinline__ float source_at(const SceneView &s, float x, float y)
         return f;
     }
     return 0.0f;
+}
+
+// ---------------------------------------------------------------------------
+// walk steps (include/wos.cuh:14-42, 115-142)
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ float bound_d2(const SceneView &s, float r) {
+    float b = fmaf(r, 1.00001f, 2.0f * s.tol);
+    return b * b;
+}
+
+template <int D>
+__device__ __forceinline__ void load_point(const float *p, long i, float *q) {
+    q[0] = p[D * i];
+    q[1] = p[D * i + 1];
+    q[2] = (D == 3) ? p[D * i + 2] : 0.0f;
+}
+
+template <int D>
+__device__ __forceinline__ void source_step(const SceneView &s, const float *p, float d, Pcg32 &rng,
+                                            float &contrib) {
+    // include/wos.cuh:115-139: uniform point in the ball, G_2D / pdf
+    float u0 = rng.next_1d(), u1 = rng.next_1d();
+    float sn, cs;
+    sincos_turns(u0, sn, cs);
+    float r = sqrtf(u1);
+    float dyx = (r * cs) * d, dyy = (r * sn) * d;
+    float area = kPi * (d * d);
+    float pdf = 1.0f / area;
+    float rr = fmaxf(sqrtf(dot2(dyx, dyy, dyx, dyy)), 1e-4f);
+    float G = log_poly(d / rr) * kInv2Pi;
+    if (pdf > 1e-5f) contrib = fmaf(source_at(s, p[0] + dyx, p[1] + dyy) * G, area, contrib);
+}
+
+template <int D>
+__device__ __forceinline__ void move_point(float *p, float d, Pcg32 &rng) {
+    if constexpr (D == 2) {
+        float u = rng.next_1d();
+        float sn, cs;
+        sincos_turns(u, sn, cs);
+        p[0] = fmaf(cs, d, p[0]);
+        p[1] = fmaf(sn, d, p[1]);
+    } else {
+        float u0 = rng.next_1d(), u1 = rng.next_1d();
+        float z = 1.0f - 2.0f * u0;
+        float r = sqrtf(fmaxf(0.0f, 1.0f - z * z));
+        float sn, cs;
+        sincos_turns(u1, sn, cs);
+        p[0] = fmaf(r * cs, d, p[0]);
+        p[1] = fmaf(r * sn, d, p[1]);
+        p[2] = fmaf(z, d, p[2]);
+    }
 }
 
 // wave helpers ---------------------------------------------------------------

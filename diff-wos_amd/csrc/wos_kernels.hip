@@ -53,18 +53,6 @@ static int grid_of(long n, int cap = kGridCap) {
     const int slot = threadIdx.x;                                               \
     const long lane_g = (long)blockIdx.x * kBlock + threadIdx.x;
 
-__device__ __forceinline__ float bound_d2(const SceneView &s, float r) {
-    float b = fmaf(r, 1.00001f, 2.0f * s.tol);
-    return b * b;
-}
-
-template <int D>
-__device__ __forceinline__ void load_point(const float *p, long i, float *q) {
-    q[0] = p[D * i];
-    q[1] = p[D * i + 1];
-    q[2] = (D == 3) ? p[D * i + 2] : 0.0f;
-}
-
 // ---------------------------------------------------------------------------
 // point queries
 // ---------------------------------------------------------------------------
@@ -175,42 +163,6 @@ __global__ void k_dirichlet(SceneView s, const uint8_t *valid, const int *prim, 
 // ---------------------------------------------------------------------------
 // one walk on one lane (include/wos.cuh:97-145)
 // ---------------------------------------------------------------------------
-template <int D>
-__device__ __forceinline__ void source_step(const SceneView &s, const float *p, float d, Pcg32 &rng,
-                                            float &contrib) {
-    // include/wos.cuh:115-139: uniform point in the ball, G_2D / pdf
-    float u0 = rng.next_1d(), u1 = rng.next_1d();
-    float sn, cs;
-    sincos_turns(u0, sn, cs);
-    float r = sqrtf(u1);
-    float dyx = (r * cs) * d, dyy = (r * sn) * d;
-    float area = kPi * (d * d);
-    float pdf = 1.0f / area;
-    float rr = fmaxf(sqrtf(dot2(dyx, dyy, dyx, dyy)), 1e-4f);
-    float G = log_poly(d / rr) * kInv2Pi;
-    if (pdf > 1e-5f) contrib = fmaf(source_at(s, p[0] + dyx, p[1] + dyy) * G, area, contrib);
-}
-
-template <int D>
-__device__ __forceinline__ void move_point(float *p, float d, Pcg32 &rng) {
-    if constexpr (D == 2) {
-        float u = rng.next_1d();
-        float sn, cs;
-        sincos_turns(u, sn, cs);
-        p[0] = fmaf(cs, d, p[0]);
-        p[1] = fmaf(sn, d, p[1]);
-    } else {
-        float u0 = rng.next_1d(), u1 = rng.next_1d();
-        float z = 1.0f - 2.0f * u0;
-        float r = sqrtf(fmaxf(0.0f, 1.0f - z * z));
-        float sn, cs;
-        sincos_turns(u1, sn, cs);
-        p[0] = fmaf(r * cs, d, p[0]);
-        p[1] = fmaf(r * sn, d, p[1]);
-        p[2] = fmaf(z, d, p[2]);
-    }
-}
-
 // init_prim >= 0: the closest primitive of p0 is already known (init_d2).
 template <int D>
 __device__ void walk_one(const SceneView &s, const WalkSettings &w, const float *p0, Pcg32 &rng, int init_prim,
@@ -274,6 +226,7 @@ struct PersistArgs {
     long long total;                 // n_points * nwalks
     unsigned long long *counter;     // zero before launch
     int chunk;                       // walks claimed per queue visit
+    int window;                      // node visits per lane between refills
     const int *init_prim;            // per point: closest primitive of the start point
     const float *init_sd;            // per point: signed distance of the start point
     float *walk_value;
@@ -301,7 +254,7 @@ __global__ __launch_bounds__(kBlock) void k_point_init(SceneView s, const float 
     }
 }
 
-enum : int { kIdle = 0, kStep = 2 };
+enum : int { kIdle = 0, kStart = 1, kQuery = 2, kReady = 3 };
 
 template <int D, bool BASIS>
 __device__ __forceinline__ void write_walk(const SceneView &s, const PersistArgs &a, long long k, bool valid,
@@ -329,11 +282,15 @@ __device__ __forceinline__ void write_walk(const SceneView &s, const PersistArgs
     }
 }
 
-// Each wave keeps 64 walks in flight.  Lanes whose walk ends take the next
-// walk index of the wave's chunk (ballot + mbcnt), and the wave claims a new
-// chunk from the global counter when its chunk runs dry, so divergent walk
-// lengths do not idle the SIMD.  Every walk re-derives its random stream from
-// (seed, point, walk index), so results do not depend on the schedule.
+// Each wave keeps 64 walks in flight.  A lane's life cycle: kStart (sample
+// the source term, jump to the sphere, begin the closest-primitive query) ->
+// kQuery (the query advances a few node visits per round) -> kReady (the
+// closest primitive of the new point is known: stop on the boundary or take
+// the next step).  Lanes whose walk ends take the next walk index of the
+// wave's chunk (ballot + mbcnt) and the wave claims a new chunk from the
+// global counter when its chunk runs dry, so neither walk lengths nor query
+// lengths idle the SIMD for long.  Every walk re-derives its random stream
+// from (seed, point, walk index), so results do not depend on the schedule.
 template <int D, bool SRC, bool BASIS>
 __global__ __launch_bounds__(kBlock) void k_walk_persistent(SceneView s, PersistArgs a) {
     DECLARE_STACK(a.spill, a.spill_stride);
@@ -345,23 +302,31 @@ __global__ __launch_bounds__(kBlock) void k_walk_persistent(SceneView s, Persist
     int step = 0, prim = -1;
     Pcg32 rng;
     rng.state = rng.inc = 0;
+    Query q;
+    q.node = q.sp = 0;
+    q.best = q.thr = 0.0f;
+    q.best_id = -1;
     float bas[BASIS ? kMaxBasis : 1];
 #pragma unroll
-    for (int q = 0; q < (BASIS ? kMaxBasis : 1); ++q) bas[q] = 0.0f;
+    for (int i = 0; i < (BASIS ? kMaxBasis : 1); ++i) bas[i] = 0.0f;
     long long c_next = 0, c_end = 0;
     bool exhausted = false;
     const int W = a.w.nwalks;
 
     for (;;) {
-        // 1. walks that reached the boundary or ran out of steps
-        //    (include/wos.cuh:107-113, 144)
-        if (mode == kStep) {
+        // 1. completed queries: stop on the boundary (include/wos.cuh:108-113),
+        //    run out of steps (144), or take the next step
+        if (mode == kReady) {
+            prim = q.best_id;
+            d = sqrtf(q.best);
             bool done_valid = d < a.w.epsilon && step < a.w.nsteps;
             bool done_invalid = !done_valid && step >= a.w.nsteps;
             if (done_valid || done_invalid) {
                 write_walk<D, BASIS>(s, a, k, done_valid, prim, p, contrib, bas);
                 mode = kIdle;
                 k = -1;
+            } else {
+                mode = kStart;
             }
         }
         // 2. refill idle lanes from the wave's chunk
@@ -397,7 +362,7 @@ __global__ __launch_bounds__(kBlock) void k_walk_persistent(SceneView s, Persist
                     load_point<D>(a.points, i, p);
                     contrib = 0.0f;
 #pragma unroll
-                    for (int q = 0; q < (BASIS ? kMaxBasis : 1); ++q) bas[q] = 0.0f;
+                    for (int j = 0; j < (BASIS ? kMaxBasis : 1); ++j) bas[j] = 0.0f;
                     // include/wos.cuh:100-106, then the first loop test
                     bool outside = ip < 0 || (sd >= 0.0f && !a.w.double_sided);
                     float d0 = fabsf(sd);
@@ -412,7 +377,7 @@ __global__ __launch_bounds__(kBlock) void k_walk_persistent(SceneView s, Persist
                         prim = ip;
                         d = d0;
                         step = 0;
-                        mode = kStep;
+                        mode = kStart;
                     }
                 }
             }
@@ -428,9 +393,9 @@ __global__ __launch_bounds__(kBlock) void k_walk_persistent(SceneView s, Persist
         }
         if (exhausted && __ballot(mode != kIdle) == 0) break;
 
-        // 3. stepping lanes sample the source term, jump to the sphere and
-        //    query the closest primitive within 2d of the new point
-        if (mode == kStep) {
+        // 3. a new step: source sample, jump to the sphere, begin the query
+        //    within 2d of the new point, seeded with the previous primitive
+        if (mode == kStart) {
             if (SRC) {
                 if constexpr (BASIS) {
                     float u0 = rng.next_1d(), u1 = rng.next_1d();
@@ -446,12 +411,12 @@ __global__ __launch_bounds__(kBlock) void k_walk_persistent(SceneView s, Persist
                         float yx = p[0] + dyx, yy = p[1] + dyy;
                         float f = 0.0f;
 #pragma unroll
-                        for (int q = 0; q < kMaxBasis; ++q) {
-                            if (q < a.K) {
-                                const float *c = s.src + 4 * q;
+                        for (int j = 0; j < kMaxBasis; ++j) {
+                            if (j < a.K) {
+                                const float *c = s.src + 4 * j;
                                 float b = sin_cw(fmaf(c[0], yx, fmaf(c[1], yy, c[2])));
                                 f = fmaf(c[3], b, f);
-                                bas[q] = fmaf(b * G, area, bas[q]);
+                                bas[j] = fmaf(b * G, area, bas[j]);
                             }
                         }
                         contrib = fmaf(f * G, area, contrib);
@@ -462,10 +427,35 @@ __global__ __launch_bounds__(kBlock) void k_walk_persistent(SceneView s, Persist
             }
             move_point<D>(p, d, rng);
             ++step;
-            if (s.stats) atomicAdd(&s.stats[3], 1ull);
-            float d2;
-            prim = closest_prim<D>(s, p, bound_d2(s, 2.0f * d), d2, st, slot, lane_g);
-            d = sqrtf(d2);
+            if (s.stats) {
+                atomicAdd(&s.stats[3], 1ull);
+                if (s.use_bvh) {
+                    atomicAdd(&s.stats[2], 1ull);
+                    atomicAdd(&s.stats[1], 1ull);   // the seed primitive
+                }
+            }
+            float seed_d2 = prim_d2<D>(s, prim, p);
+            if (!s.use_bvh) {
+                float d2;
+                int id = closest_brute<D>(s, p, d2);
+                q.best = d2;
+                q.best_id = id;
+                mode = kReady;
+            } else if (s.n_prims == 1) {
+                q.best = seed_d2;
+                q.best_id = prim;
+                mode = kReady;
+            } else {
+                query_begin(q, bound_d2(s, 2.0f * d), prim, seed_d2, s.tol);
+                mode = kQuery;
+            }
+        }
+        // 4. advance every pending query by up to `window` node visits
+        for (int it = 0; it < a.window; ++it) {
+            if (mode == kQuery) {
+                if (query_step<D>(s, p, q, st, slot, lane_g, s.tol)) mode = kReady;
+            }
+            if (__ballot(mode == kQuery) == 0) break;
         }
     }
 }
@@ -832,6 +822,7 @@ int launch_walk_reduce(const SceneView &s, const WalkSettings &w, const float *p
     a.counter = launch_counter(st);
     if (!a.counter) return -1;
     a.chunk = chunk > 0 ? chunk : (g_tuning.chunk > 0 ? g_tuning.chunk : 256);
+    a.window = g_tuning.window > 0 ? g_tuning.window : 8;
     a.walk_value = o.walk_value;
     a.rec_prim = o.rec_prim;
     a.rec_t = o.rec_t;

@@ -235,3 +235,54 @@ def decimated_star_mesh(n_tris=200_000, subdiv=7, amp=0.15, seed=0):
     noise = SmoothNoise(seed)
     r = 1.0 + amp * noise(U)
     return (U * r[:, None]).astype(np.float32), U.astype(np.float32), F.astype(np.int32), noise
+
+
+def random_star_mesh(n_tris, amp=0.15, seed=0):
+    """Closed star-shaped triangle mesh with n_tris = 2N - 4 triangles: convex
+    hull of N seeded random directions, displaced radially by 1 + amp n."""
+    from scipy.spatial import ConvexHull
+    rng = np.random.default_rng(seed)
+    N = max(n_tris // 2 + 2, 4)
+    U = rng.normal(size=(N, 3))
+    U /= np.linalg.norm(U, axis=1, keepdims=True)
+    F = ConvexHull(U).simplices.astype(np.int64)
+    c = U[F].mean(axis=1)
+    nrm = np.cross(U[F[:, 1]] - U[F[:, 0]], U[F[:, 2]] - U[F[:, 0]])
+    flip = (nrm * c).sum(axis=1) < 0
+    F[flip] = F[flip][:, [0, 2, 1]]
+    noise = SmoothNoise(seed)
+    return (U * (1.0 + amp * noise(U))[:, None]).astype(np.float32), F.astype(np.int32), noise
+
+
+def config5(scale=1.0, seed=0):
+    """64 uneven scenes: 32 clover/star polylines with 256..16,384 segments
+    and 32 displaced spheres with 1k..100k triangles (log-uniform sizes),
+    4k..1M query points each (log-uniform, times `scale`), 16..128 walks,
+    half of the 2D scenes Poisson (Fourier source), the rest Laplace."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for j in range(64):
+        n_pts = int(np.exp(rng.uniform(np.log(4e3), np.log(1e6))) * scale)
+        n_pts = max(n_pts, 16)
+        W = int(rng.integers(16, 129))
+        if j < 32:
+            nseg = int(np.exp(rng.uniform(np.log(256), np.log(16384))))
+            k = int(rng.integers(3, 7))
+            a = float(rng.uniform(0.1, 0.3))
+            v, ix = star_polyline(nseg, a=a, k=k, phase=float(rng.uniform(0, 2 * np.pi)))
+            vals = g_saddle(v).astype(np.float32)
+            r = np.sqrt(rng.random(n_pts)) * (1 - a) * 0.98
+            t = rng.random(n_pts) * 2 * np.pi
+            p = np.stack([r * np.cos(t), r * np.sin(t)], 1).astype(np.float32)
+            item = dict(dim=2, vertices=v, indices=ix, values=vals, points=p, nwalks=W, nsteps=256, epsilon=1e-3,
+                        seed=j)
+            if j % 2:
+                item.update(source_type=3, source_params=fourier_params())
+        else:
+            ntri = int(np.exp(rng.uniform(np.log(1e3), np.log(1e5))))
+            v, f, noise = random_star_mesh(ntri, seed=j)
+            p = points_in_star(n_pts, noise, seed=j)
+            item = dict(dim=3, vertices=v, indices=f, values=g_harmonic3(v).astype(np.float32), points=p, nwalks=W,
+                        nsteps=256, epsilon=1e-3, seed=j)
+        out.append(item)
+    return out

@@ -84,8 +84,17 @@ _sig("wos_dirichlet", _vp, ctypes.POINTER(_Record), _i, _vp, _vp)
 _sig("wos_single_walk", _vp, ctypes.POINTER(_WalkParams), _vp, _i, _vp, _vp, ctypes.POINTER(_Record), _vp)
 _sig("wos_walk_reduce", _vp, ctypes.POINTER(_WalkParams), _vp, _i, _i, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp)
 _sig("wos_walk_reduce_counts", _vp, ctypes.POINTER(_WalkParams), _vp, _i, _i, _i, _vp, _vp, _vp)
-_sig("wos_set_tuning", _i, _i, _i)
-_sig("wos_get_tuning", _vp, _vp, _vp)
+class _BatchItem(ctypes.Structure):
+    _fields_ = [("scene", _vp), ("params", _WalkParams), ("points", _vp), ("n_points", _i), ("seed", _i),
+                ("mean", _vp), ("mean64", _vp), ("var64", _vp), ("walk_value", _vp), ("rec_prim", _vp),
+                ("rec_t", _vp), ("basis", _vp)]
+
+
+_sig("wos_batch_create", ctypes.POINTER(_BatchItem), _i, ctypes.POINTER(_vp))
+_sig("wos_batch_run", _vp, _vp)
+_sig("wos_batch_destroy", _vp)
+_sig("wos_set_tuning", _i, _i, _i, _i)
+_sig("wos_get_tuning", _vp, _vp, _vp, _vp)
 _sig("wos_adjoint_values", _vp, _i, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp)
 _sig("wos_shape_gradient", _vp, ctypes.POINTER(_GradParams), _vp, _i, _i, _vp, _vp, _vp, _vp)
 
@@ -94,20 +103,22 @@ EXPORTED = ("wos_last_error", "wos_scene_create", "wos_scene_destroy", "wos_scen
             "wos_scene_get_bvh", "wos_closest_point_index", "wos_sdf", "wos_closest_point",
             "wos_largest_inscribed_ball", "wos_dirichlet", "wos_single_walk", "wos_walk_reduce",
             "wos_walk_reduce_counts", "wos_adjoint_values", "wos_shape_gradient", "wos_set_tuning",
-            "wos_get_tuning")
+            "wos_get_tuning", "wos_batch_create", "wos_batch_run", "wos_batch_destroy")
 
 
-def set_tuning(chunk=None, wgs_per_cu=None, top_levels=None):
+def set_tuning(chunk=None, wgs_per_cu=None, top_levels=None, window=None):
     """Process-wide launch tuning (walks per queue claim, workgroups per CU,
-    BVH levels in LDS); unspecified values keep their current setting."""
-    c, w, t = get_tuning()
-    _check(lib.wos_set_tuning(int(chunk or c), int(wgs_per_cu or w), int(t if top_levels is None else top_levels)))
+    BVH levels in LDS, node visits per lane between refills); unspecified
+    values keep their current setting."""
+    c, w, t, v = get_tuning()
+    _check(lib.wos_set_tuning(int(chunk or c), int(wgs_per_cu or w), int(t if top_levels is None else top_levels),
+                              int(window or v)))
 
 
 def get_tuning():
-    c, w, t = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
-    _check(lib.wos_get_tuning(ctypes.byref(c), ctypes.byref(w), ctypes.byref(t)))
-    return c.value, w.value, t.value
+    c, w, t, v = ctypes.c_int(), ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+    _check(lib.wos_get_tuning(ctypes.byref(c), ctypes.byref(w), ctypes.byref(t), ctypes.byref(v)))
+    return c.value, w.value, t.value, v.value
 
 
 def _check(rc):
@@ -555,3 +566,55 @@ def shape_gradient(scene, points, seed, grad_out, *, nwalks, nsteps, epsilon, ep
     _check(lib.wos_shape_gradient(scene.handle, ctypes.byref(gp), _p(q), P, int(seed), _p(go),
                                   _p(gv if go is not None else None), _p(sg), _stream()))
     return gv, sg
+
+
+class Batch:
+    """Many (scene, walk settings, points, seed) items walked by one
+    persistent launch per dimension (``wos_batch_run``).  Item j's outputs
+    are ``outputs[j]`` (``mean``, ``mean64``, ``var64``, ``walk_value`` and,
+    on request, ``rec_prim``/``rec_t``/``basis``); each equals what
+    ``WoS.walk_reduce`` returns for that item alone.  ``run()`` allocates
+    nothing, so it can be captured with ``torch.cuda.graph``."""
+
+    def __init__(self, items):
+        self._h = _vp(None)
+        self.outputs, self._keep = [], []
+        arr = (_BatchItem * len(items))()
+        for j, it in enumerate(items):
+            scene, cw = it["scene"], it["wos"]
+            q = _dev(it["points"], torch.float32, scene.dim)
+            P, W = q.shape[0], cw.nwalks
+            dv = q.device
+            o = {"mean": torch.empty(P, dtype=torch.float32, device=dv),
+                 "mean64": torch.empty(P, dtype=torch.float64, device=dv),
+                 "var64": torch.empty(P, dtype=torch.float64, device=dv),
+                 "walk_value": torch.empty(P * W, dtype=torch.float32, device=dv)}
+            if it.get("records"):
+                o["rec_prim"] = torch.empty(P * W, dtype=torch.int32, device=dv)
+                o["rec_t"] = torch.empty(P * W * (scene.dim - 1), dtype=torch.float32, device=dv)
+            if it.get("basis"):
+                K = scene.c_source_params.numel() // 4
+                o["basis"] = torch.empty(K * P * W, dtype=torch.float32, device=dv)
+            self.outputs.append(o)
+            self._keep += [q, scene]
+            g = o.get
+            arr[j] = _BatchItem(scene.handle, cw._params(), q.data_ptr(), P, int(it.get("seed", 0)),
+                                g("mean").data_ptr(), g("mean64").data_ptr(), g("var64").data_ptr(),
+                                g("walk_value").data_ptr(), _p(g("rec_prim")), _p(g("rec_t")), _p(g("basis")))
+        h = _vp()
+        torch.cuda.synchronize()
+        _check(lib.wos_batch_create(arr, len(items), ctypes.byref(h)))
+        self._h = h
+
+    def run(self):
+        _check(lib.wos_batch_run(self._h, _stream()))
+        return self.outputs
+
+    def __del__(self):
+        h = self.__dict__.get("_h")
+        if h is not None and h.value:
+            try:
+                lib.wos_batch_destroy(h)
+            except Exception:
+                pass
+            self._h = _vp(None)

@@ -74,10 +74,11 @@ int wos_scene_create(int dim, const float *vertices, int n_vertices, const int *
                      int n_source_params, void *stream, wos_scene **out);
 /* Launch tuning of every walk launch (no counterpart in the reference, whose
  * thrust launches are untuned): walks a wave claims per queue visit (chunk),
- * persistent workgroups per CU, and BVH levels kept in LDS.  Process-wide;
- * chosen by the autotuner in the Python package. */
-int wos_set_tuning(int chunk, int wgs_per_cu, int top_levels);
-int wos_get_tuning(int *chunk, int *wgs_per_cu, int *top_levels);
+ * persistent workgroups per CU, BVH levels kept in LDS, and node visits per
+ * lane between refills of the walk kernel (window).  Process-wide; chosen by
+ * the autotuner in the Python package. */
+int wos_set_tuning(int chunk, int wgs_per_cu, int top_levels, int window);
+int wos_get_tuning(int *chunk, int *wgs_per_cu, int *top_levels, int *window);
 /* Scene destructor (the reference frees its thrust buffers with the object). */
 int wos_scene_destroy(wos_scene *scene);
 /* Scene.values / Scene.source_params setters (def_readwrite in
@@ -161,6 +162,33 @@ int wos_adjoint_values(const wos_scene *scene, int n_points, int nwalks, const i
  * n_points*nwalks*dim floats (0 for samples that do not contribute). */
 int wos_shape_gradient(const wos_scene *scene, const wos_grad_params *params, const float *points, int n_points,
                        int seed, const float *grad_out, float *grad_vertices, float *sample_grad_u, void *stream);
+
+/* ---- batches of scenes ----------------------------------------------------
+ * No counterpart in the reference, which solves one scene per call: a batch
+ * runs Solver.walk_reduce (wos/solver.py:49-65) for many (scene, settings,
+ * points, seed) items in one persistent launch per dimension.  Each item's
+ * walks use exactly the streams of a single wos_walk_reduce call with the
+ * same seed.  wos_batch_create uploads the item table and allocates every
+ * buffer a run needs (synchronous); wos_batch_run allocates nothing and can
+ * be captured into a graph; runs of one batch on concurrent streams race. */
+typedef struct wos_batch wos_batch;
+typedef struct {
+    const wos_scene *scene;
+    wos_walk_params  params;
+    const float     *points;
+    int              n_points;
+    int              seed;
+    float           *mean;        /* nullable */
+    double          *mean64;      /* nullable */
+    double          *var64;       /* nullable */
+    float           *walk_value;  /* required: n_points * nwalks */
+    int             *rec_prim;    /* nullable */
+    float           *rec_t;       /* nullable */
+    float           *basis;       /* nullable: Fourier sources only */
+} wos_batch_item;
+int wos_batch_create(const wos_batch_item *items, int n_items, wos_batch **out);
+int wos_batch_run(wos_batch *batch, void *stream);
+int wos_batch_destroy(wos_batch *batch);
 
 #ifdef __cplusplus
 }
