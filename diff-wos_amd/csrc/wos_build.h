@@ -54,10 +54,10 @@ struct WalkOutputs {
 
 // Launch tuning shared by all walk launches (set through wos_set_tuning).
 struct Tuning {
-    int chunk = 256;        // walks a wave claims per queue visit
+    int chunk = 128;        // walks a wave claims per queue visit
     int wgs_per_cu = 8;     // persistent workgroups per CU
     int top_levels = 6;     // BVH levels kept in LDS
-    int window = 8;         // node visits per lane between refills of the walk kernel
+    int window = 16;        // node visits per lane between refills of the walk kernel
 };
 extern Tuning g_tuning;
 

@@ -822,7 +822,7 @@ int launch_walk_reduce(const SceneView &s, const WalkSettings &w, const float *p
     a.counter = launch_counter(st);
     if (!a.counter) return -1;
     a.chunk = chunk > 0 ? chunk : (g_tuning.chunk > 0 ? g_tuning.chunk : 256);
-    a.window = g_tuning.window > 0 ? g_tuning.window : 8;
+    a.window = g_tuning.window > 0 ? g_tuning.window : 16;
     a.walk_value = o.walk_value;
     a.rec_prim = o.rec_prim;
     a.rec_t = o.rec_t;

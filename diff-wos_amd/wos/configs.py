@@ -286,3 +286,25 @@ def config5(scale=1.0, seed=0):
                         nsteps=256, epsilon=1e-3, seed=j)
         out.append(item)
     return out
+
+
+def points_in_ball(n, radius, seed=2):
+    rng = np.random.default_rng(seed)
+    d = rng.normal(size=(n, 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    r = radius * rng.random(n) ** (1.0 / 3.0)
+    return (d * r[:, None]).astype(np.float32)
+
+
+def config4(n_tris=200_000, n_points=262_144, seed=0):
+    """3D shape gradient: the seeded decimation of the subdivision-7 sphere
+    (200k triangles), displaced by 1 + 0.15 n.  Vertex values carry g = x^2 -
+    y^2 + z of each vertex's undisplaced position (the data painted on the
+    sphere), so the target -- the solution on the undisplaced sphere -- is
+    g itself.  Query points: uniform in the ball of radius 0.8 (inside both
+    surfaces).  Loss L = mean (u - g)^2; gradient w.r.t. all vertex positions."""
+    V, U, F, noise = decimated_star_mesh(n_tris, seed=seed)
+    vals = g_harmonic3(U).astype(np.float32)
+    p = points_in_ball(n_points, 0.8, seed=seed + 2)
+    return dict(vertices=V, sphere_vertices=U, indices=F, values=vals, points=p, target=g_harmonic3(p),
+                nwalks=16, nsteps=256, epsilon=1e-3, epsilon2=1e-3, clamping=0.1, seed=0)

@@ -98,3 +98,42 @@ def test_custom_solver_shape_gradient_directional_fd():
     fd = (loss_at(1 + h, fwd) - loss_at(1 - h, fwd)) / (2 * h)
     assert np.isfinite(dL)
     assert abs(dL - fd) < 0.1 * abs(fd) + 1e-3, (dL, fd)
+
+
+def test_ours3d_directional_fd_on_64_vertices():
+    """Scaled-down shape-gradient workload: gradient w.r.t. vertex positions
+    (Ours3D through CustomSolver) against a central difference of the loss
+    along random directions on 64 random vertices, common seeds."""
+    from wos.scene3d import Scene3D
+    from wos.solver import CustomSolver
+    from wos.wos3d import WoS3DCUDA
+    from wos.wos_grad_3d import Ours3D
+    c = C.config4(n_tris=5000, n_points=4096)
+    V0 = _t(c["vertices"])
+    vals = _t(c["values"])
+    pts = _t(c["points"])
+    target = _t(c["target"].astype(np.float32))
+    W = 64
+    fwd = WoS3DCUDA(nwalks=W, nsteps=256, epsilon=1e-3)
+    bwd = Ours3D(nwalks=W, nsteps=256, epsilon=1e-3, epsilon2=1e-3, clamping=0.1)
+
+    def loss(V, solver):
+        u = solver.walk(pts, Scene3D(V, _t(c["indices"]), vals), seed=5)
+        return ((u - target) ** 2).mean()
+
+    V = V0.clone().requires_grad_(True)
+    loss(V, CustomSolver(fwd, bwd)).backward()
+    grad = V.grad.detach()
+    rng = np.random.default_rng(9)
+    sel = torch.from_numpy(rng.choice(V0.shape[0], 64, replace=False)).cuda()
+    fwd_fd = WoS3DCUDA(nwalks=1024, nsteps=256, epsilon=1e-3)
+    agree = 0
+    for trial in range(3):
+        dirn = torch.zeros_like(V0)
+        dirn[sel] = torch.from_numpy(rng.normal(size=(64, 3)).astype(np.float32)).cuda()
+        h = 2e-2
+        fd = (loss(V0 + h * dirn, fwd_fd).item() - loss(V0 - h * dirn, fwd_fd).item()) / (2 * h)
+        ad = float((grad * dirn).sum())
+        assert np.isfinite(ad)
+        agree += abs(ad - fd) < 0.25 * abs(fd) + 2e-5
+    assert agree >= 2
