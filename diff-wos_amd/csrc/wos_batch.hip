@@ -49,9 +49,7 @@ __global__ __launch_bounds__(kBlock) void k_batch_init(const BatchItem *items, i
     __shared__ long long s_begin[kMaxBatchItems];
     for (int i = threadIdx.x; i < n_items; i += kBlock) s_begin[i] = items[i].point_begin;
     __syncthreads();
-    __shared__ int s_stack_node[kLdsStack * kBlock];
-    __shared__ float s_stack_d2[kLdsStack * kBlock];
-    TravStack st{nullptr, s_stack_node, s_stack_d2, spill, stride};
+    TravStack st{spill, stride};
     const long lane_g = (long)blockIdx.x * kBlock + threadIdx.x;
     for (long long g = lane_g; g < n_total_pts; g += (long long)gridDim.x * kBlock) {
         int it = find_item(s_begin, n_items, g);
@@ -100,9 +98,7 @@ __global__ __launch_bounds__(kBlock) void k_batch_walk(const BatchItem *items, i
     __shared__ long long s_begin[kMaxBatchItems];
     for (int i = threadIdx.x; i < n_items; i += kBlock) s_begin[i] = items[i].walk_begin;
     __syncthreads();
-    __shared__ int s_stack_node[kLdsStack * kBlock];
-    __shared__ float s_stack_d2[kLdsStack * kBlock];
-    TravStack st{nullptr, s_stack_node, s_stack_d2, spill, stride};
+    TravStack st{spill, stride};
     const long lane_g = (long)blockIdx.x * kBlock + threadIdx.x;
     const int lane = __lane_id();
 

@@ -19,7 +19,7 @@ struct BuildInputs {
 };
 
 struct BuildOutputs {
-    float4 *nodes;          // (n_prims - 1) * (dim == 2 ? 3 : 4) float4
+    float4 *nodes;          // (n_prims - 1) * (dim == 2 ? 3 : 4) float4; 2D nodes hold leaf segments inline
     float4 *leaves;         // n_prims * (dim == 2 ? 2 : 3) float4
     float *normals;         // n_vertices * dim
     int *sorted_prim;       // n_prims

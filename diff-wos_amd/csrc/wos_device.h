@@ -268,13 +268,26 @@ struct SceneView {
 // Per-lane traversal stack: the first kLdsStack entries live in LDS (lane
 // stride kBlock), deeper entries spill to a global buffer (lane stride
 // spill_stride).  Depth never exceeds the tree depth, which sizes the spill.
+// The LDS part and the LDS copy of the top nodes are file-scope so every
+// access compiles to a ds_* instruction.
+static __shared__ int g_stack_node[kLdsStack * kBlock];
+static __shared__ float g_stack_d2[kLdsStack * kBlock];
+extern __shared__ float4 g_top_nodes[];
+
 struct TravStack {
-    const float4 *top;    // LDS copy of nodes [0, n_top)
-    int *lds_node;
-    float *lds_d2;
     int2 *spill;          // may be null when the tree fits the LDS part
     long spill_stride;
 };
+
+// Copies the first s.n_top nodes (depth-major order) to LDS; every lane of
+// the workgroup must call it before any traversal.
+__device__ __forceinline__ void load_top_nodes(const SceneView &s) {
+    if (s.n_top > 0 && s.use_bvh) {
+        const int nw = s.n_top * (s.dim == 2 ? 3 : 4);
+        for (int i = threadIdx.x; i < nw; i += kBlock) g_top_nodes[i] = s.nodes[i];
+        __syncthreads();
+    }
+}
 
 // Box distance^2 from p to [lo, hi].
 __device__ __forceinline__ float box_d2_2(float px, float py, float4 b) {
@@ -305,263 +318,6 @@ __device__ __forceinline__ float prune_threshold(float best_d2, float tol) {
     return d * d;
 }
 
-template <int D>
-__device__ __forceinline__ float leaf_d2(const SceneView &s, int j, const float *p, int &prim) {
-    if constexpr (D == 2) {
-        float4 l0 = s.leaves[2 * j], l1 = s.leaves[2 * j + 1];
-        prim = __float_as_int(l1.y);
-        float h, dx, dy;
-        return seg_d2(p[0], p[1], l0.x, l0.y, l0.z, l0.w, l1.x, h, dx, dy);
-    } else {
-        float4 l0 = s.leaves[3 * j], l1 = s.leaves[3 * j + 1], l2 = s.leaves[3 * j + 2];
-        prim = __float_as_int(l2.y);
-        float a[3] = {l0.x, l0.y, l0.z}, b[3] = {l0.w, l1.x, l1.y}, c[3] = {l1.z, l1.w, l2.x};
-        float q[3], bary[2];
-        return tri_closest(a, b, c, p, q, bary);
-    }
-}
-
-// Closest primitive by brute force in index order (include/scene.cuh:102-128).
-template <int D>
-__device__ __forceinline__ int closest_brute(const SceneView &s, const float *p, float &best_d2) {
-    float best = __builtin_inff();
-    int id = -1;
-    for (int f = 0; f < s.n_prims; ++f) {
-        float d2;
-        if constexpr (D == 2) {
-            int i0 = s.idx[2 * f], i1 = s.idx[2 * f + 1];
-            float ax = s.vtx[2 * i0], ay = s.vtx[2 * i0 + 1];
-            float bax = s.vtx[2 * i1] - ax, bay = s.vtx[2 * i1 + 1] - ay;
-            float inv = 1.0f / dot2(bax, bay, bax, bay);
-            float h, dx, dy;
-            d2 = seg_d2(p[0], p[1], ax, ay, bax, bay, inv, h, dx, dy);
-        } else {
-            int i0 = s.idx[3 * f], i1 = s.idx[3 * f + 1], i2 = s.idx[3 * f + 2];
-            float a[3] = {s.vtx[3 * i0], s.vtx[3 * i0 + 1], s.vtx[3 * i0 + 2]};
-            float b[3] = {s.vtx[3 * i1], s.vtx[3 * i1 + 1], s.vtx[3 * i1 + 2]};
-            float c[3] = {s.vtx[3 * i2], s.vtx[3 * i2 + 1], s.vtx[3 * i2 + 2]};
-            float q[3], bary[2];
-            d2 = tri_closest(a, b, c, p, q, bary);
-        }
-        if (d2 < best) { best = d2; id = f; }
-    }
-    if (s.stats) {
-        atomicAdd(&s.stats[1], (unsigned long long)s.n_prims);
-        atomicAdd(&s.stats[2], 1ull);
-    }
-    best_d2 = best;
-    return id;
-}
-
-// Nearest-first LBVH traversal.  bound_d2: squared distance within which the
-// closest primitive is known to lie (infinity when unknown).  Returns the
-// primitive index (-1 if none lies within the bound) and its squared distance.
-template <int D>
-__device__ int closest_bvh(const SceneView &s, const float *p, float bound_d2, float &best_d2,
-                           const TravStack &st, int lane_slot, long spill_lane) {
-    float best = __builtin_inff();
-    int best_id = 0x7fffffff;
-    float thr = bound_d2;
-    if (s.n_prims == 1) {
-        int prim;
-        float d2 = leaf_d2<D>(s, 0, p, prim);
-        if (s.stats) {
-            atomicAdd(&s.stats[1], 1ull);
-            atomicAdd(&s.stats[2], 1ull);
-        }
-        best_d2 = d2;
-        return prim;
-    }
-    int node = 0, sp = 0;
-    unsigned nbox = 0, nprim = 0;
-    for (;;) {
-        float dL, dR;
-        nbox += 2;
-        int cL, cR;
-        if constexpr (D == 2) {
-            const float4 *nb = (node < s.n_top) ? st.top + 3 * node : s.nodes + 3 * node;
-            float4 b0 = nb[0], b1 = nb[1], c = nb[2];
-            dL = box_d2_2(p[0], p[1], b0);
-            dR = box_d2_2(p[0], p[1], b1);
-            cL = __float_as_int(c.x);
-            cR = __float_as_int(c.y);
-        } else {
-            const float4 *nb = (node < s.n_top) ? st.top + 4 * node : s.nodes + 4 * node;
-            float4 n0 = nb[0], n1 = nb[1], n2 = nb[2], c = nb[3];
-            dL = box_d2_3(p, n0.x, n0.y, n0.z, n0.w, n1.x, n1.y);
-            dR = box_d2_3(p, n1.z, n1.w, n2.x, n2.y, n2.z, n2.w);
-            cL = __float_as_int(c.x);
-            cR = __float_as_int(c.y);
-        }
-        bool goL = false, goR = false;
-        if (dL <= thr) {
-            if (cL < 0) {
-                int prim;
-                ++nprim;
-                float d2 = leaf_d2<D>(s, ~cL, p, prim);
-                if (better(d2, prim, best, best_id)) {
-                    best = d2; best_id = prim;
-                    thr = fminf(thr, prune_threshold(best, s.tol));
-                }
-            } else goL = true;
-        }
-        if (dR <= thr) {
-            if (cR < 0) {
-                int prim;
-                ++nprim;
-                float d2 = leaf_d2<D>(s, ~cR, p, prim);
-                if (better(d2, prim, best, best_id)) {
-                    best = d2; best_id = prim;
-                    thr = fminf(thr, prune_threshold(best, s.tol));
-                }
-            } else goR = true;
-        }
-        goL = goL && dL <= thr;
-        if (goL && goR) {
-            bool leftNear = dL <= dR;
-            int nearN = leftNear ? cL : cR, farN = leftNear ? cR : cL;
-            float farD = leftNear ? dR : dL;
-            if (sp < kLdsStack) {
-                st.lds_node[sp * kBlock + lane_slot] = farN;
-                st.lds_d2[sp * kBlock + lane_slot] = farD;
-            } else {
-                st.spill[(long)(sp - kLdsStack) * st.spill_stride + spill_lane] = make_int2(farN, __float_as_int(farD));
-            }
-            ++sp;
-            node = nearN;
-            continue;
-        }
-        if (goL) { node = cL; continue; }
-        if (goR) { node = cR; continue; }
-        // pop until an entry survives the current threshold
-        bool found = false;
-        while (sp > 0) {
-            --sp;
-            int n;
-            float dd;
-            if (sp < kLdsStack) {
-                n = st.lds_node[sp * kBlock + lane_slot];
-                dd = st.lds_d2[sp * kBlock + lane_slot];
-            } else {
-                int2 e = st.spill[(long)(sp - kLdsStack) * st.spill_stride + spill_lane];
-                n = e.x;
-                dd = __int_as_float(e.y);
-            }
-            if (dd <= thr) { node = n; found = true; break; }
-        }
-        if (!found) break;
-    }
-    if (s.stats) {
-        atomicAdd(&s.stats[0], (unsigned long long)nbox);
-        atomicAdd(&s.stats[1], (unsigned long long)nprim);
-        atomicAdd(&s.stats[2], 1ull);
-    }
-    best_d2 = best;
-    return best_id == 0x7fffffff ? -1 : best_id;
-}
-
-// Resumable traversal: the walk kernel advances every lane's query by a few
-// node visits per round, so lanes with short and long queries stay busy.
-struct Query {
-    int node, sp;
-    float best, thr;   // best squared distance so far, box pruning threshold
-    int best_id;
-};
-
-// Starts a query whose answer is known to lie within bound_d2, seeded with a
-// candidate primitive (seed_id, seed_d2) -- e.g. the previous step's closest
-// primitive -- which tightens pruning from the root.
-__device__ __forceinline__ void query_begin(Query &q, float bound_d2, int seed_id, float seed_d2, float tol) {
-    q.node = 0;
-    q.sp = 0;
-    q.best = seed_d2;
-    q.best_id = seed_id;
-    q.thr = fminf(bound_d2, prune_threshold(seed_d2, tol));
-}
-
-// One node visit; returns true when the query is complete.  Same pruning and
-// (d2, index) minimum as closest_bvh.
-template <int D>
-__device__ __forceinline__ bool query_step(const SceneView &s, const float *p, Query &q, const TravStack &st,
-                                           int lane_slot, long spill_lane, float tol) {
-    const int node = q.node;
-    float dL, dR;
-    int cL, cR;
-    if constexpr (D == 2) {
-        const float4 *nb = (node < s.n_top) ? st.top + 3 * node : s.nodes + 3 * node;
-        float4 b0 = nb[0], b1 = nb[1], c = nb[2];
-        dL = box_d2_2(p[0], p[1], b0);
-        dR = box_d2_2(p[0], p[1], b1);
-        cL = __float_as_int(c.x);
-        cR = __float_as_int(c.y);
-    } else {
-        const float4 *nb = (node < s.n_top) ? st.top + 4 * node : s.nodes + 4 * node;
-        float4 n0 = nb[0], n1 = nb[1], n2 = nb[2], c = nb[3];
-        dL = box_d2_3(p, n0.x, n0.y, n0.z, n0.w, n1.x, n1.y);
-        dR = box_d2_3(p, n1.z, n1.w, n2.x, n2.y, n2.z, n2.w);
-        cL = __float_as_int(c.x);
-        cR = __float_as_int(c.y);
-    }
-    if (s.stats) atomicAdd(&s.stats[0], 2ull);
-    bool goL = false, goR = false;
-    if (dL <= q.thr) {
-        if (cL < 0) {
-            int prim;
-            float d2 = leaf_d2<D>(s, ~cL, p, prim);
-            if (s.stats) atomicAdd(&s.stats[1], 1ull);
-            if (better(d2, prim, q.best, q.best_id)) {
-                q.best = d2;
-                q.best_id = prim;
-                q.thr = fminf(q.thr, prune_threshold(d2, tol));
-            }
-        } else goL = true;
-    }
-    if (dR <= q.thr) {
-        if (cR < 0) {
-            int prim;
-            float d2 = leaf_d2<D>(s, ~cR, p, prim);
-            if (s.stats) atomicAdd(&s.stats[1], 1ull);
-            if (better(d2, prim, q.best, q.best_id)) {
-                q.best = d2;
-                q.best_id = prim;
-                q.thr = fminf(q.thr, prune_threshold(d2, tol));
-            }
-        } else goR = true;
-    }
-    goL = goL && dL <= q.thr;
-    if (goL && goR) {
-        bool leftNear = dL <= dR;
-        int nearN = leftNear ? cL : cR, farN = leftNear ? cR : cL;
-        float farD = leftNear ? dR : dL;
-        if (q.sp < kLdsStack) {
-            st.lds_node[q.sp * kBlock + lane_slot] = farN;
-            st.lds_d2[q.sp * kBlock + lane_slot] = farD;
-        } else {
-            st.spill[(long)(q.sp - kLdsStack) * st.spill_stride + spill_lane] = make_int2(farN, __float_as_int(farD));
-        }
-        ++q.sp;
-        q.node = nearN;
-        return false;
-    }
-    if (goL) { q.node = cL; return false; }
-    if (goR) { q.node = cR; return false; }
-    while (q.sp > 0) {
-        --q.sp;
-        int n;
-        float dd;
-        if (q.sp < kLdsStack) {
-            n = st.lds_node[q.sp * kBlock + lane_slot];
-            dd = st.lds_d2[q.sp * kBlock + lane_slot];
-        } else {
-            int2 e = st.spill[(long)(q.sp - kLdsStack) * st.spill_stride + spill_lane];
-            n = e.x;
-            dd = __int_as_float(e.y);
-        }
-        if (dd <= q.thr) { q.node = n; return false; }
-    }
-    return true;
-}
-
 // Squared distance from p to primitive f computed from the scene arrays
 // (identical to the leaf test of the same primitive).
 template <int D>
@@ -581,6 +337,237 @@ __device__ __forceinline__ float prim_d2(const SceneView &s, int f, const float 
         float q[3], bary[2];
         return tri_closest(a, b, c, p, q, bary);
     }
+}
+
+// 3D leaf: triangle a, b, c and the primitive index, in Morton order.
+__device__ __forceinline__ float leaf_d2_3(const SceneView &s, int j, const float *p, int &prim) {
+    float4 l0 = s.leaves[3 * j], l1 = s.leaves[3 * j + 1], l2 = s.leaves[3 * j + 2];
+    prim = __float_as_int(l2.y);
+    float a[3] = {l0.x, l0.y, l0.z}, b[3] = {l0.w, l1.x, l1.y}, c[3] = {l1.z, l1.w, l2.x};
+    float q[3], bary[2];
+    return tri_closest(a, b, c, p, q, bary);
+}
+
+// Closest primitive by brute force in index order (include/scene.cuh:102-128).
+template <int D>
+__device__ __forceinline__ int closest_brute(const SceneView &s, const float *p, float &best_d2) {
+    float best = __builtin_inff();
+    int id = -1;
+    for (int f = 0; f < s.n_prims; ++f) {
+        float d2 = prim_d2<D>(s, f, p);
+        if (d2 < best) { best = d2; id = f; }
+    }
+    if (s.stats) {
+        atomicAdd(&s.stats[1], (unsigned long long)s.n_prims);
+        atomicAdd(&s.stats[2], 1ull);
+    }
+    best_d2 = best;
+    return id;
+}
+
+// Resumable nearest-first LBVH traversal.  The walk kernel advances every
+// lane's query by a few node visits per round, so lanes with short and long
+// queries stay busy; the one-shot queries loop query_step to completion.
+struct Query {
+    int node, sp;
+    float best, thr;   // best squared distance so far, box pruning threshold
+    int best_id;
+    unsigned nbox, nprim;   // work counters (instrumented launches only)
+};
+
+// Starts a query whose answer lies within bound_d2, optionally seeded with a
+// candidate primitive (seed_id, seed_d2) -- e.g. the previous step's closest
+// primitive -- which tightens pruning from the root.
+__device__ __forceinline__ void query_begin(Query &q, float bound_d2, int seed_id, float seed_d2, float tol) {
+    q.node = 0;
+    q.sp = 0;
+    q.best = seed_d2;
+    q.best_id = seed_id;
+    q.thr = fminf(bound_d2, prune_threshold(seed_d2, tol));
+    q.nbox = q.nprim = 0;
+}
+
+__device__ __forceinline__ void stack_push(Query &q, const TravStack &st, int slot, long spill_lane, int node,
+                                           float d2) {
+    if (q.sp < kLdsStack) {
+        g_stack_node[q.sp * kBlock + slot] = node;
+        g_stack_d2[q.sp * kBlock + slot] = d2;
+    } else {
+        st.spill[(long)(q.sp - kLdsStack) * st.spill_stride + spill_lane] = make_int2(node, __float_as_int(d2));
+    }
+    ++q.sp;
+}
+
+// Pops until an entry survives the threshold; false when the stack is empty.
+__device__ __forceinline__ bool stack_pop(Query &q, const TravStack &st, int slot, long spill_lane) {
+    while (q.sp > 0) {
+        --q.sp;
+        int n;
+        float dd;
+        if (q.sp < kLdsStack) {
+            n = g_stack_node[q.sp * kBlock + slot];
+            dd = g_stack_d2[q.sp * kBlock + slot];
+        } else {
+            int2 e = st.spill[(long)(q.sp - kLdsStack) * st.spill_stride + spill_lane];
+            n = e.x;
+            dd = __int_as_float(e.y);
+        }
+        if (dd <= q.thr) {
+            q.node = n;
+            return true;
+        }
+    }
+    return false;
+}
+
+// One node visit; true when the query is complete.
+//
+// 2D nodes carry their leaf children inline (3 x float4):
+//   slot k (float4): box {lo.xy, hi.xy} of an internal child, or segment
+//                    {a.xy, (b-a).xy} of a leaf child
+//   aux (float4):    {inv0, id0, inv1, id1}; inv = 1/|b-a|^2 > 0 marks a leaf
+//                    (id = primitive index), inv = -1 an internal child (id =
+//                    node index)
+// so both children are evaluated without branches and without a second load.
+template <int D, bool STATS>
+__device__ __forceinline__ bool query_step(const SceneView &s, const float *p, Query &q, const TravStack &st,
+                                           int slot, long spill_lane, float tol) {
+    const int node = q.node;
+    if constexpr (D == 2) {
+        float4 n0, n1, n2;
+        if (node < s.n_top) {
+            n0 = g_top_nodes[3 * node];
+            n1 = g_top_nodes[3 * node + 1];
+            n2 = g_top_nodes[3 * node + 2];
+            // the empty asm keeps the LDS and global loads apart: merged,
+            // they become one generic (flat) load for both address spaces
+            __asm__ volatile("" ::: "memory");
+        } else {
+            n0 = s.nodes[3 * node];
+            n1 = s.nodes[3 * node + 1];
+            n2 = s.nodes[3 * node + 2];
+        }
+        const bool leaf0 = n2.x > 0.0f, leaf1 = n2.z > 0.0f;
+        const int id0 = __float_as_int(n2.y), id1 = __float_as_int(n2.w);
+        float h, dx, dy;
+        float s0 = seg_d2(p[0], p[1], n0.x, n0.y, n0.z, n0.w, n2.x, h, dx, dy);
+        float s1 = seg_d2(p[0], p[1], n1.x, n1.y, n1.z, n1.w, n2.z, h, dx, dy);
+        float d0 = leaf0 ? s0 : box_d2_2(p[0], p[1], n0);
+        float d1 = leaf1 ? s1 : box_d2_2(p[0], p[1], n1);
+        if constexpr (STATS) {
+            q.nbox += (leaf0 ? 0u : 1u) + (leaf1 ? 0u : 1u);
+            q.nprim += (leaf0 ? 1u : 0u) + (leaf1 ? 1u : 0u);
+        }
+        bool u0 = leaf0 && better(d0, id0, q.best, q.best_id);
+        if (u0) { q.best = d0; q.best_id = id0; }
+        bool u1 = leaf1 && better(d1, id1, q.best, q.best_id);
+        if (u1) { q.best = d1; q.best_id = id1; }
+        if (u0 || u1) q.thr = fminf(q.thr, prune_threshold(q.best, tol));
+        const bool g0 = !leaf0 && d0 <= q.thr, g1 = !leaf1 && d1 <= q.thr;
+        if (g0 && g1) {
+            const bool near0 = d0 <= d1;
+            stack_push(q, st, slot, spill_lane, near0 ? id1 : id0, near0 ? d1 : d0);
+            q.node = near0 ? id0 : id1;
+            return false;
+        }
+        if (g0 || g1) {
+            q.node = g0 ? id0 : id1;
+            return false;
+        }
+        return !stack_pop(q, st, slot, spill_lane);
+    } else {
+        float4 n0, n1, n2, c;
+        if (node < s.n_top) {
+            n0 = g_top_nodes[4 * node];
+            n1 = g_top_nodes[4 * node + 1];
+            n2 = g_top_nodes[4 * node + 2];
+            c = g_top_nodes[4 * node + 3];
+            __asm__ volatile("" ::: "memory");
+        } else {
+            n0 = s.nodes[4 * node];
+            n1 = s.nodes[4 * node + 1];
+            n2 = s.nodes[4 * node + 2];
+            c = s.nodes[4 * node + 3];
+        }
+        float dL = box_d2_3(p, n0.x, n0.y, n0.z, n0.w, n1.x, n1.y);
+        float dR = box_d2_3(p, n1.z, n1.w, n2.x, n2.y, n2.z, n2.w);
+        int cL = __float_as_int(c.x), cR = __float_as_int(c.y);
+        if constexpr (STATS) q.nbox += 2u;
+        bool goL = false, goR = false;
+        if (dL <= q.thr) {
+            if (cL < 0) {
+                int prim;
+                float d2 = leaf_d2_3(s, ~cL, p, prim);
+                if constexpr (STATS) q.nprim += 1u;
+                if (better(d2, prim, q.best, q.best_id)) {
+                    q.best = d2;
+                    q.best_id = prim;
+                    q.thr = fminf(q.thr, prune_threshold(d2, tol));
+                }
+            } else goL = true;
+        }
+        if (dR <= q.thr) {
+            if (cR < 0) {
+                int prim;
+                float d2 = leaf_d2_3(s, ~cR, p, prim);
+                if constexpr (STATS) q.nprim += 1u;
+                if (better(d2, prim, q.best, q.best_id)) {
+                    q.best = d2;
+                    q.best_id = prim;
+                    q.thr = fminf(q.thr, prune_threshold(d2, tol));
+                }
+            } else goR = true;
+        }
+        goL = goL && dL <= q.thr;
+        if (goL && goR) {
+            const bool nearL = dL <= dR;
+            stack_push(q, st, slot, spill_lane, nearL ? cR : cL, nearL ? dR : dL);
+            q.node = nearL ? cL : cR;
+            return false;
+        }
+        if (goL || goR) {
+            q.node = goL ? cL : cR;
+            return false;
+        }
+        return !stack_pop(q, st, slot, spill_lane);
+    }
+}
+
+// Adds a finished query's box and primitive tests to the launch counters.
+template <bool STATS>
+__device__ __forceinline__ void query_flush_stats(const SceneView &s, const Query &q) {
+    if constexpr (STATS) {
+        if (s.stats) {
+            atomicAdd(&s.stats[0], (unsigned long long)q.nbox);
+            atomicAdd(&s.stats[1], (unsigned long long)q.nprim);
+        }
+    }
+}
+
+// One-shot nearest-first query.  bound_d2: squared distance within which
+// the closest primitive is known to lie (infinity when unknown).  Returns the
+// primitive index (-1 if none lies within the bound) and its squared distance.
+template <int D>
+__device__ int closest_bvh(const SceneView &s, const float *p, float bound_d2, float &best_d2,
+                           const TravStack &st, int lane_slot, long spill_lane) {
+    if (s.n_prims == 1) {
+        best_d2 = prim_d2<D>(s, 0, p);
+        if (s.stats) {
+            atomicAdd(&s.stats[1], 1ull);
+            atomicAdd(&s.stats[2], 1ull);
+        }
+        return 0;
+    }
+    Query q;
+    query_begin(q, bound_d2, 0x7fffffff, __builtin_inff(), s.tol);
+    while (!query_step<D, true>(s, p, q, st, lane_slot, spill_lane, s.tol)) {
+    }
+    query_flush_stats<true>(s, q);
+    if (s.stats) atomicAdd(&s.stats[2], 1ull);
+    best_d2 = q.best;
+    // a leaf is accepted whatever the bound; beyond it, the answer is not
+    // guaranteed to be the closest, so report nothing and let the caller widen
+    return (q.best_id == 0x7fffffff || q.best > bound_d2) ? -1 : q.best_id;
 }
 
 // Closest primitive with an optional distance bound; falls back to an

@@ -41,15 +41,8 @@ static int grid_of(long n, int cap = kGridCap) {
 // Per-workgroup traversal state: the LDS stack and, when s.n_top > 0, an LDS
 // copy of the first s.n_top nodes (dynamic shared memory).
 #define DECLARE_STACK(spill_, stride_)                                          \
-    __shared__ int s_stack_node[kLdsStack * kBlock];                            \
-    __shared__ float s_stack_d2[kLdsStack * kBlock];                            \
-    extern __shared__ float4 s_top_nodes[];                                     \
-    if (s.n_top > 0 && s.use_bvh) {                                             \
-        const int nw_ = s.n_top * (s.dim == 2 ? 3 : 4);                         \
-        for (int i_ = threadIdx.x; i_ < nw_; i_ += kBlock) s_top_nodes[i_] = s.nodes[i_]; \
-        __syncthreads();                                                        \
-    }                                                                           \
-    TravStack st{s_top_nodes, s_stack_node, s_stack_d2, (spill_), (stride_)};   \
+    load_top_nodes(s);                                                          \
+    TravStack st{(spill_), (stride_)};                                          \
     const int slot = threadIdx.x;                                               \
     const long lane_g = (long)blockIdx.x * kBlock + threadIdx.x;
 
@@ -291,7 +284,7 @@ __device__ __forceinline__ void write_walk(const SceneView &s, const PersistArgs
 // global counter when its chunk runs dry, so neither walk lengths nor query
 // lengths idle the SIMD for long.  Every walk re-derives its random stream
 // from (seed, point, walk index), so results do not depend on the schedule.
-template <int D, bool SRC, bool BASIS>
+template <int D, bool SRC, bool BASIS, bool STATS>
 __global__ __launch_bounds__(kBlock) void k_walk_persistent(SceneView s, PersistArgs a) {
     DECLARE_STACK(a.spill, a.spill_stride);
     const int lane = __lane_id();
@@ -306,6 +299,7 @@ __global__ __launch_bounds__(kBlock) void k_walk_persistent(SceneView s, Persist
     q.node = q.sp = 0;
     q.best = q.thr = 0.0f;
     q.best_id = -1;
+    q.nbox = q.nprim = 0;
     float bas[BASIS ? kMaxBasis : 1];
 #pragma unroll
     for (int i = 0; i < (BASIS ? kMaxBasis : 1); ++i) bas[i] = 0.0f;
@@ -427,7 +421,7 @@ __global__ __launch_bounds__(kBlock) void k_walk_persistent(SceneView s, Persist
             }
             move_point<D>(p, d, rng);
             ++step;
-            if (s.stats) {
+            if constexpr (STATS) {
                 atomicAdd(&s.stats[3], 1ull);
                 if (s.use_bvh) {
                     atomicAdd(&s.stats[2], 1ull);
@@ -453,7 +447,10 @@ __global__ __launch_bounds__(kBlock) void k_walk_persistent(SceneView s, Persist
         // 4. advance every pending query by up to `window` node visits
         for (int it = 0; it < a.window; ++it) {
             if (mode == kQuery) {
-                if (query_step<D>(s, p, q, st, slot, lane_g, s.tol)) mode = kReady;
+                if (query_step<D, STATS>(s, p, q, st, slot, lane_g, s.tol)) {
+                    mode = kReady;
+                    query_flush_stats<STATS>(s, q);
+                }
             }
             if (__ballot(mode == kQuery) == 0) break;
         }
@@ -852,11 +849,18 @@ int launch_walk_reduce(const SceneView &s, const WalkSettings &w, const float *p
     bool src = s.dim == 2 && s.source_type != 0;
     bool bas = src && o.basis != nullptr && K > 0;
     if (s.dim == 2) {
-        if (bas) k_walk_persistent<2, true, true><<<g, kBlock, tb, st>>>(s, a);
-        else if (src) k_walk_persistent<2, true, false><<<g, kBlock, tb, st>>>(s, a);
-        else k_walk_persistent<2, false, false><<<g, kBlock, tb, st>>>(s, a);
+        if (s.stats) {
+            if (bas) k_walk_persistent<2, true, true, true><<<g, kBlock, tb, st>>>(s, a);
+            else if (src) k_walk_persistent<2, true, false, true><<<g, kBlock, tb, st>>>(s, a);
+            else k_walk_persistent<2, false, false, true><<<g, kBlock, tb, st>>>(s, a);
+        } else {
+            if (bas) k_walk_persistent<2, true, true, false><<<g, kBlock, tb, st>>>(s, a);
+            else if (src) k_walk_persistent<2, true, false, false><<<g, kBlock, tb, st>>>(s, a);
+            else k_walk_persistent<2, false, false, false><<<g, kBlock, tb, st>>>(s, a);
+        }
     } else {
-        k_walk_persistent<3, false, false><<<g, kBlock, tb, st>>>(s, a);
+        if (s.stats) k_walk_persistent<3, false, false, true><<<g, kBlock, tb, st>>>(s, a);
+        else k_walk_persistent<3, false, false, false><<<g, kBlock, tb, st>>>(s, a);
     }
     (void)hipFreeAsync(iprim, st);
     (void)hipFreeAsync(isd, st);

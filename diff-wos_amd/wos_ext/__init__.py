@@ -270,9 +270,9 @@ class _SceneBase:
         if h is not None and h.value:
             try:
                 lib.wos_scene_destroy(h)
-            except Exception:
+            except Exception:   # interpreter shutdown: module globals already cleared
                 pass
-            self.__dict__["_h"] = _vp(None)
+            self.__dict__["_h"] = None
 
     @property
     def handle(self):
@@ -615,6 +615,6 @@ class Batch:
         if h is not None and h.value:
             try:
                 lib.wos_batch_destroy(h)
-            except Exception:
+            except Exception:   # interpreter shutdown: module globals already cleared
                 pass
-            self._h = _vp(None)
+            self.__dict__["_h"] = None

@@ -102,8 +102,15 @@ def test_custom_solver_shape_gradient_directional_fd():
 
 def test_ours3d_directional_fd_on_64_vertices():
     """Scaled-down shape-gradient workload: gradient w.r.t. vertex positions
-    (Ours3D through CustomSolver) against a central difference of the loss
-    along random directions on 64 random vertices, common seeds."""
+    (Ours3D through CustomSolver) against central differences of the loss,
+    common seeds, along two deformations: 64 random vertices pushed along
+    their normals, and a smooth anisotropic stretch of the whole mesh.
+
+    The backward pass replays the forward seed (wos/solver.py:105-110), so
+    the adjoint estimates the derivative of E[L] at the backward walk count;
+    that bias shrinks as 1/W, hence 512 backward walks.  Directions with
+    large random tangential moves fold the 0.05-edge mesh at these step sizes
+    and make the difference quotient meaningless, so they are not used."""
     from wos.scene3d import Scene3D
     from wos.solver import CustomSolver
     from wos.wos3d import WoS3DCUDA
@@ -112,28 +119,35 @@ def test_ours3d_directional_fd_on_64_vertices():
     V0 = _t(c["vertices"])
     vals = _t(c["values"])
     pts = _t(c["points"])
+    ix = _t(c["indices"])
     target = _t(c["target"].astype(np.float32))
-    W = 64
+    W = 512
     fwd = WoS3DCUDA(nwalks=W, nsteps=256, epsilon=1e-3)
     bwd = Ours3D(nwalks=W, nsteps=256, epsilon=1e-3, epsilon2=1e-3, clamping=0.1)
 
-    def loss(V, solver):
-        u = solver.walk(pts, Scene3D(V, _t(c["indices"]), vals), seed=5)
+    def loss(V, solver, seed):
+        u = solver.walk(pts, Scene3D(V, ix, vals), seed=seed)
         return ((u - target) ** 2).mean()
 
-    V = V0.clone().requires_grad_(True)
-    loss(V, CustomSolver(fwd, bwd)).backward()
-    grad = V.grad.detach()
+    grads = []
+    for seed in range(5, 9):
+        V = V0.clone().requires_grad_(True)
+        loss(V, CustomSolver(fwd, bwd), seed).backward()
+        grads.append(V.grad.detach().clone())
     rng = np.random.default_rng(9)
     sel = torch.from_numpy(rng.choice(V0.shape[0], 64, replace=False)).cuda()
-    fwd_fd = WoS3DCUDA(nwalks=1024, nsteps=256, epsilon=1e-3)
-    agree = 0
-    for trial in range(3):
-        dirn = torch.zeros_like(V0)
-        dirn[sel] = torch.from_numpy(rng.normal(size=(64, 3)).astype(np.float32)).cuda()
-        h = 2e-2
-        fd = (loss(V0 + h * dirn, fwd_fd).item() - loss(V0 - h * dirn, fwd_fd).item()) / (2 * h)
-        ad = float((grad * dirn).sum())
+    bump = torch.zeros_like(V0)
+    bump[sel] = V0[sel] / V0[sel].norm(dim=1, keepdim=True)
+    stretch = torch.zeros_like(V0)
+    stretch[:, 0] = 0.5 * V0[:, 0]
+    stretch[:, 2] = -0.3 * V0[:, 2]
+    fwd_fd = WoS3DCUDA(nwalks=4096, nsteps=256, epsilon=1e-3)
+    h = 2e-2
+    for name, dirn in (("bump64", bump), ("stretch", stretch)):
+        ads = np.array([float((g * dirn).sum()) for g in grads])
+        ad, se = ads.mean(), ads.std(ddof=1) / np.sqrt(len(ads))
+        fd = np.mean([(loss(V0 + h * dirn, fwd_fd, s).item() - loss(V0 - h * dirn, fwd_fd, s).item()) / (2 * h)
+                      for s in (5, 6)])
+        print(f"{name}: adjoint {ad:+.4e} +- {se:.1e}, central difference {fd:+.4e}")
         assert np.isfinite(ad)
-        agree += abs(ad - fd) < 0.25 * abs(fd) + 2e-5
-    assert agree >= 2
+        assert abs(ad - fd) < 0.15 * abs(fd) + 3 * se, (name, ad, se, fd)
