@@ -19,7 +19,7 @@ struct BuildInputs {
 };
 
 struct BuildOutputs {
-    float4 *nodes;          // (n_prims - 1) * (dim == 2 ? 3 : 4) float4; 2D nodes hold leaf segments inline
+    float4 *nodes;          // (n_prims - 1) * 4 float4 (2D: capsule bounds with leaf segments inline)
     float4 *leaves;         // n_prims * (dim == 2 ? 2 : 3) float4
     float *normals;         // n_vertices * dim
     int *sorted_prim;       // n_prims
@@ -32,6 +32,31 @@ struct BuildOutputs {
 };
 
 int build_scene_gpu(BuildInputs &in, hipStream_t stream, BuildOutputs &out);
+
+// Near-boundary candidate grid of a 2D scene (see grid_closest).
+struct GridSettings {
+    float cell_factor = 1.0f;   // cell side / mean segment length; <= 0 disables the grid
+    int band_cells = 16;        // cells within this many cell sides of the boundary get lists
+    int max_cand = 48;          // longer lists are dropped (those cells ask the tree)
+    long max_cells = 1L << 23;  // the cell side grows to respect this
+};
+struct GridInputs {
+    int n_prims, max_depth;
+    const float *vtx;
+    const int *idx;
+    const float4 *nodes, *leaves;
+    const int *sorted_prim;
+    float amin[2], amax[2];
+    float tol;
+};
+struct GridOutputs {
+    uint32_t *start = nullptr;   // nx * ny + 1 (hipMalloc, owned by the caller afterwards)
+    int *cand = nullptr;         // n_cand leaf indices (hipMalloc)
+    uint32_t n_cand = 0;
+    float ox = 0, oy = 0, inv_cell = 0, cell = 0;
+    int nx = 0, ny = 0;
+};
+int build_grid_2d(const GridInputs &in, const GridSettings &gs, hipStream_t stream, GridOutputs &out);
 
 // ---- solver launches (wos_kernels.hip) -------------------------------------
 struct Workspace;   // per-device scratch owned by the C ABI layer

@@ -297,43 +297,104 @@ __global__ void k_leaves(const int *sorted_prim, int n, int D, const float *vtx,
     }
 }
 
-__device__ __forceinline__ void write_child_box(float4 *nodes, int D, int parent, int slot, const float *lo,
-                                                const float *hi) {
-    if (D == 2) {
-        nodes[3 * parent + slot] = make_float4(lo[0], lo[1], hi[0], hi[1]);
-    } else {
-        float *f = reinterpret_cast<float *>(nodes + 4 * parent);
-        int o = slot * 6;
-        f[o + 0] = lo[0]; f[o + 1] = lo[1]; f[o + 2] = lo[2];
-        f[o + 3] = hi[0]; f[o + 4] = hi[1]; f[o + 5] = hi[2];
-    }
+// 3D child boxes: six floats per child in words 0..2 of the node.
+__device__ __forceinline__ void write_child_box(float4 *nodes, int node, int slot, const float *lo, const float *hi) {
+    float *f = reinterpret_cast<float *>(nodes + 4 * node);
+    int o = slot * 6;
+    f[o + 0] = lo[0]; f[o + 1] = lo[1]; f[o + 2] = lo[2];
+    f[o + 3] = hi[0]; f[o + 4] = hi[1]; f[o + 5] = hi[2];
 }
 
-__device__ __forceinline__ void read_child_box(const float4 *nodes, int D, int node, int slot, float *lo,
-                                               float *hi) {
-    if (D == 2) {
-        float4 b = nodes[3 * node + slot];
-        lo[0] = b.x; lo[1] = b.y; hi[0] = b.z; hi[1] = b.w;
-        lo[2] = hi[2] = 0.0f;
-    } else {
-        const float *f = reinterpret_cast<const float *>(nodes + 4 * node);
-        int o = slot * 6;
-        lo[0] = f[o + 0]; lo[1] = f[o + 1]; lo[2] = f[o + 2];
-        hi[0] = f[o + 3]; hi[1] = f[o + 4]; hi[2] = f[o + 5];
-    }
+__device__ __forceinline__ void read_child_box(const float4 *nodes, int node, int slot, float *lo, float *hi) {
+    const float *f = reinterpret_cast<const float *>(nodes + 4 * node);
+    int o = slot * 6;
+    lo[0] = f[o + 0]; lo[1] = f[o + 1]; lo[2] = f[o + 2];
+    hi[0] = f[o + 3]; hi[1] = f[o + 4]; hi[2] = f[o + 5];
 }
 
-// Writes the children words of every node (left, right).
-__global__ void k_node_children(const int *children, int n, int D, float4 *nodes) {
+// Writes the children word (the last of W words) of every node.
+__global__ void k_node_children(const int *children, int n, int W, float4 *nodes) {
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n - 1; i += gridDim.x * blockDim.x) {
-        int s = (D == 2) ? 3 * i + 2 : 4 * i + 3;
-        nodes[s] = make_float4(__int_as_float(children[2 * i]), __int_as_float(children[2 * i + 1]), 0.0f, 0.0f);
+        nodes[W * i + W - 1] =
+            make_float4(__int_as_float(children[2 * i]), __int_as_float(children[2 * i + 1]), 0.0f, 0.0f);
     }
 }
 
-// Bottom-up boxes: the second child to arrive at a node forms its union and
-// carries on to the parent.  Release/acquire at agent scope orders the child
-// box stores before the counter update across XCDs.
+// 2D bounding capsules: the set of points within r of the segment [a, b].
+// The union bound of two capsules takes the farthest pair of their four
+// endpoints as its axis (so endpoints are always exact vertex coordinates)
+// and the largest endpoint distance to that axis plus the endpoint's own
+// radius as its radius -- distance to a segment is convex, so the endpoints
+// bound the whole child capsule.  Computed in double and rounded up.
+__device__ double seg_dist_d(double px, double py, double ax, double ay, double bx, double by) {
+    double dx = bx - ax, dy = by - ay, qx = px - ax, qy = py - ay;
+    double dd = dx * dx + dy * dy;
+    double h = dd > 0.0 ? (qx * dx + qy * dy) / dd : 0.0;
+    h = h < 0.0 ? 0.0 : (h > 1.0 ? 1.0 : h);
+    double ex = qx - h * dx, ey = qy - h * dy;
+    return sqrt(ex * ex + ey * ey);
+}
+
+__device__ void capsule_union(const float4 c0, float r0, const float4 c1, float r1, float4 &c, float &r) {
+    const float ex[4] = {c0.x, c0.z, c1.x, c1.z}, ey[4] = {c0.y, c0.w, c1.y, c1.w};
+    const float er[4] = {r0, r0, r1, r1};
+    int bi = 0, bj = 1;
+    double best = -1.0;
+    for (int i = 0; i < 4; ++i)
+        for (int j = i + 1; j < 4; ++j) {
+            double dx = (double)ex[j] - ex[i], dy = (double)ey[j] - ey[i];
+            double d2 = dx * dx + dy * dy;
+            if (d2 > best) { best = d2; bi = i; bj = j; }
+        }
+    double rr = 0.0;
+    for (int k = 0; k < 4; ++k) {
+        double d = seg_dist_d(ex[k], ey[k], ex[bi], ey[bi], ex[bj], ey[bj]) + er[k];
+        rr = d > rr ? d : rr;
+    }
+    c = make_float4(ex[bi], ey[bi], ex[bj], ey[bj]);
+    r = __double2float_ru(rr * (1.0 + 1e-12));
+}
+
+// Bottom-up bounds: the second child to arrive at a node forms the union
+// and carries on to the parent.  Release/acquire at agent scope orders the
+// child stores before the counter update across XCDs.
+//
+// 2D build layout (kBuildWords2 words per node): words 0, 1 the capsule
+// axes {a, b} of the children, word 2 their radii, words 3, 4 their boxes
+// {lo, hi}, word 5 the children.
+constexpr int kBuildWords2 = 6;
+
+__global__ void k_refit_2d(int n, const int *children, const int *par_int, const int *par_leaf, const float *vtx,
+                           const int *idx, const int *sorted_prim, float4 *nodes, uint32_t *flags) {
+    const int W = kBuildWords2;
+    for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += gridDim.x * blockDim.x) {
+        int f = sorted_prim[j];
+        int i0 = idx[2 * f], i1 = idx[2 * f + 1];
+        float ax = vtx[2 * i0], ay = vtx[2 * i0 + 1], bx = vtx[2 * i1], by = vtx[2 * i1 + 1];
+        float4 cap = make_float4(ax, ay, bx, by);
+        float4 box = make_float4(fminf(ax, bx), fminf(ay, by), fmaxf(ax, bx), fmaxf(ay, by));
+        float r = 0.0f;
+        int child = ~j;
+        int node = par_leaf[j];
+        while (node >= 0) {
+            int slot = (children[2 * node] == child) ? 0 : 1;
+            nodes[W * node + slot] = cap;
+            reinterpret_cast<float *>(nodes + W * node + 2)[slot] = r;
+            nodes[W * node + 3 + slot] = box;
+            __threadfence();
+            uint32_t old = __hip_atomic_fetch_add(&flags[node], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+            if (old == 0u) break;
+            __threadfence();
+            float4 rw = nodes[W * node + 2];
+            capsule_union(nodes[W * node], rw.x, nodes[W * node + 1], rw.y, cap, r);
+            float4 b0 = nodes[W * node + 3], b1 = nodes[W * node + 4];
+            box = make_float4(fminf(b0.x, b1.x), fminf(b0.y, b1.y), fmaxf(b0.z, b1.z), fmaxf(b0.w, b1.w));
+            child = node;
+            node = par_int[node];
+        }
+    }
+}
+
 __global__ void k_refit(int n, int D, const int *children, const int *par_int, const int *par_leaf,
                         const float *vtx, const int *idx, const int *sorted_prim, float4 *nodes, uint32_t *flags) {
     for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += gridDim.x * blockDim.x) {
@@ -352,14 +413,14 @@ __global__ void k_refit(int n, int D, const int *children, const int *par_int, c
         int node = par_leaf[j];
         while (node >= 0) {
             int slot = (children[2 * node] == child) ? 0 : 1;
-            write_child_box(nodes, D, node, slot, lo, hi);
+            write_child_box(nodes, node, slot, lo, hi);
             __threadfence();
             uint32_t old = __hip_atomic_fetch_add(&flags[node], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
             if (old == 0u) break;
             __threadfence();
             float l0[3], h0[3], l1[3], h1[3];
-            read_child_box(nodes, D, node, 0, l0, h0);
-            read_child_box(nodes, D, node, 1, l1, h1);
+            read_child_box(nodes, node, 0, l0, h0);
+            read_child_box(nodes, node, 1, l1, h1);
             for (int k = 0; k < D; ++k) {
                 lo[k] = fminf(l0[k], l1[k]);
                 hi[k] = fmaxf(h0[k], h1[k]);
@@ -397,8 +458,7 @@ __global__ void k_scatter_pos(const int *sorted_old, int n_int, int *pos) {
 }
 
 // Copies every node to its depth-major slot, renumbering internal children.
-__global__ void k_reorder_nodes(const float4 *src, const int *pos, int n_int, int D, float4 *dst) {
-    const int W = (D == 2) ? 3 : 4;
+__global__ void k_reorder_nodes(const float4 *src, const int *pos, int n_int, int W, float4 *dst) {
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n_int; i += gridDim.x * blockDim.x) {
         int j = pos[i];
         for (int k = 0; k < W - 1; ++k) dst[W * j + k] = src[W * i + k];
@@ -410,30 +470,66 @@ __global__ void k_reorder_nodes(const float4 *src, const int *pos, int n_int, in
     }
 }
 
-// 2D: folds every leaf child into its parent node -- slot k holds the
-// child's box {lo, hi} or, for a leaf, its segment {a, b - a}; the aux word
-// holds {inv, id} per slot with inv = 1/|b - a|^2 for a leaf (id = primitive)
-// and inv = -1 for an internal child (id = node).  A traversal step then
-// needs one 48-byte fetch and no branch on the child kind.
-__global__ void k_pack_nodes_2d(const float4 *src, const float4 *leaves, int n_int, float4 *dst) {
+// 2D node layout used by the traversal (4 x float4):
+//   word k (k = 0, 1): bound of child k -- a leaf's segment {a.xy, b.xy}, or
+//                      for an internal child whichever of its capsule axis
+//                      {a.xy, b.xy} and its box {lo.xy, hi.xy} has the
+//                      smaller area
+//   word 2:            {inv0, r0, inv1, r1}: r = 0 for a leaf (inv as
+//                      computed for the leaf), r > 0 a capsule radius (inv =
+//                      1/|b - a|^2, 0 for a point), r < 0 a box
+//   word 3:            {id0, id1}: internal child index, or ~primitive
+// A leaf's distance comes from the same arithmetic as a brute-force scan, so
+// both children go through one code path and a node fetch carries everything
+// a visit needs.  Capsule radii grow by 2 tol (tol = 1e-6 max(1,
+// |coordinate|)) to cover the float evaluation of the bound.
+__global__ void k_pack_nodes_2d(const float4 *src, const float4 *leaves, int n_int, const uint32_t *meta,
+                                float4 *dst) {
+    const int W = kBuildWords2;
+    float m = 1.0f;
+    for (int k = 0; k < 2; ++k) {
+        uint32_t lo = meta[k], hi = meta[3 + k];
+        uint32_t blo = (lo & 0x80000000u) ? (lo & 0x7fffffffu) : ~lo;
+        uint32_t bhi = (hi & 0x80000000u) ? (hi & 0x7fffffffu) : ~hi;
+        m = fmaxf(m, fmaxf(fabsf(__uint_as_float(blo)), fabsf(__uint_as_float(bhi))));
+    }
+    const float slack = 2e-6f * m;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n_int; i += gridDim.x * blockDim.x) {
-        float4 c = src[3 * i + 2];
+        float4 rw = src[W * i + 2], c = src[W * i + W - 1];
         int ch[2] = {__float_as_int(c.x), __float_as_int(c.y)};
+        float r[2] = {rw.x, rw.y};
         float aux[4];
+        int ids[2];
         for (int k = 0; k < 2; ++k) {
+            float4 g = src[W * i + k];
             if (ch[k] < 0) {
-                int j = ~ch[k];
-                dst[3 * i + k] = leaves[2 * j];
-                float4 l1 = leaves[2 * j + 1];
+                float4 l1 = leaves[2 * (~ch[k]) + 1];
+                dst[4 * i + k] = g;
                 aux[2 * k] = l1.x;
-                aux[2 * k + 1] = l1.y;
-            } else {
-                dst[3 * i + k] = src[3 * i + k];
-                aux[2 * k] = -1.0f;
-                aux[2 * k + 1] = __int_as_float(ch[k]);
+                aux[2 * k + 1] = 0.0f;
+                ids[k] = ~__float_as_int(l1.y);
+                continue;
             }
+            float4 bx = src[W * i + 3 + k];
+            float bx_area = (bx.z - bx.x) * (bx.w - bx.y);
+            float rr = r[k] + slack;
+            float len = sqrtf(dot2(g.z - g.x, g.w - g.y, g.z - g.x, g.w - g.y));
+            float cap_area = fmaf(2.0f * rr, len, kPi * rr * rr);
+            if (cap_area < bx_area) {
+                float ex = g.z - g.x, ey = g.w - g.y;
+                float dd = dot2(ex, ey, ex, ey);
+                dst[4 * i + k] = g;
+                aux[2 * k] = dd > 0.0f ? 1.0f / dd : 0.0f;
+                aux[2 * k + 1] = rr;
+            } else {
+                dst[4 * i + k] = bx;
+                aux[2 * k] = 0.0f;
+                aux[2 * k + 1] = -1.0f;
+            }
+            ids[k] = ch[k];
         }
-        dst[3 * i + 2] = make_float4(aux[0], aux[1], aux[2], aux[3]);
+        dst[4 * i + 2] = make_float4(aux[0], aux[1], aux[2], aux[3]);
+        dst[4 * i + 3] = make_float4(__int_as_float(ids[0]), __int_as_float(ids[1]), 0.0f, 0.0f);
     }
 }
 
@@ -523,7 +619,7 @@ int build_scene_gpu(BuildInputs &in, hipStream_t stream, BuildOutputs &out) {
     uint32_t *dhist = nullptr, *ndepth = nullptr;
     int *dvals[2] = {nullptr, nullptr}, *pos = nullptr;
     uint32_t *dkeys = nullptr;
-    float4 *tmp_nodes = nullptr;
+    float4 *tmp_nodes = nullptr, *work_nodes = nullptr;
 
     BCHK(hipMallocAsync((void **)&meta, (14 + 64) * sizeof(uint32_t), stream));
     dhist = meta + 14;
@@ -560,16 +656,25 @@ int build_scene_gpu(BuildInputs &in, hipStream_t stream, BuildOutputs &out) {
     BCHK(hipMemcpyAsync(out.sorted_codes, keys[0], sizeof(uint32_t) * (size_t)n, hipMemcpyDeviceToDevice, stream));
     k_leaves<<<grid_for(n), kBlock, 0, stream>>>(vals[0], n, D, in.vtx, in.idx, out.leaves);
     if (n > 1) {
+        const int ni = n - 1;
+        // 2D builds capsules and boxes in a wider scratch layout and packs
+        // them at the end; 3D builds its final layout in place
+        const int W = (D == 2) ? kBuildWords2 : 4;
+        float4 *work = out.nodes;
+        if (D == 2) BCHK(hipMallocAsync((void **)&work_nodes, sizeof(float4) * W * (size_t)ni, stream));
+        if (D == 2) work = work_nodes;
         k_karras<<<grid_for(n - 1), kBlock, 0, stream>>>(keys[0], n, children, par_int, par_leaf);
-        k_node_children<<<grid_for(n - 1), kBlock, 0, stream>>>(children, n, D, out.nodes);
-        k_refit<<<grid_for(n), kBlock, 0, stream>>>(n, D, children, par_int, par_leaf, in.vtx, in.idx, vals[0],
-                                                  out.nodes, flags);
+        k_node_children<<<grid_for(n - 1), kBlock, 0, stream>>>(children, n, W, work);
+        if (D == 2)
+            k_refit_2d<<<grid_for(n), kBlock, 0, stream>>>(n, children, par_int, par_leaf, in.vtx, in.idx, vals[0],
+                                                         work, flags);
+        else
+            k_refit<<<grid_for(n), kBlock, 0, stream>>>(n, D, children, par_int, par_leaf, in.vtx, in.idx, vals[0],
+                                                      work, flags);
         k_depth<<<grid_for(n), kBlock, 0, stream>>>(n, par_int, par_leaf, meta + 13);
         // depth-major node order: the top levels become a contiguous prefix
         // (root stays 0), which the traversal can keep in LDS
-        const int ni = n - 1;
         const int nb = (ni + kRsTile - 1) / kRsTile;
-        const int W = (D == 2) ? 3 : 4;
         BCHK(hipMallocAsync((void **)&ndepth, sizeof(uint32_t) * (size_t)ni, stream));
         BCHK(hipMallocAsync((void **)&dkeys, sizeof(uint32_t) * (size_t)ni, stream));
         for (int b = 0; b < 2; ++b) BCHK(hipMallocAsync((void **)&dvals[b], sizeof(int) * (size_t)ni, stream));
@@ -584,9 +689,9 @@ int build_scene_gpu(BuildInputs &in, hipStream_t stream, BuildOutputs &out) {
         k_rs_scan<<<1, 1024, 0, stream>>>(hist2, 256L * nb);
         k_rs_scatter<<<nb, kBlock, 0, stream>>>(ndepth, dvals[0], ni, 0, nb, hist2, dkeys, dvals[1]);
         k_scatter_pos<<<grid_for(ni), kBlock, 0, stream>>>(dvals[1], ni, pos);
-        k_reorder_nodes<<<grid_for(ni), kBlock, 0, stream>>>(out.nodes, pos, ni, D, tmp_nodes);
+        k_reorder_nodes<<<grid_for(ni), kBlock, 0, stream>>>(work, pos, ni, W, tmp_nodes);
         if (D == 2)
-            k_pack_nodes_2d<<<grid_for(ni), kBlock, 0, stream>>>(tmp_nodes, out.leaves, ni, out.nodes);
+            k_pack_nodes_2d<<<grid_for(ni), kBlock, 0, stream>>>(tmp_nodes, out.leaves, ni, meta, out.nodes);
         else
             BCHK(hipMemcpyAsync(out.nodes, tmp_nodes, sizeof(float4) * W * (size_t)ni, hipMemcpyDeviceToDevice,
                                 stream));
@@ -641,6 +746,269 @@ int build_scene_gpu(BuildInputs &in, hipStream_t stream, BuildOutputs &out) {
         if (dvals[b]) (void)hipFreeAsync(dvals[b], stream);
     if (pos) (void)hipFreeAsync(pos, stream);
     if (tmp_nodes) (void)hipFreeAsync(tmp_nodes, stream);
+    if (work_nodes) (void)hipFreeAsync(work_nodes, stream);
+    return rc;
+}
+
+// ---------------------------------------------------------------------------
+// near-boundary candidate grid (2D)
+// ---------------------------------------------------------------------------
+constexpr int kGridStack = 64;   // private traversal stack of the grid builder
+constexpr int kGridBlock = 8;    // fine cells per coarse block side
+
+// Sum of segment lengths in a fixed order (one workgroup), so the cell size
+// and with it the grid layout are reproducible.
+__global__ __launch_bounds__(1024) void k_segment_length_sum(const float *vtx, const int *idx, int n,
+                                                              double *out) {
+    __shared__ double part[1024];
+    double acc = 0.0;
+    for (int f = threadIdx.x; f < n; f += 1024) {
+        int i0 = idx[2 * f], i1 = idx[2 * f + 1];
+        double dx = (double)vtx[2 * i1] - vtx[2 * i0], dy = (double)vtx[2 * i1 + 1] - vtx[2 * i0 + 1];
+        acc += sqrt(dx * dx + dy * dy);
+    }
+    part[threadIdx.x] = acc;
+    __syncthreads();
+    for (int off = 512; off > 0; off >>= 1) {
+        if ((int)threadIdx.x < off) part[threadIdx.x] += part[threadIdx.x + off];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *out = part[0];
+}
+
+__global__ void k_leaf_of_prim(const int *sorted_prim, int n, int *leaf_of_prim) {
+    for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += gridDim.x * blockDim.x)
+        leaf_of_prim[sorted_prim[j]] = j;
+}
+
+// Child k of a packed 2D node: squared distance to its segment / capsule
+// axis, and whether it is a leaf.
+__device__ __forceinline__ float child_axis_d2(float px, float py, float4 g, float inv) {
+    float h, dx, dy;
+    return seg_d2(px, py, g.x, g.y, g.z - g.x, g.w - g.y, inv, h, dx, dy);
+}
+
+// Squared distance from (px, py) to the boundary over the packed 2D nodes.
+// Pruning can only drop subtrees no closer than the current best, so the
+// result never undershoots the true distance by more than rounding.
+__device__ float grid_nearest_d2(const float4 *nodes, const float4 *leaves, int n_prims, float px, float py) {
+    float h, dx, dy;
+    if (n_prims == 1) {
+        float4 l0 = leaves[0], l1 = leaves[1];
+        return seg_d2(px, py, l0.x, l0.y, l0.z, l0.w, l1.x, h, dx, dy);
+    }
+    int stack[kGridStack];
+    float sd[kGridStack];
+    int sp = 0, node = 0;
+    float best = __builtin_inff();   // squared
+    for (;;) {
+        float4 n0 = nodes[4 * node], n1 = nodes[4 * node + 1], n2 = nodes[4 * node + 2], n3 = nodes[4 * node + 3];
+        int c0 = __float_as_int(n3.x), c1 = __float_as_int(n3.y);
+        float s0 = child_axis_d2(px, py, n0, n2.x), s1 = child_axis_d2(px, py, n1, n2.z);
+        if (c0 < 0) best = fminf(best, s0);
+        if (c1 < 0) best = fminf(best, s1);
+        // capsule lower bounds, compared in the distance domain
+        float lb0 = child_bound_2d(px, py, n0, s0, n2.y), lb1 = child_bound_2d(px, py, n1, s1, n2.w);
+        float rb = sqrtf(best);
+        bool g0 = c0 >= 0 && lb0 < rb, g1 = c1 >= 0 && lb1 < rb;
+        if (g0 && g1) {
+            bool near0 = lb0 <= lb1;
+            stack[sp] = near0 ? c1 : c0;
+            sd[sp] = near0 ? lb1 : lb0;
+            ++sp;
+            node = near0 ? c0 : c1;
+            continue;
+        }
+        if (g0 || g1) {
+            node = g0 ? c0 : c1;
+            continue;
+        }
+        bool found = false;
+        while (sp > 0) {
+            --sp;
+            if (sd[sp] < sqrtf(best)) {
+                node = stack[sp];
+                found = true;
+                break;
+            }
+        }
+        if (!found) return best;
+    }
+}
+
+// Leaves whose segment lies within r (not squared) of (px, py): counted, or
+// written to out.  Returns the count, or -1 once it exceeds cap.
+template <bool FILL>
+__device__ int grid_range(const float4 *nodes, const float4 *leaves, const int *leaf_of_prim, int n_prims,
+                          float px, float py, float r, int cap, int *out) {
+    float h, dx, dy;
+    int cnt = 0;
+    const float r2 = r * r;
+    if (n_prims == 1) {
+        float4 l0 = leaves[0], l1 = leaves[1];
+        if (seg_d2(px, py, l0.x, l0.y, l0.z, l0.w, l1.x, h, dx, dy) <= r2) {
+            if (FILL) out[0] = 0;
+            cnt = 1;
+        }
+        return cnt > cap ? -1 : cnt;
+    }
+    int stack[kGridStack];
+    int sp = 0;
+    stack[sp++] = 0;
+    while (sp > 0) {
+        int node = stack[--sp];
+        float4 n2 = nodes[4 * node + 2], n3 = nodes[4 * node + 3];
+        for (int k = 0; k < 2; ++k) {
+            float4 g = nodes[4 * node + k];
+            int id = __float_as_int(k ? n3.y : n3.x);
+            float d2 = child_axis_d2(px, py, g, k ? n2.z : n2.x);
+            if (id < 0) {
+                if (d2 <= r2) {
+                    if (cnt >= cap) return -1;
+                    if (FILL) out[cnt] = leaf_of_prim[~id];
+                    ++cnt;
+                }
+            } else if (child_bound_2d(px, py, g, d2, k ? n2.w : n2.y) <= r) {
+                stack[sp++] = id;
+            }
+        }
+    }
+    return cnt;
+}
+
+struct GridGeom {
+    float ox, oy, cell;
+    int nx, ny;
+    float band, margin;
+    int cap;
+};
+
+__device__ __forceinline__ void cell_center(const GridGeom &g, float cx, float cy, float &x, float &y) {
+    x = fmaf(cx, g.cell, g.ox);
+    y = fmaf(cy, g.cell, g.oy);
+}
+
+// Coarse blocks of kGridBlock^2 cells: a block whose centre is farther than
+// band + its half-diagonal from the boundary holds no cell of the band.
+__global__ void k_grid_blocks(const float4 *nodes, const float4 *leaves, int n_prims, GridGeom g, int nbx, int nby,
+                              uint8_t *active) {
+    const float half_diag = 0.70710678f * kGridBlock * g.cell;
+    for (long b = blockIdx.x * (long)blockDim.x + threadIdx.x; b < (long)nbx * nby; b += (long)gridDim.x * blockDim.x) {
+        int bx = (int)(b % nbx), by = (int)(b / nbx);
+        float x, y;
+        cell_center(g, bx * kGridBlock + 0.5f * kGridBlock, by * kGridBlock + 0.5f * kGridBlock, x, y);
+        float d = sqrtf(grid_nearest_d2(nodes, leaves, n_prims, x, y));
+        active[b] = d - half_diag <= g.band * 1.0001f + g.margin ? 1 : 0;
+    }
+}
+
+template <bool FILL>
+__global__ void k_grid_cells(const float4 *nodes, const float4 *leaves, const int *leaf_of_prim, int n_prims,
+                             GridGeom g, int nbx, const uint8_t *active, uint32_t *counts, int *cand) {
+    const float two_rho = 1.41421356f * g.cell;
+    for (long c = blockIdx.x * (long)blockDim.x + threadIdx.x; c < (long)g.nx * g.ny; c += (long)gridDim.x * blockDim.x) {
+        int cx = (int)(c % g.nx), cy = (int)(c / g.nx);
+        if (FILL) {
+            if (counts[c + 1] == counts[c]) continue;
+        } else if (!active[(cy / kGridBlock) * nbx + cx / kGridBlock]) {
+            counts[c] = 0;
+            continue;
+        }
+        float x, y;
+        cell_center(g, cx + 0.5f, cy + 0.5f, x, y);
+        float d = sqrtf(grid_nearest_d2(nodes, leaves, n_prims, x, y));
+        if (!FILL && d > g.band) {
+            counts[c] = 0;
+            continue;
+        }
+        float r = (d + two_rho) * 1.00001f + g.margin;
+        // the fill pass repeats the count pass exactly; its capacity is the
+        // counted length, so it cannot write past the cell's range
+        int n = grid_range<FILL>(nodes, leaves, leaf_of_prim, n_prims, x, y, r,
+                                 FILL ? (int)(counts[c + 1] - counts[c]) : g.cap, FILL ? cand + counts[c] : nullptr);
+        if (!FILL) counts[c] = n < 0 ? 0u : (uint32_t)n;
+    }
+}
+
+int build_grid_2d(const GridInputs &in, const GridSettings &gs, hipStream_t stream, GridOutputs &out) {
+    out = GridOutputs{};
+    if (in.n_prims < 1 || gs.cell_factor <= 0.0f || gs.band_cells <= 0 || gs.max_cand <= 0) return 0;
+    // the range traversal holds at most depth + 2 entries; deeper (degenerate)
+    // trees keep to the tree queries
+    if (in.max_depth + 2 > kGridStack) return 0;
+    double *dsum = nullptr;
+    int *leaf_of_prim = nullptr;
+    uint8_t *active = nullptr;
+    uint32_t *counts = nullptr;
+    int *cand = nullptr;
+    int rc = -1;
+    double sum = 0.0;
+    uint32_t total = 0;
+    GridGeom g;
+    long ncells = 0;
+    int nbx = 0, nby = 0;
+    BCHK(hipMallocAsync((void **)&dsum, sizeof(double), stream));
+    k_segment_length_sum<<<1, 1024, 0, stream>>>(in.vtx, in.idx, in.n_prims, dsum);
+    BCHK(hipMemcpyAsync(&sum, dsum, sizeof(double), hipMemcpyDeviceToHost, stream));
+    BCHK(hipStreamSynchronize(stream));
+    {
+        double cell = gs.cell_factor * sum / in.n_prims;
+        double band = gs.band_cells * cell;
+        double w = (double)in.amax[0] - in.amin[0] + 2.0 * band + 2.0 * cell;
+        double h = (double)in.amax[1] - in.amin[1] + 2.0 * band + 2.0 * cell;
+        double cells = (w / cell) * (h / cell);
+        if (cells > (double)gs.max_cells) {
+            cell *= sqrt(cells / (double)gs.max_cells);
+            band = gs.band_cells * cell;
+            w = (double)in.amax[0] - in.amin[0] + 2.0 * band + 2.0 * cell;
+            h = (double)in.amax[1] - in.amin[1] + 2.0 * band + 2.0 * cell;
+        }
+        g.cell = (float)cell;
+        g.ox = (float)(in.amin[0] - band - cell);
+        g.oy = (float)(in.amin[1] - band - cell);
+        g.nx = (int)ceil(w / cell);
+        g.ny = (int)ceil(h / cell);
+        g.band = (float)band;
+        g.margin = 8.0f * in.tol;   // rounding of cell centres, cell lookup and distances
+        g.cap = gs.max_cand;
+    }
+    ncells = (long)g.nx * g.ny;
+    nbx = (g.nx + kGridBlock - 1) / kGridBlock;
+    nby = (g.ny + kGridBlock - 1) / kGridBlock;
+    BCHK(hipMallocAsync((void **)&leaf_of_prim, sizeof(int) * (size_t)in.n_prims, stream));
+    BCHK(hipMallocAsync((void **)&active, (size_t)nbx * nby, stream));
+    BCHK(hipMalloc((void **)&counts, sizeof(uint32_t) * (size_t)(ncells + 1)));
+    k_leaf_of_prim<<<grid_for(in.n_prims), kBlock, 0, stream>>>(in.sorted_prim, in.n_prims, leaf_of_prim);
+    k_grid_blocks<<<grid_for((long)nbx * nby), kBlock, 0, stream>>>(in.nodes, in.leaves, in.n_prims, g, nbx, nby,
+                                                                      active);
+    k_fill_u32<<<1, 64, 0, stream>>>(counts + ncells, 0u, 1);
+    k_grid_cells<false><<<grid_for(ncells), kBlock, 0, stream>>>(in.nodes, in.leaves, leaf_of_prim, in.n_prims, g,
+                                                                  nbx, active, counts, nullptr);
+    k_rs_scan<<<1, 1024, 0, stream>>>(counts, ncells + 1);
+    BCHK(hipMemcpyAsync(&total, counts + ncells, sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
+    BCHK(hipStreamSynchronize(stream));
+    BCHK(hipMalloc((void **)&cand, sizeof(int) * (size_t)(total > 0 ? total : 1)));
+    if (total > 0)
+        k_grid_cells<true><<<grid_for(ncells), kBlock, 0, stream>>>(in.nodes, in.leaves, leaf_of_prim, in.n_prims,
+                                                                     g, nbx, active, counts, cand);
+    BCHK(hipGetLastError());
+    out.start = counts;
+    out.cand = cand;
+    out.n_cand = total;
+    out.ox = g.ox;
+    out.oy = g.oy;
+    out.inv_cell = 1.0f / g.cell;
+    out.cell = g.cell;
+    out.nx = g.nx;
+    out.ny = g.ny;
+    counts = nullptr;
+    cand = nullptr;
+    rc = 0;
+    (void)hipFreeAsync(dsum, stream);
+    (void)hipFreeAsync(leaf_of_prim, stream);
+    (void)hipFreeAsync(active, stream);
+    if (counts) (void)hipFree(counts);
+    if (cand) (void)hipFree(cand);
     return rc;
 }
 

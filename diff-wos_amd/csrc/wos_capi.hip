@@ -21,6 +21,7 @@ struct wos_scene {
     float tol = 0.0f;
     int max_depth = 0;
     int top_counts[17] = {0};
+    wos::GridOutputs grid;        // 2D near-boundary candidate grid (start == null: none)
 };
 
 namespace wos {
@@ -111,9 +112,16 @@ static SceneView view_of(const wos_scene *s) {
     v.stats = nullptr;
     int lv = std::min(std::max(g_tuning.top_levels, 0), 16);
     int nt = s->n_prims > 1 ? s->top_counts[lv] : 0;
-    const int node_bytes = (s->dim == 2 ? 3 : 4) * (int)sizeof(float4);
+    const int node_bytes = 4 * (int)sizeof(float4);
     while (nt > 0 && (size_t)nt * node_bytes > 32768) nt = s->top_counts[--lv];
     v.n_top = nt;
+    v.grid_start = s->grid.start;
+    v.grid_cand = s->grid.cand;
+    v.grid_ox = s->grid.ox;
+    v.grid_oy = s->grid.oy;
+    v.grid_inv_cell = s->grid.inv_cell;
+    v.grid_nx = s->grid.nx;
+    v.grid_ny = s->grid.ny;
     return v;
 }
 
@@ -154,7 +162,23 @@ static void free_scene(wos_scene *s) {
     (void)hipFree(s->sorted_codes);
     (void)hipFree(s->nodes);
     (void)hipFree(s->leaves);
+    (void)hipFree(s->grid.start);
+    (void)hipFree(s->grid.cand);
     delete s;
+}
+
+static int build_grid(wos_scene *s, const GridSettings &gs, hipStream_t st) {
+    GridInputs gi;
+    gi.n_prims = s->n_prims;
+    gi.max_depth = s->max_depth;
+    gi.vtx = s->vtx;
+    gi.idx = s->idx;
+    gi.nodes = s->nodes;
+    gi.leaves = s->leaves;
+    gi.sorted_prim = s->sorted_prim;
+    for (int k = 0; k < 2; ++k) { gi.amin[k] = s->amin[k]; gi.amax[k] = s->amax[k]; }
+    gi.tol = s->tol;
+    return build_grid_2d(gi, gs, st, s->grid);
 }
 
 static int check_source(int source_type, const float *params, int n) {
@@ -204,7 +228,7 @@ int wos_scene_create(int dim, const float *vertices, int n_vertices, const int *
     ACHK(hipMalloc((void **)&s->sorted_prim, sizeof(int) * (size_t)n_prims));
     ACHK(hipMalloc((void **)&s->sorted_codes, sizeof(uint32_t) * (size_t)n_prims));
     ACHK(hipMalloc((void **)&s->children, sizeof(int) * 2 * (size_t)nn));
-    ACHK(hipMalloc((void **)&s->nodes, sizeof(float4) * (dim == 2 ? 3 : 4) * (size_t)nn));
+    ACHK(hipMalloc((void **)&s->nodes, sizeof(float4) * 4 * (size_t)nn));
     ACHK(hipMalloc((void **)&s->leaves, sizeof(float4) * (dim == 2 ? 2 : 3) * (size_t)n_prims));
     ACHK(hipMalloc((void **)&s->src, sizeof(float) * (size_t)std::max(1, s->n_src)));
     ACHK(hipMemcpyAsync(s->vtx, vertices, vb, hipMemcpyDefault, st));
@@ -243,7 +267,35 @@ int wos_scene_create(int dim, const float *vertices, int n_vertices, const int *
     s->tol = 1e-6f * m;
     s->max_depth = bo.max_depth;
     for (int k = 0; k < 17; ++k) s->top_counts[k] = bo.top_counts[k];
+    if (dim == 2 && build_grid(s, GridSettings{}, st) != 0) {
+        free_scene(s);
+        return -1;
+    }
     *out = s;
+    return 0;
+}
+
+int wos_scene_set_grid(wos_scene *s, float cell_factor, int band_cells, int max_cand, void *stream) {
+    REQUIRE(s, "scene is NULL");
+    REQUIRE(s->dim == 2, "the candidate grid is built for 2D scenes only");
+    REQUIRE(band_cells >= 0 && max_cand >= 0, "band_cells and max_cand must be non-negative");
+    CHK(hipDeviceSynchronize());   // kernels in flight may still read the old grid
+    (void)hipFree(s->grid.start);
+    (void)hipFree(s->grid.cand);
+    s->grid = GridOutputs{};
+    GridSettings gs;
+    gs.cell_factor = cell_factor;
+    gs.band_cells = band_cells;
+    gs.max_cand = max_cand;
+    return build_grid(s, gs, S(stream));
+}
+
+int wos_scene_get_grid(const wos_scene *s, int *nx, int *ny, float *cell, unsigned int *n_cand) {
+    REQUIRE(s, "scene is NULL");
+    if (nx) *nx = s->grid.nx;
+    if (ny) *ny = s->grid.ny;
+    if (cell) *cell = s->grid.cell;
+    if (n_cand) *n_cand = s->grid.n_cand;
     return 0;
 }
 

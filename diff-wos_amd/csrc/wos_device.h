@@ -263,6 +263,13 @@ struct SceneView {
     int max_depth;
     unsigned long long *stats;  // null, or {box tests, primitive tests, queries, walk steps}
     int n_top;             // leading nodes (depth-major order) the kernel keeps in LDS
+    // near-boundary candidate grid (2D, optional): cell c lists, in
+    // grid_cand[grid_start[c] .. grid_start[c + 1]), the leaves that can be
+    // closest to any point of the cell; an empty range means "ask the tree"
+    const uint32_t *grid_start;   // grid_nx * grid_ny + 1 entries, null when absent
+    const int *grid_cand;         // leaf indices (Morton order)
+    float grid_ox, grid_oy, grid_inv_cell;
+    int grid_nx, grid_ny;
 };
 
 // Per-lane traversal stack: the first kLdsStack entries live in LDS (lane
@@ -283,7 +290,7 @@ struct TravStack {
 // the workgroup must call it before any traversal.
 __device__ __forceinline__ void load_top_nodes(const SceneView &s) {
     if (s.n_top > 0 && s.use_bvh) {
-        const int nw = s.n_top * (s.dim == 2 ? 3 : 4);
+        const int nw = s.n_top * 4;
         for (int i = threadIdx.x; i < nw; i += kBlock) g_top_nodes[i] = s.nodes[i];
         __syncthreads();
     }
@@ -370,30 +377,42 @@ __device__ __forceinline__ int closest_brute(const SceneView &s, const float *p,
 // queries stay busy; the one-shot queries loop query_step to completion.
 struct Query {
     int node, sp;
-    float best, thr;   // best squared distance so far, box pruning threshold
+    float best;        // best squared distance so far
+    float thr;         // pruning threshold: a distance in 2D (capsule bounds),
+                       // a squared distance in 3D (box bounds)
     int best_id;
     unsigned nbox, nprim;   // work counters (instrumented launches only)
 };
 
+// Pruning threshold for the current best squared distance (see
+// prune_threshold; the 2D form stays in the distance domain).
+template <int D>
+__device__ __forceinline__ float prune_for(float best_d2, float tol) {
+    if constexpr (D == 2) return fmaf(__builtin_amdgcn_sqrtf(best_d2), 1.00000095367431640625f, tol);
+    else return prune_threshold(best_d2, tol);
+}
+
 // Starts a query whose answer lies within bound_d2, optionally seeded with a
 // candidate primitive (seed_id, seed_d2) -- e.g. the previous step's closest
 // primitive -- which tightens pruning from the root.
+template <int D>
 __device__ __forceinline__ void query_begin(Query &q, float bound_d2, int seed_id, float seed_d2, float tol) {
     q.node = 0;
     q.sp = 0;
     q.best = seed_d2;
     q.best_id = seed_id;
-    q.thr = fminf(bound_d2, prune_threshold(seed_d2, tol));
+    const float b = (D == 2) ? __builtin_amdgcn_sqrtf(bound_d2) * 1.00000095367431640625f : bound_d2;
+    q.thr = fminf(b, prune_for<D>(seed_d2, tol));
     q.nbox = q.nprim = 0;
 }
 
 __device__ __forceinline__ void stack_push(Query &q, const TravStack &st, int slot, long spill_lane, int node,
-                                           float d2) {
+                                           float key) {
     if (q.sp < kLdsStack) {
         g_stack_node[q.sp * kBlock + slot] = node;
-        g_stack_d2[q.sp * kBlock + slot] = d2;
+        g_stack_d2[q.sp * kBlock + slot] = key;
     } else {
-        st.spill[(long)(q.sp - kLdsStack) * st.spill_stride + spill_lane] = make_int2(node, __float_as_int(d2));
+        st.spill[(long)(q.sp - kLdsStack) * st.spill_stride + spill_lane] = make_int2(node, __float_as_int(key));
     }
     ++q.sp;
 }
@@ -420,58 +439,67 @@ __device__ __forceinline__ bool stack_pop(Query &q, const TravStack &st, int slo
     return false;
 }
 
+// Distance lower bound of an internal child of a packed 2D node: r >= 0 is
+// a capsule (s_axis = squared distance to its axis), r < 0 a box {lo, hi}.
+__device__ __forceinline__ float child_bound_2d(float px, float py, float4 g, float s_axis, float r) {
+    const float b = box_d2_2(px, py, g);
+    return __builtin_amdgcn_sqrtf(r < 0.0f ? b : s_axis) - fmaxf(r, 0.0f);
+}
+
 // One node visit; true when the query is complete.
 //
-// 2D nodes carry their leaf children inline (3 x float4):
-//   slot k (float4): box {lo.xy, hi.xy} of an internal child, or segment
-//                    {a.xy, (b-a).xy} of a leaf child
-//   aux (float4):    {inv0, id0, inv1, id1}; inv = 1/|b-a|^2 > 0 marks a leaf
-//                    (id = primitive index), inv = -1 an internal child (id =
-//                    node index)
-// so both children are evaluated without branches and without a second load.
+// 2D nodes (4 x float4): words 0 and 1 hold the bound of each child -- a
+// leaf's segment {a.xy, b.xy}, an internal child's capsule axis {a.xy, b.xy}
+// or box {lo.xy, hi.xy} --, word 2 holds {inv0, r0, inv1, r1} (r = 0 leaf,
+// r > 0 capsule radius, r < 0 box; inv = 1/|b - a|^2) and word 3 {id0, id1}
+// (child node index, or ~primitive for a leaf).  Both children take the same
+// arithmetic: the squared distance to the segment, which for a leaf is the
+// exact primitive distance of a brute-force scan, and the child's lower
+// bound, which for a capsule is |p - axis| - r.
 template <int D, bool STATS>
 __device__ __forceinline__ bool query_step(const SceneView &s, const float *p, Query &q, const TravStack &st,
                                            int slot, long spill_lane, float tol) {
     const int node = q.node;
     if constexpr (D == 2) {
-        float4 n0, n1, n2;
+        float4 n0, n1, n2, n3;
         if (node < s.n_top) {
-            n0 = g_top_nodes[3 * node];
-            n1 = g_top_nodes[3 * node + 1];
-            n2 = g_top_nodes[3 * node + 2];
+            n0 = g_top_nodes[4 * node];
+            n1 = g_top_nodes[4 * node + 1];
+            n2 = g_top_nodes[4 * node + 2];
+            n3 = g_top_nodes[4 * node + 3];
             // the empty asm keeps the LDS and global loads apart: merged,
             // they become one generic (flat) load for both address spaces
             __asm__ volatile("" ::: "memory");
         } else {
-            n0 = s.nodes[3 * node];
-            n1 = s.nodes[3 * node + 1];
-            n2 = s.nodes[3 * node + 2];
+            n0 = s.nodes[4 * node];
+            n1 = s.nodes[4 * node + 1];
+            n2 = s.nodes[4 * node + 2];
+            n3 = s.nodes[4 * node + 3];
         }
-        const bool leaf0 = n2.x > 0.0f, leaf1 = n2.z > 0.0f;
-        const int id0 = __float_as_int(n2.y), id1 = __float_as_int(n2.w);
+        const int c0 = __float_as_int(n3.x), c1 = __float_as_int(n3.y);
+        const bool leaf0 = c0 < 0, leaf1 = c1 < 0;
         float h, dx, dy;
-        float s0 = seg_d2(p[0], p[1], n0.x, n0.y, n0.z, n0.w, n2.x, h, dx, dy);
-        float s1 = seg_d2(p[0], p[1], n1.x, n1.y, n1.z, n1.w, n2.z, h, dx, dy);
-        float d0 = leaf0 ? s0 : box_d2_2(p[0], p[1], n0);
-        float d1 = leaf1 ? s1 : box_d2_2(p[0], p[1], n1);
+        const float s0 = seg_d2(p[0], p[1], n0.x, n0.y, n0.z - n0.x, n0.w - n0.y, n2.x, h, dx, dy);
+        const float s1 = seg_d2(p[0], p[1], n1.x, n1.y, n1.z - n1.x, n1.w - n1.y, n2.z, h, dx, dy);
         if constexpr (STATS) {
             q.nbox += (leaf0 ? 0u : 1u) + (leaf1 ? 0u : 1u);
             q.nprim += (leaf0 ? 1u : 0u) + (leaf1 ? 1u : 0u);
         }
-        bool u0 = leaf0 && better(d0, id0, q.best, q.best_id);
-        if (u0) { q.best = d0; q.best_id = id0; }
-        bool u1 = leaf1 && better(d1, id1, q.best, q.best_id);
-        if (u1) { q.best = d1; q.best_id = id1; }
-        if (u0 || u1) q.thr = fminf(q.thr, prune_threshold(q.best, tol));
-        const bool g0 = !leaf0 && d0 <= q.thr, g1 = !leaf1 && d1 <= q.thr;
+        bool u0 = leaf0 && better(s0, ~c0, q.best, q.best_id);
+        if (u0) { q.best = s0; q.best_id = ~c0; }
+        bool u1 = leaf1 && better(s1, ~c1, q.best, q.best_id);
+        if (u1) { q.best = s1; q.best_id = ~c1; }
+        if (u0 || u1) q.thr = fminf(q.thr, prune_for<2>(q.best, tol));
+        const float l0 = child_bound_2d(p[0], p[1], n0, s0, n2.y), l1 = child_bound_2d(p[0], p[1], n1, s1, n2.w);
+        const bool g0 = !leaf0 && l0 <= q.thr, g1 = !leaf1 && l1 <= q.thr;
         if (g0 && g1) {
-            const bool near0 = d0 <= d1;
-            stack_push(q, st, slot, spill_lane, near0 ? id1 : id0, near0 ? d1 : d0);
-            q.node = near0 ? id0 : id1;
+            const bool near0 = l0 <= l1;
+            stack_push(q, st, slot, spill_lane, near0 ? c1 : c0, near0 ? l1 : l0);
+            q.node = near0 ? c0 : c1;
             return false;
         }
         if (g0 || g1) {
-            q.node = g0 ? id0 : id1;
+            q.node = g0 ? c0 : c1;
             return false;
         }
         return !stack_pop(q, st, slot, spill_lane);
@@ -502,7 +530,7 @@ __device__ __forceinline__ bool query_step(const SceneView &s, const float *p, Q
                 if (better(d2, prim, q.best, q.best_id)) {
                     q.best = d2;
                     q.best_id = prim;
-                    q.thr = fminf(q.thr, prune_threshold(d2, tol));
+                    q.thr = fminf(q.thr, prune_for<3>(d2, tol));
                 }
             } else goL = true;
         }
@@ -514,7 +542,7 @@ __device__ __forceinline__ bool query_step(const SceneView &s, const float *p, Q
                 if (better(d2, prim, q.best, q.best_id)) {
                     q.best = d2;
                     q.best_id = prim;
-                    q.thr = fminf(q.thr, prune_threshold(d2, tol));
+                    q.thr = fminf(q.thr, prune_for<3>(d2, tol));
                 }
             } else goR = true;
         }
@@ -544,6 +572,57 @@ __device__ __forceinline__ void query_flush_stats(const SceneView &s, const Quer
     }
 }
 
+// Closest primitive from the candidate grid: -1 when p lies outside the grid
+// or in a cell without a list.  A cell of half-diagonal rho whose centre x is
+// at distance D from the boundary lists every primitive within D + 2 rho (+ a
+// rounding margin) of x; the closest primitive of any point p of the cell is
+// within |p - x| + dist(p, boundary) <= rho + (D + rho) of x, so the
+// lexicographic minimum over the list is the global one.
+template <bool STATS>
+__device__ __forceinline__ int grid_closest(const SceneView &s, const float *p, float &best_d2) {
+    const float fx = (p[0] - s.grid_ox) * s.grid_inv_cell;
+    const float fy = (p[1] - s.grid_oy) * s.grid_inv_cell;
+    // written so that NaN coordinates fail the test
+    if (!(fx >= 0.0f && fy >= 0.0f && fx < (float)s.grid_nx && fy < (float)s.grid_ny)) return -1;
+    const int c = (int)fy * s.grid_nx + (int)fx;
+    const uint32_t b = s.grid_start[c], e = s.grid_start[c + 1];
+    if (b == e) return -1;
+    float best = __builtin_inff();
+    int best_id = 0x7fffffff;
+    // four candidates per round with every load issued before the first
+    // test, so a round costs two memory latencies instead of eight; indices
+    // past the end repeat the last candidate, which never wins twice
+    for (uint32_t j = b; j < e; j += 4) {
+        int lf[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) lf[u] = s.grid_cand[j + u < e ? j + u : e - 1];
+        float4 l0[4], l1[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            l0[u] = s.leaves[2 * lf[u]];
+            l1[u] = s.leaves[2 * lf[u] + 1];
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            float h, dx, dy;
+            const float d2 = seg_d2(p[0], p[1], l0[u].x, l0[u].y, l0[u].z, l0[u].w, l1[u].x, h, dx, dy);
+            const int id = __float_as_int(l1[u].y);
+            if (better(d2, id, best, best_id)) {
+                best = d2;
+                best_id = id;
+            }
+        }
+    }
+    if constexpr (STATS) {
+        if (s.stats) {
+            atomicAdd(&s.stats[1], (unsigned long long)(e - b));
+            atomicAdd(&s.stats[2], 1ull);
+        }
+    }
+    best_d2 = best;
+    return best_id;
+}
+
 // One-shot nearest-first query.  bound_d2: squared distance within which
 // the closest primitive is known to lie (infinity when unknown).  Returns the
 // primitive index (-1 if none lies within the bound) and its squared distance.
@@ -559,7 +638,7 @@ __device__ int closest_bvh(const SceneView &s, const float *p, float bound_d2, f
         return 0;
     }
     Query q;
-    query_begin(q, bound_d2, 0x7fffffff, __builtin_inff(), s.tol);
+    query_begin<D>(q, bound_d2, 0x7fffffff, __builtin_inff(), s.tol);
     while (!query_step<D, true>(s, p, q, st, lane_slot, spill_lane, s.tol)) {
     }
     query_flush_stats<true>(s, q);
@@ -576,6 +655,12 @@ template <int D>
 __device__ __forceinline__ int closest_prim(const SceneView &s, const float *p, float bound_d2, float &best_d2,
                                             const TravStack &st, int lane_slot, long spill_lane) {
     if (!s.use_bvh) return closest_brute<D>(s, p, best_d2);
+    if constexpr (D == 2) {
+        if (s.grid_start) {
+            int g = grid_closest<true>(s, p, best_d2);
+            if (g >= 0) return g;
+        }
+    }
     int id = closest_bvh<D>(s, p, bound_d2, best_d2, st, lane_slot, spill_lane);
     if (id < 0 && bound_d2 < __builtin_inff())
         id = closest_bvh<D>(s, p, __builtin_inff(), best_d2, st, lane_slot, spill_lane);

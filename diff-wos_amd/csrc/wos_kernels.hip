@@ -421,27 +421,27 @@ __global__ __launch_bounds__(kBlock) void k_walk_persistent(SceneView s, Persist
             }
             move_point<D>(p, d, rng);
             ++step;
-            if constexpr (STATS) {
-                atomicAdd(&s.stats[3], 1ull);
-                if (s.use_bvh) {
-                    atomicAdd(&s.stats[2], 1ull);
-                    atomicAdd(&s.stats[1], 1ull);   // the seed primitive
-                }
-            }
-            float seed_d2 = prim_d2<D>(s, prim, p);
+            if constexpr (STATS) atomicAdd(&s.stats[3], 1ull);
             if (!s.use_bvh) {
                 float d2;
                 int id = closest_brute<D>(s, p, d2);
                 q.best = d2;
                 q.best_id = id;
                 mode = kReady;
-            } else if (s.n_prims == 1) {
-                q.best = seed_d2;
-                q.best_id = prim;
-                mode = kReady;
             } else {
-                query_begin(q, bound_d2(s, 2.0f * d), prim, seed_d2, s.tol);
-                mode = kQuery;
+                if constexpr (STATS) {
+                    atomicAdd(&s.stats[2], 1ull);
+                    atomicAdd(&s.stats[1], 1ull);   // the seed primitive
+                }
+                float seed_d2 = prim_d2<D>(s, prim, p);
+                if (s.n_prims == 1) {
+                    q.best = seed_d2;
+                    q.best_id = prim;
+                    mode = kReady;
+                } else {
+                    query_begin<D>(q, bound_d2(s, 2.0f * d), prim, seed_d2, s.tol);
+                    mode = kQuery;
+                }
             }
         }
         // 4. advance every pending query by up to `window` node visits
@@ -711,7 +711,7 @@ __global__ __launch_bounds__(kBlock) void k_shape_gradient(SceneView s, GradSett
 Tuning g_tuning;
 
 static size_t top_bytes(const SceneView &s) {
-    return s.use_bvh ? (size_t)s.n_top * sizeof(float4) * (s.dim == 2 ? 3 : 4) : 0;
+    return s.use_bvh ? (size_t)s.n_top * sizeof(float4) * 4 : 0;
 }
 
 static int occupancy_grid() {

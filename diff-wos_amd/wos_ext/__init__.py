@@ -76,6 +76,8 @@ _sig("wos_scene_set_use_bvh", _vp, _i)
 _sig("wos_scene_get_normals", _vp, _vp, _vp)
 _sig("wos_scene_get_info", _vp, _vp, _vp, _vp, _vp, _vp, _vp)
 _sig("wos_scene_get_bvh", _vp, _vp, _vp, _vp, _vp)
+_sig("wos_scene_set_grid", _vp, _f, _i, _i, _vp)
+_sig("wos_scene_get_grid", _vp, _vp, _vp, _vp, _vp)
 _sig("wos_closest_point_index", _vp, _vp, _i, _vp, _vp, _vp)
 _sig("wos_sdf", _vp, _vp, _i, _vp, _vp)
 _sig("wos_closest_point", _vp, _vp, _i, ctypes.POINTER(_Record), _vp)
@@ -100,7 +102,7 @@ _sig("wos_shape_gradient", _vp, ctypes.POINTER(_GradParams), _vp, _i, _i, _vp, _
 
 EXPORTED = ("wos_last_error", "wos_scene_create", "wos_scene_destroy", "wos_scene_set_values",
             "wos_scene_set_source", "wos_scene_set_use_bvh", "wos_scene_get_normals", "wos_scene_get_info",
-            "wos_scene_get_bvh", "wos_closest_point_index", "wos_sdf", "wos_closest_point",
+            "wos_scene_get_bvh", "wos_scene_set_grid", "wos_scene_get_grid", "wos_closest_point_index", "wos_sdf", "wos_closest_point",
             "wos_largest_inscribed_ball", "wos_dirichlet", "wos_single_walk", "wos_walk_reduce",
             "wos_walk_reduce_counts", "wos_adjoint_values", "wos_shape_gradient", "wos_set_tuning",
             "wos_get_tuning", "wos_batch_create", "wos_batch_run", "wos_batch_destroy")
@@ -345,6 +347,18 @@ class _SceneBase:
     def use_bvh(self, flag):
         _check(lib.wos_scene_set_use_bvh(self.handle, int(bool(flag))))
         self.__dict__["_use_bvh"] = bool(flag)
+
+    def set_grid(self, cell_factor=1.0, band_cells=16, max_cand=48):
+        """Rebuilds the 2D near-boundary candidate grid (cell_factor <= 0
+        removes it).  Query results are identical with and without it."""
+        _check(lib.wos_scene_set_grid(self.handle, float(cell_factor), int(band_cells), int(max_cand), _stream()))
+
+    def grid_info(self):
+        nx, ny, nc = ctypes.c_int(), ctypes.c_int(), ctypes.c_uint()
+        cell = ctypes.c_float()
+        _check(lib.wos_scene_get_grid(self.handle, ctypes.byref(nx), ctypes.byref(ny), ctypes.byref(cell),
+                                      ctypes.byref(nc)))
+        return {"nx": nx.value, "ny": ny.value, "cell": cell.value, "entries": nc.value}
 
     def bvh_topology(self):
         n = self._indices.shape[0]

@@ -98,6 +98,18 @@ int wos_scene_get_info(const wos_scene *scene, float *aabb_min3, float *aabb_max
  * the structure lbvh::bvh builds at include/scene.cuh:452. */
 int wos_scene_get_bvh(const wos_scene *scene, int *sorted_prim, uint32_t *sorted_codes, int *children,
                       void *stream);
+/* Near-boundary candidate grid of a 2D scene: an acceleration structure with
+ * no counterpart in the reference, answering the same closest-primitive
+ * queries as its LBVH (include/scene.cuh:130-146) with identical results.
+ * Cells of side cell_factor x the mean segment length that lie within
+ * band_cells cells of the boundary list every segment that can be closest to
+ * a point of the cell (lists longer than max_cand are dropped); queries in
+ * other cells use the tree.  wos_scene_create builds it with (1.0, 16, 48);
+ * this call rebuilds it, or removes it when cell_factor <= 0.  Synchronises
+ * the device. */
+int wos_scene_set_grid(wos_scene *scene, float cell_factor, int band_cells, int max_cand, void *stream);
+/* Grid shape (nx = ny = 0 when there is none), cell side and list entries. */
+int wos_scene_get_grid(const wos_scene *scene, int *nx, int *ny, float *cell, unsigned int *n_cand);
 
 /* Scene.closest_point_preliminary(p) -> Int (include/scene.cuh:497-537).
  * Ties: equal squared distance keeps the smallest primitive index.
