@@ -363,6 +363,9 @@ __device__ void capsule_union(const float4 c0, float r0, const float4 c1, float 
 // axes {a, b} of the children, word 2 their radii, words 3, 4 their boxes
 // {lo, hi}, word 5 the children.
 constexpr int kBuildWords2 = 6;
+#ifndef WOS_BOUND_R0
+#define WOS_BOUND_R0 0.3f
+#endif
 
 __global__ void k_refit_2d(int n, const int *children, const int *par_int, const int *par_leaf, const float *vtx,
                            const int *idx, const int *sorted_prim, float4 *nodes, uint32_t *flags) {
@@ -510,12 +513,16 @@ __global__ void k_pack_nodes_2d(const float4 *src, const float4 *leaves, int n_i
                 ids[k] = ~__float_as_int(l1.y);
                 continue;
             }
+            // keep the bound whose dilation by a ball of radius r0 (a
+            // typical query distance) has the smaller area
             float4 bx = src[W * i + 3 + k];
-            float bx_area = (bx.z - bx.x) * (bx.w - bx.y);
+            float bw = bx.z - bx.x, bh = bx.w - bx.y;
             float rr = r[k] + slack;
             float len = sqrtf(dot2(g.z - g.x, g.w - g.y, g.z - g.x, g.w - g.y));
-            float cap_area = fmaf(2.0f * rr, len, kPi * rr * rr);
-            if (cap_area < bx_area) {
+            const float r0 = WOS_BOUND_R0 * m;
+            float bx_cost = fmaf(bw, bh, 2.0f * (bw + bh) * r0);
+            float cap_cost = fmaf(2.0f * rr, len, kPi * rr * rr) + (2.0f * len + 2.0f * kPi * rr) * r0;
+            if (cap_cost < bx_cost) {
                 float ex = g.z - g.x, ey = g.w - g.y;
                 float dd = dot2(ex, ey, ex, ey);
                 dst[4 * i + k] = g;
