@@ -58,6 +58,16 @@ struct GridOutputs {
 };
 int build_grid_2d(const GridInputs &in, const GridSettings &gs, hipStream_t stream, GridOutputs &out);
 
+// Start-node grid of a 2D scene (see start_node_of): `resolution` cells along
+// the longer side of the bounding box; 0 builds none.
+constexpr int kStartGridResolution = 512;
+struct StartGridOutputs {
+    int *start = nullptr;   // nx * ny node indices (hipMalloc, owned by the caller afterwards)
+    float ox = 0, oy = 0, inv_cell = 0;
+    int nx = 0, ny = 0;
+};
+int build_start_grid_2d(const GridInputs &in, int resolution, hipStream_t stream, StartGridOutputs &out);
+
 // ---- solver launches (wos_kernels.hip) -------------------------------------
 struct Workspace;   // per-device scratch owned by the C ABI layer
 
@@ -82,7 +92,7 @@ struct Tuning {
     int chunk = 128;        // walks a wave claims per queue visit
     int wgs_per_cu = 8;     // persistent workgroups per CU
     int top_levels = 6;     // BVH levels kept in LDS
-    int window = 16;        // node visits per lane between refills of the walk kernel
+    int window = 8;         // node visits per lane between refills of the walk kernel
 };
 extern Tuning g_tuning;
 

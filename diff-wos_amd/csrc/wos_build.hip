@@ -363,9 +363,10 @@ __device__ void capsule_union(const float4 c0, float r0, const float4 c1, float 
 // axes {a, b} of the children, word 2 their radii, words 3, 4 their boxes
 // {lo, hi}, word 5 the children.
 constexpr int kBuildWords2 = 6;
-#ifndef WOS_BOUND_R0
-#define WOS_BOUND_R0 0.3f
-#endif
+// Query radius (x scene scale) at which the 2D packer compares the dilated
+// areas of a child's capsule and box; walk queries are dominated by steps far
+// from the boundary, and larger values measured marginally faster.
+constexpr float kBoundR0 = 0.3f;
 
 __global__ void k_refit_2d(int n, const int *children, const int *par_int, const int *par_leaf, const float *vtx,
                            const int *idx, const int *sorted_prim, float4 *nodes, uint32_t *flags) {
@@ -519,7 +520,7 @@ __global__ void k_pack_nodes_2d(const float4 *src, const float4 *leaves, int n_i
             float bw = bx.z - bx.x, bh = bx.w - bx.y;
             float rr = r[k] + slack;
             float len = sqrtf(dot2(g.z - g.x, g.w - g.y, g.z - g.x, g.w - g.y));
-            const float r0 = WOS_BOUND_R0 * m;
+            const float r0 = kBoundR0 * m;
             float bx_cost = fmaf(bw, bh, 2.0f * (bw + bh) * r0);
             float cap_cost = fmaf(2.0f * rr, len, kPi * rr * rr) + (2.0f * len + 2.0f * kPi * rr) * r0;
             if (cap_cost < bx_cost) {
@@ -1017,6 +1018,74 @@ int build_grid_2d(const GridInputs &in, const GridSettings &gs, hipStream_t stre
     if (counts) (void)hipFree(counts);
     if (cand) (void)hipFree(cand);
     return rc;
+}
+
+// ---------------------------------------------------------------------------
+// start-node grid (2D)
+// ---------------------------------------------------------------------------
+// For every cell, the deepest node N such that every primitive that can be
+// closest to a point of the cell lies in N's subtree.  For a cell with centre
+// x, half-diagonal rho and boundary distance D at x, those primitives lie
+// within R = D + 2 rho of x (as for the candidate grid); descending from the
+// root while exactly one child's bound comes within R of x keeps them all,
+// so a query from any point of the cell may start its traversal at N.
+__global__ void k_start_cells_2d(const float4 *nodes, const float4 *leaves, int n_prims, GridGeom g, int *start) {
+    const float two_rho = 1.41421356f * g.cell;
+    for (long c = blockIdx.x * (long)blockDim.x + threadIdx.x; c < (long)g.nx * g.ny; c += (long)gridDim.x * blockDim.x) {
+        float x, y;
+        cell_center(g, (float)(c % g.nx) + 0.5f, (float)(c / g.nx) + 0.5f, x, y);
+        int node = 0;
+        if (n_prims > 1) {
+            const float R = (sqrtf(grid_nearest_d2(nodes, leaves, n_prims, x, y)) + two_rho) * 1.00001f + g.margin;
+            for (;;) {
+                const float4 *nd = nodes + 4 * node;
+                float4 n0 = nd[0], n1 = nd[1], n2 = nd[2], n3 = nd[3];
+                int c0 = __float_as_int(n3.x), c1 = __float_as_int(n3.y);
+                float s0 = child_axis_d2(x, y, n0, n2.x), s1 = child_axis_d2(x, y, n1, n2.z);
+                float lb0 = c0 < 0 ? sqrtf(s0) : child_bound_2d(x, y, n0, s0, n2.y);
+                float lb1 = c1 < 0 ? sqrtf(s1) : child_bound_2d(x, y, n1, s1, n2.w);
+                bool p0 = lb0 <= R, p1 = lb1 <= R;
+                if (p0 == p1) break;
+                int next = p0 ? c0 : c1;
+                if (next < 0) break;
+                node = next;
+            }
+        }
+        start[c] = node;
+    }
+}
+
+int build_start_grid_2d(const GridInputs &in, int resolution, hipStream_t stream, StartGridOutputs &out) {
+    out = StartGridOutputs{};
+    if (in.n_prims < 2 || resolution <= 0) return 0;
+    GridGeom g;
+    double w = (double)in.amax[0] - in.amin[0], h = (double)in.amax[1] - in.amin[1];
+    double cell = (w > h ? w : h) / resolution;
+    if (!(cell > 0.0)) return 0;
+    g.cell = (float)cell;
+    g.ox = (float)(in.amin[0] - cell);
+    g.oy = (float)(in.amin[1] - cell);
+    g.nx = (int)ceil(w / cell) + 2;
+    g.ny = (int)ceil(h / cell) + 2;
+    g.band = 0.0f;
+    g.margin = 8.0f * in.tol;
+    g.cap = 0;
+    int *start = nullptr;
+    BCHK(hipMalloc((void **)&start, sizeof(int) * (size_t)g.nx * g.ny));
+    k_start_cells_2d<<<grid_for((long)g.nx * g.ny), kBlock, 0, stream>>>(in.nodes, in.leaves, in.n_prims, g, start);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) {
+        (void)hipFree(start);
+        set_error("start grid: %s", hipGetErrorString(e));
+        return -1;
+    }
+    out.start = start;
+    out.ox = g.ox;
+    out.oy = g.oy;
+    out.inv_cell = 1.0f / g.cell;
+    out.nx = g.nx;
+    out.ny = g.ny;
+    return 0;
 }
 
 }  // namespace wos

@@ -22,6 +22,7 @@ struct wos_scene {
     int max_depth = 0;
     int top_counts[17] = {0};
     wos::GridOutputs grid;        // 2D near-boundary candidate grid (start == null: none)
+    wos::StartGridOutputs sgrid;  // 2D start-node grid (start == null: none)
 };
 
 namespace wos {
@@ -122,6 +123,12 @@ static SceneView view_of(const wos_scene *s) {
     v.grid_inv_cell = s->grid.inv_cell;
     v.grid_nx = s->grid.nx;
     v.grid_ny = s->grid.ny;
+    v.start_node = s->sgrid.start;
+    v.st_ox = s->sgrid.ox;
+    v.st_oy = s->sgrid.oy;
+    v.st_inv_cell = s->sgrid.inv_cell;
+    v.st_nx = s->sgrid.nx;
+    v.st_ny = s->sgrid.ny;
     return v;
 }
 
@@ -164,10 +171,11 @@ static void free_scene(wos_scene *s) {
     (void)hipFree(s->leaves);
     (void)hipFree(s->grid.start);
     (void)hipFree(s->grid.cand);
+    (void)hipFree(s->sgrid.start);
     delete s;
 }
 
-static int build_grid(wos_scene *s, const GridSettings &gs, hipStream_t st) {
+static GridInputs grid_inputs(const wos_scene *s) {
     GridInputs gi;
     gi.n_prims = s->n_prims;
     gi.max_depth = s->max_depth;
@@ -178,7 +186,11 @@ static int build_grid(wos_scene *s, const GridSettings &gs, hipStream_t st) {
     gi.sorted_prim = s->sorted_prim;
     for (int k = 0; k < 2; ++k) { gi.amin[k] = s->amin[k]; gi.amax[k] = s->amax[k]; }
     gi.tol = s->tol;
-    return build_grid_2d(gi, gs, st, s->grid);
+    return gi;
+}
+
+static int build_grid(wos_scene *s, const GridSettings &gs, hipStream_t st) {
+    return build_grid_2d(grid_inputs(s), gs, st, s->grid);
 }
 
 static int check_source(int source_type, const float *params, int n) {
@@ -267,7 +279,8 @@ int wos_scene_create(int dim, const float *vertices, int n_vertices, const int *
     s->tol = 1e-6f * m;
     s->max_depth = bo.max_depth;
     for (int k = 0; k < 17; ++k) s->top_counts[k] = bo.top_counts[k];
-    if (dim == 2 && build_grid(s, GridSettings{}, st) != 0) {
+    if (dim == 2 && (build_grid(s, GridSettings{}, st) != 0 ||
+                     build_start_grid_2d(grid_inputs(s), kStartGridResolution, st, s->sgrid) != 0)) {
         free_scene(s);
         return -1;
     }
@@ -288,6 +301,23 @@ int wos_scene_set_grid(wos_scene *s, float cell_factor, int band_cells, int max_
     gs.band_cells = band_cells;
     gs.max_cand = max_cand;
     return build_grid(s, gs, S(stream));
+}
+
+int wos_scene_set_start_grid(wos_scene *s, int resolution, void *stream) {
+    REQUIRE(s, "scene is NULL");
+    REQUIRE(s->dim == 2, "the start-node grid is built for 2D scenes only");
+    REQUIRE(resolution >= 0 && resolution <= 8192, "resolution must be in [0, 8192] (got %d)", resolution);
+    CHK(hipDeviceSynchronize());   // kernels in flight may still read the old grid
+    (void)hipFree(s->sgrid.start);
+    s->sgrid = StartGridOutputs{};
+    return build_start_grid_2d(grid_inputs(s), resolution, S(stream), s->sgrid);
+}
+
+int wos_scene_get_start_grid(const wos_scene *s, int *nx, int *ny) {
+    REQUIRE(s, "scene is NULL");
+    if (nx) *nx = s->sgrid.nx;
+    if (ny) *ny = s->sgrid.ny;
+    return 0;
 }
 
 int wos_scene_get_grid(const wos_scene *s, int *nx, int *ny, float *cell, unsigned int *n_cand) {

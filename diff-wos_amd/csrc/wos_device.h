@@ -270,6 +270,11 @@ struct SceneView {
     const int *grid_cand;         // leaf indices (Morton order)
     float grid_ox, grid_oy, grid_inv_cell;
     int grid_nx, grid_ny;
+    // start-node grid (2D, optional): the node a query from a point of the
+    // cell may start at (every primitive that can be closest lies below it)
+    const int *start_node;        // st_nx * st_ny, null when absent
+    float st_ox, st_oy, st_inv_cell;
+    int st_nx, st_ny;
 };
 
 // Per-lane traversal stack: the first kLdsStack entries live in LDS (lane
@@ -392,12 +397,24 @@ __device__ __forceinline__ float prune_for(float best_d2, float tol) {
     else return prune_threshold(best_d2, tol);
 }
 
+// Node a closest-primitive query from p may start at: the cell's entry of
+// the start-node grid, or the root.
+__device__ __forceinline__ int start_node_of(const SceneView &s, const float *p) {
+    if (!s.start_node) return 0;
+    const float fx = (p[0] - s.st_ox) * s.st_inv_cell;
+    const float fy = (p[1] - s.st_oy) * s.st_inv_cell;
+    // written so that NaN coordinates fail the test
+    if (!(fx >= 0.0f && fy >= 0.0f && fx < (float)s.st_nx && fy < (float)s.st_ny)) return 0;
+    return s.start_node[(int)fy * s.st_nx + (int)fx];
+}
+
 // Starts a query whose answer lies within bound_d2, optionally seeded with a
 // candidate primitive (seed_id, seed_d2) -- e.g. the previous step's closest
 // primitive -- which tightens pruning from the root.
 template <int D>
-__device__ __forceinline__ void query_begin(Query &q, float bound_d2, int seed_id, float seed_d2, float tol) {
-    q.node = 0;
+__device__ __forceinline__ void query_begin(Query &q, float bound_d2, int seed_id, float seed_d2, float tol,
+                                            int start = 0) {
+    q.node = start;
     q.sp = 0;
     q.best = seed_d2;
     q.best_id = seed_id;
@@ -638,7 +655,7 @@ __device__ int closest_bvh(const SceneView &s, const float *p, float bound_d2, f
         return 0;
     }
     Query q;
-    query_begin<D>(q, bound_d2, 0x7fffffff, __builtin_inff(), s.tol);
+    query_begin<D>(q, bound_d2, 0x7fffffff, __builtin_inff(), s.tol, D == 2 ? start_node_of(s, p) : 0);
     while (!query_step<D, true>(s, p, q, st, lane_slot, spill_lane, s.tol)) {
     }
     query_flush_stats<true>(s, q);

@@ -388,21 +388,41 @@ __global__ __launch_bounds__(kBlock) void k_walk_persistent(SceneView s, Persist
         if (exhausted && __ballot(mode != kIdle) == 0) break;
 
         // 3. a new step: source sample, jump to the sphere, begin the query
-        //    within 2d of the new point, seeded with the previous primitive
+        //    within 2d of the new point, seeded with the previous primitive.
+        //    The random numbers are drawn in stream order (source sample,
+        //    then direction); the loads for the new point are issued before
+        //    the source arithmetic at the old point so that they overlap it.
         if (mode == kStart) {
+            float su0 = 0.0f, su1 = 0.0f;
             if (SRC) {
-                if constexpr (BASIS) {
-                    float u0 = rng.next_1d(), u1 = rng.next_1d();
-                    float sn, cs;
-                    sincos_turns(u0, sn, cs);
-                    float r = sqrtf(u1);
-                    float dyx = (r * cs) * d, dyy = (r * sn) * d;
-                    float area = kPi * (d * d);
-                    float pdf = 1.0f / area;
-                    float rr = fmaxf(sqrtf(dot2(dyx, dyy, dyx, dyy)), 1e-4f);
-                    float G = log_poly(d / rr) * kInv2Pi;
-                    if (pdf > 1e-5f) {
-                        float yx = p[0] + dyx, yy = p[1] + dyy;
+                su0 = rng.next_1d();
+                su1 = rng.next_1d();
+            }
+            const float px0 = p[0], py0 = p[1];
+            move_point<D>(p, d, rng);
+            ++step;
+            if constexpr (STATS) atomicAdd(&s.stats[3], 1ull);
+            int i0 = 0, i1 = 0, start = 0;
+            if (s.use_bvh) {
+                if constexpr (D == 2) {
+                    i0 = s.idx[2 * prim];
+                    i1 = s.idx[2 * prim + 1];
+                    start = start_node_of(s, p);
+                }
+            }
+            if (SRC) {
+                // include/wos.cuh:115-139: uniform point in the ball, G_2D / pdf
+                float sn, cs;
+                sincos_turns(su0, sn, cs);
+                float r = sqrtf(su1);
+                float dyx = (r * cs) * d, dyy = (r * sn) * d;
+                float area = kPi * (d * d);
+                float pdf = 1.0f / area;
+                float rr = fmaxf(sqrtf(dot2(dyx, dyy, dyx, dyy)), 1e-4f);
+                float G = log_poly(d / rr) * kInv2Pi;
+                if (pdf > 1e-5f) {
+                    float yx = px0 + dyx, yy = py0 + dyy;
+                    if constexpr (BASIS) {
                         float f = 0.0f;
 #pragma unroll
                         for (int j = 0; j < kMaxBasis; ++j) {
@@ -414,14 +434,11 @@ __global__ __launch_bounds__(kBlock) void k_walk_persistent(SceneView s, Persist
                             }
                         }
                         contrib = fmaf(f * G, area, contrib);
+                    } else {
+                        contrib = fmaf(source_at(s, yx, yy) * G, area, contrib);
                     }
-                } else {
-                    source_step<D>(s, p, d, rng, contrib);
                 }
             }
-            move_point<D>(p, d, rng);
-            ++step;
-            if constexpr (STATS) atomicAdd(&s.stats[3], 1ull);
             if (!s.use_bvh) {
                 float d2;
                 int id = closest_brute<D>(s, p, d2);
@@ -433,13 +450,22 @@ __global__ __launch_bounds__(kBlock) void k_walk_persistent(SceneView s, Persist
                     atomicAdd(&s.stats[2], 1ull);
                     atomicAdd(&s.stats[1], 1ull);   // the seed primitive
                 }
-                float seed_d2 = prim_d2<D>(s, prim, p);
+                float seed_d2;
+                if constexpr (D == 2) {
+                    float ax = s.vtx[2 * i0], ay = s.vtx[2 * i0 + 1];
+                    float bax = s.vtx[2 * i1] - ax, bay = s.vtx[2 * i1 + 1] - ay;
+                    float inv = 1.0f / dot2(bax, bay, bax, bay);
+                    float h, dx, dy;
+                    seed_d2 = seg_d2(p[0], p[1], ax, ay, bax, bay, inv, h, dx, dy);
+                } else {
+                    seed_d2 = prim_d2<D>(s, prim, p);
+                }
                 if (s.n_prims == 1) {
                     q.best = seed_d2;
                     q.best_id = prim;
                     mode = kReady;
                 } else {
-                    query_begin<D>(q, bound_d2(s, 2.0f * d), prim, seed_d2, s.tol);
+                    query_begin<D>(q, bound_d2(s, 2.0f * d), prim, seed_d2, s.tol, start);
                     mode = kQuery;
                 }
             }

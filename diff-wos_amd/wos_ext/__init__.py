@@ -78,6 +78,8 @@ _sig("wos_scene_get_info", _vp, _vp, _vp, _vp, _vp, _vp, _vp)
 _sig("wos_scene_get_bvh", _vp, _vp, _vp, _vp, _vp)
 _sig("wos_scene_set_grid", _vp, _f, _i, _i, _vp)
 _sig("wos_scene_get_grid", _vp, _vp, _vp, _vp, _vp)
+_sig("wos_scene_set_start_grid", _vp, _i, _vp)
+_sig("wos_scene_get_start_grid", _vp, _vp, _vp)
 _sig("wos_closest_point_index", _vp, _vp, _i, _vp, _vp, _vp)
 _sig("wos_sdf", _vp, _vp, _i, _vp, _vp)
 _sig("wos_closest_point", _vp, _vp, _i, ctypes.POINTER(_Record), _vp)
@@ -102,7 +104,8 @@ _sig("wos_shape_gradient", _vp, ctypes.POINTER(_GradParams), _vp, _i, _i, _vp, _
 
 EXPORTED = ("wos_last_error", "wos_scene_create", "wos_scene_destroy", "wos_scene_set_values",
             "wos_scene_set_source", "wos_scene_set_use_bvh", "wos_scene_get_normals", "wos_scene_get_info",
-            "wos_scene_get_bvh", "wos_scene_set_grid", "wos_scene_get_grid", "wos_closest_point_index", "wos_sdf", "wos_closest_point",
+            "wos_scene_get_bvh", "wos_scene_set_grid", "wos_scene_get_grid",
+            "wos_scene_set_start_grid", "wos_scene_get_start_grid", "wos_closest_point_index", "wos_sdf", "wos_closest_point",
             "wos_largest_inscribed_ball", "wos_dirichlet", "wos_single_walk", "wos_walk_reduce",
             "wos_walk_reduce_counts", "wos_adjoint_values", "wos_shape_gradient", "wos_set_tuning",
             "wos_get_tuning", "wos_batch_create", "wos_batch_run", "wos_batch_destroy")
@@ -359,6 +362,16 @@ class _SceneBase:
         _check(lib.wos_scene_get_grid(self.handle, ctypes.byref(nx), ctypes.byref(ny), ctypes.byref(cell),
                                       ctypes.byref(nc)))
         return {"nx": nx.value, "ny": ny.value, "cell": cell.value, "entries": nc.value}
+
+    def set_start_grid(self, resolution=512):
+        """Rebuilds the 2D start-node grid (resolution 0 removes it).  Query
+        results are identical with and without it."""
+        _check(lib.wos_scene_set_start_grid(self.handle, int(resolution), _stream()))
+
+    def start_grid_info(self):
+        nx, ny = ctypes.c_int(), ctypes.c_int()
+        _check(lib.wos_scene_get_start_grid(self.handle, ctypes.byref(nx), ctypes.byref(ny)))
+        return {"nx": nx.value, "ny": ny.value}
 
     def bvh_topology(self):
         n = self._indices.shape[0]

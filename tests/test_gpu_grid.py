@@ -1,7 +1,8 @@
-"""GPU parity of the 2D near-boundary candidate grid: closest primitive and
-squared distance bit-exact against the oracle's in-order brute-force scan for
-points at distances 1e-7 .. 1e-1 on both sides of the boundary, under several
-grid shapes, and walks identical with and without the grid."""
+"""GPU parity of the 2D query accelerators (near-boundary candidate grid,
+start-node grid): closest primitive and squared distance bit-exact against
+the oracle's in-order brute-force scan for points at distances 1e-7 .. 1e-1
+on both sides of the boundary, under several grid shapes, and walks
+identical with and without them."""
 import numpy as np
 import pytest
 import torch
@@ -58,6 +59,27 @@ def test_grid_closest_bitexact(kind, n, grid):
     np.testing.assert_array_equal(d2.cpu().numpy(), d_ref)
 
 
+@pytest.mark.parametrize("kind,n", [("circle", 1024), ("clover", 4096), ("star", 2048), ("star", 7)])
+@pytest.mark.parametrize("res", [0, 16, 512, 3000])
+def test_start_grid_closest_bitexact(kind, n, res):
+    import wos_ext
+    v, ix = _polyline(kind, n)
+    vals = C.g_circle(v).astype(np.float32)
+    sc = wos_ext.Scene(torch.from_numpy(v).cuda(), torch.from_numpy(ix).cuda(), torch.from_numpy(vals).cuda())
+    sc.set_grid(0.0)           # isolate the start-node grid
+    sc.set_start_grid(res)
+    info = sc.start_grid_info()
+    assert (info["nx"] > 0) == (res > 0), info
+    rng = np.random.default_rng(n + res)
+    p = np.concatenate([_near_boundary_points(v, ix, rng, 20000),
+                        rng.uniform(-1.6, 1.6, size=(4000, 2)).astype(np.float32),
+                        rng.uniform(-40, 40, size=(500, 2)).astype(np.float32)])
+    idx, d2 = sc.closest_point_index_d2(torch.from_numpy(p).cuda())
+    i_ref, d_ref = O.Scene(v, ix, vals).closest_index(p)
+    np.testing.assert_array_equal(idx.cpu().numpy(), i_ref)
+    np.testing.assert_array_equal(d2.cpu().numpy(), d_ref)
+
+
 def test_grid_removal():
     import wos_ext
     v, ix = C.circle_polyline(256)
@@ -79,5 +101,8 @@ def test_walks_identical_with_and_without_grid():
     a = w.walk_reduce(pts, sc, seed=3, records=True, basis=True)
     sc.set_grid(0.0)
     b = w.walk_reduce(pts, sc, seed=3, records=True, basis=True)
+    sc.set_start_grid(0)
+    c0 = w.walk_reduce(pts, sc, seed=3, records=True, basis=True)
     for k in ("mean", "walk_value", "rec_prim", "rec_t", "basis"):
         np.testing.assert_array_equal(a[k].cpu().numpy(), b[k].cpu().numpy(), err_msg=k)
+        np.testing.assert_array_equal(a[k].cpu().numpy(), c0[k].cpu().numpy(), err_msg=k)
