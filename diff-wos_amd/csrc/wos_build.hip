@@ -399,25 +399,34 @@ __global__ void k_refit_2d(int n, const int *children, const int *par_int, const
     }
 }
 
-__global__ void k_refit(int n, int D, const int *children, const int *par_int, const int *par_leaf,
-                        const float *vtx, const int *idx, const int *sorted_prim, float4 *nodes, uint32_t *flags) {
+// 3D build layout (4 words per node): both child boxes in words 0..2, the
+// children in word 3.  Alongside, nsum[2 node + slot] accumulates the
+// area-weighted normal sum of each child's triangles (the slab direction).
+__global__ void k_refit_3d(int n, const int *children, const int *par_int, const int *par_leaf, const float *vtx,
+                           const int *idx, const int *sorted_prim, float4 *nodes, float4 *nsum, uint32_t *flags) {
     for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += gridDim.x * blockDim.x) {
         int f = sorted_prim[j];
+        float v[3][3];
         float lo[3] = {__builtin_inff(), __builtin_inff(), __builtin_inff()};
         float hi[3] = {-__builtin_inff(), -__builtin_inff(), -__builtin_inff()};
-        for (int c = 0; c < D; ++c) {
-            int v = idx[(long)f * D + c];
-            for (int k = 0; k < D; ++k) {
-                float x = vtx[(long)v * D + k];
-                lo[k] = fminf(lo[k], x);
-                hi[k] = fmaxf(hi[k], x);
+        for (int c = 0; c < 3; ++c) {
+            int vi = idx[3L * f + c];
+            for (int k = 0; k < 3; ++k) {
+                v[c][k] = vtx[3L * vi + k];
+                lo[k] = fminf(lo[k], v[c][k]);
+                hi[k] = fmaxf(hi[k], v[c][k]);
             }
         }
+        float e1[3] = {v[1][0] - v[0][0], v[1][1] - v[0][1], v[1][2] - v[0][2]};
+        float e2[3] = {v[2][0] - v[0][0], v[2][1] - v[0][1], v[2][2] - v[0][2]};
+        float4 S = make_float4(0.5f * (e1[1] * e2[2] - e1[2] * e2[1]), 0.5f * (e1[2] * e2[0] - e1[0] * e2[2]),
+                               0.5f * (e1[0] * e2[1] - e1[1] * e2[0]), 0.0f);
         int child = ~j;
         int node = par_leaf[j];
         while (node >= 0) {
             int slot = (children[2 * node] == child) ? 0 : 1;
             write_child_box(nodes, node, slot, lo, hi);
+            nsum[2 * node + slot] = S;
             __threadfence();
             uint32_t old = __hip_atomic_fetch_add(&flags[node], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
             if (old == 0u) break;
@@ -425,9 +434,73 @@ __global__ void k_refit(int n, int D, const int *children, const int *par_int, c
             float l0[3], h0[3], l1[3], h1[3];
             read_child_box(nodes, node, 0, l0, h0);
             read_child_box(nodes, node, 1, l1, h1);
-            for (int k = 0; k < D; ++k) {
+            for (int k = 0; k < 3; ++k) {
                 lo[k] = fminf(l0[k], l1[k]);
                 hi[k] = fmaxf(h0[k], h1[k]);
+            }
+            float4 s0 = nsum[2 * node], s1 = nsum[2 * node + 1];
+            S = make_float4(s0.x + s1.x, s0.y + s1.y, s0.z + s1.z, 0.0f);
+            child = node;
+            node = par_int[node];
+        }
+    }
+}
+
+// Unit slab direction of a normal sum (false when the sum vanishes).
+__device__ __forceinline__ bool slab_normal(float4 S, float *n) {
+    float l = sqrtf(fmaf(S.x, S.x, fmaf(S.y, S.y, S.z * S.z)));
+    if (!(l > 0.0f)) return false;
+    float il = 1.0f / l;
+    n[0] = S.x * il;
+    n[1] = S.y * il;
+    n[2] = S.z * il;
+    return true;
+}
+
+// Order-preserving float <-> uint32 map for atomic min / max.
+__device__ __forceinline__ uint32_t f2ord(float f) {
+    uint32_t u = __float_as_uint(f);
+    return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+__device__ __forceinline__ float ord2f(uint32_t u) {
+    return __uint_as_float((u & 0x80000000u) ? (u & 0x7fffffffu) : ~u);
+}
+
+// Empty ranges: min = +max ordinal, max = 0.
+__global__ void k_fill_slab_range(uint32_t *mm, long n_slots) {
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n_slots; i += (long)gridDim.x * blockDim.x) {
+        mm[2 * i] = 0xffffffffu;
+        mm[2 * i + 1] = 0u;
+    }
+}
+
+// Extent of every child's triangles along its slab direction: each leaf
+// walks up its ancestors and folds its three vertices into the range of the
+// child slot it came through (mm[2 (2 node + slot)] = min, [+1] = max).
+__global__ void k_slab_extent(int n, const int *children, const int *par_int, const int *par_leaf,
+                              const float *vtx, const int *idx, const int *sorted_prim, const float4 *nsum,
+                              uint32_t *mm) {
+    for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += gridDim.x * blockDim.x) {
+        int f = sorted_prim[j];
+        float v[3][3];
+        for (int c = 0; c < 3; ++c) {
+            int vi = idx[3L * f + c];
+            for (int k = 0; k < 3; ++k) v[c][k] = vtx[3L * vi + k];
+        }
+        int child = ~j;
+        int node = par_leaf[j];
+        while (node >= 0) {
+            int slot = (children[2 * node] == child) ? 0 : 1;
+            float nn[3];
+            if (slab_normal(nsum[2 * node + slot], nn)) {
+                float lo = __builtin_inff(), hi = -__builtin_inff();
+                for (int c = 0; c < 3; ++c) {
+                    float t = fmaf(nn[0], v[c][0], fmaf(nn[1], v[c][1], nn[2] * v[c][2]));
+                    lo = fminf(lo, t);
+                    hi = fmaxf(hi, t);
+                }
+                atomicMin(&mm[2 * (2 * node + slot)], f2ord(lo));
+                atomicMax(&mm[2 * (2 * node + slot) + 1], f2ord(hi));
             }
             child = node;
             node = par_int[node];
@@ -541,6 +614,57 @@ __global__ void k_pack_nodes_2d(const float4 *src, const float4 *leaves, int n_i
     }
 }
 
+// 3D node layout used by the traversal (kNodeWords3 float4), one block of
+// three words per child:
+//   internal child: {lo.xyz, hi.x}, {hi.yz, n.xy}, {n.z, c, h, id}: its box
+//                   and its slab |n.x - c| <= h (n the unit area-weighted
+//                   normal of its triangles); the lower bound of a visit is
+//                   max(box distance, |n.p - c| - h)
+//   leaf child:     {a.xyz, b.x}, {b.yz, c.xy}, {c.z, 0, 0, ~primitive}
+// Slab half-widths grow by 4e-6 max(1, |coordinate|) to cover the float
+// evaluation of n.x at build and query time.
+__global__ void k_pack_nodes_3d(const float4 *src, const int *old_of_new, const float4 *nsum, const uint32_t *mm,
+                                const float4 *leaves, int n_int, const uint32_t *meta, float4 *dst) {
+    const int V = kNodeWords3;
+    float m = 1.0f;
+    for (int k = 0; k < 3; ++k) {
+        uint32_t lo = meta[k], hi = meta[3 + k];
+        uint32_t blo = (lo & 0x80000000u) ? (lo & 0x7fffffffu) : ~lo;
+        uint32_t bhi = (hi & 0x80000000u) ? (hi & 0x7fffffffu) : ~hi;
+        m = fmaxf(m, fmaxf(fabsf(__uint_as_float(blo)), fabsf(__uint_as_float(bhi))));
+    }
+    const float slack = 4e-6f * m;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n_int; i += gridDim.x * blockDim.x) {
+        const int old = old_of_new[i];
+        const float *bx = reinterpret_cast<const float *>(src + 4 * i);
+        float4 c = src[4 * i + 3];
+        int ch[2] = {__float_as_int(c.x), __float_as_int(c.y)};
+        for (int k = 0; k < 2; ++k) {
+            float4 *o = dst + V * i + 3 * k;
+            if (ch[k] < 0) {
+                int j = ~ch[k];
+                float4 l0 = leaves[3 * j], l1 = leaves[3 * j + 1], l2 = leaves[3 * j + 2];
+                o[0] = l0;
+                o[1] = l1;
+                o[2] = make_float4(l2.x, 0.0f, 0.0f, __int_as_float(~__float_as_int(l2.y)));
+                continue;
+            }
+            const float *b = bx + 6 * k;
+            float nn[3] = {0.0f, 0.0f, 0.0f}, cc = 0.0f, hh = 3.0e38f;
+            if (slab_normal(nsum[2 * old + k], nn)) {
+                float lo = ord2f(mm[2 * (2 * old + k)]), hi = ord2f(mm[2 * (2 * old + k) + 1]);
+                cc = 0.5f * (lo + hi);
+                hh = 0.5f * (hi - lo) + slack;
+            } else {
+                nn[0] = nn[1] = nn[2] = 0.0f;
+            }
+            o[0] = make_float4(b[0], b[1], b[2], b[3]);
+            o[1] = make_float4(b[4], b[5], nn[0], nn[1]);
+            o[2] = make_float4(nn[2], cc, hh, __int_as_float(ch[k]));
+        }
+    }
+}
+
 // 2D vertex normals: sum of the unit normals (dy, -dx) of adjacent segments
 // (include/scene.cuh:369-380).  Two contributions per vertex sum exactly in
 // either order.
@@ -627,7 +751,8 @@ int build_scene_gpu(BuildInputs &in, hipStream_t stream, BuildOutputs &out) {
     uint32_t *dhist = nullptr, *ndepth = nullptr;
     int *dvals[2] = {nullptr, nullptr}, *pos = nullptr;
     uint32_t *dkeys = nullptr;
-    float4 *tmp_nodes = nullptr, *work_nodes = nullptr;
+    float4 *tmp_nodes = nullptr, *work_nodes = nullptr, *nsum = nullptr;
+    uint32_t *slab_mm = nullptr;
 
     BCHK(hipMallocAsync((void **)&meta, (14 + 64) * sizeof(uint32_t), stream));
     dhist = meta + 14;
@@ -668,17 +793,22 @@ int build_scene_gpu(BuildInputs &in, hipStream_t stream, BuildOutputs &out) {
         // 2D builds capsules and boxes in a wider scratch layout and packs
         // them at the end; 3D builds its final layout in place
         const int W = (D == 2) ? kBuildWords2 : 4;
-        float4 *work = out.nodes;
-        if (D == 2) BCHK(hipMallocAsync((void **)&work_nodes, sizeof(float4) * W * (size_t)ni, stream));
-        if (D == 2) work = work_nodes;
+        BCHK(hipMallocAsync((void **)&work_nodes, sizeof(float4) * W * (size_t)ni, stream));
+        float4 *work = work_nodes;
         k_karras<<<grid_for(n - 1), kBlock, 0, stream>>>(keys[0], n, children, par_int, par_leaf);
         k_node_children<<<grid_for(n - 1), kBlock, 0, stream>>>(children, n, W, work);
-        if (D == 2)
+        if (D == 2) {
             k_refit_2d<<<grid_for(n), kBlock, 0, stream>>>(n, children, par_int, par_leaf, in.vtx, in.idx, vals[0],
                                                          work, flags);
-        else
-            k_refit<<<grid_for(n), kBlock, 0, stream>>>(n, D, children, par_int, par_leaf, in.vtx, in.idx, vals[0],
-                                                      work, flags);
+        } else {
+            BCHK(hipMallocAsync((void **)&nsum, sizeof(float4) * 2 * (size_t)ni, stream));
+            BCHK(hipMallocAsync((void **)&slab_mm, sizeof(uint32_t) * 4 * (size_t)ni, stream));
+            k_refit_3d<<<grid_for(n), kBlock, 0, stream>>>(n, children, par_int, par_leaf, in.vtx, in.idx, vals[0],
+                                                         work, nsum, flags);
+            k_fill_slab_range<<<grid_for(2L * ni), kBlock, 0, stream>>>(slab_mm, 2L * ni);
+            k_slab_extent<<<grid_for(n), kBlock, 0, stream>>>(n, children, par_int, par_leaf, in.vtx, in.idx,
+                                                            vals[0], nsum, slab_mm);
+        }
         k_depth<<<grid_for(n), kBlock, 0, stream>>>(n, par_int, par_leaf, meta + 13);
         // depth-major node order: the top levels become a contiguous prefix
         // (root stays 0), which the traversal can keep in LDS
@@ -701,8 +831,8 @@ int build_scene_gpu(BuildInputs &in, hipStream_t stream, BuildOutputs &out) {
         if (D == 2)
             k_pack_nodes_2d<<<grid_for(ni), kBlock, 0, stream>>>(tmp_nodes, out.leaves, ni, meta, out.nodes);
         else
-            BCHK(hipMemcpyAsync(out.nodes, tmp_nodes, sizeof(float4) * W * (size_t)ni, hipMemcpyDeviceToDevice,
-                                stream));
+            k_pack_nodes_3d<<<grid_for(ni), kBlock, 0, stream>>>(tmp_nodes, dvals[1], nsum, slab_mm, out.leaves, ni,
+                                                               meta, out.nodes);
         (void)hipFreeAsync(hist2, stream);
         if (out.children) BCHK(hipMemcpyAsync(out.children, children, sizeof(int) * 2 * (size_t)(n - 1),
                                               hipMemcpyDeviceToDevice, stream));
@@ -755,6 +885,8 @@ int build_scene_gpu(BuildInputs &in, hipStream_t stream, BuildOutputs &out) {
     if (pos) (void)hipFreeAsync(pos, stream);
     if (tmp_nodes) (void)hipFreeAsync(tmp_nodes, stream);
     if (work_nodes) (void)hipFreeAsync(work_nodes, stream);
+    if (nsum) (void)hipFreeAsync(nsum, stream);
+    if (slab_mm) (void)hipFreeAsync(slab_mm, stream);
     return rc;
 }
 

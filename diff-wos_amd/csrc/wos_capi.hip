@@ -113,7 +113,7 @@ static SceneView view_of(const wos_scene *s) {
     v.stats = nullptr;
     int lv = std::min(std::max(g_tuning.top_levels, 0), 16);
     int nt = s->n_prims > 1 ? s->top_counts[lv] : 0;
-    const int node_bytes = 4 * (int)sizeof(float4);
+    const int node_bytes = node_words(s->dim) * (int)sizeof(float4);
     while (nt > 0 && (size_t)nt * node_bytes > 32768) nt = s->top_counts[--lv];
     v.n_top = nt;
     v.grid_start = s->grid.start;
@@ -240,7 +240,7 @@ int wos_scene_create(int dim, const float *vertices, int n_vertices, const int *
     ACHK(hipMalloc((void **)&s->sorted_prim, sizeof(int) * (size_t)n_prims));
     ACHK(hipMalloc((void **)&s->sorted_codes, sizeof(uint32_t) * (size_t)n_prims));
     ACHK(hipMalloc((void **)&s->children, sizeof(int) * 2 * (size_t)nn));
-    ACHK(hipMalloc((void **)&s->nodes, sizeof(float4) * 4 * (size_t)nn));
+    ACHK(hipMalloc((void **)&s->nodes, sizeof(float4) * node_words(dim) * (size_t)nn));
     ACHK(hipMalloc((void **)&s->leaves, sizeof(float4) * (dim == 2 ? 2 : 3) * (size_t)n_prims));
     ACHK(hipMalloc((void **)&s->src, sizeof(float) * (size_t)std::max(1, s->n_src)));
     ACHK(hipMemcpyAsync(s->vtx, vertices, vb, hipMemcpyDefault, st));

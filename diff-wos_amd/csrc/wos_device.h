@@ -17,6 +17,9 @@ constexpr float kPi = 3.14159265358979323846f;
 constexpr float kInv2Pi = 0.15915494309189533577f;
 constexpr int kBlock = 256;          // threads per workgroup (4 wave64)
 constexpr int kLdsStack = 8;         // traversal stack entries kept in LDS per lane
+constexpr int kNodeWords2 = 4;       // float4 words per 2D internal node
+constexpr int kNodeWords3 = 6;       // float4 words per 3D internal node
+__host__ __device__ constexpr int node_words(int dim) { return dim == 2 ? kNodeWords2 : kNodeWords3; }
 
 // ---------------------------------------------------------------------------
 // random numbers
@@ -295,7 +298,7 @@ struct TravStack {
 // the workgroup must call it before any traversal.
 __device__ __forceinline__ void load_top_nodes(const SceneView &s) {
     if (s.n_top > 0 && s.use_bvh) {
-        const int nw = s.n_top * 4;
+        const int nw = s.n_top * node_words(s.dim);
         for (int i = threadIdx.x; i < nw; i += kBlock) g_top_nodes[i] = s.nodes[i];
         __syncthreads();
     }
@@ -322,14 +325,6 @@ __device__ __forceinline__ bool better(float d2, int id, float best_d2, int best
     return d2 < best_d2 || (d2 == best_d2 && id < best_id);
 }
 
-// Prune threshold for boxes: (best + tol)^2, which absorbs the rounding gap
-// between a box distance and the distance of a primitive on its face.
-__device__ __forceinline__ float prune_threshold(float best_d2, float tol) {
-    // hardware square root (about 1 ulp) inflated by 2^-20: still an upper bound
-    float d = fmaf(__builtin_amdgcn_sqrtf(best_d2), 1.00000095367431640625f, tol);
-    return d * d;
-}
-
 // Squared distance from p to primitive f computed from the scene arrays
 // (identical to the leaf test of the same primitive).
 template <int D>
@@ -349,15 +344,6 @@ __device__ __forceinline__ float prim_d2(const SceneView &s, int f, const float 
         float q[3], bary[2];
         return tri_closest(a, b, c, p, q, bary);
     }
-}
-
-// 3D leaf: triangle a, b, c and the primitive index, in Morton order.
-__device__ __forceinline__ float leaf_d2_3(const SceneView &s, int j, const float *p, int &prim) {
-    float4 l0 = s.leaves[3 * j], l1 = s.leaves[3 * j + 1], l2 = s.leaves[3 * j + 2];
-    prim = __float_as_int(l2.y);
-    float a[3] = {l0.x, l0.y, l0.z}, b[3] = {l0.w, l1.x, l1.y}, c[3] = {l1.z, l1.w, l2.x};
-    float q[3], bary[2];
-    return tri_closest(a, b, c, p, q, bary);
 }
 
 // Closest primitive by brute force in index order (include/scene.cuh:102-128).
@@ -383,18 +369,17 @@ __device__ __forceinline__ int closest_brute(const SceneView &s, const float *p,
 struct Query {
     int node, sp;
     float best;        // best squared distance so far
-    float thr;         // pruning threshold: a distance in 2D (capsule bounds),
-                       // a squared distance in 3D (box bounds)
+    float thr;         // pruning threshold (a distance): children whose lower
+                       // bound exceeds it cannot hold a closer primitive
     int best_id;
     unsigned nbox, nprim;   // work counters (instrumented launches only)
 };
 
-// Pruning threshold for the current best squared distance (see
-// prune_threshold; the 2D form stays in the distance domain).
+// Pruning threshold for the current best squared distance: sqrt(best) + tol,
+// with the hardware square root (about 1 ulp) inflated by 2^-20.
 template <int D>
 __device__ __forceinline__ float prune_for(float best_d2, float tol) {
-    if constexpr (D == 2) return fmaf(__builtin_amdgcn_sqrtf(best_d2), 1.00000095367431640625f, tol);
-    else return prune_threshold(best_d2, tol);
+    return fmaf(__builtin_amdgcn_sqrtf(best_d2), 1.00000095367431640625f, tol);
 }
 
 // Node a closest-primitive query from p may start at: the cell's entry of
@@ -418,7 +403,7 @@ __device__ __forceinline__ void query_begin(Query &q, float bound_d2, int seed_i
     q.sp = 0;
     q.best = seed_d2;
     q.best_id = seed_id;
-    const float b = (D == 2) ? __builtin_amdgcn_sqrtf(bound_d2) * 1.00000095367431640625f : bound_d2;
+    const float b = __builtin_amdgcn_sqrtf(bound_d2) * 1.00000095367431640625f;
     q.thr = fminf(b, prune_for<D>(seed_d2, tol));
     q.nbox = q.nprim = 0;
 }
@@ -521,57 +506,50 @@ __device__ __forceinline__ bool query_step(const SceneView &s, const float *p, Q
         }
         return !stack_pop(q, st, slot, spill_lane);
     } else {
-        float4 n0, n1, n2, c;
+        constexpr int W = kNodeWords3;
+        float4 w[6];
         if (node < s.n_top) {
-            n0 = g_top_nodes[4 * node];
-            n1 = g_top_nodes[4 * node + 1];
-            n2 = g_top_nodes[4 * node + 2];
-            c = g_top_nodes[4 * node + 3];
+#pragma unroll
+            for (int k = 0; k < 6; ++k) w[k] = g_top_nodes[W * node + k];
             __asm__ volatile("" ::: "memory");
         } else {
-            n0 = s.nodes[4 * node];
-            n1 = s.nodes[4 * node + 1];
-            n2 = s.nodes[4 * node + 2];
-            c = s.nodes[4 * node + 3];
+#pragma unroll
+            for (int k = 0; k < 6; ++k) w[k] = s.nodes[W * node + k];
         }
-        float dL = box_d2_3(p, n0.x, n0.y, n0.z, n0.w, n1.x, n1.y);
-        float dR = box_d2_3(p, n1.z, n1.w, n2.x, n2.y, n2.z, n2.w);
-        int cL = __float_as_int(c.x), cR = __float_as_int(c.y);
-        if constexpr (STATS) q.nbox += 2u;
-        bool goL = false, goR = false;
-        if (dL <= q.thr) {
-            if (cL < 0) {
-                int prim;
-                float d2 = leaf_d2_3(s, ~cL, p, prim);
+        const int c0 = __float_as_int(w[2].w), c1 = __float_as_int(w[5].w);
+        float lb[2];
+#pragma unroll
+        for (int k = 0; k < 2; ++k) {
+            const float4 x0 = w[3 * k], x1 = w[3 * k + 1], x2 = w[3 * k + 2];
+            const int ck = k ? c1 : c0;
+            if (ck < 0) {
+                // leaf: the triangle, exactly as the brute-force scan tests it
+                float a[3] = {x0.x, x0.y, x0.z}, b[3] = {x0.w, x1.x, x1.y}, c[3] = {x1.z, x1.w, x2.x};
+                float qq[3], bary[2];
+                const float d2 = tri_closest(a, b, c, p, qq, bary);
                 if constexpr (STATS) q.nprim += 1u;
-                if (better(d2, prim, q.best, q.best_id)) {
+                if (better(d2, ~ck, q.best, q.best_id)) {
                     q.best = d2;
-                    q.best_id = prim;
+                    q.best_id = ~ck;
                     q.thr = fminf(q.thr, prune_for<3>(d2, tol));
                 }
-            } else goL = true;
+                lb[k] = __builtin_inff();
+            } else {
+                if constexpr (STATS) q.nbox += 1u;
+                const float bd = box_d2_3(p, x0.x, x0.y, x0.z, x0.w, x1.x, x1.y);
+                const float sd = fabsf(fmaf(x1.z, p[0], fmaf(x1.w, p[1], fmaf(x2.x, p[2], -x2.y)))) - x2.z;
+                lb[k] = fmaxf(__builtin_amdgcn_sqrtf(bd), sd);
+            }
         }
-        if (dR <= q.thr) {
-            if (cR < 0) {
-                int prim;
-                float d2 = leaf_d2_3(s, ~cR, p, prim);
-                if constexpr (STATS) q.nprim += 1u;
-                if (better(d2, prim, q.best, q.best_id)) {
-                    q.best = d2;
-                    q.best_id = prim;
-                    q.thr = fminf(q.thr, prune_for<3>(d2, tol));
-                }
-            } else goR = true;
-        }
-        goL = goL && dL <= q.thr;
-        if (goL && goR) {
-            const bool nearL = dL <= dR;
-            stack_push(q, st, slot, spill_lane, nearL ? cR : cL, nearL ? dR : dL);
-            q.node = nearL ? cL : cR;
+        const bool g0 = c0 >= 0 && lb[0] <= q.thr, g1 = c1 >= 0 && lb[1] <= q.thr;
+        if (g0 && g1) {
+            const bool near0 = lb[0] <= lb[1];
+            stack_push(q, st, slot, spill_lane, near0 ? c1 : c0, near0 ? lb[1] : lb[0]);
+            q.node = near0 ? c0 : c1;
             return false;
         }
-        if (goL || goR) {
-            q.node = goL ? cL : cR;
+        if (g0 || g1) {
+            q.node = g0 ? c0 : c1;
             return false;
         }
         return !stack_pop(q, st, slot, spill_lane);

@@ -737,7 +737,7 @@ __global__ __launch_bounds__(kBlock) void k_shape_gradient(SceneView s, GradSett
 Tuning g_tuning;
 
 static size_t top_bytes(const SceneView &s) {
-    return s.use_bvh ? (size_t)s.n_top * sizeof(float4) * 4 : 0;
+    return s.use_bvh ? (size_t)s.n_top * sizeof(float4) * node_words(s.dim) : 0;
 }
 
 static int occupancy_grid() {
