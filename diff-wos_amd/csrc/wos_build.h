@@ -58,15 +58,23 @@ struct GridOutputs {
 };
 int build_grid_2d(const GridInputs &in, const GridSettings &gs, hipStream_t stream, GridOutputs &out);
 
-// Start-node grid of a 2D scene (see start_node_of): `resolution` cells along
-// the longer side of the bounding box; 0 builds none.
-constexpr int kStartGridResolution = 512;
+// Start-node grid (see start_node_of): `resolution` cells along the longest
+// side of the bounding box; 0 builds none.
+constexpr int kStartGridResolution2 = 512;
+constexpr int kStartGridResolution3 = 96;
 struct StartGridOutputs {
-    int *start = nullptr;   // nx * ny node indices (hipMalloc, owned by the caller afterwards)
-    float ox = 0, oy = 0, inv_cell = 0;
-    int nx = 0, ny = 0;
+    int *start = nullptr;   // nx * ny * nz node indices (hipMalloc, owned by the caller afterwards)
+    float ox = 0, oy = 0, oz = 0, inv_cell = 0;
+    int nx = 0, ny = 0, nz = 0;
 };
 int build_start_grid_2d(const GridInputs &in, int resolution, hipStream_t stream, StartGridOutputs &out);
+struct GridInputs3 {
+    int n_prims, max_depth;
+    const float4 *nodes, *leaves;
+    float amin[3], amax[3];
+    float tol;
+};
+int build_start_grid_3d(const GridInputs3 &in, int resolution, hipStream_t stream, StartGridOutputs &out);
 
 // ---- solver launches (wos_kernels.hip) -------------------------------------
 struct Workspace;   // per-device scratch owned by the C ABI layer

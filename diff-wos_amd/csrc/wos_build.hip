@@ -1217,6 +1217,151 @@ int build_start_grid_2d(const GridInputs &in, int resolution, hipStream_t stream
     out.inv_cell = 1.0f / g.cell;
     out.nx = g.nx;
     out.ny = g.ny;
+    out.nz = 1;
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// start-node grid (3D)
+// ---------------------------------------------------------------------------
+// Lower bound of an internal child block of a packed 3D node (see
+// k_pack_nodes_3d) and the triangle distance of a leaf block.
+__device__ __forceinline__ float child_bound_3d(const float *p, float4 x0, float4 x1, float4 x2) {
+    float bd = box_d2_3(p, x0.x, x0.y, x0.z, x0.w, x1.x, x1.y);
+    float sd = fabsf(fmaf(x1.z, p[0], fmaf(x1.w, p[1], fmaf(x2.x, p[2], -x2.y)))) - x2.z;
+    return fmaxf(sqrtf(bd), sd);
+}
+__device__ __forceinline__ float leaf_block_d2(const float *p, float4 x0, float4 x1, float4 x2) {
+    float a[3] = {x0.x, x0.y, x0.z}, b[3] = {x0.w, x1.x, x1.y}, c[3] = {x1.z, x1.w, x2.x};
+    float q[3], bary[2];
+    return tri_closest(a, b, c, p, q, bary);
+}
+
+// Squared distance from p to the boundary over the packed 3D nodes.
+__device__ float nearest_d2_3d(const float4 *nodes, const float4 *leaves, int n_prims, const float *p) {
+    if (n_prims == 1) {
+        float4 l2 = leaves[2];
+        return leaf_block_d2(p, leaves[0], leaves[1], make_float4(l2.x, 0.0f, 0.0f, 0.0f));
+    }
+    int stack[kGridStack];
+    float sd[kGridStack];
+    int sp = 0, node = 0;
+    float best = __builtin_inff();   // squared
+    for (;;) {
+        const float4 *nd = nodes + kNodeWords3 * node;
+        float lb[2];
+        int ch[2];
+        for (int k = 0; k < 2; ++k) {
+            float4 x0 = nd[3 * k], x1 = nd[3 * k + 1], x2 = nd[3 * k + 2];
+            ch[k] = __float_as_int(x2.w);
+            if (ch[k] < 0) {
+                best = fminf(best, leaf_block_d2(p, x0, x1, x2));
+                lb[k] = __builtin_inff();
+            } else {
+                lb[k] = child_bound_3d(p, x0, x1, x2);
+            }
+        }
+        float rb = sqrtf(best);
+        bool g0 = ch[0] >= 0 && lb[0] < rb, g1 = ch[1] >= 0 && lb[1] < rb;
+        if (g0 && g1) {
+            bool near0 = lb[0] <= lb[1];
+            stack[sp] = near0 ? ch[1] : ch[0];
+            sd[sp] = near0 ? lb[1] : lb[0];
+            ++sp;
+            node = near0 ? ch[0] : ch[1];
+            continue;
+        }
+        if (g0 || g1) {
+            node = g0 ? ch[0] : ch[1];
+            continue;
+        }
+        bool found = false;
+        while (sp > 0) {
+            --sp;
+            if (sd[sp] < sqrtf(best)) {
+                node = stack[sp];
+                found = true;
+                break;
+            }
+        }
+        if (!found) return best;
+    }
+}
+
+struct GridGeom3 {
+    float ox, oy, oz, cell, margin;
+    int nx, ny, nz;
+};
+
+// As k_start_cells_2d, for cubic cells (half-diagonal sqrt(3)/2 cell).
+__global__ void k_start_cells_3d(const float4 *nodes, const float4 *leaves, int n_prims, GridGeom3 g, int *start) {
+    const float two_rho = 1.7320508f * g.cell;
+    const long nxy = (long)g.nx * g.ny;
+    for (long c = blockIdx.x * (long)blockDim.x + threadIdx.x; c < nxy * g.nz; c += (long)gridDim.x * blockDim.x) {
+        long cz = c / nxy, r = c % nxy;
+        float p[3] = {fmaf((float)(r % g.nx) + 0.5f, g.cell, g.ox), fmaf((float)(r / g.nx) + 0.5f, g.cell, g.oy),
+                      fmaf((float)cz + 0.5f, g.cell, g.oz)};
+        int node = 0;
+        if (n_prims > 1) {
+            const float R = (sqrtf(nearest_d2_3d(nodes, leaves, n_prims, p)) + two_rho) * 1.00001f + g.margin;
+            for (;;) {
+                const float4 *nd = nodes + kNodeWords3 * node;
+                bool pass[2];
+                int ch[2];
+                for (int k = 0; k < 2; ++k) {
+                    float4 x0 = nd[3 * k], x1 = nd[3 * k + 1], x2 = nd[3 * k + 2];
+                    ch[k] = __float_as_int(x2.w);
+                    float lb = ch[k] < 0 ? sqrtf(leaf_block_d2(p, x0, x1, x2)) : child_bound_3d(p, x0, x1, x2);
+                    pass[k] = lb <= R;
+                }
+                if (pass[0] == pass[1]) break;
+                int next = pass[0] ? ch[0] : ch[1];
+                if (next < 0) break;
+                node = next;
+            }
+        }
+        start[c] = node;
+    }
+}
+
+int build_start_grid_3d(const GridInputs3 &in, int resolution, hipStream_t stream, StartGridOutputs &out) {
+    out = StartGridOutputs{};
+    if (in.n_prims < 2 || resolution <= 0) return 0;
+    if (in.max_depth + 2 > kGridStack) return 0;
+    GridGeom3 g;
+    double ext[3], mx = 0.0;
+    for (int k = 0; k < 3; ++k) {
+        ext[k] = (double)in.amax[k] - in.amin[k];
+        mx = ext[k] > mx ? ext[k] : mx;
+    }
+    double cell = mx / resolution;
+    if (!(cell > 0.0)) return 0;
+    g.cell = (float)cell;
+    g.ox = (float)(in.amin[0] - cell);
+    g.oy = (float)(in.amin[1] - cell);
+    g.oz = (float)(in.amin[2] - cell);
+    g.nx = (int)ceil(ext[0] / cell) + 2;
+    g.ny = (int)ceil(ext[1] / cell) + 2;
+    g.nz = (int)ceil(ext[2] / cell) + 2;
+    g.margin = 8.0f * in.tol;
+    int *start = nullptr;
+    const long ncells = (long)g.nx * g.ny * g.nz;
+    BCHK(hipMalloc((void **)&start, sizeof(int) * (size_t)ncells));
+    k_start_cells_3d<<<grid_for(ncells), kBlock, 0, stream>>>(in.nodes, in.leaves, in.n_prims, g, start);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) {
+        (void)hipFree(start);
+        set_error("start grid: %s", hipGetErrorString(e));
+        return -1;
+    }
+    out.start = start;
+    out.ox = g.ox;
+    out.oy = g.oy;
+    out.oz = g.oz;
+    out.inv_cell = 1.0f / g.cell;
+    out.nx = g.nx;
+    out.ny = g.ny;
+    out.nz = g.nz;
     return 0;
 }
 

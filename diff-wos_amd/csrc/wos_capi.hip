@@ -126,9 +126,11 @@ static SceneView view_of(const wos_scene *s) {
     v.start_node = s->sgrid.start;
     v.st_ox = s->sgrid.ox;
     v.st_oy = s->sgrid.oy;
+    v.st_oz = s->sgrid.oz;
     v.st_inv_cell = s->sgrid.inv_cell;
     v.st_nx = s->sgrid.nx;
     v.st_ny = s->sgrid.ny;
+    v.st_nz = s->sgrid.nz;
     return v;
 }
 
@@ -191,6 +193,18 @@ static GridInputs grid_inputs(const wos_scene *s) {
 
 static int build_grid(wos_scene *s, const GridSettings &gs, hipStream_t st) {
     return build_grid_2d(grid_inputs(s), gs, st, s->grid);
+}
+
+static int build_start_grid(wos_scene *s, int resolution, hipStream_t st) {
+    if (s->dim == 2) return build_start_grid_2d(grid_inputs(s), resolution, st, s->sgrid);
+    GridInputs3 gi;
+    gi.n_prims = s->n_prims;
+    gi.max_depth = s->max_depth;
+    gi.nodes = s->nodes;
+    gi.leaves = s->leaves;
+    for (int k = 0; k < 3; ++k) { gi.amin[k] = s->amin[k]; gi.amax[k] = s->amax[k]; }
+    gi.tol = s->tol;
+    return build_start_grid_3d(gi, resolution, st, s->sgrid);
 }
 
 static int check_source(int source_type, const float *params, int n) {
@@ -279,8 +293,8 @@ int wos_scene_create(int dim, const float *vertices, int n_vertices, const int *
     s->tol = 1e-6f * m;
     s->max_depth = bo.max_depth;
     for (int k = 0; k < 17; ++k) s->top_counts[k] = bo.top_counts[k];
-    if (dim == 2 && (build_grid(s, GridSettings{}, st) != 0 ||
-                     build_start_grid_2d(grid_inputs(s), kStartGridResolution, st, s->sgrid) != 0)) {
+    if (build_start_grid(s, dim == 2 ? kStartGridResolution2 : kStartGridResolution3, st) != 0 ||
+        (dim == 2 && build_grid(s, GridSettings{}, st) != 0)) {
         free_scene(s);
         return -1;
     }
@@ -305,18 +319,19 @@ int wos_scene_set_grid(wos_scene *s, float cell_factor, int band_cells, int max_
 
 int wos_scene_set_start_grid(wos_scene *s, int resolution, void *stream) {
     REQUIRE(s, "scene is NULL");
-    REQUIRE(s->dim == 2, "the start-node grid is built for 2D scenes only");
-    REQUIRE(resolution >= 0 && resolution <= 8192, "resolution must be in [0, 8192] (got %d)", resolution);
+    const int cap = s->dim == 2 ? 8192 : 512;
+    REQUIRE(resolution >= 0 && resolution <= cap, "resolution must be in [0, %d] (got %d)", cap, resolution);
     CHK(hipDeviceSynchronize());   // kernels in flight may still read the old grid
     (void)hipFree(s->sgrid.start);
     s->sgrid = StartGridOutputs{};
-    return build_start_grid_2d(grid_inputs(s), resolution, S(stream), s->sgrid);
+    return build_start_grid(s, resolution, S(stream));
 }
 
-int wos_scene_get_start_grid(const wos_scene *s, int *nx, int *ny) {
+int wos_scene_get_start_grid(const wos_scene *s, int *nx, int *ny, int *nz) {
     REQUIRE(s, "scene is NULL");
     if (nx) *nx = s->sgrid.nx;
     if (ny) *ny = s->sgrid.ny;
+    if (nz) *nz = s->sgrid.nz;
     return 0;
 }
 

@@ -275,9 +275,9 @@ struct SceneView {
     int grid_nx, grid_ny;
     // start-node grid (2D, optional): the node a query from a point of the
     // cell may start at (every primitive that can be closest lies below it)
-    const int *start_node;        // st_nx * st_ny, null when absent
-    float st_ox, st_oy, st_inv_cell;
-    int st_nx, st_ny;
+    const int *start_node;        // st_nx * st_ny * st_nz, null when absent
+    float st_ox, st_oy, st_oz, st_inv_cell;
+    int st_nx, st_ny, st_nz;
 };
 
 // Per-lane traversal stack: the first kLdsStack entries live in LDS (lane
@@ -384,13 +384,20 @@ __device__ __forceinline__ float prune_for(float best_d2, float tol) {
 
 // Node a closest-primitive query from p may start at: the cell's entry of
 // the start-node grid, or the root.
+template <int D>
 __device__ __forceinline__ int start_node_of(const SceneView &s, const float *p) {
     if (!s.start_node) return 0;
     const float fx = (p[0] - s.st_ox) * s.st_inv_cell;
     const float fy = (p[1] - s.st_oy) * s.st_inv_cell;
     // written so that NaN coordinates fail the test
     if (!(fx >= 0.0f && fy >= 0.0f && fx < (float)s.st_nx && fy < (float)s.st_ny)) return 0;
-    return s.start_node[(int)fy * s.st_nx + (int)fx];
+    long c = (long)(int)fy * s.st_nx + (int)fx;
+    if constexpr (D == 3) {
+        const float fz = (p[2] - s.st_oz) * s.st_inv_cell;
+        if (!(fz >= 0.0f && fz < (float)s.st_nz)) return 0;
+        c += (long)(int)fz * s.st_nx * s.st_ny;
+    }
+    return s.start_node[c];
 }
 
 // Starts a query whose answer lies within bound_d2, optionally seeded with a
@@ -633,7 +640,7 @@ __device__ int closest_bvh(const SceneView &s, const float *p, float bound_d2, f
         return 0;
     }
     Query q;
-    query_begin<D>(q, bound_d2, 0x7fffffff, __builtin_inff(), s.tol, D == 2 ? start_node_of(s, p) : 0);
+    query_begin<D>(q, bound_d2, 0x7fffffff, __builtin_inff(), s.tol, start_node_of<D>(s, p));
     while (!query_step<D, true>(s, p, q, st, lane_slot, spill_lane, s.tol)) {
     }
     query_flush_stats<true>(s, q);

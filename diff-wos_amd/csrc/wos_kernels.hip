@@ -404,10 +404,10 @@ __global__ __launch_bounds__(kBlock) void k_walk_persistent(SceneView s, Persist
             if constexpr (STATS) atomicAdd(&s.stats[3], 1ull);
             int i0 = 0, i1 = 0, start = 0;
             if (s.use_bvh) {
+                start = start_node_of<D>(s, p);
                 if constexpr (D == 2) {
                     i0 = s.idx[2 * prim];
                     i1 = s.idx[2 * prim + 1];
-                    start = start_node_of(s, p);
                 }
             }
             if (SRC) {

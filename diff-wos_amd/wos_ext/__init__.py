@@ -79,7 +79,7 @@ _sig("wos_scene_get_bvh", _vp, _vp, _vp, _vp, _vp)
 _sig("wos_scene_set_grid", _vp, _f, _i, _i, _vp)
 _sig("wos_scene_get_grid", _vp, _vp, _vp, _vp, _vp)
 _sig("wos_scene_set_start_grid", _vp, _i, _vp)
-_sig("wos_scene_get_start_grid", _vp, _vp, _vp)
+_sig("wos_scene_get_start_grid", _vp, _vp, _vp, _vp)
 _sig("wos_closest_point_index", _vp, _vp, _i, _vp, _vp, _vp)
 _sig("wos_sdf", _vp, _vp, _i, _vp, _vp)
 _sig("wos_closest_point", _vp, _vp, _i, ctypes.POINTER(_Record), _vp)
@@ -363,15 +363,18 @@ class _SceneBase:
                                       ctypes.byref(nc)))
         return {"nx": nx.value, "ny": ny.value, "cell": cell.value, "entries": nc.value}
 
-    def set_start_grid(self, resolution=512):
-        """Rebuilds the 2D start-node grid (resolution 0 removes it).  Query
+    def set_start_grid(self, resolution=None):
+        """Rebuilds the start-node grid with `resolution` cells along the
+        longest side (default 512 in 2D, 96 in 3D; 0 removes it).  Query
         results are identical with and without it."""
+        if resolution is None:
+            resolution = 512 if self._dim == 2 else 96
         _check(lib.wos_scene_set_start_grid(self.handle, int(resolution), _stream()))
 
     def start_grid_info(self):
-        nx, ny = ctypes.c_int(), ctypes.c_int()
-        _check(lib.wos_scene_get_start_grid(self.handle, ctypes.byref(nx), ctypes.byref(ny)))
-        return {"nx": nx.value, "ny": ny.value}
+        nx, ny, nz = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        _check(lib.wos_scene_get_start_grid(self.handle, ctypes.byref(nx), ctypes.byref(ny), ctypes.byref(nz)))
+        return {"nx": nx.value, "ny": ny.value, "nz": nz.value}
 
     def bvh_topology(self):
         n = self._indices.shape[0]

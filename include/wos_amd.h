@@ -110,14 +110,15 @@ int wos_scene_get_bvh(const wos_scene *scene, int *sorted_prim, uint32_t *sorted
 int wos_scene_set_grid(wos_scene *scene, float cell_factor, int band_cells, int max_cand, void *stream);
 /* Grid shape (nx = ny = 0 when there is none), cell side and list entries. */
 int wos_scene_get_grid(const wos_scene *scene, int *nx, int *ny, float *cell, unsigned int *n_cand);
-/* Start-node grid of a 2D scene (no counterpart in the reference; results of
- * every query are unchanged): each of resolution^2-ish square cells over the
- * bounding box records the deepest LBVH node below which every segment that
- * can be closest to a point of the cell lies, and traversals from that cell
- * start there instead of at the root.  wos_scene_create builds it with 512
- * cells along the longer side; 0 removes it.  Synchronises the device. */
+/* Start-node grid (no counterpart in the reference; results of every query
+ * are unchanged): each square (2D) or cubic (3D) cell over the bounding box
+ * records the deepest LBVH node below which every primitive that can be
+ * closest to a point of the cell lies, and traversals from that cell start
+ * there instead of at the root.  wos_scene_create builds it with 512 (2D) /
+ * 96 (3D) cells along the longest side; 0 removes it.  Synchronises the
+ * device. */
 int wos_scene_set_start_grid(wos_scene *scene, int resolution, void *stream);
-int wos_scene_get_start_grid(const wos_scene *scene, int *nx, int *ny);
+int wos_scene_get_start_grid(const wos_scene *scene, int *nx, int *ny, int *nz);
 
 /* Scene.closest_point_preliminary(p) -> Int (include/scene.cuh:497-537).
  * Ties: equal squared distance keeps the smallest primitive index.

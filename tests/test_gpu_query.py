@@ -53,6 +53,21 @@ def test_closest_index_bitexact_2d(kind, n):
     np.testing.assert_array_equal(d_bvh.cpu().numpy(), d_ref)
 
 
+@pytest.mark.parametrize("res", [0, 12, 96])
+def test_closest_index_bitexact_3d_start_grid(res):
+    rng = np.random.default_rng(5)
+    sc, ref, v, noise = _scene3(4)
+    sc.set_start_grid(res)
+    assert (sc.start_grid_info()["nz"] > 0) == (res > 0)
+    p = np.concatenate([rng.uniform(-1.3, 1.3, size=(6000, 3)), C.points_in_star(4000, noise),
+                        v[:500] * (1 + rng.normal(scale=1e-3, size=(500, 1))), rng.uniform(-9, 9, size=(300, 3))]
+                       ).astype(np.float32)
+    idx, d2 = sc.closest_point_index_d2(torch.from_numpy(p).cuda())
+    i_ref, d_ref = ref.closest_index(p)
+    np.testing.assert_array_equal(idx.cpu().numpy(), i_ref)
+    np.testing.assert_array_equal(d2.cpu().numpy(), d_ref)
+
+
 def test_closest_index_bitexact_3d():
     rng = np.random.default_rng(2)
     sc, ref, v, noise = _scene3(4)
