@@ -150,6 +150,108 @@ def primal_workload(args, torch, wos_ext, C):
                       "counts": cnt}))
 
 
+def shape3d_workload(args, torch, wos_ext, C):
+    """3D shape gradient (200k-triangle decimated displaced sphere, 262,144
+    points, 16 walks): forward walk_reduce, loss L = mean (u - g)^2, Ours3D
+    backward onto all vertex positions.  One "primal+grad walk" is a forward
+    walk plus a gradient sample (primary walk, LIB, two secondary walks)."""
+    c = C.config4()
+    dv = torch.device("cuda", 0)
+    t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dv)
+    V, F, G, pts, target = t(c["vertices"]), t(c["indices"]), t(c["values"]), t(c["points"]), t(
+        c["target"].astype(np.float32))
+    P, W = pts.shape[0], c["nwalks"]
+    torch.cuda.synchronize()
+    tb = time.perf_counter()
+    scene = wos_ext.Scene3D(V, F, G, use_bvh=True)
+    torch.cuda.synchronize()
+    build_ms = (time.perf_counter() - tb) * 1e3
+    cw = wos_ext.WoS3D(nwalks=W, nsteps=c["nsteps"], epsilon=c["epsilon"])
+    kw = dict(nwalks=W, nsteps=c["nsteps"], epsilon=c["epsilon"], epsilon2=c["epsilon2"], clamping=c["clamping"])
+
+    def step(seed):
+        ev = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
+        ev[0].record()
+        u = cw.walk_reduce(pts, scene, seed=seed)["mean"]
+        g = (2.0 / P) * (u - target)
+        ev[1].record()
+        gv, _ = wos_ext.shape_gradient(scene, pts, seed, g, **kw)
+        ev[2].record()
+        return ev, gv
+
+    for i in range(args.warmup):
+        step(i)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    evs = [step(100 + i)[0] for i in range(args.steps)]
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / args.steps
+    fwd = float(np.mean([e[0].elapsed_time(e[1]) for e in evs]))
+    bwd = float(np.mean([e[1].elapsed_time(e[2]) for e in evs]))
+    log(f"shape3d: {ms:.2f} ms/step (forward {fwd:.2f}, gradient {bwd:.2f}), {P * W / ms * 1e3:.3e} walks/s")
+    print(json.dumps({"metric": "primal+grad walks/s (3D shape gradient)", "value": P * W / ms * 1e3,
+                      "unit": "walks/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms,
+                      "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+                      "data": "synthetic",
+                      "config": {"workload": "3d-shape-gradient-200k-tris-262144pts-16walks", "points": P,
+                                 "walks_per_point": W, "triangles": int(c["indices"].shape[0])},
+                      "forward_ms": fwd, "gradient_ms": bwd, "scene_build_ms": build_ms}))
+
+
+def batch_workload(args, torch, wos_ext, C):
+    """64 uneven scenes (32 2D, 32 3D; sizes log-uniform; points scaled by
+    --batch-scale): one batched launch per dimension, eager and replayed from
+    a captured graph, against the same items walked one call at a time."""
+    items_np = C.config5(scale=args.batch_scale)
+    dv = torch.device("cuda", 0)
+    t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dv)
+    items, total = [], 0
+    for it in items_np:
+        if it["dim"] == 2:
+            sc = wos_ext.Scene(t(it["vertices"]), t(it["indices"]), t(it["values"]), use_bvh=True,
+                               source_type=it.get("source_type", 0),
+                               source_params=t(it["source_params"]) if "source_params" in it else None)
+            cw = wos_ext.WoS(nwalks=it["nwalks"], nsteps=it["nsteps"], epsilon=it["epsilon"])
+        else:
+            sc = wos_ext.Scene3D(t(it["vertices"]), t(it["indices"]), t(it["values"]), use_bvh=True)
+            cw = wos_ext.WoS3D(nwalks=it["nwalks"], nsteps=it["nsteps"], epsilon=it["epsilon"])
+        items.append(dict(scene=sc, wos=cw, points=t(it["points"]), seed=it["seed"], records=True,
+                          basis="source_params" in it))
+        total += it["points"].shape[0] * it["nwalks"]
+    batch = wos_ext.Batch(items)
+
+    def sequential():
+        for it in items:
+            it["wos"].walk_reduce(it["points"], it["scene"], seed=it["seed"], records=True, basis=it["basis"])
+
+    def timed(fn):
+        fn()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            fn()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) * 1e3 / args.steps
+
+    seq_ms = timed(sequential)
+    eager_ms = timed(batch.run)
+    s = torch.cuda.Stream()
+    g = torch.cuda.CUDAGraph()
+    batch.run()
+    torch.cuda.synchronize()
+    with torch.cuda.graph(g, stream=s):
+        batch.run()
+    graph_ms = timed(g.replay)
+    log(f"batch: {total} walks; sequential {seq_ms:.2f} ms, batched {eager_ms:.2f} ms, graph {graph_ms:.2f} ms")
+    print(json.dumps({"metric": "walks/s (64 batched scenes)", "value": total / graph_ms * 1e3, "unit": "walks/s",
+                      "n_gpus": 1, "steps": args.steps, "warmup": 1, "ms_per_step": graph_ms,
+                      "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+                      "data": "synthetic",
+                      "config": {"workload": f"64-scene-batch-scale-{args.batch_scale}", "walks": total},
+                      "sequential_ms": seq_ms, "batched_eager_ms": eager_ms, "graph_replay_ms": graph_ms,
+                      "sequential_walks_per_s": total / seq_ms * 1e3}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -163,9 +265,12 @@ def main():
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run only a 1 GiB device-to-device copy (3 times), the reference kernel for "
                          "scaling FETCH_SIZE / WRITE_SIZE counters")
-    ap.add_argument("--workload", choices=["clover", "circle", "sphere"], default="clover",
+    ap.add_argument("--workload", choices=["clover", "circle", "sphere", "shape3d", "batch"], default="clover",
                     help="clover: the headline primal+adjoint workload; circle / sphere: primal-only "
-                         "Laplace workloads (1024-segment circle, 81,920-triangle displaced icosphere)")
+                         "Laplace workloads (1024-segment circle, 81,920-triangle displaced icosphere); "
+                         "shape3d: 3D shape gradient (200k triangles); batch: 64 uneven scenes")
+    ap.add_argument("--batch-scale", type=float, default=0.1,
+                    help="query points per batched scene, as a fraction of the full workload's")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU; multi-GPU runs are not part of it",
@@ -185,6 +290,10 @@ def main():
         torch.cuda.synchronize()
         print(json.dumps({"calibration": "copy", "bytes_read": a.numel() * 4, "bytes_written": b.numel() * 4}))
         return
+    if args.workload == "shape3d":
+        return shape3d_workload(args, torch, wos_ext, C)
+    if args.workload == "batch":
+        return batch_workload(args, torch, wos_ext, C)
     if args.workload != "clover":
         return primal_workload(args, torch, wos_ext, C)
     c, d = build_inputs(torch, C)

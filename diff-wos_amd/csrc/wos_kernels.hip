@@ -574,155 +574,368 @@ __global__ __launch_bounds__(kBlock) void k_adjoint_values(SceneView s, int n_po
 // ---------------------------------------------------------------------------
 // shape gradient (OursCUDA / Ours3D), one lane per (point, walk) sample
 // ---------------------------------------------------------------------------
+struct GradArgs {
+    const float *points;
+    int n_points;
+    GradSettings g;
+    int32_t seed;
+    long long total;                 // n_points * nwalks samples
+    unsigned long long *counter;     // zero before launch
+    int chunk, window;
+    const int *init_prim;            // per point: closest primitive of the start point
+    const float *init_sd;            // per point: signed distance of the start point
+    const float *grad_out;
+    float *grad_vertices;
+    float *sample_grad_u;
+    int2 *spill;
+    long spill_stride;
+};
+
+// Phases of a shape-gradient sample (wos/wos_grad.py:442-463 with
+// wos/wos.py:59-129; wos/wos_grad_3d.py:295-315 with wos/wos3d.py:97-156).
+enum : int { kPrimary = 0, kLib = 1, kSecQuery = 2, kSecWalk = 3 };
+
+// Half the smallest bounding-box extent: the upper end of the LIB bisection
+// (include/scene.cuh:225-249).
 template <int D>
-__device__ float secondary_u(const SceneView &s, const WalkSettings &w2, const float *y, Pcg32 &rng,
-                             float R, const TravStack &st, int slot, long lane_g) {
-    float d2;
-    int id = closest_prim<D>(s, y, bound_d2(s, 2.0f * R), d2, st, slot, lane_g);
-    float sd = signed_dist<D>(s, id, y);
-    if (fabsf(sd) < w2.epsilon) {
-        // wos/wos.py:221-223; the walk would stop at once and draw nothing
-        Rec c;
-        closest_record<D>(s, id, y, c);
-        return dirichlet_at<D>(s, id, c.t, c.uv[0], c.uv[1]);
-    }
-    Rec r;
-    float contrib;
-    walk_one<D>(s, w2, y, rng, id, d2, st, slot, lane_g, r, contrib);
-    if (!r.valid) return 0.0f;
-    return dirichlet_at<D>(s, r.prim, r.t, r.uv[0], r.uv[1]) + contrib;
+__device__ __forceinline__ float lib_upper(const SceneView &s) {
+    float b = fminf(fabsf(s.amax[0] - s.amin[0]), fabsf(s.amax[1] - s.amin[1]));
+    if (D == 3) b = fminf(b, fabsf(s.amax[2] - s.amin[2]));
+    return b / 2.0f;
 }
 
+// Point y of secondary evaluation `side` on the sphere of radius R around
+// c = x_b - n R (n = inward normal), from the sample's angle draws.
 template <int D>
-__global__ __launch_bounds__(kBlock) void k_shape_gradient(SceneView s, GradSettings g, const float *points,
-                                                           int n_points, int32_t seed, const float *grad_out,
-                                                           float *grad_vertices, float *sample_grad_u,
-                                                           int2 *spill, long stride) {
-    DECLARE_STACK(spill, stride);
+__device__ __forceinline__ void secondary_point(const float *xb, const float *n, float R, const float *dr, int side,
+                                                float *y) {
+    float c[3];
+    for (int q = 0; q < 3; ++q) c[q] = fmaf(-n[q], R, xb[q]);
+    if constexpr (D == 2) {
+        const float sn = dr[0], cs = dr[1];
+        const float pd[2] = {-n[1], n[0]};
+        const float ss = side ? -sn : sn;
+        y[0] = fmaf(R, fmaf(n[0], cs, pd[0] * ss), c[0]);
+        y[1] = fmaf(R, fmaf(n[1], cs, pd[1] * ss), c[1]);
+        y[2] = 0.0f;
+    } else {
+        const float ct = dr[0], sn = dr[1], cs = dr[2];
+        float up[3] = {0.0f, 0.0f, 1.0f};
+        if (!(fabsf(n[2]) < 0.9f)) { up[0] = 1.0f; up[2] = 0.0f; }
+        float pd[3];
+        cross3(n, up, pd);
+        float l = sqrtf(dot3(pd, pd));
+        pd[0] = pd[0] / l; pd[1] = pd[1] / l; pd[2] = pd[2] / l;
+        float qd[3];
+        cross3(n, pd, qd);
+        const float r = sqrtf(fmaxf(0.0f, 1.0f - ct * ct));
+        const float ss = side ? -sn : sn, cc = side ? -cs : cs;
+        for (int q = 0; q < 3; ++q) y[q] = fmaf(R, fmaf(ct, n[q], r * fmaf(cc, pd[q], ss * qd[q])), c[q]);
+    }
+}
+
+// Tangential derivative of the boundary data on primitive f (wos/scene.py:
+// 451-460, wos/wos3d.py:137-156), added to gu.
+template <int D>
+__device__ __forceinline__ void add_tangent_derivative(const SceneView &s, int f, float *gu) {
+    if constexpr (D == 2) {
+        int i0 = s.idx[2 * f], i1 = s.idx[2 * f + 1];
+        float ex = s.vtx[2 * i1] - s.vtx[2 * i0], ey = s.vtx[2 * i1 + 1] - s.vtx[2 * i0 + 1];
+        float len = sqrtf(dot2(ex, ey, ex, ey));
+        float dv = (s.val[i1] - s.val[i0]) / len;
+        gu[0] += dv * (ex / len);
+        gu[1] += dv * (ey / len);
+    } else {
+        int i0 = s.idx[3 * f], i1 = s.idx[3 * f + 1], i2 = s.idx[3 * f + 2];
+        float a[3], ab[3], ac[3];
+        for (int q = 0; q < 3; ++q) {
+            a[q] = s.vtx[3 * i0 + q];
+            ab[q] = s.vtx[3 * i1 + q] - a[q];
+            ac[q] = s.vtx[3 * i2 + q] - a[q];
+        }
+        float lab = sqrtf(dot3(ab, ab));
+        float t[3] = {ab[0] / lab, ab[1] / lab, ab[2] / lab};
+        float fn[3];
+        cross3(ab, ac, fn);
+        float lf = sqrtf(dot3(fn, fn));
+        fn[0] = fn[0] / lf; fn[1] = fn[1] / lf; fn[2] = fn[2] / lf;
+        float nn[3];
+        cross3(fn, t, nn);
+        float ln = sqrtf(dot3(nn, nn));
+        nn[0] = nn[0] / ln; nn[1] = nn[1] / ln; nn[2] = nn[2] / ln;
+        float va = s.val[i0], vb = s.val[i1], vc = s.val[i2];
+        float gt = (vb - va) / lab;
+        float gn = ((vc - va) - gt * dot3(ac, t)) / dot3(ac, nn);
+        for (int q = 0; q < 3; ++q) gu[q] += fmaf(gt, t[q], gn * nn[q]);
+    }
+}
+
+// Shape-gradient samples (OursCUDA / Ours3D), one per (point, walk), on the
+// same persistent machinery as the walk kernel: every closest-primitive
+// query of a sample -- the primary walk's steps, the ten LIB bisection
+// queries, the two secondary starts and their walks' steps -- is a
+// resumable query advanced `window` node visits per round, and lanes take
+// new samples from the wave's chunk as theirs finish.  Queries are seeded
+// with a primitive known to be near (the previous step's, or the primary
+// exit primitive, which is within m of a LIB point and 2R of a secondary
+// start), so they need no distance bound; the answer is the exact closest
+// primitive either way, and the random draws follow the single-lane order
+// (primary walk, angle draws, secondary walk 0, secondary walk 1).
+template <int D>
+__global__ __launch_bounds__(kBlock) void k_grad_persistent(SceneView s, GradArgs a) {
+    DECLARE_STACK(a.spill, a.spill_stride);
+    const int lane = __lane_id();
+    const GradSettings g = a.g;
     const int W = g.nwalks;
-    long long total = (long long)n_points * W;
-    WalkSettings w1{g.nwalks, g.nsteps, g.epsilon, g.double_sided, 0};
-    WalkSettings w2{g.nwalks, g.nsteps, g.epsilon2, g.double_sided, 0};
-    for (long long k = lane_g; k < total; k += (long long)gridDim.x * kBlock) {
-        long long i = k / W;
-        Pcg32 rng = walk_stream(seed, i, k);
-        float p[3];
-        load_point<D>(points, i, p);
-        float d2;
-        int id0 = closest_prim<D>(s, p, __builtin_inff(), d2, st, slot, lane_g);
-        float gu[3] = {0.0f, 0.0f, 0.0f};
-        bool ok = false;
-        Rec rec;
-        rec.valid = 0;
-        if (id0 >= 0) {
-            float sd0 = signed_dist<D>(s, id0, p);
-            bool inside = g.double_sided || sd0 < 0.0f;
-            float contrib;
-            walk_one<D>(s, w1, p, rng, id0, d2, st, slot, lane_g, rec, contrib);
-            if (rec.valid) {
-                ok = inside;
-                float n[3] = {-rec.n[0], -rec.n[1], D == 3 ? -rec.n[2] : 0.0f};
-                float u_ref = dirichlet_at<D>(s, rec.prim, rec.t, rec.uv[0], rec.uv[1]);
-                float R = lib_one<D>(s, rec.p, rec.n, 1e-3f, st, slot, lane_g);
-                float c[3];
-                for (int q = 0; q < 3; ++q) c[q] = fmaf(-n[q], R, rec.p[q]);
-                float grad = 0.0f;
-                if constexpr (D == 2) {
-                    float t = (float)rng.next_f64();
-                    float lo = g.clamping * kInv2Pi;
-                    t = fminf(fmaxf(t, lo), 1.0f - lo);
-                    float sn, cs;
-                    sincos_turns(t, sn, cs);
-                    float pd[2] = {-n[1], n[0]};
-                    float P = 1.0f / (cs - 1.0f);
-                    for (int side = 0; side < 2; ++side) {
-                        float ss = side ? -sn : sn;
-                        float y[3] = {fmaf(R, fmaf(n[0], cs, pd[0] * ss), c[0]),
-                                      fmaf(R, fmaf(n[1], cs, pd[1] * ss), c[1]), 0.0f};
-                        float u = secondary_u<D>(s, w2, y, rng, R, st, slot, lane_g);
-                        float num = g.control_variates ? (u - u_ref) : u;
-                        grad = grad + P * num / R;
+    long long k = -1;
+    int mode = kIdle, phase = kPrimary;
+    float p[3] = {0.0f, 0.0f, 0.0f};
+    float d = 0.0f;
+    int step = 0, prim = -1;
+    Pcg32 rng;
+    rng.state = rng.inc = 0;
+    Query q;
+    q.node = q.sp = 0;
+    q.best = q.thr = 0.0f;
+    q.best_id = -1;
+    q.nbox = q.nprim = 0;
+    // primary exit record and the sample's derived quantities
+    float xb[3] = {0.0f, 0.0f, 0.0f}, nb[3] = {0.0f, 0.0f, 0.0f};   // exit point, rec.n
+    int rprim = -1;
+    float rt0 = 0.0f, rt1 = 0.0f, u_ref = 0.0f;
+    float la = 0.0f, lb = 0.0f, R = 0.0f, grad = 0.0f;
+    float dr[3] = {0.0f, 0.0f, 0.0f};
+    int lit = 0, side = 0;
+    bool start_known = false;
+    long long c_next = 0, c_end = 0;
+    bool exhausted = false;
+
+    for (;;) {
+        // 1. completed queries
+        if (mode == kReady) {
+            const int found = q.best_id;
+            bool finish = false, ok = false;
+            float u_side = 0.0f;
+            bool have_u = false;
+            if (phase == kPrimary || phase == kSecWalk) {
+                if (!start_known) {
+                    prim = found;
+                    d = sqrtf(q.best);
+                }
+                start_known = false;
+                const float eps = phase == kPrimary ? g.epsilon : g.epsilon2;
+                if (d < eps && step < g.nsteps) {
+                    Rec r;
+                    closest_record<D>(s, prim, p, r);
+                    if (phase == kPrimary) {
+                        for (int c = 0; c < 3; ++c) { xb[c] = r.p[c]; nb[c] = r.n[c]; }
+                        rprim = prim;
+                        rt0 = r.t;
+                        if (D == 3) { rt0 = r.uv[0]; rt1 = r.uv[1]; }
+                        u_ref = dirichlet_at<D>(s, prim, r.t, r.uv[0], r.uv[1]);
+                        la = 1e-3f;
+                        lb = lib_upper<D>(s);
+                        lit = 0;
+                        phase = kLib;
+                    } else {
+                        u_side = dirichlet_at<D>(s, prim, r.t, r.uv[0], r.uv[1]);
+                        have_u = true;
                     }
+                } else if (step >= g.nsteps) {
+                    if (phase == kPrimary) finish = true;
+                    else { u_side = 0.0f; have_u = true; }
                 } else {
-                    double z64 = 1.0 - 2.0 * rng.next_f64();
-                    double ph64 = rng.next_f64();
-                    float ct = fminf(fmaxf((float)z64, -1.0f), 1.0f - g.clamping);
-                    float up[3] = {0.0f, 0.0f, 1.0f};
-                    if (!(fabsf(n[2]) < 0.9f)) { up[0] = 1.0f; up[2] = 0.0f; }
-                    float pd[3];
-                    cross3(n, up, pd);
-                    float l = sqrtf(dot3(pd, pd));
-                    pd[0] = pd[0] / l; pd[1] = pd[1] / l; pd[2] = pd[2] / l;
-                    float qd[3];
-                    cross3(n, pd, qd);
-                    float r = sqrtf(fmaxf(0.0f, 1.0f - ct * ct));
-                    float sn, cs;
-                    sincos_turns((float)ph64, sn, cs);
-                    float om = 1.0f - ct;
-                    float P = -1.0f / (1.41421356237309504880f * (om * sqrtf(om)));
-                    for (int side = 0; side < 2; ++side) {
-                        float ss = side ? -sn : sn, cc = side ? -cs : cs;
-                        float y[3];
-                        for (int q = 0; q < 3; ++q) y[q] = fmaf(R, fmaf(ct, n[q], r * fmaf(cc, pd[q], ss * qd[q])), c[q]);
-                        float u = secondary_u<D>(s, w2, y, rng, R, st, slot, lane_g);
-                        grad = grad + P * (u - u_ref) / R;
+                    mode = kStart;
+                }
+            } else if (phase == kLib) {
+                const float m = (la + lb) / 2.0f;
+                const float dd = fabsf(signed_dist<D>(s, found, p));
+                if (fabsf(dd - m) < 1e-5f) la = m; else lb = m;
+                if (++lit == 10) {
+                    R = la;
+                    if constexpr (D == 2) {
+                        float t = (float)rng.next_f64();
+                        const float lo = g.clamping * kInv2Pi;
+                        t = fminf(fmaxf(t, lo), 1.0f - lo);
+                        sincos_turns(t, dr[0], dr[1]);
+                    } else {
+                        const double z64 = 1.0 - 2.0 * rng.next_f64();
+                        const double ph64 = rng.next_f64();
+                        dr[0] = fminf(fmaxf((float)z64, -1.0f), 1.0f - g.clamping);
+                        sincos_turns((float)ph64, dr[1], dr[2]);
+                    }
+                    side = 0;
+                    grad = 0.0f;
+                    phase = kSecQuery;
+                }
+            } else {   // kSecQuery: the start of secondary walk `side`
+                const float sd = signed_dist<D>(s, found, p);
+                if (fabsf(sd) < g.epsilon2) {
+                    // wos/wos.py:221-223: the walk would stop at once
+                    Rec r;
+                    closest_record<D>(s, found, p, r);
+                    u_side = dirichlet_at<D>(s, found, r.t, r.uv[0], r.uv[1]);
+                    have_u = true;
+                } else if ((sd >= 0.0f && !g.double_sided) || g.nsteps <= 0) {
+                    u_side = 0.0f;   // invalid walk (include/wos.cuh:102-103, 144)
+                    have_u = true;
+                } else {
+                    prim = found;
+                    d = fabsf(sd);
+                    step = 0;
+                    phase = kSecWalk;
+                    mode = kStart;
+                }
+            }
+            if (have_u) {
+                float P;
+                if constexpr (D == 2) P = 1.0f / (dr[1] - 1.0f);
+                else {
+                    const float om = 1.0f - dr[0];
+                    P = -1.0f / (1.41421356237309504880f * (om * sqrtf(om)));
+                }
+                const float num = (D == 3 || g.control_variates) ? (u_side - u_ref) : u_side;
+                grad = grad + P * num / R;
+                if (++side == 2) {
+                    finish = true;
+                    ok = true;
+                } else {
+                    phase = kSecQuery;
+                }
+            }
+            if (finish) {
+                const long long i = k / W;
+                float gu[3] = {0.0f, 0.0f, 0.0f};
+                if (ok) {
+                    const float n[3] = {-nb[0], -nb[1], D == 3 ? -nb[2] : 0.0f};
+                    const float gr = grad / 2.0f;
+                    for (int c = 0; c < D; ++c) gu[c] = gr * n[c];
+                    add_tangent_derivative<D>(s, rprim, gu);
+                }
+                if (a.sample_grad_u)
+                    for (int c = 0; c < D; ++c) a.sample_grad_u[k * D + c] = gu[c];
+                if (ok && a.grad_vertices && a.grad_out) {
+                    const float gi = a.grad_out[i] / (float)W;
+                    if constexpr (D == 2) {
+                        int i0 = s.idx[2 * rprim], i1 = s.idx[2 * rprim + 1];
+                        for (int c = 0; c < 2; ++c) {
+                            atomicAdd(&a.grad_vertices[2 * i0 + c], -gi * (1.0f - rt0) * gu[c]);
+                            atomicAdd(&a.grad_vertices[2 * i1 + c], -gi * rt0 * gu[c]);
+                        }
+                    } else {
+                        int i0 = s.idx[3 * rprim], i1 = s.idx[3 * rprim + 1], i2 = s.idx[3 * rprim + 2];
+                        const float wa = 1.0f - rt0 - rt1;
+                        for (int c = 0; c < 3; ++c) {
+                            atomicAdd(&a.grad_vertices[3 * i0 + c], -gi * wa * gu[c]);
+                            atomicAdd(&a.grad_vertices[3 * i1 + c], -gi * rt0 * gu[c]);
+                            atomicAdd(&a.grad_vertices[3 * i2 + c], -gi * rt1 * gu[c]);
+                        }
                     }
                 }
-                grad = grad / 2.0f;
-                for (int q = 0; q < D; ++q) gu[q] = grad * n[q];
-                // tangential derivative (wos/scene.py:451-460, wos/wos3d.py:137-156)
-                if constexpr (D == 2) {
-                    int i0 = s.idx[2 * rec.prim], i1 = s.idx[2 * rec.prim + 1];
-                    float ex = s.vtx[2 * i1] - s.vtx[2 * i0], ey = s.vtx[2 * i1 + 1] - s.vtx[2 * i0 + 1];
-                    float len = sqrtf(dot2(ex, ey, ex, ey));
-                    float dv = (s.val[i1] - s.val[i0]) / len;
-                    gu[0] += dv * (ex / len);
-                    gu[1] += dv * (ey / len);
+                mode = kIdle;
+                k = -1;
+            } else if (mode == kReady) {
+                // a LIB or secondary-start query comes next: p and the seed
+                if (phase == kLib) {
+                    const float m = (la + lb) / 2.0f;
+                    for (int c = 0; c < D; ++c) p[c] = fmaf(nb[c], m, xb[c]);
                 } else {
-                    int i0 = s.idx[3 * rec.prim], i1 = s.idx[3 * rec.prim + 1], i2 = s.idx[3 * rec.prim + 2];
-                    float a[3], ab[3], ac[3];
-                    for (int q = 0; q < 3; ++q) {
-                        a[q] = s.vtx[3 * i0 + q];
-                        ab[q] = s.vtx[3 * i1 + q] - a[q];
-                        ac[q] = s.vtx[3 * i2 + q] - a[q];
-                    }
-                    float lab = sqrtf(dot3(ab, ab));
-                    float t[3] = {ab[0] / lab, ab[1] / lab, ab[2] / lab};
-                    float fn[3];
-                    cross3(ab, ac, fn);
-                    float lf = sqrtf(dot3(fn, fn));
-                    fn[0] = fn[0] / lf; fn[1] = fn[1] / lf; fn[2] = fn[2] / lf;
-                    float nn[3];
-                    cross3(fn, t, nn);
-                    float ln = sqrtf(dot3(nn, nn));
-                    nn[0] = nn[0] / ln; nn[1] = nn[1] / ln; nn[2] = nn[2] / ln;
-                    float va = s.val[i0], vb = s.val[i1], vc = s.val[i2];
-                    float gt = (vb - va) / lab;
-                    float gn = ((vc - va) - gt * dot3(ac, t)) / dot3(ac, nn);
-                    for (int q = 0; q < 3; ++q) gu[q] += fmaf(gt, t[q], gn * nn[q]);
+                    const float n[3] = {-nb[0], -nb[1], D == 3 ? -nb[2] : 0.0f};
+                    secondary_point<D>(xb, n, R, dr, side, p);
+                }
+                const float seed_d2 = prim_d2<D>(s, rprim, p);
+                if (s.n_prims == 1) {
+                    q.best = seed_d2;
+                    q.best_id = rprim;
+                } else {
+                    query_begin<D>(q, __builtin_inff(), rprim, seed_d2, s.tol, start_node_of<D>(s, p));
+                    mode = kQuery;
                 }
             }
         }
-        if (sample_grad_u)
-            for (int q = 0; q < D; ++q) sample_grad_u[k * D + q] = ok ? gu[q] : 0.0f;
-        if (ok && grad_vertices && grad_out) {
-            float gi = grad_out[i] / (float)W;
-            if constexpr (D == 2) {
-                int i0 = s.idx[2 * rec.prim], i1 = s.idx[2 * rec.prim + 1];
-                float t = rec.t;
-                for (int q = 0; q < 2; ++q) {
-                    atomicAdd(&grad_vertices[2 * i0 + q], -gi * (1.0f - t) * gu[q]);
-                    atomicAdd(&grad_vertices[2 * i1 + q], -gi * t * gu[q]);
-                }
-            } else {
-                int i0 = s.idx[3 * rec.prim], i1 = s.idx[3 * rec.prim + 1], i2 = s.idx[3 * rec.prim + 2];
-                float u = rec.uv[0], v = rec.uv[1], wa = 1.0f - u - v;
-                for (int q = 0; q < 3; ++q) {
-                    atomicAdd(&grad_vertices[3 * i0 + q], -gi * wa * gu[q]);
-                    atomicAdd(&grad_vertices[3 * i1 + q], -gi * u * gu[q]);
-                    atomicAdd(&grad_vertices[3 * i2 + q], -gi * v * gu[q]);
+        // 2. refill idle lanes from the wave's chunk
+        uint64_t idle = __ballot(mode == kIdle);
+        if (idle != 0 && !exhausted) {
+            int need = __popcll(idle);
+            long long avail = c_end - c_next;
+            long long base2 = 0, end2 = 0;
+            bool fresh = false;
+            if (avail < need) {
+                unsigned long long b = 0;
+                if (lane == 0) b = atomicAdd(a.counter, (unsigned long long)a.chunk);
+                uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)b);
+                uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(b >> 32));
+                long long bb = (long long)(((uint64_t)hi << 32) | lo);
+                if (bb >= a.total) {
+                    exhausted = true;
+                } else {
+                    fresh = true;
+                    base2 = bb;
+                    end2 = bb + a.chunk < a.total ? bb + a.chunk : a.total;
                 }
             }
+            if (mode == kIdle) {
+                int r = rank_in_mask(idle);
+                long long kk = -1;
+                if (r < avail) kk = c_next + r;
+                else if (fresh && (r - avail) < end2 - base2) kk = base2 + (r - avail);
+                if (kk >= 0) {
+                    const long long i = kk / W;
+                    const int ip = a.init_prim[i];
+                    const float sd = a.init_sd[i];
+                    k = kk;
+                    load_point<D>(a.points, i, p);
+                    rng = walk_stream(a.seed, i, kk);
+                    phase = kPrimary;
+                    // include/wos.cuh:100-106 (no first-step halving here)
+                    const bool outside = ip < 0 || (sd >= 0.0f && !g.double_sided);
+                    if (outside || g.nsteps <= 0) {
+                        if (a.sample_grad_u)
+                            for (int c = 0; c < D; ++c) a.sample_grad_u[kk * D + c] = 0.0f;
+                        k = -1;
+                    } else {
+                        prim = ip;
+                        d = fabsf(sd);
+                        step = 0;
+                        start_known = true;   // the first test uses d = |sd| as is
+                        mode = kReady;
+                    }
+                }
+            }
+            if (avail >= need) {
+                c_next += need;
+            } else if (fresh) {
+                long long used2 = need - avail;
+                c_next = base2 + used2 < end2 ? base2 + used2 : end2;
+                c_end = end2;
+            } else {
+                c_next = c_end;
+            }
+        }
+        if (exhausted && __ballot(mode != kIdle) == 0) break;
+
+        // 3. a walk step (primary or secondary): jump to the sphere and begin
+        //    the query within 2d of the new point, seeded with the previous
+        //    primitive (Laplace: no source term)
+        if (mode == kStart) {
+            move_point<D>(p, d, rng);
+            ++step;
+            const float seed_d2 = prim_d2<D>(s, prim, p);
+            if (s.n_prims == 1) {
+                q.best = seed_d2;
+                q.best_id = prim;
+                mode = kReady;
+            } else {
+                query_begin<D>(q, bound_d2(s, 2.0f * d), prim, seed_d2, s.tol, start_node_of<D>(s, p));
+                mode = kQuery;
+            }
+        }
+        // 4. advance every pending query by up to `window` node visits
+        for (int it = 0; it < a.window; ++it) {
+            if (mode == kQuery) {
+                if (query_step<D, false>(s, p, q, st, slot, lane_g, s.tol)) mode = kReady;
+            }
+            if (__ballot(mode == kQuery) == 0) break;
         }
     }
 }
@@ -931,14 +1144,48 @@ int launch_adjoint_values(const SceneView &s, int n_points, int nwalks, const in
 int launch_shape_gradient(const SceneView &s, const GradSettings &g, const float *points, int n_points, int32_t seed,
                           const float *grad_out, float *grad_vertices, float *sample_grad_u, hipStream_t st) {
     if (n_points <= 0) return 0;
-    long long total = (long long)n_points * g.nwalks;
-    int gr = grid_of((long)total, occupancy_grid());
-    long stride;
-    int2 *spill = spill_buffer(s, (long)gr * kBlock, &stride, st);
-    if (s.dim == 2)
-        k_shape_gradient<2><<<gr, kBlock, top_bytes(s), st>>>(s, g, points, n_points, seed, grad_out, grad_vertices, sample_grad_u, spill, stride);
-    else
-        k_shape_gradient<3><<<gr, kBlock, top_bytes(s), st>>>(s, g, points, n_points, seed, grad_out, grad_vertices, sample_grad_u, spill, stride);
+    if (!s.use_bvh) {
+        set_error("the shape gradient needs the scene's BVH (use_bvh)");
+        return -1;
+    }
+    GradArgs a;
+    a.points = points;
+    a.n_points = n_points;
+    a.g = g;
+    a.seed = seed;
+    a.total = (long long)n_points * g.nwalks;
+    a.counter = launch_counter(st);
+    if (!a.counter) return -1;
+    a.chunk = g_tuning.chunk > 0 ? g_tuning.chunk : 128;
+    a.window = g_tuning.window > 0 ? g_tuning.window : 8;
+    a.grad_out = grad_out;
+    a.grad_vertices = grad_vertices;
+    a.sample_grad_u = sample_grad_u;
+    long waves_needed = (long)((a.total + 63) / 64);
+    int gr = (int)((waves_needed + 3) / 4);
+    int cap = occupancy_grid();
+    if (gr > cap) gr = cap;
+    if (gr < 1) gr = 1;
+    int gi = grid_of(n_points, cap);
+    long lanes = (long)std::max(gr, gi) * kBlock;
+    a.spill = spill_buffer(s, lanes, &a.spill_stride, st);
+    int *iprim = nullptr;
+    float *isd = nullptr;
+    if (hipMallocAsync((void **)&iprim, sizeof(int) * (size_t)n_points, st) != hipSuccess ||
+        hipMallocAsync((void **)&isd, sizeof(float) * (size_t)n_points, st) != hipSuccess) {
+        set_error("cannot allocate %d start records", n_points);
+        return -1;
+    }
+    a.init_prim = iprim;
+    a.init_sd = isd;
+    size_t tb = top_bytes(s);
+    if (s.dim == 2) k_point_init<2><<<gi, kBlock, tb, st>>>(s, points, n_points, iprim, isd, a.spill, a.spill_stride);
+    else k_point_init<3><<<gi, kBlock, tb, st>>>(s, points, n_points, iprim, isd, a.spill, a.spill_stride);
+    k_zero_u64<<<1, 1, 0, st>>>(a.counter);
+    if (s.dim == 2) k_grad_persistent<2><<<gr, kBlock, tb, st>>>(s, a);
+    else k_grad_persistent<3><<<gr, kBlock, tb, st>>>(s, a);
+    (void)hipFreeAsync(iprim, st);
+    (void)hipFreeAsync(isd, st);
     KCHK();
     return 0;
 }
