@@ -91,10 +91,16 @@ __device__ __forceinline__ void batch_write(const BatchItem &b, long long k, boo
     }
 }
 
+// The persistent walk of the single-scene kernel (k_walk_persistent) over
+// the concatenated walk space of a dimension group: a lane resolves a new
+// walk to its item by binary search, keeps the item's node pointer in
+// registers for its resumable queries and reads the rest of the item's
+// scene view from the item table (cached; items are few and shared).
 template <int D>
 __global__ __launch_bounds__(kBlock) void k_batch_walk(const BatchItem *items, int n_items, long long total,
-                                                       unsigned long long *counter, int chunk, const int *iprim,
-                                                       const float *isd, int2 *spill, long stride) {
+                                                       unsigned long long *counter, int chunk, int window,
+                                                       const int *iprim, const float *isd, int2 *spill,
+                                                       long stride) {
     __shared__ long long s_begin[kMaxBatchItems];
     for (int i = threadIdx.x; i < n_items; i += kBlock) s_begin[i] = items[i].walk_begin;
     __syncthreads();
@@ -104,29 +110,44 @@ __global__ __launch_bounds__(kBlock) void k_batch_walk(const BatchItem *items, i
 
     int item = 0;
     long long k = -1;      // walk index within the item
-    bool active = false;
+    int mode = kIdle;
     float p[3] = {0.0f, 0.0f, 0.0f};
     float d = 0.0f, contrib = 0.0f;
     int step = 0, prim = -1;
     Pcg32 rng;
     rng.state = rng.inc = 0;
+    Query q;
+    q.node = q.sp = 0;
+    q.best = q.thr = 0.0f;
+    q.best_id = -1;
+    q.nbox = q.nprim = 0;
+    SceneView tv;            // what a traversal step reads: the item's nodes (no LDS copy)
+    tv.nodes = nullptr;
+    tv.n_top = 0;
+    float tol = 0.0f;
     float bas[kMaxBatchBasis];
 #pragma unroll
-    for (int q = 0; q < kMaxBatchBasis; ++q) bas[q] = 0.0f;
+    for (int j = 0; j < kMaxBatchBasis; ++j) bas[j] = 0.0f;
     long long c_next = 0, c_end = 0;
     bool exhausted = false;
 
     for (;;) {
-        if (active) {
+        // 1. completed queries: stop, give up, or take the next step
+        if (mode == kReady) {
             const BatchItem &b = items[item];
+            prim = q.best_id;
+            d = sqrtf(q.best);
             bool done_valid = d < b.w.epsilon && step < b.w.nsteps;
             bool done_invalid = !done_valid && step >= b.w.nsteps;
             if (done_valid || done_invalid) {
                 batch_write<D>(b, k, done_valid, prim, p, contrib, bas);
-                active = false;
+                mode = kIdle;
+            } else {
+                mode = kStart;
             }
         }
-        uint64_t idle = __ballot(!active);
+        // 2. refill idle lanes from the wave's chunk
+        uint64_t idle = __ballot(mode == kIdle);
         if (idle != 0 && !exhausted) {
             int need = __popcll(idle);
             long long avail = c_end - c_next;
@@ -146,7 +167,7 @@ __global__ __launch_bounds__(kBlock) void k_batch_walk(const BatchItem *items, i
                     end2 = bb + chunk < total ? bb + chunk : total;
                 }
             }
-            if (!active) {
+            if (mode == kIdle) {
                 int r = rank_in_mask(idle);
                 long long gk = -1;
                 if (r < avail) gk = c_next + r;
@@ -162,7 +183,8 @@ __global__ __launch_bounds__(kBlock) void k_batch_walk(const BatchItem *items, i
                     load_point<D>(b.points, i, p);
                     contrib = 0.0f;
 #pragma unroll
-                    for (int q = 0; q < kMaxBatchBasis; ++q) bas[q] = 0.0f;
+                    for (int j = 0; j < kMaxBatchBasis; ++j) bas[j] = 0.0f;
+                    // include/wos.cuh:100-106, then the first loop test
                     bool outside = ip < 0 || (sd >= 0.0f && !b.w.double_sided);
                     float d0 = fabsf(sd);
                     if (b.w.prevent_fd_artifacts) d0 *= 0.5f;
@@ -176,7 +198,9 @@ __global__ __launch_bounds__(kBlock) void k_batch_walk(const BatchItem *items, i
                         prim = ip;
                         d = d0;
                         step = 0;
-                        active = true;
+                        tv.nodes = b.v.nodes;
+                        tol = b.v.tol;
+                        mode = kStart;
                     }
                 }
             }
@@ -190,9 +214,11 @@ __global__ __launch_bounds__(kBlock) void k_batch_walk(const BatchItem *items, i
                 c_next = c_end;
             }
         }
-        if (exhausted && __ballot(active) == 0) break;
-        if (active) {
-            const SceneView s = items[item].v;
+        if (exhausted && __ballot(mode != kIdle) == 0) break;
+
+        // 3. a new step: source sample, jump to the sphere, begin the query
+        if (mode == kStart) {
+            const SceneView &s = items[item].v;
             if (D == 2 && s.source_type != 0) {
                 const int K = items[item].K;
                 float u0 = rng.next_1d(), u1 = rng.next_1d();
@@ -209,12 +235,12 @@ __global__ __launch_bounds__(kBlock) void k_batch_walk(const BatchItem *items, i
                     if (K > 0) {
                         float f = 0.0f;
 #pragma unroll
-                        for (int q = 0; q < kMaxBatchBasis; ++q) {
-                            if (q < K) {
-                                const float *c = s.src + 4 * q;
+                        for (int j = 0; j < kMaxBatchBasis; ++j) {
+                            if (j < K) {
+                                const float *c = s.src + 4 * j;
                                 float bq = sin_cw(fmaf(c[0], yx, fmaf(c[1], yy, c[2])));
                                 f = fmaf(c[3], bq, f);
-                                bas[q] = fmaf(bq * G, area, bas[q]);
+                                bas[j] = fmaf(bq * G, area, bas[j]);
                             }
                         }
                         contrib = fmaf(f * G, area, contrib);
@@ -225,9 +251,30 @@ __global__ __launch_bounds__(kBlock) void k_batch_walk(const BatchItem *items, i
             }
             move_point<D>(p, d, rng);
             ++step;
-            float d2;
-            prim = closest_prim<D>(s, p, bound_d2(s, 2.0f * d), d2, st, threadIdx.x, lane_g);
-            d = sqrtf(d2);
+            if (!s.use_bvh) {
+                float d2;
+                int id = closest_brute<D>(s, p, d2);
+                q.best = d2;
+                q.best_id = id;
+                mode = kReady;
+            } else {
+                const float seed_d2 = prim_d2<D>(s, prim, p);
+                if (s.n_prims == 1) {
+                    q.best = seed_d2;
+                    q.best_id = prim;
+                    mode = kReady;
+                } else {
+                    query_begin<D>(q, bound_d2(s, 2.0f * d), prim, seed_d2, tol, start_node_of<D>(s, p));
+                    mode = kQuery;
+                }
+            }
+        }
+        // 4. advance every pending query by up to `window` node visits
+        for (int it = 0; it < window; ++it) {
+            if (mode == kQuery) {
+                if (query_step<D, false>(tv, p, q, st, threadIdx.x, lane_g, tol)) mode = kReady;
+            }
+            if (__ballot(mode == kQuery) == 0) break;
         }
     }
 }
@@ -426,14 +473,15 @@ extern "C" int wos_batch_run(wos_batch *B, void *stream) {
         long gw = (g.walks / 64 + 4) / 4;
         int gwk = (int)std::min<long>(std::max<long>(gw, 1), cap);
         int chunk = g_tuning.chunk > 0 ? g_tuning.chunk : 256;
+        int window = g_tuning.window > 0 ? g_tuning.window : 8;
         if (d == 0) {
             k_batch_init<2><<<gi, kBlock, 0, st>>>(g.items, g.n_items, g.points, g.iprim, g.isd, spill, stride);
-            k_batch_walk<2><<<gwk, kBlock, 0, st>>>(g.items, g.n_items, g.walks, B->counters + d, chunk, g.iprim,
-                                                    g.isd, spill, stride);
+            k_batch_walk<2><<<gwk, kBlock, 0, st>>>(g.items, g.n_items, g.walks, B->counters + d, chunk, window,
+                                                    g.iprim, g.isd, spill, stride);
         } else {
             k_batch_init<3><<<gi, kBlock, 0, st>>>(g.items, g.n_items, g.points, g.iprim, g.isd, spill, stride);
-            k_batch_walk<3><<<gwk, kBlock, 0, st>>>(g.items, g.n_items, g.walks, B->counters + d, chunk, g.iprim,
-                                                    g.isd, spill, stride);
+            k_batch_walk<3><<<gwk, kBlock, 0, st>>>(g.items, g.n_items, g.walks, B->counters + d, chunk, window,
+                                                    g.iprim, g.isd, spill, stride);
         }
         long gr = std::min<long>((g.points * 64 + kBlock - 1) / kBlock, 8192);
         k_batch_reduce<<<(int)std::max<long>(gr, 1), kBlock, 0, st>>>(g.items, g.n_items, g.points);

@@ -846,6 +846,10 @@ __device__ __forceinline__ void move_point(float *p, float d, Pcg32 &rng) {
     }
 }
 
+// Lane modes of the persistent walk kernels: no work, a step to take, a
+// query in progress, a query just completed.
+enum : int { kIdle = 0, kStart = 1, kQuery = 2, kReady = 3 };
+
 // wave helpers ---------------------------------------------------------------
 __device__ __forceinline__ int lane_id() { return __lane_id(); }
 __device__ __forceinline__ int rank_in_mask(uint64_t m) {

@@ -247,7 +247,6 @@ __global__ __launch_bounds__(kBlock) void k_point_init(SceneView s, const float 
     }
 }
 
-enum : int { kIdle = 0, kStart = 1, kQuery = 2, kReady = 3 };
 
 template <int D, bool BASIS>
 __device__ __forceinline__ void write_walk(const SceneView &s, const PersistArgs &a, long long k, bool valid,
