@@ -219,6 +219,9 @@ def batch_workload(args, torch, wos_ext, C):
                           basis="source_params" in it))
         total += it["points"].shape[0] * it["nwalks"]
     batch = wos_ext.Batch(items)
+    tuning = batch.autotune() if args.autotune else None
+    if tuning:
+        log(f"autotuned: {tuning}")
 
     def sequential():
         for it in items:
@@ -249,6 +252,7 @@ def batch_workload(args, torch, wos_ext, C):
                       "data": "synthetic",
                       "config": {"workload": f"64-scene-batch-scale-{args.batch_scale}", "walks": total},
                       "sequential_ms": seq_ms, "batched_eager_ms": eager_ms, "graph_replay_ms": graph_ms,
+                      "tuning": tuning,
                       "sequential_walks_per_s": total / seq_ms * 1e3}))
 
 
@@ -269,6 +273,8 @@ def main():
                     help="clover: the headline primal+adjoint workload; circle / sphere: primal-only "
                          "Laplace workloads (1024-segment circle, 81,920-triangle displaced icosphere); "
                          "shape3d: 3D shape gradient (200k triangles); batch: 64 uneven scenes")
+    ap.add_argument("--autotune", action="store_true",
+                    help="batch workload: choose the launch tuning first (cached per device)")
     ap.add_argument("--batch-scale", type=float, default=0.1,
                     help="query points per batched scene, as a fraction of the full workload's")
     args = ap.parse_args()

@@ -640,6 +640,15 @@ class Batch:
         _check(lib.wos_batch_run(self._h, _stream()))
         return self.outputs
 
+    def autotune(self, key=None, reps=2):
+        """Chooses the launch tuning for this batch (wos_ext.autotune);
+        outputs are unchanged by it.  `key` names the workload in the
+        tuning cache (default: the batch's item count and walk total)."""
+        from .autotune import autotune
+        if key is None:
+            key = f"batch-{len(self.outputs)}-{sum(o['walk_value'].numel() for o in self.outputs)}"
+        return autotune(self.run, key, reps=reps)
+
     def __del__(self):
         h = self.__dict__.get("_h")
         if h is not None and h.value:

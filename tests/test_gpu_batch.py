@@ -62,3 +62,26 @@ def test_batch_graph_replay_matches_eager():
         torch.cuda.synchronize()
         for e, o in zip(eager, b.outputs):
             assert torch.equal(e, o["walk_value"]), rep
+
+
+def test_autotune_keeps_results_and_caches(tmp_path, monkeypatch):
+    """Tuning changes only speed: outputs are identical under the tuned
+    setting, and a second call is answered from the cache."""
+    import wos_ext
+    from wos_ext.autotune import autotune
+    monkeypatch.setenv("WOS_AMD_TUNING_CACHE", str(tmp_path / "tuning.json"))
+    before = wos_ext.get_tuning()
+    items = _build(0.005)
+    b = wos_ext.Batch(items)
+    ref = [o["walk_value"].clone() for o in b.run()]
+    small = {"chunk": (64, 256), "wgs_per_cu": (4, 8), "top_levels": (0, 6), "window": (4, 16)}
+    try:
+        t = autotune(b.run, "test-batch", reps=1, candidates=small)
+        assert not t["cached"]
+        for r, o in zip(ref, b.run()):
+            assert torch.equal(r, o["walk_value"])
+        t2 = autotune(b.run, "test-batch", reps=1, candidates=small)
+        assert t2["cached"] and [t2[k] for k in ("chunk", "wgs_per_cu", "top_levels", "window")] == \
+            [t[k] for k in ("chunk", "wgs_per_cu", "top_levels", "window")]
+    finally:
+        wos_ext.set_tuning(*before)

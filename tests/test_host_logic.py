@@ -42,3 +42,23 @@ def test_config_sizes():
     n = C.SmoothNoise(0)
     probe, _ = C.icosphere(4)
     assert abs(np.abs(n(probe)).max() - 1.0) < 0.05
+
+
+def test_autotune_coordinate_descent_finds_separable_minimum():
+    from wos_ext.autotune import CANDIDATES, coordinate_descent
+    target = (128, 6, 4, 12)
+
+    def measure(t):
+        return sum((a - b) ** 2 / (b * b) for a, b in zip(t, target)) + 1.0
+
+    best, ms, trials = coordinate_descent(measure, (64, 8, 0, 4))
+    assert best == target and ms == 1.0
+    assert trials <= sum(len(v) for v in CANDIDATES.values()) * 2 + 1
+
+
+def test_autotune_cache_round_trip(tmp_path):
+    from wos_ext import autotune as A
+    p = str(tmp_path / "sub" / "tuning.json")
+    A._store(p, {"dev|k": {"tuning": [128, 8, 6, 8], "ms": 1.5}})
+    assert A._load(p)["dev|k"]["tuning"] == [128, 8, 6, 8]
+    assert A._load(str(tmp_path / "missing.json")) == {}
