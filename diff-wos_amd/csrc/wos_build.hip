@@ -190,6 +190,71 @@ __global__ __launch_bounds__(1024) void k_rs_scan(uint32_t *hist, long m) {
     }
 }
 
+// Exclusive scan of a long array in three passes: per-tile sums, one
+// workgroup scanning the tile sums, then per-tile scans plus their offsets.
+constexpr int kScanTile = 4096;   // elements per workgroup of 1024 threads
+
+__device__ __forceinline__ uint32_t block_exclusive_1024(uint32_t v, uint32_t *sh, uint32_t &total) {
+    const int t = threadIdx.x, lane = __lane_id(), w = t >> 6;
+    uint32_t x = v;
+    for (int off = 1; off < 64; off <<= 1) {
+        uint32_t y = __shfl_up(x, off);
+        if (lane >= off) x += y;
+    }
+    if (lane == 63) sh[w] = x;
+    __syncthreads();
+    if (t < 16) {
+        uint32_t sv = sh[t];
+        for (int off = 1; off < 16; off <<= 1) {
+            uint32_t y = __shfl_up(sv, off, 16);
+            if (t >= off) sv += y;
+        }
+        sh[16 + t] = sv;
+    }
+    __syncthreads();
+    total = sh[16 + 15];
+    uint32_t base = w ? sh[16 + w - 1] : 0u;
+    uint32_t excl = base + x - v;
+    __syncthreads();
+    return excl;
+}
+
+__global__ __launch_bounds__(1024) void k_scan_tile_sums(const uint32_t *a, long n, uint32_t *sums) {
+    __shared__ uint32_t sh[32];
+    const long base = (long)blockIdx.x * kScanTile;
+    uint32_t v = 0;
+    for (int j = 0; j < kScanTile / 1024; ++j) {
+        long i = base + (long)j * 1024 + threadIdx.x;
+        if (i < n) v += a[i];
+    }
+    uint32_t tot;
+    (void)block_exclusive_1024(v, sh, tot);
+    if (threadIdx.x == 0) sums[blockIdx.x] = tot;
+}
+
+__global__ __launch_bounds__(1024) void k_scan_tiles(uint32_t *a, long n, const uint32_t *offs) {
+    __shared__ uint32_t sh[32];
+    const long base = (long)blockIdx.x * kScanTile;
+    uint32_t run = offs[blockIdx.x];
+    for (int j = 0; j < kScanTile / 1024; ++j) {
+        long i = base + (long)j * 1024 + threadIdx.x;
+        uint32_t v = i < n ? a[i] : 0u;
+        uint32_t tot;
+        uint32_t e = block_exclusive_1024(v, sh, tot);
+        if (i < n) a[i] = run + e;
+        run += tot;
+    }
+}
+
+// In-place exclusive scan of n entries; `scratch` holds ceil(n / kScanTile)
+// entries.
+static void exclusive_scan_u32(uint32_t *a, long n, uint32_t *scratch, hipStream_t stream) {
+    const long tiles = (n + kScanTile - 1) / kScanTile;
+    k_scan_tile_sums<<<(int)tiles, 1024, 0, stream>>>(a, n, scratch);
+    k_rs_scan<<<1, 1024, 0, stream>>>(scratch, tiles);
+    k_scan_tiles<<<(int)tiles, 1024, 0, stream>>>(a, n, scratch);
+}
+
 __global__ __launch_bounds__(kBlock) void k_rs_scatter(const uint32_t *keys, const int *vals, int n, int shift,
                                                        int nblocks, const uint32_t *offs, uint32_t *okeys,
                                                        int *ovals) {
@@ -1044,12 +1109,14 @@ __global__ void k_grid_blocks(const float4 *nodes, const float4 *leaves, int n_p
 
 template <bool FILL>
 __global__ void k_grid_cells(const float4 *nodes, const float4 *leaves, const int *leaf_of_prim, int n_prims,
-                             GridGeom g, int nbx, const uint8_t *active, uint32_t *counts, int *cand) {
+                             GridGeom g, int nbx, const uint8_t *active, uint32_t *counts, int *cand,
+                             uint32_t n_cand) {
     const float two_rho = 1.41421356f * g.cell;
     for (long c = blockIdx.x * (long)blockDim.x + threadIdx.x; c < (long)g.nx * g.ny; c += (long)gridDim.x * blockDim.x) {
         int cx = (int)(c % g.nx), cy = (int)(c / g.nx);
         if (FILL) {
-            if (counts[c + 1] == counts[c]) continue;
+            // ranges outside the list (impossible for a correct scan) are skipped
+            if (counts[c + 1] == counts[c] || counts[c + 1] < counts[c] || counts[c + 1] > n_cand) continue;
         } else if (!active[(cy / kGridBlock) * nbx + cx / kGridBlock]) {
             counts[c] = 0;
             continue;
@@ -1079,7 +1146,7 @@ int build_grid_2d(const GridInputs &in, const GridSettings &gs, hipStream_t stre
     double *dsum = nullptr;
     int *leaf_of_prim = nullptr;
     uint8_t *active = nullptr;
-    uint32_t *counts = nullptr;
+    uint32_t *counts = nullptr, *scan_tmp = nullptr;
     int *cand = nullptr;
     int rc = -1;
     double sum = 0.0;
@@ -1123,14 +1190,15 @@ int build_grid_2d(const GridInputs &in, const GridSettings &gs, hipStream_t stre
                                                                       active);
     k_fill_u32<<<1, 64, 0, stream>>>(counts + ncells, 0u, 1);
     k_grid_cells<false><<<grid_for(ncells), kBlock, 0, stream>>>(in.nodes, in.leaves, leaf_of_prim, in.n_prims, g,
-                                                                  nbx, active, counts, nullptr);
-    k_rs_scan<<<1, 1024, 0, stream>>>(counts, ncells + 1);
+                                                                  nbx, active, counts, nullptr, 0u);
+    BCHK(hipMallocAsync((void **)&scan_tmp, sizeof(uint32_t) * (size_t)((ncells + 1) / kScanTile + 1), stream));
+    exclusive_scan_u32(counts, ncells + 1, scan_tmp, stream);
     BCHK(hipMemcpyAsync(&total, counts + ncells, sizeof(uint32_t), hipMemcpyDeviceToHost, stream));
     BCHK(hipStreamSynchronize(stream));
     BCHK(hipMalloc((void **)&cand, sizeof(int) * (size_t)(total > 0 ? total : 1)));
     if (total > 0)
         k_grid_cells<true><<<grid_for(ncells), kBlock, 0, stream>>>(in.nodes, in.leaves, leaf_of_prim, in.n_prims,
-                                                                     g, nbx, active, counts, cand);
+                                                                     g, nbx, active, counts, cand, total);
     BCHK(hipGetLastError());
     out.start = counts;
     out.cand = cand;
@@ -1147,6 +1215,7 @@ int build_grid_2d(const GridInputs &in, const GridSettings &gs, hipStream_t stre
     (void)hipFreeAsync(dsum, stream);
     (void)hipFreeAsync(leaf_of_prim, stream);
     (void)hipFreeAsync(active, stream);
+    if (scan_tmp) (void)hipFreeAsync(scan_tmp, stream);
     if (counts) (void)hipFree(counts);
     if (cand) (void)hipFree(cand);
     return rc;

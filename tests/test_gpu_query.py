@@ -57,6 +57,7 @@ def test_closest_index_bitexact_2d(kind, n):
 def test_closest_index_bitexact_3d_start_grid(res):
     rng = np.random.default_rng(5)
     sc, ref, v, noise = _scene3(4)
+    assert sc.start_grid_info()["nz"] == 0    # 3D scenes are built without one
     sc.set_start_grid(res)
     assert (sc.start_grid_info()["nz"] > 0) == (res > 0)
     p = np.concatenate([rng.uniform(-1.3, 1.3, size=(6000, 3)), C.points_in_star(4000, noise),
