@@ -120,7 +120,10 @@ class Polyline(wos_ext.Scene):
         return self.sdf_native(p)
 
     def dirichlet(self, its):
-        out = super().dirichlet(its)
+        """wos/scene.py:353-357: lerp of the segment's vertex values at t (0
+        for invalid records), attached to ``values``."""
+        f = self.indices[its.prim_id.clamp(min=0).long()].long()
+        out = torch.lerp(self.values[f[:, 0]], self.values[f[:, 1]], its.t.float())
         if its.valid is not None:
             out = torch.where(its.valid.bool(), out, torch.zeros_like(out))
         return out
@@ -128,6 +131,34 @@ class Polyline(wos_ext.Scene):
     def closest_point(self, p):
         r = super().closest_point(p)
         rec = ClosestPointRecord(valid=r.valid, p=r.p, n=r.n, t=r.t, prim_id=r.prim_id, contrib=r.contrib)
+        rec.val = self.dirichlet(rec)
+        return rec
+
+    # ---- queries attached to the vertex tensor (for autograd estimators) -----
+    def _segment(self, p):
+        """Closest segment of each point (selected by the kernels, detached)
+        and the clamped projection parameter recomputed from ``vertices``."""
+        idx = self.closest_point_preliminary(p.detach()).long()
+        f = self.indices[idx.clamp(min=0)].long()
+        a, b = self.vertices[f[:, 0]], self.vertices[f[:, 1]]
+        ba = b - a
+        h = (((p - a) * ba).sum(1) / (ba * ba).sum(1)).clamp(0.0, 1.0)
+        return idx, a, ba, h
+
+    def sdf_attached(self, p):
+        """wos/scene.py:325-339 with the distance differentiable in p and the
+        vertices (the sign, piecewise constant, comes from the kernels)."""
+        idx, a, ba, h = self._segment(p)
+        dist = (p - a - ba * h[:, None]).norm(dim=1)
+        sgn = torch.where(self.sdf(p.detach()) >= 0, 1.0, -1.0)
+        return sgn * dist
+
+    def closest_point_attached(self, p):
+        """wos/scene.py:303-323 with p and t differentiable."""
+        idx, a, ba, h = self._segment(p)
+        r = super().closest_point(p.detach())
+        rec = ClosestPointRecord(valid=idx >= 0, p=a + ba * h[:, None], n=r.n, t=h, prim_id=idx.int(),
+                                 contrib=r.contrib)
         rec.val = self.dirichlet(rec)
         return rec
 

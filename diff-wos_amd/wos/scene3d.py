@@ -91,7 +91,12 @@ class Scene3D(wos_ext.Scene3D):
         return self.sdf_native(p)
 
     def dirichlet(self, its):
-        out = super().dirichlet(its)
+        """wos/scene3d.py:216-223: b u + c v + a (1 - u - v) (0 for invalid
+        records), attached to ``values``."""
+        f = self.indices[its.prim_id.clamp(min=0).long()].long()
+        va, vb, vc = self.values[f[:, 0]], self.values[f[:, 1]], self.values[f[:, 2]]
+        u, v = its.uv[:, 0].float(), its.uv[:, 1].float()
+        out = vb * u + vc * v + va * (1 - u - v)
         if its.valid is not None:
             out = torch.where(its.valid.bool(), out, torch.zeros_like(out))
         return out
@@ -99,6 +104,30 @@ class Scene3D(wos_ext.Scene3D):
     def closest_point(self, p):
         r = super().closest_point(p)
         rec = ClosestPointRecord3D(valid=r.valid, p=r.p, n=r.n, uv=r.uv, prim_id=r.prim_id, contrib=r.contrib)
+        rec.val = self.dirichlet(rec)
+        return rec
+
+    # ---- queries attached to the vertex tensor (for autograd estimators) -----
+    def _triangle(self, p):
+        from wos.utils import closest_point_triangle
+        idx = self.closest_point_preliminary(p.detach()).long()
+        f = self.indices[idx.clamp(min=0)].long()
+        a, b, c = self.vertices[f[:, 0]], self.vertices[f[:, 1]], self.vertices[f[:, 2]]
+        pt, uv, d = closest_point_triangle(p, a, b, c)
+        return idx, pt, uv, d
+
+    def sdf_attached(self, p):
+        """Signed distance differentiable in p and the vertices (sign from
+        the kernels)."""
+        idx, pt, uv, d = self._triangle(p)
+        sgn = torch.where(self.sdf(p.detach()) >= 0, 1.0, -1.0)
+        return sgn * d
+
+    def closest_point_attached(self, p):
+        """wos/scene3d.py:179-199 with p and uv differentiable."""
+        idx, pt, uv, d = self._triangle(p)
+        r = super().closest_point(p.detach())
+        rec = ClosestPointRecord3D(valid=idx >= 0, p=pt, n=r.n, uv=uv, prim_id=idx.int(), contrib=r.contrib)
         rec.val = self.dirichlet(rec)
         return rec
 

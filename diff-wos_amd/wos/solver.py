@@ -15,6 +15,7 @@ return zero-valued, gradient-carrying results.
 """
 from __future__ import annotations
 
+import math
 from dataclasses import dataclass
 from enum import Enum
 
@@ -125,9 +126,49 @@ class Solver:
     def single_walk(self, p, scene, sampler):
         raise NotImplementedError
 
-    # subclasses provide _walk
+    # ---- sampling helpers (wos/solver.py:25-42); draws advance every lane ----
+    def rand_on_circle(self, sampler):
+        angle = sampler.next_float64() * (2.0 * math.pi)
+        return torch.stack([torch.cos(angle), torch.sin(angle)], dim=1).float()
+
+    def rand_in_half_circle(self, n, sampler):
+        d = self.rand_on_circle(sampler)
+        return torch.where(((n * d).sum(1) > 0)[:, None], d, -d)
+
+    def sample_uniform(self, sampler):
+        return (sampler.next_float64() * (2.0 * math.pi)).float()
+
+    def pdf_uniform(self, theta):
+        return 1.0 / (2.0 * math.pi)
+
+    # ---- the generic walk_reduce (wos/solver.py:49-65) ------------------------
     def _walk(self, p, scene, seed, nwalks):
-        raise NotImplementedError
+        """Walk k = i*W + w of point i on the PCG32 stream keyed by TEA(seed,
+        i) and k, with this solver's single_walk over all P*W walks at once
+        (torch, one pass per step); the per-point mean stays attached to
+        whatever the walk values are attached to."""
+        P = p.shape[0]
+        n = P * nwalks
+        idx = torch.arange(n, dtype=torch.int64, device=p.device)
+        if nwalks > 1:
+            idx = idx // nwalks
+        v0, v1 = sample_tea_32(seed, idx)
+        sampler = PCG32(n, v0.to(torch.int64), v1.to(torch.int64))
+        value = self.single_walk(p.repeat_interleave(nwalks, dim=0), scene, sampler)
+        out = torch.zeros(P, dtype=value.dtype, device=p.device).index_add(0, idx, value)
+        return out / nwalks if nwalks > 1 else out
+
+    def _walk_with(self, p, scene, seed, nwalks, vertices, values, source_params):
+        """_walk with the scene's vertex / value / source tensors replaced by
+        the given ones for the duration of the call (CustomSolver backward)."""
+        saved = (scene.vertices, scene.values, getattr(scene, "source_params", None))
+        scene.vertices, scene.values = vertices, values
+        if source_params is not None:
+            scene.source_params = source_params
+        try:
+            return self._walk(p, scene, seed, nwalks)
+        finally:
+            scene.vertices, scene.values, scene.source_params = saved
 
 
 class _CustomWalk(torch.autograd.Function):
