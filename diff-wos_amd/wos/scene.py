@@ -8,6 +8,7 @@ autograd functions return gradients to them.
 """
 from __future__ import annotations
 
+import math
 from dataclasses import dataclass
 
 import numpy as np
@@ -15,6 +16,7 @@ import torch
 
 import wos_ext
 from wos.fwd import Array2, Array2i, Float, device
+from wos.utils import DiscreteDistribution
 
 
 @dataclass
@@ -69,6 +71,19 @@ class ClosestPointRecord:
         return wos_ext.ClosestPointRecord(self.valid, self.p, self.n, self.t, self.prim_id, self.contrib)
 
 
+@dataclass
+class BoundarySamplingRecord:
+    """wos/scene.py:86-92: p attached (to the vertices), n detached, the
+    one-sided Dirichlet value, primitive, parameter and pdf (per length or
+    area)."""
+    p: torch.Tensor = None
+    n: torch.Tensor = None
+    val: torch.Tensor = None
+    prim_id: torch.Tensor = None
+    t: torch.Tensor = None
+    pdf: torch.Tensor = None
+
+
 class Polyline(wos_ext.Scene):
     """wos/scene.py:192-475 over the native scene.  ``use_bvh`` defaults to
     True: the LBVH query returns the same primitive as the brute-force scan
@@ -83,6 +98,38 @@ class Polyline(wos_ext.Scene):
         if source_params is not None and not isinstance(source_params, torch.Tensor):
             self.source_params = Float(source_params)
         super().__init__(self.vertices, self.indices, self.values, use_bvh, source_type, self.source_params)
+        self.init_dist(self.vertices, self.indices)
+
+    def init_dist(self, vertices, indices):
+        """wos/scene.py:275-279: segments sampled proportionally to length."""
+        f = indices.long()
+        self.edge_distr = DiscreteDistribution((vertices[f[:, 1]] - vertices[f[:, 0]]).detach().norm(dim=1))
+
+    def sample_boundary(self, sampler):
+        """wos/scene.py:435-449: a point uniform in arc length, attached to
+        the vertices; n = (-ab.y, ab.x) normalised and detached."""
+        rnd = sampler.next_float32()
+        prim, t, prob = self.edge_distr.sample_reuse_pmf(rnd)
+        f = self.indices[prim].long()
+        a, b = self.vertices[f[:, 0]], self.vertices[f[:, 1]]
+        ab = b - a
+        val = torch.lerp(self.values[f[:, 0]], self.values[f[:, 1]], t)
+        x = a + ab * t[:, None]
+        n = torch.stack([-ab[:, 1], ab[:, 0]], dim=1).detach()
+        n = n / n.norm(dim=1, keepdim=True)
+        return BoundarySamplingRecord(p=x, n=n, val=val, prim_id=prim.int(), t=rnd, pdf=prob / ab.norm(dim=1))
+
+    def J(self, prim_id):
+        """wos/scene.py:409-414: l / detach(l) of the segment."""
+        f = self.indices[prim_id.clamp(min=0).long()].long()
+        l = (self.vertices[f[:, 1]] - self.vertices[f[:, 0]]).norm(dim=1)
+        return l / l.detach()
+
+    def geometric(self, x, y, n):
+        """wos/scene.py:401-407: cos(angle(y - x, n)) / (2 pi |y - x|)."""
+        d = y - x
+        r = d.norm(dim=1)
+        return ((d / r[:, None]) * n).sum(1) / (2 * math.pi * r)
 
     @property
     def source_type(self):

@@ -8,7 +8,8 @@ import torch
 
 import wos_ext
 from wos.fwd import Array3, Array3i, Float, device
-from wos.scene import Detector
+from wos.scene import BoundarySamplingRecord, Detector
+from wos.utils import DiscreteDistribution
 
 
 @dataclass
@@ -61,6 +62,34 @@ class Scene3D(wos_ext.Scene3D):
         self.values = values if isinstance(values, torch.Tensor) else Float(values)
         self.source_params = None
         super().__init__(self.vertices, self.indices, self.values, use_bvh)
+        self.init_dist(self.vertices, self.indices)
+
+    def init_dist(self, vertices, indices):
+        """wos/scene3d.py:150-155: triangles sampled proportionally to area."""
+        f = indices.long()
+        a, b, c = vertices[f[:, 0]], vertices[f[:, 1]], vertices[f[:, 2]]
+        self.edge_distr = DiscreteDistribution(torch.cross(b - a, c - a, dim=1).detach().norm(dim=1))
+
+    def sample_boundary(self, sampler):
+        """wos/scene3d.py:241-262: a triangle by area, a point uniform on it
+        (u = sqrt(r1)(1 - r2), v = sqrt(r1) r2), attached to the vertices;
+        n = -(b - a) x (c - a) normalised, detached."""
+        rnd = sampler.next_float32()
+        prim, _, prob = self.edge_distr.sample_reuse_pmf(rnd)
+        f = self.indices[prim].long()
+        a, b, c = self.vertices[f[:, 0]], self.vertices[f[:, 1]], self.vertices[f[:, 2]]
+        va, vb, vc = self.values[f[:, 0]], self.values[f[:, 1]], self.values[f[:, 2]]
+        cr = torch.cross(b - a, c - a, dim=1)
+        area = cr.norm(dim=1)
+        r1 = sampler.next_float32()
+        r2 = sampler.next_float32()
+        u = torch.sqrt(r1) * (1 - r2)
+        v = torch.sqrt(r1) * r2
+        p = b * u[:, None] + c * v[:, None] + a * (1 - u - v)[:, None]
+        val = vb * u + vc * v + va * (1 - u - v)
+        n = -(cr / area[:, None]).detach()
+        return BoundarySamplingRecord(p=p, n=n, val=val, prim_id=prim.int(), t=torch.stack([u, v], dim=1),
+                                      pdf=prob / area)
 
     @property
     def source_type(self):

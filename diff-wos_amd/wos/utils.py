@@ -108,3 +108,24 @@ def rotate_euler(v, angles):
 def concat(a, b):
     """wos/utils.py:217-224: concatenate along the element axis."""
     return torch.cat([a, b], dim=0)
+
+
+class DiscreteDistribution:
+    """The sampling of mi.DiscreteDistribution used here: CDF over an
+    unnormalised pmf; sample_reuse_pmf(u) returns the index whose CDF
+    interval holds u * sum, the position of u inside that interval rescaled
+    to [0, 1), and the normalised pmf of the index.  (Mitsuba's own tie and
+    edge handling is not pinned here.)"""
+
+    def __init__(self, pmf):
+        pmf = pmf.double()
+        self.pmf = pmf
+        self.cdf = torch.cumsum(pmf, 0)
+        self.sum = float(self.cdf[-1])
+
+    def sample_reuse_pmf(self, u):
+        v = u.double() * self.sum
+        idx = torch.searchsorted(self.cdf, v, right=True).clamp(max=self.pmf.numel() - 1)
+        lo = torch.where(idx > 0, self.cdf[(idx - 1).clamp(min=0)], torch.zeros_like(v))
+        t = ((v - lo) / self.pmf[idx]).clamp(0.0, 1.0)
+        return idx, t.float(), (self.pmf[idx] / self.sum).float()

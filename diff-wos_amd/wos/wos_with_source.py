@@ -18,28 +18,8 @@ import torch
 
 import wos_ext
 from wos.solver import ValueSolverMixin
+from wos.utils import DiscreteDistribution
 from wos.wos import WoS, _sel
-
-
-class _DiscreteDistribution:
-    """The sampling of mi.DiscreteDistribution used here: CDF over an
-    unnormalised pmf; sample_reuse_pmf(u) returns the index whose CDF
-    interval holds u * sum, the position of u inside that interval rescaled
-    to [0, 1), and the normalised pmf of the index.  (Mitsuba's own tie and
-    edge handling is not pinned here.)"""
-
-    def __init__(self, pmf):
-        pmf = pmf.double()
-        self.pmf = pmf
-        self.cdf = torch.cumsum(pmf, 0)
-        self.sum = float(self.cdf[-1])
-
-    def sample_reuse_pmf(self, u):
-        v = u.double() * self.sum
-        idx = torch.searchsorted(self.cdf, v, right=True).clamp(max=self.pmf.numel() - 1)
-        lo = torch.where(idx > 0, self.cdf[(idx - 1).clamp(min=0)], torch.zeros_like(v))
-        t = ((v - lo) / self.pmf[idx]).clamp(0.0, 1.0)
-        return idx, t.float(), (self.pmf[idx] / self.sum).float()
 
 
 @dataclass
@@ -60,7 +40,7 @@ class WoSWithSource(WoS):
 
     def _distr(self, dev):
         if self.G_distr is None or self.G_distr.pmf.device != dev:
-            self.G_distr = _DiscreteDistribution(self._pmf.to(dev))
+            self.G_distr = DiscreteDistribution(self._pmf.to(dev))
         return self.G_distr
 
     def eval_G(self, p, R, r_clamp=1e-4):

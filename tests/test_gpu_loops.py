@@ -143,3 +143,42 @@ def test_loop_ours_agrees_with_native_and_baseline_runs():
     u = Baseline(nwalks=4, nsteps=64, epsilon=5e-3).walk(pts[:64], scene, seed=2)
     u.sum().backward()
     assert torch.isfinite(sv.grad) and torch.isfinite(u).all()
+
+
+def test_boundary_sampling_and_boundary_term():
+    """Arc-length sampling: 1/pdf equals the polyline length for every
+    sample; the boundary term's derivative under a uniform scaling s of the
+    unit circle with g = x^2 integrates g (x . n) = -g over the boundary:
+    d/ds sum(image) = npixels * (closed integral of x^2) / detector size."""
+    from wos.fwd import PCG32
+    from wos.scene import Detector, Polyline
+    from wos.wos_boundary import WoSBoundary
+    v, ix = C.circle_polyline(1024)
+    base = _t(v)
+    L = float(np.linalg.norm(v[ix[:, 1]] - v[ix[:, 0]], axis=1).sum())
+    sc = Polyline(base, _t(ix), _t(v[:, 0] ** 2))
+    b = sc.sample_boundary(PCG32(4096, 3))
+    np.testing.assert_allclose((1.0 / b.pdf).cpu().numpy(), L, rtol=1e-5)
+
+    s = torch.tensor(1.0, device="cuda", requires_grad=True)
+    scene = Polyline(base * s, _t(ix), _t(v[:, 0] ** 2))
+    det = Detector(vmin=(-1.2, -1.2), vmax=(1.2, 1.2), res=(32, 32))
+    img = WoSBoundary(nwalks=64).walk_detector(scene, det, seed=7)
+    assert float(img.abs().max()) == 0.0
+    img.sum().backward()
+    expect = 32 * 32 * math.pi / det.size()
+    assert abs(s.grad.item() - expect) < 0.05 * expect, (s.grad.item(), expect)
+
+
+def test_boundary_sampling_3d_uniform_on_area():
+    from wos.fwd import PCG32
+    from wos.scene3d import Scene3D
+    v, f = C.icosphere(2)
+    v = v.astype(np.float32)
+    sc = Scene3D(_t(v), _t(f.astype(np.int32)), _t(v[:, 2]))
+    area = float(np.linalg.norm(np.cross(v[f[:, 1]] - v[f[:, 0]], v[f[:, 2]] - v[f[:, 0]]), axis=1).sum())
+    b = sc.sample_boundary(PCG32(20000, 5))
+    # 1/pdf = total |cross| (twice the area) for every sample; z is uniform on the sphere in the mean
+    np.testing.assert_allclose((1.0 / b.pdf).cpu().numpy(), area, rtol=1e-4)
+    assert abs(float(b.p[:, 2].mean())) < 0.03
+    assert torch.allclose((b.p * b.n).sum(1), -b.p.norm(dim=1), atol=0.06)
