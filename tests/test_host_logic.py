@@ -62,3 +62,54 @@ def test_autotune_cache_round_trip(tmp_path):
     A._store(p, {"dev|k": {"tuning": [128, 8, 6, 8], "ms": 1.5}})
     assert A._load(p)["dev|k"]["tuning"] == [128, 8, 6, 8]
     assert A._load(str(tmp_path / "missing.json")) == {}
+
+
+def test_exr_round_trip_and_layout(tmp_path):
+    """Uncompressed scanline EXR: float and half round trips, and the byte
+    layout of a 2x1 single-channel FLOAT file assembled by hand from the
+    OpenEXR file-layout specification."""
+    import struct
+    from wos import io
+    rng = np.random.default_rng(0)
+    img = rng.normal(size=(5, 7, 3)).astype(np.float32)
+    io.write_exr(str(tmp_path / "a.exr"), img)
+    np.testing.assert_array_equal(io.read_exr(str(tmp_path / "a.exr")), img)
+    io.write_exr(str(tmp_path / "h.exr"), img, half=True)
+    np.testing.assert_array_equal(io.read_exr(str(tmp_path / "h.exr")), img.astype(np.float16).astype(np.float32))
+    io.write_exr(str(tmp_path / "y.exr"), np.array([[1.5, -2.0]], np.float32))
+    b = open(tmp_path / "y.exr", "rb").read()
+    assert b[:8] == struct.pack("<ii", 20000630, 2)
+    assert b"channels\0chlist\0" + struct.pack("<i", 19) + b"Y\0" + struct.pack("<iB3xii", 2, 0, 1, 1) + b"\0" in b
+    assert b"compression\0compression\0" + struct.pack("<i", 1) + b"\0" in b
+    assert b"dataWindow\0box2i\0" + struct.pack("<iiii", 0, 0, 1, 0) in b
+    assert b.endswith(struct.pack("<ii", 0, 8) + struct.pack("<ff", 1.5, -2.0))
+    off = struct.unpack_from("<Q", b, len(b) - 16 - 8)[0]
+    assert off == len(b) - 16
+
+
+def test_obj_round_trips(tmp_path):
+    from wos import io
+    v = np.array([[0.0, 0.0], [1.0, 0.0], [0.0, 1.0]])
+    ix = np.array([[0, 1], [1, 2], [2, 0]])
+    c = np.array([0.5, -1.0, 2.0])
+    io.write_2d_obj(str(tmp_path / "p.obj"), v, ix, c)
+    v2, ix2, c2 = io.read_2d_obj(str(tmp_path / "p.obj"))
+    np.testing.assert_allclose(v2, v)
+    np.testing.assert_array_equal(ix2, ix)
+    np.testing.assert_allclose(c2, c)
+    _, ixf, _ = io.read_2d_obj(str(tmp_path / "p.obj"), flip_orientation=True)
+    np.testing.assert_array_equal(ixf, ix[:, ::-1])
+    v3 = np.array([[0.0, 0.0, 0.0], [1.0, 0.0, 0.0], [0.0, 1.0, 0.0], [0.0, 0.0, 1.0]])
+    f3 = np.array([[0, 2, 1], [0, 1, 3], [0, 3, 2], [1, 2, 3]])
+    io.write_3d_obj(str(tmp_path / "t.obj"), v3, f3, v3[:, 2])
+    a, b, cc = io.read_3d_obj(str(tmp_path / "t.obj"))
+    np.testing.assert_allclose(a, v3)
+    np.testing.assert_array_equal(b, f3)
+    np.testing.assert_allclose(cc, v3[:, 2])
+
+
+def test_srgb_transfer_is_inverse():
+    from wos import io
+    x = np.linspace(0, 1, 101)
+    np.testing.assert_allclose(io.srgb_to_linear(io.linear_to_srgb(x)), x, atol=1e-12)
+    assert abs(io.linear_to_srgb(0.0031308) - 0.0031308 * 12.92) < 1e-9
