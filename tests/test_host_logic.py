@@ -81,7 +81,7 @@ def test_exr_round_trip_and_layout(tmp_path):
     assert b[:8] == struct.pack("<ii", 20000630, 2)
     assert b"channels\0chlist\0" + struct.pack("<i", 19) + b"Y\0" + struct.pack("<iB3xii", 2, 0, 1, 1) + b"\0" in b
     assert b"compression\0compression\0" + struct.pack("<i", 1) + b"\0" in b
-    assert b"dataWindow\0box2i\0" + struct.pack("<iiii", 0, 0, 1, 0) in b
+    assert b"dataWindow\0box2i\0" + struct.pack("<i", 16) + struct.pack("<iiii", 0, 0, 1, 0) in b
     assert b.endswith(struct.pack("<ii", 0, 8) + struct.pack("<ff", 1.5, -2.0))
     off = struct.unpack_from("<Q", b, len(b) - 16 - 8)[0]
     assert off == len(b) - 16
