@@ -76,6 +76,10 @@ struct GridInputs3 {
 };
 int build_start_grid_3d(const GridInputs3 &in, int resolution, hipStream_t stream, StartGridOutputs &out);
 
+// Permutation of n points into Morton order within [amin, amax] (device).
+int sort_points_morton(const float *points, int n, int D, const float *amin, const float *amax, int *order,
+                       hipStream_t stream);
+
 // ---- solver launches (wos_kernels.hip) -------------------------------------
 struct Workspace;   // per-device scratch owned by the C ABI layer
 
@@ -101,6 +105,7 @@ struct Tuning {
     int wgs_per_cu = 8;     // persistent workgroups per CU
     int top_levels = 6;     // BVH levels kept in LDS
     int window = 8;         // node visits per lane between refills of the walk kernel
+    int spatial_order = 1;  // visit query points in Morton order (0: input order)
 };
 extern Tuning g_tuning;
 

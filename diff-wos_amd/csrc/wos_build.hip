@@ -1434,4 +1434,65 @@ int build_start_grid_3d(const GridInputs3 &in, int resolution, hipStream_t strea
     return 0;
 }
 
+// ---------------------------------------------------------------------------
+// spatial order of query points
+// ---------------------------------------------------------------------------
+// Morton codes of the points inside the scene's box (points outside clamp to
+// its faces); the walk kernels visit points in this order, so the walks in
+// flight at any time start close together and share the BVH nodes they
+// touch in L2.  Results do not depend on it (walks are keyed by their
+// original point and walk index).
+__global__ void k_point_morton(const float *p, int n, int D, float3 lo, float3 inv, uint32_t *codes, int *vals) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const float l[3] = {lo.x, lo.y, lo.z}, iv[3] = {inv.x, inv.y, inv.z};
+        float u[3];
+        for (int k = 0; k < 3; ++k) u[k] = k < D ? (p[(long)i * D + k] - l[k]) * iv[k] : 0.5f;
+        uint32_t code;
+        if (D == 2) {
+            uint32_t x = (uint32_t)fminf(fmaxf(u[0] * 65536.0f, 0.0f), 65535.0f);
+            uint32_t y = (uint32_t)fminf(fmaxf(u[1] * 65536.0f, 0.0f), 65535.0f);
+            code = expand2(x) | (expand2(y) << 1);
+        } else {
+            uint32_t x = (uint32_t)fminf(fmaxf(u[0] * 1024.0f, 0.0f), 1023.0f);
+            uint32_t y = (uint32_t)fminf(fmaxf(u[1] * 1024.0f, 0.0f), 1023.0f);
+            uint32_t z = (uint32_t)fminf(fmaxf(u[2] * 1024.0f, 0.0f), 1023.0f);
+            code = (expand3(x) << 2) | (expand3(y) << 1) | expand3(z);
+        }
+        codes[i] = code;
+        vals[i] = i;
+    }
+}
+
+int sort_points_morton(const float *points, int n, int D, const float *amin, const float *amax, int *order,
+                       hipStream_t stream) {
+    uint32_t *keys[2] = {nullptr, nullptr}, *hist = nullptr;
+    int *vals[2] = {nullptr, nullptr};
+    const int nblocks = (n + kRsTile - 1) / kRsTile;
+    for (int b = 0; b < 2; ++b) {
+        BCHK(hipMallocAsync((void **)&keys[b], sizeof(uint32_t) * (size_t)n, stream));
+        BCHK(hipMallocAsync((void **)&vals[b], sizeof(int) * (size_t)n, stream));
+    }
+    BCHK(hipMallocAsync((void **)&hist, sizeof(uint32_t) * 256 * (size_t)nblocks, stream));
+    float3 lo = make_float3(amin[0], amin[1], amin[2]);
+    float e[3];
+    for (int k = 0; k < 3; ++k) e[k] = amax[k] - amin[k];
+    float3 inv = make_float3(e[0] > 0 ? 1.0f / e[0] : 0.0f, e[1] > 0 ? 1.0f / e[1] : 0.0f, e[2] > 0 ? 1.0f / e[2] : 0.0f);
+    k_point_morton<<<grid_for(n), kBlock, 0, stream>>>(points, n, D, lo, inv, keys[0], vals[0]);
+    for (int pass = 0; pass < 4; ++pass) {
+        int src = pass & 1, dst = src ^ 1;
+        k_rs_hist<<<nblocks, kBlock, 0, stream>>>(keys[src], n, 8 * pass, nblocks, hist);
+        k_rs_scan<<<1, 1024, 0, stream>>>(hist, 256L * nblocks);
+        k_rs_scatter<<<nblocks, kBlock, 0, stream>>>(keys[src], vals[src], n, 8 * pass, nblocks, hist, keys[dst],
+                                                     vals[dst]);
+    }
+    BCHK(hipMemcpyAsync(order, vals[0], sizeof(int) * (size_t)n, hipMemcpyDeviceToDevice, stream));
+    for (int b = 0; b < 2; ++b) {
+        (void)hipFreeAsync(keys[b], stream);
+        (void)hipFreeAsync(vals[b], stream);
+    }
+    (void)hipFreeAsync(hist, stream);
+    BCHK(hipGetLastError());
+    return 0;
+}
+
 }  // namespace wos

@@ -356,6 +356,11 @@ int wos_set_tuning(int chunk, int wgs_per_cu, int top_levels, int window) {
     return 0;
 }
 
+int wos_set_spatial_order(int enable) {
+    g_tuning.spatial_order = enable ? 1 : 0;
+    return 0;
+}
+
 int wos_get_tuning(int *chunk, int *wgs_per_cu, int *top_levels, int *window) {
     if (window) *window = g_tuning.window;
     if (chunk) *chunk = g_tuning.chunk;

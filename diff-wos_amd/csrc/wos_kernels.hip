@@ -222,6 +222,7 @@ struct PersistArgs {
     int window;                      // node visits per lane between refills
     const int *init_prim;            // per point: closest primitive of the start point
     const float *init_sd;            // per point: signed distance of the start point
+    const int *order;                // visiting order of the points (nullable: input order)
     float *walk_value;
     int *rec_prim;
     float *rec_t;
@@ -345,11 +346,13 @@ __global__ __launch_bounds__(kBlock) void k_walk_persistent(SceneView s, Persist
             }
             if (mode == kIdle) {
                 int r = rank_in_mask(idle);
-                long long kk = -1;
-                if (r < avail) kk = c_next + r;
-                else if (fresh && (r - avail) < end2 - base2) kk = base2 + (r - avail);
-                if (kk >= 0) {
-                    long long i = kk / W;
+                long long slot = -1;
+                if (r < avail) slot = c_next + r;
+                else if (fresh && (r - avail) < end2 - base2) slot = base2 + (r - avail);
+                if (slot >= 0) {
+                    // slot -> (point, walk): points in the visiting order
+                    const long long i = a.order ? (long long)a.order[slot / W] : slot / W;
+                    const long long kk = i * W + slot % W;
                     int ip = a.init_prim[i];
                     float sd = a.init_sd[i];
                     load_point<D>(a.points, i, p);
@@ -583,6 +586,7 @@ struct GradArgs {
     int chunk, window;
     const int *init_prim;            // per point: closest primitive of the start point
     const float *init_sd;            // per point: signed distance of the start point
+    const int *order;                // visiting order of the points (nullable: input order)
     const float *grad_out;
     float *grad_vertices;
     float *sample_grad_u;
@@ -875,11 +879,12 @@ __global__ __launch_bounds__(kBlock) void k_grad_persistent(SceneView s, GradArg
             }
             if (mode == kIdle) {
                 int r = rank_in_mask(idle);
-                long long kk = -1;
-                if (r < avail) kk = c_next + r;
-                else if (fresh && (r - avail) < end2 - base2) kk = base2 + (r - avail);
-                if (kk >= 0) {
-                    const long long i = kk / W;
+                long long slot = -1;
+                if (r < avail) slot = c_next + r;
+                else if (fresh && (r - avail) < end2 - base2) slot = base2 + (r - avail);
+                if (slot >= 0) {
+                    const long long i = a.order ? (long long)a.order[slot / W] : slot / W;
+                    const long long kk = i * W + slot % W;
                     const int ip = a.init_prim[i];
                     const float sd = a.init_sd[i];
                     k = kk;
@@ -961,6 +966,20 @@ static int occupancy_grid() {
         if (cus <= 0) cus = 256;
     }
     return cus * (g_tuning.wgs_per_cu > 0 ? g_tuning.wgs_per_cu : 8);
+}
+
+// Visiting order of the query points for the walk kernels: Morton order
+// within the scene's box for point sets large enough to pay for the sort
+// (null: input order).
+constexpr int kOrderMinPoints = 4096;
+static int spatial_order(const SceneView &s, const float *points, int n_points, int **order, hipStream_t st) {
+    *order = nullptr;
+    if (n_points < kOrderMinPoints || g_tuning.spatial_order == 0) return 0;
+    if (hipMallocAsync((void **)order, sizeof(int) * (size_t)n_points, st) != hipSuccess) {
+        set_error("cannot allocate the point order (%d points)", n_points);
+        return -1;
+    }
+    return sort_points_morton(points, n_points, s.dim, s.amin, s.amax, *order, st);
 }
 
 int launch_closest_index(const SceneView &s, const float *p, int n, int *out, float *out_d2, hipStream_t st) {
@@ -1080,6 +1099,9 @@ int launch_walk_reduce(const SceneView &s, const WalkSettings &w, const float *p
     }
     a.init_prim = iprim;
     a.init_sd = isd;
+    int *order = nullptr;
+    if (spatial_order(s, points, n_points, &order, st) != 0) return -1;
+    a.order = order;
     size_t tb = top_bytes(s);
     if (s.dim == 2) k_point_init<2><<<gi, kBlock, tb, st>>>(s, points, n_points, iprim, isd, a.spill, a.spill_stride);
     else k_point_init<3><<<gi, kBlock, tb, st>>>(s, points, n_points, iprim, isd, a.spill, a.spill_stride);
@@ -1102,6 +1124,7 @@ int launch_walk_reduce(const SceneView &s, const WalkSettings &w, const float *p
     }
     (void)hipFreeAsync(iprim, st);
     (void)hipFreeAsync(isd, st);
+    if (order) (void)hipFreeAsync(order, st);
     KCHK();
     if (o.mean || o.mean64 || o.var64) {
         if (!o.walk_value) {
@@ -1177,6 +1200,9 @@ int launch_shape_gradient(const SceneView &s, const GradSettings &g, const float
     }
     a.init_prim = iprim;
     a.init_sd = isd;
+    int *order = nullptr;
+    if (spatial_order(s, points, n_points, &order, st) != 0) return -1;
+    a.order = order;
     size_t tb = top_bytes(s);
     if (s.dim == 2) k_point_init<2><<<gi, kBlock, tb, st>>>(s, points, n_points, iprim, isd, a.spill, a.spill_stride);
     else k_point_init<3><<<gi, kBlock, tb, st>>>(s, points, n_points, iprim, isd, a.spill, a.spill_stride);
@@ -1185,6 +1211,7 @@ int launch_shape_gradient(const SceneView &s, const GradSettings &g, const float
     else k_grad_persistent<3><<<gr, kBlock, tb, st>>>(s, a);
     (void)hipFreeAsync(iprim, st);
     (void)hipFreeAsync(isd, st);
+    if (order) (void)hipFreeAsync(order, st);
     KCHK();
     return 0;
 }

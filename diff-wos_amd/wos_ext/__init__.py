@@ -99,6 +99,7 @@ _sig("wos_batch_run", _vp, _vp)
 _sig("wos_batch_destroy", _vp)
 _sig("wos_set_tuning", _i, _i, _i, _i)
 _sig("wos_get_tuning", _vp, _vp, _vp, _vp)
+_sig("wos_set_spatial_order", _i)
 _sig("wos_adjoint_values", _vp, _i, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp)
 _sig("wos_shape_gradient", _vp, ctypes.POINTER(_GradParams), _vp, _i, _i, _vp, _vp, _vp, _vp)
 
@@ -108,7 +109,7 @@ EXPORTED = ("wos_last_error", "wos_scene_create", "wos_scene_destroy", "wos_scen
             "wos_scene_set_start_grid", "wos_scene_get_start_grid", "wos_closest_point_index", "wos_sdf", "wos_closest_point",
             "wos_largest_inscribed_ball", "wos_dirichlet", "wos_single_walk", "wos_walk_reduce",
             "wos_walk_reduce_counts", "wos_adjoint_values", "wos_shape_gradient", "wos_set_tuning",
-            "wos_get_tuning", "wos_batch_create", "wos_batch_run", "wos_batch_destroy")
+            "wos_get_tuning", "wos_set_spatial_order", "wos_batch_create", "wos_batch_run", "wos_batch_destroy")
 
 
 def set_tuning(chunk=None, wgs_per_cu=None, top_levels=None, window=None):
@@ -118,6 +119,12 @@ def set_tuning(chunk=None, wgs_per_cu=None, top_levels=None, window=None):
     c, w, t, v = get_tuning()
     _check(lib.wos_set_tuning(int(chunk or c), int(wgs_per_cu or w), int(t if top_levels is None else top_levels),
                               int(window or v)))
+
+
+def set_spatial_order(enable=True):
+    """Visit query points in Morton order (default) or input order; results
+    are identical, only cache behaviour changes."""
+    _check(lib.wos_set_spatial_order(int(bool(enable))))
 
 
 def get_tuning():

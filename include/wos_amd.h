@@ -79,6 +79,11 @@ int wos_scene_create(int dim, const float *vertices, int n_vertices, const int *
  * the autotuner in the Python package. */
 int wos_set_tuning(int chunk, int wgs_per_cu, int top_levels, int window);
 int wos_get_tuning(int *chunk, int *wgs_per_cu, int *top_levels, int *window);
+/* Whether walk launches visit the query points in Morton order within the
+ * scene's box (default 1) or in input order (0).  Results are identical
+ * either way; the order only changes which BVH nodes the walks in flight
+ * share in L2.  Process-wide; no counterpart in the reference. */
+int wos_set_spatial_order(int enable);
 /* Scene destructor (the reference frees its thrust buffers with the object). */
 int wos_scene_destroy(wos_scene *scene);
 /* Scene.values / Scene.source_params setters (def_readwrite in

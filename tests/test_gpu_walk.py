@@ -147,3 +147,31 @@ def test_adjoint_values_and_coeffs():
         fd = float(g.astype(np.float64) @ ((up - dn) / (2 * h)))
         assert abs(fd - float(gc[k])) <= 2e-3 * (abs(fd) + 1.0), (k, fd, float(gc[k]))
     assert gc.shape[0] == K
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_spatial_order_leaves_results_unchanged(dim):
+    """Visiting the points in Morton order only reschedules walks: every
+    walk's value, exit record and the per-point means are bit-identical."""
+    import wos_ext
+    t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).cuda()
+    if dim == 2:
+        c = C.config2(res=96)
+        sc = wos_ext.Scene(t(c["vertices"]), t(c["indices"]), t(c["values"]), use_bvh=True, source_type=3,
+                           source_params=t(c["source_params"]))
+        w = wos_ext.WoS(nwalks=16, nsteps=256, epsilon=1e-4)
+        pts = t(c["points"])
+    else:
+        v, f, noise = C.displaced_sphere(3)
+        sc = wos_ext.Scene3D(t(v), t(f), t(C.g_harmonic3(v).astype(np.float32)), use_bvh=True)
+        w = wos_ext.WoS3D(nwalks=16, nsteps=256, epsilon=1e-3)
+        pts = t(C.points_in_star(8000, noise))
+    try:
+        wos_ext.set_spatial_order(True)
+        a = w.walk_reduce(pts, sc, seed=4, records=True, basis=dim == 2)
+        wos_ext.set_spatial_order(False)
+        b = w.walk_reduce(pts, sc, seed=4, records=True, basis=dim == 2)
+    finally:
+        wos_ext.set_spatial_order(True)
+    for k in a:
+        assert torch.equal(a[k], b[k]), k
