@@ -121,8 +121,9 @@ __global__ __launch_bounds__(kBlock) void k_batch_walk(const BatchItem *items, i
     q.best = q.thr = 0.0f;
     q.best_id = -1;
     q.nbox = q.nprim = 0;
-    SceneView tv;            // what a traversal step reads: the item's nodes (no LDS copy)
+    SceneView tv;            // what a traversal step reads: the item's nodes and leaf triangles (no LDS copy)
     tv.nodes = nullptr;
+    tv.leaves = nullptr;
     tv.n_top = 0;
     float tol = 0.0f;
     float bas[kMaxBatchBasis];
@@ -199,6 +200,7 @@ __global__ __launch_bounds__(kBlock) void k_batch_walk(const BatchItem *items, i
                         d = d0;
                         step = 0;
                         tv.nodes = b.v.nodes;
+                        tv.leaves = b.v.leaves;
                         tol = b.v.tol;
                         mode = kStart;
                     }
