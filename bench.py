@@ -354,7 +354,7 @@ def main():
         achieved = flops / launch_s / 1e12
         roofline = {"bound": "valu", "achieved": round(achieved, 3), "peak": PEAK_FP32 / 1e12,
                     "unit": "TFLOP/s", "frac": round(achieved * 1e12 / PEAK_FP32, 4), "traffic": measured_traffic(),
-                    "kernel": "k_walk_persistent<2,true,true,false>", "flops_per_launch": float(flops),
+                    "kernel": "k_walk_persistent<2,true,8,false>", "flops_per_launch": float(flops),
                     "launch_ms": round(np.mean(kernel_ms), 3),
                     "per_query": {"box_tests": counts["box_tests"] / counts["queries"],
                                   "prim_tests": counts["prim_tests"] / counts["queries"]},

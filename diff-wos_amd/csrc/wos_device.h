@@ -847,8 +847,9 @@ __device__ __forceinline__ void move_point(float *p, float d, Pcg32 &rng) {
 }
 
 // Lane modes of the persistent walk kernels: no work, a step to take, a
-// query in progress, a query just completed.
-enum : int { kIdle = 0, kStart = 1, kQuery = 2, kReady = 3 };
+// query in progress, a query just completed, a finished walk whose outputs
+// are not yet written.
+enum : int { kIdle = 0, kStart = 1, kQuery = 2, kReady = 3, kDone = 4 };
 
 // wave helpers ---------------------------------------------------------------
 __device__ __forceinline__ int lane_id() { return __lane_id(); }

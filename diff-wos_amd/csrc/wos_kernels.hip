@@ -20,6 +20,16 @@
 namespace wos {
 
 constexpr int kMaxBasis = 16;   // Fourier source terms carried per lane
+// Basis template argument of the walk kernel: KB > 0 carries exactly KB
+// Fourier terms (no per-term guards), kAnyBasis up to kMaxBasis terms guarded
+// by the run-time count, 0 none.
+constexpr int kAnyBasis = -1;
+__host__ __device__ constexpr int basis_cap(int KB) { return KB < 0 ? kMaxBasis : KB; }
+// Walks that end wait (outputs held in registers) until this many lanes of
+// the wave are finished or idle, then write their outputs and are refilled
+// together: the write / refill path then runs with several lanes active
+// instead of once per round for one or two.
+constexpr int kFinishBatch = 8;
 constexpr int kGridCap = 4096;  // workgroups of a grid-stride launch
 
 #define KCHK()                                                        \
@@ -249,7 +259,7 @@ __global__ __launch_bounds__(kBlock) void k_point_init(SceneView s, const float 
 }
 
 
-template <int D, bool BASIS>
+template <int D, int KB>
 __device__ __forceinline__ void write_walk(const SceneView &s, const PersistArgs &a, long long k, bool valid,
                                            int prim, const float *p, float contrib, const float *bas) {
     float v = 0.0f, t0 = 0.0f, t1 = 0.0f;
@@ -268,10 +278,13 @@ __device__ __forceinline__ void write_walk(const SceneView &s, const PersistArgs
         if (D == 2) a.rec_t[k] = t0;
         else { a.rec_t[2 * k] = t0; a.rec_t[2 * k + 1] = t1; }
     }
-    if constexpr (BASIS) {
+    if constexpr (KB != 0) {
+        float *bp = a.basis + k;
 #pragma unroll
-        for (int q = 0; q < kMaxBasis; ++q)
-            if (q < a.K) a.basis[(long long)q * a.total + k] = valid ? bas[q] : 0.0f;
+        for (int q = 0; q < basis_cap(KB); ++q) {
+            if (KB > 0 || q < a.K) *bp = valid ? bas[q] : 0.0f;
+            bp += a.total;
+        }
     }
 }
 
@@ -279,13 +292,18 @@ __device__ __forceinline__ void write_walk(const SceneView &s, const PersistArgs
 // the source term, jump to the sphere, begin the closest-primitive query) ->
 // kQuery (the query advances a few node visits per round) -> kReady (the
 // closest primitive of the new point is known: stop on the boundary or take
-// the next step).  Lanes whose walk ends take the next walk index of the
-// wave's chunk (ballot + mbcnt) and the wave claims a new chunk from the
+// the next step) -> kDone when the walk ends.  Finished walks write their
+// outputs in batches of kFinishBatch lanes; the lanes then take the next walk
+// indices of the wave's chunk (ballot + mbcnt) and the wave claims a new chunk from the
 // global counter when its chunk runs dry, so neither walk lengths nor query
 // lengths idle the SIMD for long.  Every walk re-derives its random stream
 // from (seed, point, walk index), so results do not depend on the schedule.
-template <int D, bool SRC, bool BASIS, bool STATS>
-__global__ __launch_bounds__(kBlock) void k_walk_persistent(SceneView s, PersistArgs a) {
+// KB: Fourier basis sums carried per lane (see kAnyBasis).  The 2D kernels
+// are held to 64 VGPRs so that 8 waves share each SIMD (measured 14 % faster
+// on the clover than 6 waves at 65-78 VGPRs); the 3D kernel measured best
+// without the limit.
+template <int D, bool SRC, int KB, bool STATS>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(D == 2 ? 8 : 1))) void k_walk_persistent(SceneView s, PersistArgs a) {
     DECLARE_STACK(a.spill, a.spill_stride);
     const int lane = __lane_id();
     long long k = -1;
@@ -300,11 +318,11 @@ __global__ __launch_bounds__(kBlock) void k_walk_persistent(SceneView s, Persist
     q.best = q.thr = 0.0f;
     q.best_id = -1;
     q.nbox = q.nprim = 0;
-    float bas[BASIS ? kMaxBasis : 1];
+    float bas[KB != 0 ? basis_cap(KB) : 1];
 #pragma unroll
-    for (int i = 0; i < (BASIS ? kMaxBasis : 1); ++i) bas[i] = 0.0f;
+    for (int i = 0; i < (KB != 0 ? basis_cap(KB) : 1); ++i) bas[i] = 0.0f;
     long long c_next = 0, c_end = 0;
-    bool exhausted = false;
+    bool exhausted = false, done_valid = false;
     const int W = a.w.nwalks;
 
     for (;;) {
@@ -313,17 +331,21 @@ __global__ __launch_bounds__(kBlock) void k_walk_persistent(SceneView s, Persist
         if (mode == kReady) {
             prim = q.best_id;
             d = sqrtf(q.best);
-            bool done_valid = d < a.w.epsilon && step < a.w.nsteps;
-            bool done_invalid = !done_valid && step >= a.w.nsteps;
-            if (done_valid || done_invalid) {
-                write_walk<D, BASIS>(s, a, k, done_valid, prim, p, contrib, bas);
+            done_valid = d < a.w.epsilon && step < a.w.nsteps;
+            const bool done_invalid = !done_valid && step >= a.w.nsteps;
+            mode = (done_valid || done_invalid) ? kDone : kStart;
+        }
+        // 2. write finished walks once kFinishBatch lanes are done or idle
+        //    (or nothing else is left to run), then refill idle lanes from
+        //    the wave's chunk
+        if (__popcll(__ballot(mode == kDone || mode == kIdle)) >= kFinishBatch || exhausted ||
+            __ballot(mode == kStart || mode == kQuery) == 0) {
+            if (mode == kDone) {
+                write_walk<D, KB>(s, a, k, done_valid, prim, p, contrib, bas);
                 mode = kIdle;
                 k = -1;
-            } else {
-                mode = kStart;
             }
         }
-        // 2. refill idle lanes from the wave's chunk
         uint64_t idle = __ballot(mode == kIdle);
         if (idle != 0 && !exhausted) {
             int need = __popcll(idle);
@@ -358,15 +380,15 @@ __global__ __launch_bounds__(kBlock) void k_walk_persistent(SceneView s, Persist
                     load_point<D>(a.points, i, p);
                     contrib = 0.0f;
 #pragma unroll
-                    for (int j = 0; j < (BASIS ? kMaxBasis : 1); ++j) bas[j] = 0.0f;
+                    for (int j = 0; j < (KB != 0 ? basis_cap(KB) : 1); ++j) bas[j] = 0.0f;
                     // include/wos.cuh:100-106, then the first loop test
                     bool outside = ip < 0 || (sd >= 0.0f && !a.w.double_sided);
                     float d0 = fabsf(sd);
                     if (a.w.prevent_fd_artifacts) d0 *= 0.5f;
                     if (outside || a.w.nsteps <= 0) {
-                        write_walk<D, BASIS>(s, a, kk, false, ip, p, 0.0f, bas);
+                        write_walk<D, KB>(s, a, kk, false, ip, p, 0.0f, bas);
                     } else if (d0 < a.w.epsilon) {
-                        write_walk<D, BASIS>(s, a, kk, true, ip, p, 0.0f, bas);
+                        write_walk<D, KB>(s, a, kk, true, ip, p, 0.0f, bas);
                     } else {
                         k = kk;
                         rng = walk_stream(a.seed, i, kk);
@@ -424,15 +446,18 @@ __global__ __launch_bounds__(kBlock) void k_walk_persistent(SceneView s, Persist
                 float G = log_poly(d / rr) * kInv2Pi;
                 if (pdf > 1e-5f) {
                     float yx = px0 + dyx, yy = py0 + dyy;
-                    if constexpr (BASIS) {
+                    if constexpr (KB != 0) {
                         float f = 0.0f;
 #pragma unroll
-                        for (int j = 0; j < kMaxBasis; ++j) {
-                            if (j < a.K) {
+                        for (int j = 0; j < basis_cap(KB); ++j) {
+                            if (KB > 0 || j < a.K) {
                                 const float *c = s.src + 4 * j;
                                 float b = sin_cw(fmaf(c[0], yx, fmaf(c[1], yy, c[2])));
                                 f = fmaf(c[3], b, f);
                                 bas[j] = fmaf(b * G, area, bas[j]);
+                                // one term at a time: hoisting every term's
+                                // coefficient load costs registers (occupancy)
+                                __builtin_amdgcn_sched_barrier(0);
                             }
                         }
                         contrib = fmaf(f * G, area, contrib);
@@ -1110,17 +1135,19 @@ int launch_walk_reduce(const SceneView &s, const WalkSettings &w, const float *p
     bool bas = src && o.basis != nullptr && K > 0;
     if (s.dim == 2) {
         if (s.stats) {
-            if (bas) k_walk_persistent<2, true, true, true><<<g, kBlock, tb, st>>>(s, a);
-            else if (src) k_walk_persistent<2, true, false, true><<<g, kBlock, tb, st>>>(s, a);
-            else k_walk_persistent<2, false, false, true><<<g, kBlock, tb, st>>>(s, a);
+            if (bas && a.K == 8) k_walk_persistent<2, true, 8, true><<<g, kBlock, tb, st>>>(s, a);
+            else if (bas) k_walk_persistent<2, true, kAnyBasis, true><<<g, kBlock, tb, st>>>(s, a);
+            else if (src) k_walk_persistent<2, true, 0, true><<<g, kBlock, tb, st>>>(s, a);
+            else k_walk_persistent<2, false, 0, true><<<g, kBlock, tb, st>>>(s, a);
         } else {
-            if (bas) k_walk_persistent<2, true, true, false><<<g, kBlock, tb, st>>>(s, a);
-            else if (src) k_walk_persistent<2, true, false, false><<<g, kBlock, tb, st>>>(s, a);
-            else k_walk_persistent<2, false, false, false><<<g, kBlock, tb, st>>>(s, a);
+            if (bas && a.K == 8) k_walk_persistent<2, true, 8, false><<<g, kBlock, tb, st>>>(s, a);
+            else if (bas) k_walk_persistent<2, true, kAnyBasis, false><<<g, kBlock, tb, st>>>(s, a);
+            else if (src) k_walk_persistent<2, true, 0, false><<<g, kBlock, tb, st>>>(s, a);
+            else k_walk_persistent<2, false, 0, false><<<g, kBlock, tb, st>>>(s, a);
         }
     } else {
-        if (s.stats) k_walk_persistent<3, false, false, true><<<g, kBlock, tb, st>>>(s, a);
-        else k_walk_persistent<3, false, false, false><<<g, kBlock, tb, st>>>(s, a);
+        if (s.stats) k_walk_persistent<3, false, 0, true><<<g, kBlock, tb, st>>>(s, a);
+        else k_walk_persistent<3, false, 0, false><<<g, kBlock, tb, st>>>(s, a);
     }
     (void)hipFreeAsync(iprim, st);
     (void)hipFreeAsync(isd, st);
