@@ -117,15 +117,24 @@ __device__ __forceinline__ void sincos_turns(float u, float &s, float &c) {
     apply_quadrant((int)q, s0, c0, s, c);
 }
 
+// sin x for |x| < 2^24: x = k pi + r by a three-part Cody-Waite reduction
+// (pi rounded to float, then the float roundings of the remainders), so
+// |r| <= pi/2 and sin x = (-1)^k sin r; sin r by its Taylor series through
+// r^13 (truncation below 7e-10 on |r| <= pi/2).
 __device__ __forceinline__ float sin_cw(float x) {
-    float k = rintf(x * 0.636619772367581343076f);
-    float r = fmaf(-k, 1.57079637050628662109375f, x);
-    r = fmaf(-k, -4.37113900018624283e-08f, r);
-    r = fmaf(-k, -1.7151245100059209e-15f, r);
-    float s0, c0, s, c;
-    sincos_poly(r, s0, c0);
-    apply_quadrant((int)k, s0, c0, s, c);
-    return s;
+    const float k = rintf(x * 0.3183098733425140380859375f);
+    float r = fmaf(-k, 3.1415927410125732421875f, x);
+    r = fmaf(-k, -8.742277657347585773095489e-08f, r);
+    r = fmaf(-k, -3.430249020011763745607869e-15f, r);
+    const float r2 = r * r;
+    float p = fmaf(r2, 1.605904437207428259171138e-10f, -2.505210794367940252413973e-08f);
+    p = fmaf(r2, p, 2.755731884462875314056873e-06f);
+    p = fmaf(r2, p, -1.984127011382952332496643e-04f);
+    p = fmaf(r2, p, 8.333333767950534820556641e-03f);
+    p = fmaf(r2, p, -1.666666716337203979492188e-01f);
+    const float sr = fmaf(r * r2, p, r);
+    const int odd = (int)fminf(fmaxf(k, -1073741824.0f), 1073741824.0f) & 1;
+    return __int_as_float(__float_as_int(sr) ^ (odd << 31));
 }
 
 __device__ __forceinline__ float log_poly(float x) {

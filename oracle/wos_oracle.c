@@ -130,16 +130,29 @@ void or_sincos_turns(float u, float *s, float *c) {
     quadrant((int)q, s0, c0, s, c);
 }
 
-/* sin(x) for moderate |x|: three-part Cody-Waite reduction by pi/2. */
+/* sin(x) for |x| < 2^24: x = k pi + r by a three-part Cody-Waite reduction
+ * (pi rounded to float, then the float roundings of the remainders), |r| <=
+ * pi/2, sin x = (-1)^k sin r with sin r by its Taylor series through r^13.
+ * The device sin_cw performs the same operations. */
 float or_sinf(float x) {
-    float k = rintf(x * 0.636619772367581343076f);
-    float r = fmaf(-k, 1.57079637050628662109375f, x);
-    r = fmaf(-k, -4.37113900018624283e-08f, r);
-    r = fmaf(-k, -1.7151245100059209e-15f, r);
-    float s0, c0, s, c;
-    sincos_poly(r, &s0, &c0);
-    quadrant((int)k, s0, c0, &s, &c);
-    return s;
+    const float k = rintf(x * 0.3183098733425140380859375f);
+    float r = fmaf(-k, 3.1415927410125732421875f, x);
+    r = fmaf(-k, -8.742277657347585773095489e-08f, r);
+    r = fmaf(-k, -3.430249020011763745607869e-15f, r);
+    const float r2 = r * r;
+    float p = fmaf(r2, 1.605904437207428259171138e-10f, -2.505210794367940252413973e-08f);
+    p = fmaf(r2, p, 2.755731884462875314056873e-06f);
+    p = fmaf(r2, p, -1.984127011382952332496643e-04f);
+    p = fmaf(r2, p, 8.333333767950534820556641e-03f);
+    p = fmaf(r2, p, -1.666666716337203979492188e-01f);
+    const float sr = fmaf(r * r2, p, r);
+    const int odd = (int)fminf(fmaxf(k, -1073741824.0f), 1073741824.0f) & 1;
+    uint32_t b;
+    memcpy(&b, &sr, 4);
+    b ^= (uint32_t)odd << 31;
+    float out;
+    memcpy(&out, &b, 4);
+    return out;
 }
 
 /* log(x), x > 0 finite: x = m 2^e, m in [sqrt(1/2), sqrt(2)),
