@@ -342,7 +342,9 @@ __global__ void k_karras(const uint32_t *codes, int n, int *children, int *par_i
     }
 }
 
-// Leaf payload in Morton order.
+// Leaf payload in Morton order; in 2D also a copy in primitive order after
+// it (leaves + 2 n), which step queries read to seed themselves with the
+// previous primitive.
 __global__ void k_leaves(const int *sorted_prim, int n, int D, const float *vtx, const int *idx, float4 *leaves) {
     for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < n; j += gridDim.x * blockDim.x) {
         int f = sorted_prim[j];
@@ -353,6 +355,8 @@ __global__ void k_leaves(const int *sorted_prim, int n, int D, const float *vtx,
             float inv = 1.0f / dot2(bax, bay, bax, bay);
             leaves[2 * j] = make_float4(ax, ay, bax, bay);
             leaves[2 * j + 1] = make_float4(inv, __int_as_float(f), 0.0f, 0.0f);
+            leaves[2 * (long)n + 2 * f] = make_float4(ax, ay, bax, bay);
+            leaves[2 * (long)n + 2 * f + 1] = make_float4(inv, __int_as_float(j), 0.0f, 0.0f);
         } else {
             int i0 = idx[3 * f], i1 = idx[3 * f + 1], i2 = idx[3 * f + 2];
             leaves[3 * j] = make_float4(vtx[3 * i0], vtx[3 * i0 + 1], vtx[3 * i0 + 2], vtx[3 * i1]);

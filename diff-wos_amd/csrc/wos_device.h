@@ -271,6 +271,7 @@ struct SceneView {
     const float *src;      // source parameters (device)
     const float4 *nodes;   // n_prims - 1 internal nodes
     const float4 *leaves;  // n_prims leaves
+    const float4 *pleaves; // 2D: the same leaf payload in primitive order
     float tol;             // absolute pruning margin (scale-relative)
     int max_depth;
     unsigned long long *stats;  // null, or {box tests, primitive tests, queries, walk steps}
@@ -339,12 +340,10 @@ __device__ __forceinline__ bool better(float d2, int id, float best_d2, int best
 template <int D>
 __device__ __forceinline__ float prim_d2(const SceneView &s, int f, const float *p) {
     if constexpr (D == 2) {
-        int i0 = s.idx[2 * f], i1 = s.idx[2 * f + 1];
-        float ax = s.vtx[2 * i0], ay = s.vtx[2 * i0 + 1];
-        float bax = s.vtx[2 * i1] - ax, bay = s.vtx[2 * i1 + 1] - ay;
-        float inv = 1.0f / dot2(bax, bay, bax, bay);
+        const float4 g = s.pleaves[2 * f];
+        const float inv = s.pleaves[2 * f + 1].x;
         float h, dx, dy;
-        return seg_d2(p[0], p[1], ax, ay, bax, bay, inv, h, dx, dy);
+        return seg_d2(p[0], p[1], g.x, g.y, g.z, g.w, inv, h, dx, dy);
     } else {
         int i0 = s.idx[3 * f], i1 = s.idx[3 * f + 1], i2 = s.idx[3 * f + 2];
         float a[3] = {s.vtx[3 * i0], s.vtx[3 * i0 + 1], s.vtx[3 * i0 + 2]};

@@ -426,12 +426,14 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(D == 2 ?
             move_point<D>(p, d, rng);
             ++step;
             if constexpr (STATS) atomicAdd(&s.stats[3], 1ull);
-            int i0 = 0, i1 = 0, start = 0;
+            int start = 0;
+            float4 sg = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            float sinv = 0.0f;
             if (s.use_bvh) {
                 start = start_node_of<D>(s, p);
                 if constexpr (D == 2) {
-                    i0 = s.idx[2 * prim];
-                    i1 = s.idx[2 * prim + 1];
+                    sg = s.pleaves[2 * prim];
+                    sinv = s.pleaves[2 * prim + 1].x;
                 }
             }
             if (SRC) {
@@ -479,11 +481,8 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(D == 2 ?
                 }
                 float seed_d2;
                 if constexpr (D == 2) {
-                    float ax = s.vtx[2 * i0], ay = s.vtx[2 * i0 + 1];
-                    float bax = s.vtx[2 * i1] - ax, bay = s.vtx[2 * i1 + 1] - ay;
-                    float inv = 1.0f / dot2(bax, bay, bax, bay);
                     float h, dx, dy;
-                    seed_d2 = seg_d2(p[0], p[1], ax, ay, bax, bay, inv, h, dx, dy);
+                    seed_d2 = seg_d2(p[0], p[1], sg.x, sg.y, sg.z, sg.w, sinv, h, dx, dy);
                 } else {
                     seed_d2 = prim_d2<D>(s, prim, p);
                 }
