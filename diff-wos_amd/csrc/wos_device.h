@@ -254,11 +254,15 @@ __device__ __forceinline__ float tri_closest(const float *pa, const float *pb, c
 // scene view (passed by value to every kernel)
 // ---------------------------------------------------------------------------
 //
-// BVH node layout (binary LBVH, children's boxes stored in the parent):
-//   2D node: 3 x float4  {Lminx, Lminy, Lmaxx, Lmaxy} {R...} {left, right, -, -}
-//   3D node: 4 x float4  {Lmin.xyz, Lmax.x} {Lmax.yz, Rmin.xy} {Rmin.z, Rmax.xyz} {left, right, -, -}
-//   child >= 0: internal node index; child < 0: leaf ~j (j = Morton order)
-//   2D leaf: 2 x float4  {a.xy, (b-a).xy} {1/|b-a|^2, prim, -, -}
+// BVH node layout (binary LBVH, children's bounds stored in the parent; the
+// packing kernels in wos_build.hip describe the words in detail):
+//   2D node: kNodeWords2 float4, per child a capsule axis or box, plus {inv,
+//            r} per child and the child ids (node index, or ~primitive for a
+//            leaf whose segment is stored inline)
+//   3D node: kNodeWords3 float4, per child a box and a plane slab, or the
+//            leaf triangle inline with ~primitive
+//   2D leaf: 2 x float4  {a.xy, (b-a).xy} {1/|b-a|^2, prim, -, -} in Morton
+//            order (leaves), and again in primitive order (pleaves)
 //   3D leaf: 3 x float4  {a.xyz, b.x} {b.yz, c.xy} {c.z, prim, -, -}
 struct SceneView {
     int dim, n_vertices, n_prims, use_bvh;
