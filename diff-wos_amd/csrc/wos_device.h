@@ -262,7 +262,8 @@ __device__ __forceinline__ float tri_closest(const float *pa, const float *pb, c
 //   3D node: kNodeWords3 float4, per child a box and a plane slab, or the
 //            leaf triangle inline with ~primitive
 //   2D leaf: 2 x float4  {a.xy, (b-a).xy} {1/|b-a|^2, prim, -, -} in Morton
-//            order (leaves), and again in primitive order (pleaves)
+//            order (leaves), and again in primitive order (pleaves, whose
+//            second word holds the Morton position instead of prim)
 //   3D leaf: 3 x float4  {a.xyz, b.x} {b.yz, c.xy} {c.z, prim, -, -}
 struct SceneView {
     int dim, n_vertices, n_prims, use_bvh;
