@@ -266,7 +266,8 @@ __global__ __launch_bounds__(kBlock) void k_batch_walk(const BatchItem *items, i
                     q.best_id = prim;
                     mode = kReady;
                 } else {
-                    query_begin<D>(q, bound_d2(s, 2.0f * d), prim, seed_d2, tol, start_node_of<D>(s, p));
+                    query_begin_from<D>(q, start_nodes_of<D>(s, p), bound_d2(s, 2.0f * d), prim, seed_d2, tol, st,
+                                        threadIdx.x, lane_g);
                     mode = kQuery;
                 }
             }

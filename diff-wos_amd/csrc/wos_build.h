@@ -63,7 +63,7 @@ int build_grid_2d(const GridInputs &in, const GridSettings &gs, hipStream_t stre
 constexpr int kStartGridResolution2 = 512;
 constexpr int kStartGridResolution3 = 0;   // measured no faster on the displaced spheres; opt-in
 struct StartGridOutputs {
-    int *start = nullptr;   // nx * ny * nz node indices (hipMalloc, owned by the caller afterwards)
+    int *start = nullptr;   // per cell, 2D: int4 {first node, second or -1, key bits, 0}; 3D: int node (hipMalloc, owned by the caller)
     float ox = 0, oy = 0, oz = 0, inv_cell = 0;
     int nx = 0, ny = 0, nz = 0;
 };

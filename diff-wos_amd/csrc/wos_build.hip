@@ -1234,29 +1234,61 @@ int build_grid_2d(const GridInputs &in, const GridSettings &gs, hipStream_t stre
 // within R = D + 2 rho of x (as for the candidate grid); descending from the
 // root while exactly one child's bound comes within R of x keeps them all,
 // so a query from any point of the cell may start its traversal at N.
-__global__ void k_start_cells_2d(const float4 *nodes, const float4 *leaves, int n_prims, GridGeom g, int *start) {
+// Descends from node while exactly one child's bound comes within R of
+// (x, y): every primitive within R of the point lies below the node reached.
+// Leaves never become start nodes (a start node is an internal node).  lb:
+// the bound of the node reached at (x, y), as its parent stores it (left
+// as passed in when the descent does not move).
+__device__ int descend_single(const float4 *nodes, int node, float x, float y, float R, bool &split, float &lb) {
+    for (;;) {
+        const float4 *nd = nodes + 4 * node;
+        float4 n0 = nd[0], n1 = nd[1], n2 = nd[2], n3 = nd[3];
+        int c0 = __float_as_int(n3.x), c1 = __float_as_int(n3.y);
+        float s0 = child_axis_d2(x, y, n0, n2.x), s1 = child_axis_d2(x, y, n1, n2.z);
+        float lb0 = c0 < 0 ? sqrtf(s0) : child_bound_2d(x, y, n0, s0, n2.y);
+        float lb1 = c1 < 0 ? sqrtf(s1) : child_bound_2d(x, y, n1, s1, n2.w);
+        bool p0 = lb0 <= R, p1 = lb1 <= R;
+        // both children within R: a split point when both are internal
+        split = p0 && p1 && c0 >= 0 && c1 >= 0;
+        if (p0 == p1) return node;
+        int next = p0 ? c0 : c1;
+        if (next < 0) return node;
+        node = next;
+        lb = p0 ? lb0 : lb1;
+    }
+}
+
+// Per cell: two start nodes {a, b, key} (b = -1: one suffices).  The
+// descent stops at the first node with both children within D + 2 rho of
+// the cell centre; when both are internal the cell starts at both children
+// instead, each descended the same way, so a query skips the levels where
+// only one side can hold its answer.  key is a lower bound of b's bound over
+// the whole cell (the bound is 1-Lipschitz: its value at the centre minus
+// rho, less the float margin), with which a query can prune b unvisited.
+__global__ void k_start_cells_2d(const float4 *nodes, const float4 *leaves, int n_prims, GridGeom g, int4 *start) {
     const float two_rho = 1.41421356f * g.cell;
     for (long c = blockIdx.x * (long)blockDim.x + threadIdx.x; c < (long)g.nx * g.ny; c += (long)gridDim.x * blockDim.x) {
         float x, y;
         cell_center(g, (float)(c % g.nx) + 0.5f, (float)(c / g.nx) + 0.5f, x, y);
-        int node = 0;
+        int4 out = make_int4(0, -1, 0, 0);
         if (n_prims > 1) {
             const float R = (sqrtf(grid_nearest_d2(nodes, leaves, n_prims, x, y)) + two_rho) * 1.00001f + g.margin;
-            for (;;) {
+            bool split = false;
+            float lb = 0.0f;
+            const int node = descend_single(nodes, 0, x, y, R, split, lb);
+            out.x = node;
+            if (split) {
                 const float4 *nd = nodes + 4 * node;
-                float4 n0 = nd[0], n1 = nd[1], n2 = nd[2], n3 = nd[3];
-                int c0 = __float_as_int(n3.x), c1 = __float_as_int(n3.y);
-                float s0 = child_axis_d2(x, y, n0, n2.x), s1 = child_axis_d2(x, y, n1, n2.z);
-                float lb0 = c0 < 0 ? sqrtf(s0) : child_bound_2d(x, y, n0, s0, n2.y);
-                float lb1 = c1 < 0 ? sqrtf(s1) : child_bound_2d(x, y, n1, s1, n2.w);
-                bool p0 = lb0 <= R, p1 = lb1 <= R;
-                if (p0 == p1) break;
-                int next = p0 ? c0 : c1;
-                if (next < 0) break;
-                node = next;
+                const float4 n0 = nd[0], n1 = nd[1], n2 = nd[2], n3 = nd[3];
+                float lb0 = child_bound_2d(x, y, n0, child_axis_d2(x, y, n0, n2.x), n2.y);
+                float lb1 = child_bound_2d(x, y, n1, child_axis_d2(x, y, n1, n2.z), n2.w);
+                bool s2 = false;
+                out.x = descend_single(nodes, __float_as_int(n3.x), x, y, R, s2, lb0);
+                out.y = descend_single(nodes, __float_as_int(n3.y), x, y, R, s2, lb1);
+                out.z = __float_as_int(fmaxf(0.0f, (lb1 - 0.5f * two_rho) * 0.99999f - g.margin));
             }
         }
-        start[c] = node;
+        start[c] = out;
     }
 }
 
@@ -1276,8 +1308,9 @@ int build_start_grid_2d(const GridInputs &in, int resolution, hipStream_t stream
     g.margin = 8.0f * in.tol;
     g.cap = 0;
     int *start = nullptr;
-    BCHK(hipMalloc((void **)&start, sizeof(int) * (size_t)g.nx * g.ny));
-    k_start_cells_2d<<<grid_for((long)g.nx * g.ny), kBlock, 0, stream>>>(in.nodes, in.leaves, in.n_prims, g, start);
+    BCHK(hipMalloc((void **)&start, sizeof(int4) * (size_t)g.nx * g.ny));
+    k_start_cells_2d<<<grid_for((long)g.nx * g.ny), kBlock, 0, stream>>>(in.nodes, in.leaves, in.n_prims, g,
+                                                                         reinterpret_cast<int4 *>(start));
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) {
         (void)hipFree(start);

@@ -395,22 +395,27 @@ __device__ __forceinline__ float prune_for(float best_d2, float tol) {
     return fmaf(__builtin_amdgcn_sqrtf(best_d2), 1.00000095367431640625f, tol);
 }
 
-// Node a closest-primitive query from p may start at: the cell's entry of
-// the start-node grid, or the root.
+// Nodes a closest-primitive query from p may start at: the cell's entry of
+// the start-node grid, or the root.  In 2D a cell holds two nodes (x, y;
+// y = -1 when one suffices) whose subtrees together hold every primitive
+// that can be closest to a point of the cell, and in z the bits of a lower
+// bound of y's distance over the cell; in 3D one node.
 template <int D>
-__device__ __forceinline__ int start_node_of(const SceneView &s, const float *p) {
-    if (!s.start_node) return 0;
+__device__ __forceinline__ int4 start_nodes_of(const SceneView &s, const float *p) {
+    if (!s.start_node) return make_int4(0, -1, 0, 0);
     const float fx = (p[0] - s.st_ox) * s.st_inv_cell;
     const float fy = (p[1] - s.st_oy) * s.st_inv_cell;
     // written so that NaN coordinates fail the test
-    if (!(fx >= 0.0f && fy >= 0.0f && fx < (float)s.st_nx && fy < (float)s.st_ny)) return 0;
+    if (!(fx >= 0.0f && fy >= 0.0f && fx < (float)s.st_nx && fy < (float)s.st_ny)) return make_int4(0, -1, 0, 0);
     long c = (long)(int)fy * s.st_nx + (int)fx;
     if constexpr (D == 3) {
         const float fz = (p[2] - s.st_oz) * s.st_inv_cell;
-        if (!(fz >= 0.0f && fz < (float)s.st_nz)) return 0;
+        if (!(fz >= 0.0f && fz < (float)s.st_nz)) return make_int4(0, -1, 0, 0);
         c += (long)(int)fz * s.st_nx * s.st_ny;
+        return make_int4(s.start_node[c], -1, 0, 0);
+    } else {
+        return reinterpret_cast<const int4 *>(s.start_node)[c];
     }
-    return s.start_node[c];
 }
 
 // Starts a query whose answer lies within bound_d2, optionally seeded with a
@@ -437,6 +442,21 @@ __device__ __forceinline__ void stack_push(Query &q, const TravStack &st, int sl
         st.spill[(long)(q.sp - kLdsStack) * st.spill_stride + spill_lane] = make_int2(node, __float_as_int(key));
     }
     ++q.sp;
+}
+
+// query_begin at the start nodes of p's cell: the first is visited first,
+// the second (when present) waits on the stack under its cell-wide bound.
+template <int D>
+__device__ __forceinline__ void query_begin_from(Query &q, int4 start, float bound_d2, int seed_id, float seed_d2,
+                                                 float tol, const TravStack &st, int slot, long spill_lane) {
+    query_begin<D>(q, bound_d2, seed_id, seed_d2, tol, start.x);
+    if (start.y >= 0) stack_push(q, st, slot, spill_lane, start.y, __int_as_float(start.z));
+}
+template <int D>
+__device__ __forceinline__ void query_begin_at(Query &q, const SceneView &s, const float *p, float bound_d2,
+                                               int seed_id, float seed_d2, const TravStack &st, int slot,
+                                               long spill_lane) {
+    query_begin_from<D>(q, start_nodes_of<D>(s, p), bound_d2, seed_id, seed_d2, s.tol, st, slot, spill_lane);
 }
 
 // Pops until an entry survives the threshold; false when the stack is empty.
@@ -653,7 +673,7 @@ __device__ int closest_bvh(const SceneView &s, const float *p, float bound_d2, f
         return 0;
     }
     Query q;
-    query_begin<D>(q, bound_d2, 0x7fffffff, __builtin_inff(), s.tol, start_node_of<D>(s, p));
+    query_begin_at<D>(q, s, p, bound_d2, 0x7fffffff, __builtin_inff(), st, lane_slot, spill_lane);
     while (!query_step<D, true>(s, p, q, st, lane_slot, spill_lane, s.tol)) {
     }
     query_flush_stats<true>(s, q);

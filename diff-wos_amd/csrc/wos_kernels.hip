@@ -426,11 +426,11 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(D == 2 ?
             move_point<D>(p, d, rng);
             ++step;
             if constexpr (STATS) atomicAdd(&s.stats[3], 1ull);
-            int start = 0;
+            int4 start = make_int4(0, -1, 0, 0);
             float4 sg = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
             float sinv = 0.0f;
             if (s.use_bvh) {
-                start = start_node_of<D>(s, p);
+                start = start_nodes_of<D>(s, p);
                 if constexpr (D == 2) {
                     sg = s.pleaves[2 * prim];
                     sinv = s.pleaves[2 * prim + 1].x;
@@ -491,7 +491,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(D == 2 ?
                     q.best_id = prim;
                     mode = kReady;
                 } else {
-                    query_begin<D>(q, bound_d2(s, 2.0f * d), prim, seed_d2, s.tol, start);
+                    query_begin_from<D>(q, start, bound_d2(s, 2.0f * d), prim, seed_d2, s.tol, st, slot, lane_g);
                     mode = kQuery;
                 }
             }
@@ -875,7 +875,7 @@ __global__ __launch_bounds__(kBlock) void k_grad_persistent(SceneView s, GradArg
                     q.best = seed_d2;
                     q.best_id = rprim;
                 } else {
-                    query_begin<D>(q, __builtin_inff(), rprim, seed_d2, s.tol, start_node_of<D>(s, p));
+                    query_begin_at<D>(q, s, p, __builtin_inff(), rprim, seed_d2, st, slot, lane_g);
                     mode = kQuery;
                 }
             }
@@ -954,7 +954,7 @@ __global__ __launch_bounds__(kBlock) void k_grad_persistent(SceneView s, GradArg
                 q.best_id = prim;
                 mode = kReady;
             } else {
-                query_begin<D>(q, bound_d2(s, 2.0f * d), prim, seed_d2, s.tol, start_node_of<D>(s, p));
+                query_begin_at<D>(q, s, p, bound_d2(s, 2.0f * d), prim, seed_d2, st, slot, lane_g);
                 mode = kQuery;
             }
         }
