@@ -58,9 +58,15 @@ struct GridOutputs {
 };
 int build_grid_2d(const GridInputs &in, const GridSettings &gs, hipStream_t stream, GridOutputs &out);
 
-// Start-node grid (see start_node_of): `resolution` cells along the longest
-// side of the bounding box; 0 builds none.
-constexpr int kStartGridResolution2 = 512;
+// Start-node grid (see start_nodes_of): `resolution` cells along the longest
+// side of the bounding box; 0 builds none; kStartGridAuto chooses it: in 2D
+// two cells per mean segment length (clamped to [64, kStartGridAutoMax2]:
+// finer cells start queries deeper, measured down to half a segment on the
+// clover), in 3D kStartGridAuto3.
+constexpr int kStartGridAuto = -1;
+constexpr int kStartGridAutoMax2 = 2048;
+constexpr int kStartGridAuto3 = 96;
+constexpr int kStartGridResolution2 = kStartGridAuto;
 constexpr int kStartGridResolution3 = 0;   // measured no faster on the displaced spheres; opt-in
 struct StartGridOutputs {
     int *start = nullptr;   // per cell, 2D: int4 {first node, second or -1, key bits, 0}; 3D: int node (hipMalloc, owned by the caller)

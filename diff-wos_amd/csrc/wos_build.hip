@@ -1294,9 +1294,21 @@ __global__ void k_start_cells_2d(const float4 *nodes, const float4 *leaves, int 
 
 int build_start_grid_2d(const GridInputs &in, int resolution, hipStream_t stream, StartGridOutputs &out) {
     out = StartGridOutputs{};
-    if (in.n_prims < 2 || resolution <= 0) return 0;
+    if (in.n_prims < 2 || resolution == 0) return 0;
     GridGeom g;
     double w = (double)in.amax[0] - in.amin[0], h = (double)in.amax[1] - in.amin[1];
+    if (resolution < 0) {
+        // two cells per mean segment length
+        double *dsum = nullptr, sum = 0.0;
+        BCHK(hipMallocAsync((void **)&dsum, sizeof(double), stream));
+        k_segment_length_sum<<<1, 1024, 0, stream>>>(in.vtx, in.idx, in.n_prims, dsum);
+        BCHK(hipMemcpyAsync(&sum, dsum, sizeof(double), hipMemcpyDeviceToHost, stream));
+        BCHK(hipFreeAsync(dsum, stream));
+        BCHK(hipStreamSynchronize(stream));
+        const double mean = sum / in.n_prims;
+        const double r = mean > 0.0 ? 2.0 * (w > h ? w : h) / mean : 64.0;
+        resolution = (int)std::min(std::max(std::lround(r), 64L), (long)kStartGridAutoMax2);
+    }
     double cell = (w > h ? w : h) / resolution;
     if (!(cell > 0.0)) return 0;
     g.cell = (float)cell;

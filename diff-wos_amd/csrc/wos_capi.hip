@@ -198,6 +198,7 @@ static int build_grid(wos_scene *s, const GridSettings &gs, hipStream_t st) {
 
 static int build_start_grid(wos_scene *s, int resolution, hipStream_t st) {
     if (s->dim == 2) return build_start_grid_2d(grid_inputs(s), resolution, st, s->sgrid);
+    if (resolution == kStartGridAuto) resolution = kStartGridAuto3;
     GridInputs3 gi;
     gi.n_prims = s->n_prims;
     gi.max_depth = s->max_depth;
@@ -321,7 +322,8 @@ int wos_scene_set_grid(wos_scene *s, float cell_factor, int band_cells, int max_
 int wos_scene_set_start_grid(wos_scene *s, int resolution, void *stream) {
     REQUIRE(s, "scene is NULL");
     const int cap = s->dim == 2 ? 8192 : 512;
-    REQUIRE(resolution >= 0 && resolution <= cap, "resolution must be in [0, %d] (got %d)", cap, resolution);
+    REQUIRE(resolution >= kStartGridAuto && resolution <= cap,
+            "resolution must be in [0, %d] or %d (automatic) (got %d)", cap, kStartGridAuto, resolution);
     CHK(hipDeviceSynchronize());   // kernels in flight may still read the old grid
     (void)hipFree(s->sgrid.start);
     s->sgrid = StartGridOutputs{};

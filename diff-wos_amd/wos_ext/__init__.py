@@ -372,11 +372,12 @@ class _SceneBase:
 
     def set_start_grid(self, resolution=None):
         """Rebuilds the start-node grid with `resolution` cells along the
-        longest side (default 512 in 2D, 96 in 3D, where scenes are built
+        longest side (default: two cells per mean segment length in 2D, at
+        most 2048, as scenes are built; 96 in 3D, where scenes are built
         without one; 0 removes it).  Query results are identical with and
         without it."""
         if resolution is None:
-            resolution = 512 if self._dim == 2 else 96
+            resolution = -1
         _check(lib.wos_scene_set_start_grid(self.handle, int(resolution), _stream()))
 
     def start_grid_info(self):

@@ -119,9 +119,12 @@ int wos_scene_get_grid(const wos_scene *scene, int *nx, int *ny, float *cell, un
  * are unchanged): each square (2D) or cubic (3D) cell over the bounding box
  * records the deepest LBVH node below which every primitive that can be
  * closest to a point of the cell lies, and traversals from that cell start
- * there instead of at the root.  wos_scene_create builds it with 512 cells
- * along the longer side in 2D and none in 3D (where it measured no faster);
- * 0 removes it.  Synchronises the device. */
+ * there instead of at the root (in 2D a cell may record two nodes).
+ * wos_scene_create builds it in 2D with two cells per mean segment length
+ * along the longer side (at most 2048), and none in 3D (where it measured no
+ * faster).  resolution: cells along the longer side, 0 removes the grid, -1
+ * chooses as wos_scene_create does in 2D and 96 in 3D.  Synchronises the
+ * device. */
 int wos_scene_set_start_grid(wos_scene *scene, int resolution, void *stream);
 int wos_scene_get_start_grid(const wos_scene *scene, int *nx, int *ny, int *nz);
 

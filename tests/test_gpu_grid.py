@@ -60,7 +60,7 @@ def test_grid_closest_bitexact(kind, n, grid):
 
 
 @pytest.mark.parametrize("kind,n", [("circle", 1024), ("clover", 4096), ("star", 2048), ("star", 7)])
-@pytest.mark.parametrize("res", [0, 16, 512, 3000])
+@pytest.mark.parametrize("res", [0, 16, 512, 3000, -1])
 def test_start_grid_closest_bitexact(kind, n, res):
     import wos_ext
     v, ix = _polyline(kind, n)
@@ -69,8 +69,14 @@ def test_start_grid_closest_bitexact(kind, n, res):
     sc.set_grid(0.0)           # isolate the start-node grid
     sc.set_start_grid(res)
     info = sc.start_grid_info()
-    assert (info["nx"] > 0) == (res > 0), info
-    rng = np.random.default_rng(n + res)
+    assert (info["nx"] > 0) == (res != 0), info
+    if res < 0:
+        # automatic: two cells per mean segment length along the longer side,
+        # within [64, 2048] (plus a one-cell border on each side)
+        seg = np.linalg.norm(v[ix[:, 1]] - v[ix[:, 0]], axis=1).mean()
+        side = (v.max(0) - v.min(0)).max()
+        assert max(info["nx"], info["ny"]) - 2 == pytest.approx(min(max(2 * side / seg, 64), 2048), abs=2), info
+    rng = np.random.default_rng(n + abs(res))
     p = np.concatenate([_near_boundary_points(v, ix, rng, 20000),
                         rng.uniform(-1.6, 1.6, size=(4000, 2)).astype(np.float32),
                         rng.uniform(-40, 40, size=(500, 2)).astype(np.float32)])
