@@ -69,7 +69,7 @@ constexpr int kStartGridAuto3 = 96;
 constexpr int kStartGridResolution2 = kStartGridAuto;
 constexpr int kStartGridResolution3 = 0;   // measured no faster on the displaced spheres; opt-in
 struct StartGridOutputs {
-    int *start = nullptr;   // per cell, 2D: int4 {first node, second or -1, key bits, 0}; 3D: int node (hipMalloc, owned by the caller)
+    int *start = nullptr;   // per cell int4 {first node, second or -1, key bits, 0} (hipMalloc, owned by the caller)
     float ox = 0, oy = 0, oz = 0, inv_cell = 0;
     int nx = 0, ny = 0, nz = 0;
 };

@@ -1412,33 +1412,57 @@ struct GridGeom3 {
 };
 
 // As k_start_cells_2d, for cubic cells (half-diagonal sqrt(3)/2 cell).
-__global__ void k_start_cells_3d(const float4 *nodes, const float4 *leaves, int n_prims, GridGeom3 g, int *start) {
+// 3D counterpart of descend_single: the child bounds are the packed box and
+// slab (or, for a leaf, the triangle distance).
+__device__ int descend_single_3d(const float4 *nodes, int node, const float *p, float R, bool &split, float &lb) {
+    for (;;) {
+        const float4 *nd = nodes + kNodeWords3 * node;
+        bool pass[2];
+        int ch[2];
+        float lbs[2];
+        for (int k = 0; k < 2; ++k) {
+            float4 x0 = nd[3 * k], x1 = nd[3 * k + 1], x2 = nd[3 * k + 2];
+            ch[k] = __float_as_int(x2.w);
+            lbs[k] = ch[k] < 0 ? sqrtf(leaf_block_d2(p, x0, x1, x2)) : child_bound_3d(p, x0, x1, x2);
+            pass[k] = lbs[k] <= R;
+        }
+        split = pass[0] && pass[1] && ch[0] >= 0 && ch[1] >= 0;
+        if (pass[0] == pass[1]) return node;
+        int next = pass[0] ? ch[0] : ch[1];
+        if (next < 0) return node;
+        node = next;
+        lb = pass[0] ? lbs[0] : lbs[1];
+    }
+}
+
+// Per cell {a, b, key, 0} as in 2D (k_start_cells_2d): the first node with
+// both children within D + 2 rho of the cell centre is replaced by both
+// children, each descended, when both are internal.
+__global__ void k_start_cells_3d(const float4 *nodes, const float4 *leaves, int n_prims, GridGeom3 g, int4 *start) {
     const float two_rho = 1.7320508f * g.cell;
     const long nxy = (long)g.nx * g.ny;
     for (long c = blockIdx.x * (long)blockDim.x + threadIdx.x; c < nxy * g.nz; c += (long)gridDim.x * blockDim.x) {
         long cz = c / nxy, r = c % nxy;
         float p[3] = {fmaf((float)(r % g.nx) + 0.5f, g.cell, g.ox), fmaf((float)(r / g.nx) + 0.5f, g.cell, g.oy),
                       fmaf((float)cz + 0.5f, g.cell, g.oz)};
-        int node = 0;
+        int4 out = make_int4(0, -1, 0, 0);
         if (n_prims > 1) {
             const float R = (sqrtf(nearest_d2_3d(nodes, leaves, n_prims, p)) + two_rho) * 1.00001f + g.margin;
-            for (;;) {
+            bool split = false;
+            float lb = 0.0f;
+            const int node = descend_single_3d(nodes, 0, p, R, split, lb);
+            out.x = node;
+            if (split) {
                 const float4 *nd = nodes + kNodeWords3 * node;
-                bool pass[2];
-                int ch[2];
-                for (int k = 0; k < 2; ++k) {
-                    float4 x0 = nd[3 * k], x1 = nd[3 * k + 1], x2 = nd[3 * k + 2];
-                    ch[k] = __float_as_int(x2.w);
-                    float lb = ch[k] < 0 ? sqrtf(leaf_block_d2(p, x0, x1, x2)) : child_bound_3d(p, x0, x1, x2);
-                    pass[k] = lb <= R;
-                }
-                if (pass[0] == pass[1]) break;
-                int next = pass[0] ? ch[0] : ch[1];
-                if (next < 0) break;
-                node = next;
+                float lb0 = child_bound_3d(p, nd[0], nd[1], nd[2]);
+                float lb1 = child_bound_3d(p, nd[3], nd[4], nd[5]);
+                bool s2 = false;
+                out.x = descend_single_3d(nodes, __float_as_int(nd[2].w), p, R, s2, lb0);
+                out.y = descend_single_3d(nodes, __float_as_int(nd[5].w), p, R, s2, lb1);
+                out.z = __float_as_int(fmaxf(0.0f, (lb1 - 0.5f * two_rho) * 0.99999f - g.margin));
             }
         }
-        start[c] = node;
+        start[c] = out;
     }
 }
 
@@ -1464,8 +1488,9 @@ int build_start_grid_3d(const GridInputs3 &in, int resolution, hipStream_t strea
     g.margin = 8.0f * in.tol;
     int *start = nullptr;
     const long ncells = (long)g.nx * g.ny * g.nz;
-    BCHK(hipMalloc((void **)&start, sizeof(int) * (size_t)ncells));
-    k_start_cells_3d<<<grid_for(ncells), kBlock, 0, stream>>>(in.nodes, in.leaves, in.n_prims, g, start);
+    BCHK(hipMalloc((void **)&start, sizeof(int4) * (size_t)ncells));
+    k_start_cells_3d<<<grid_for(ncells), kBlock, 0, stream>>>(in.nodes, in.leaves, in.n_prims, g,
+                                                                reinterpret_cast<int4 *>(start));
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) {
         (void)hipFree(start);

@@ -321,7 +321,7 @@ int wos_scene_set_grid(wos_scene *s, float cell_factor, int band_cells, int max_
 
 int wos_scene_set_start_grid(wos_scene *s, int resolution, void *stream) {
     REQUIRE(s, "scene is NULL");
-    const int cap = s->dim == 2 ? 8192 : 512;
+    const int cap = s->dim == 2 ? 8192 : 256;
     REQUIRE(resolution >= kStartGridAuto && resolution <= cap,
             "resolution must be in [0, %d] or %d (automatic) (got %d)", cap, kStartGridAuto, resolution);
     CHK(hipDeviceSynchronize());   // kernels in flight may still read the old grid

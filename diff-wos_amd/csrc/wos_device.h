@@ -396,10 +396,10 @@ __device__ __forceinline__ float prune_for(float best_d2, float tol) {
 }
 
 // Nodes a closest-primitive query from p may start at: the cell's entry of
-// the start-node grid, or the root.  In 2D a cell holds two nodes (x, y;
-// y = -1 when one suffices) whose subtrees together hold every primitive
-// that can be closest to a point of the cell, and in z the bits of a lower
-// bound of y's distance over the cell; in 3D one node.
+// the start-node grid, or the root.  A cell holds two nodes (x, y; y = -1
+// when one suffices) whose subtrees together hold every primitive that can
+// be closest to a point of the cell, and in z the bits of a lower bound of
+// y's distance over the cell.
 template <int D>
 __device__ __forceinline__ int4 start_nodes_of(const SceneView &s, const float *p) {
     if (!s.start_node) return make_int4(0, -1, 0, 0);
@@ -412,10 +412,8 @@ __device__ __forceinline__ int4 start_nodes_of(const SceneView &s, const float *
         const float fz = (p[2] - s.st_oz) * s.st_inv_cell;
         if (!(fz >= 0.0f && fz < (float)s.st_nz)) return make_int4(0, -1, 0, 0);
         c += (long)(int)fz * s.st_nx * s.st_ny;
-        return make_int4(s.start_node[c], -1, 0, 0);
-    } else {
-        return reinterpret_cast<const int4 *>(s.start_node)[c];
     }
+    return reinterpret_cast<const int4 *>(s.start_node)[c];
 }
 
 // Starts a query whose answer lies within bound_d2, optionally seeded with a
