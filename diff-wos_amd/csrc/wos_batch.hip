@@ -384,7 +384,7 @@ extern "C" int wos_batch_create(const wos_batch_item *items, int n_items, wos_ba
         BREQ(it.scene, "item %d has no scene", j);
         BREQ(it.params.nwalks >= 1 && it.params.nsteps >= 0, "item %d: bad walk settings", j);
         BREQ(it.n_points >= 0 && (it.n_points == 0 || it.points), "item %d: bad points", j);
-        BREQ(it.walk_value, "item %d: walk_value is required", j);
+        BREQ(it.n_points == 0 || it.walk_value, "item %d: walk_value is required", j);
         SceneView v = scene_view(it.scene);
         v.n_top = 0;
         BatchItem b;

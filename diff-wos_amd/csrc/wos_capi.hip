@@ -456,13 +456,13 @@ int wos_closest_point(const wos_scene *s, const float *p, int n, const wos_recor
 
 int wos_largest_inscribed_ball(const wos_scene *s, const wos_record *its, int n, float epsilon, float *out,
                                void *stream) {
-    REQUIRE(s && its && out && (n == 0 || (its->p && its->n)), "scene, its.p, its.n and out are required");
+    REQUIRE(s && its && (n == 0 || (its->p && its->n && out)), "scene, its.p, its.n and out are required");
     REQUIRE(n >= 0, "negative count");
     return launch_lib(view_of(s), its->valid, its->p, its->n, n, epsilon, out, S(stream));
 }
 
 int wos_dirichlet(const wos_scene *s, const wos_record *rec, int n, float *out, void *stream) {
-    REQUIRE(s && rec && out && (n == 0 || (rec->prim_id && rec->t)), "scene, rec.prim_id, rec.t and out are required");
+    REQUIRE(s && rec && (n == 0 || (rec->prim_id && rec->t && out)), "scene, rec.prim_id, rec.t and out are required");
     REQUIRE(n >= 0, "negative count");
     return launch_dirichlet(view_of(s), rec->valid, rec->prim_id, rec->t, n, out, S(stream));
 }
@@ -501,8 +501,9 @@ int wos_walk_reduce(const wos_scene *s, const wos_walk_params *w, const float *p
 
 int wos_walk_reduce_counts(const wos_scene *s, const wos_walk_params *w, const float *points, int n_points,
                            int seed, int with_basis, float *walk_value, unsigned long long *counts, void *stream) {
-    REQUIRE(s && counts && walk_value && (n_points == 0 || points), "scene, points, walk_value and counts are required");
+    REQUIRE(s && counts && (n_points == 0 || (points && walk_value)), "scene, points, walk_value and counts are required");
     if (check_walk(w)) return -1;
+    if (n_points == 0) return 0;
     SceneView v = view_of(s);
     v.stats = counts;
     WalkSettings ws{w->nwalks, w->nsteps, w->epsilon, w->double_sided, w->prevent_fd_artifacts};
@@ -532,7 +533,7 @@ int wos_shape_gradient(const wos_scene *s, const wos_grad_params *g, const float
                        const float *grad_out, float *grad_vertices, float *sample_grad_u, void *stream) {
     REQUIRE(s && g && (n_points == 0 || points), "scene, params and points are required");
     REQUIRE(n_points >= 0 && g->nwalks >= 1 && g->nsteps >= 0, "bad sizes");
-    REQUIRE(!grad_vertices || grad_out, "grad_vertices needs grad_out");
+    REQUIRE(n_points == 0 || !grad_vertices || grad_out, "grad_vertices needs grad_out");
     GradSettings gs{g->nwalks, g->nsteps, g->epsilon, g->epsilon2, g->clamping, g->double_sided, g->control_variates};
     return launch_shape_gradient(view_of(s), gs, points, n_points, (int32_t)seed, grad_out, grad_vertices,
                                  sample_grad_u, S(stream));
