@@ -143,6 +143,9 @@ struct GradSettings {
 };
 int launch_shape_gradient(const SceneView &s, const GradSettings &g, const float *points, int n_points,
                           int32_t seed, const float *grad_out, float *grad_vertices, float *sample_grad_u,
-                          hipStream_t st);
+                          hipStream_t st, const float *velocity = nullptr, float *sample_jvp = nullptr);
+int launch_shape_derivative(const SceneView &s, const GradSettings &g, const float *points, int n_points,
+                            int32_t seed, const float *velocity, float *mean, double *mean64, double *var64,
+                            hipStream_t st);
 
 }  // namespace wos

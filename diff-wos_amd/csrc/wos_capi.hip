@@ -539,4 +539,14 @@ int wos_shape_gradient(const wos_scene *s, const wos_grad_params *g, const float
                                  sample_grad_u, S(stream));
 }
 
+int wos_shape_derivative(const wos_scene *s, const wos_grad_params *g, const float *points, int n_points, int seed,
+                         const float *velocity, float *out_mean, double *out_mean64, double *out_var64,
+                         void *stream) {
+    REQUIRE(s && g && (n_points == 0 || (points && velocity)), "scene, params, points and velocity are required");
+    REQUIRE(n_points >= 0 && g->nwalks >= 1 && g->nsteps >= 0, "bad sizes");
+    GradSettings gs{g->nwalks, g->nsteps, g->epsilon, g->epsilon2, g->clamping, g->double_sided, g->control_variates};
+    return launch_shape_derivative(view_of(s), gs, points, n_points, (int32_t)seed, velocity, out_mean, out_mean64,
+                                   out_var64, S(stream));
+}
+
 }  // extern "C"

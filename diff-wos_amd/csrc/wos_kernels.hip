@@ -11,8 +11,8 @@
 //   k_reduce           per-point FP64 mean / variance of the walk values
 //   k_adjoint_values   reverse mode of the walk value w.r.t. vertex values and
 //                      Fourier source coefficients, from stored exit records
-//   k_shape_gradient   OursCUDA / Ours3D (wos/wos_grad.py:442-463,
-//                      wos/wos_grad_3d.py:295-315) transposed onto vertices
+//   k_shape_gradient   OursCUDA / Ours3D (wos/wos_grad.py:130-151,
+//                      wos/wos_grad_3d.py:117-137) transposed onto vertices
 #include <algorithm>
 
 #include "wos_build.h"
@@ -614,12 +614,14 @@ struct GradArgs {
     const float *grad_out;
     float *grad_vertices;
     float *sample_grad_u;
+    const float *velocity;           // forward mode: per-vertex velocity (n_vertices * D), nullable
+    float *sample_jvp;               // forward mode: per-sample -grad u . V(exit) (n_points * nwalks)
     int2 *spill;
     long spill_stride;
 };
 
-// Phases of a shape-gradient sample (wos/wos_grad.py:442-463 with
-// wos/wos.py:59-129; wos/wos_grad_3d.py:295-315 with wos/wos3d.py:97-156).
+// Phases of a shape-gradient sample (wos/wos_grad.py:130-151 with
+// wos/wos.py:59-129; wos/wos_grad_3d.py:117-137 with wos/wos3d.py:97-156).
 enum : int { kPrimary = 0, kLib = 1, kSecQuery = 2, kSecWalk = 3 };
 
 // Half the smallest bounding-box extent: the upper end of the LIB bisection
@@ -798,7 +800,7 @@ __global__ __launch_bounds__(kBlock) void k_grad_persistent(SceneView s, GradArg
             } else {   // kSecQuery: the start of secondary walk `side`
                 const float sd = signed_dist<D>(s, found, p);
                 if (fabsf(sd) < g.epsilon2) {
-                    // wos/wos.py:221-223: the walk would stop at once
+                    // wos/wos.py:96-98: the walk would stop at once
                     Rec r;
                     closest_record<D>(s, found, p, r);
                     u_side = dirichlet_at<D>(s, found, r.t, r.uv[0], r.uv[1]);
@@ -841,6 +843,27 @@ __global__ __launch_bounds__(kBlock) void k_grad_persistent(SceneView s, GradArg
                 }
                 if (a.sample_grad_u)
                     for (int c = 0; c < D; ++c) a.sample_grad_u[k * D + c] = gu[c];
+                if (a.sample_jvp) {
+                    // forward mode of the same integrand: the exit point moves
+                    // with the interpolated vertex velocity
+                    float jv = 0.0f;
+                    if (ok) {
+                        if constexpr (D == 2) {
+                            const int i0 = s.idx[2 * rprim], i1 = s.idx[2 * rprim + 1];
+                            for (int c = 0; c < 2; ++c)
+                                jv = fmaf(-gu[c], fmaf(rt0, a.velocity[2 * i1 + c], (1.0f - rt0) * a.velocity[2 * i0 + c]), jv);
+                        } else {
+                            const int i0 = s.idx[3 * rprim], i1 = s.idx[3 * rprim + 1], i2 = s.idx[3 * rprim + 2];
+                            const float wa = 1.0f - rt0 - rt1;
+                            for (int c = 0; c < 3; ++c) {
+                                const float vc = fmaf(rt1, a.velocity[3 * i2 + c],
+                                                      fmaf(rt0, a.velocity[3 * i1 + c], wa * a.velocity[3 * i0 + c]));
+                                jv = fmaf(-gu[c], vc, jv);
+                            }
+                        }
+                    }
+                    a.sample_jvp[k] = jv;
+                }
                 if (ok && a.grad_vertices && a.grad_out) {
                     const float gi = a.grad_out[i] / (float)W;
                     if constexpr (D == 2) {
@@ -920,6 +943,7 @@ __global__ __launch_bounds__(kBlock) void k_grad_persistent(SceneView s, GradArg
                     if (outside || g.nsteps <= 0) {
                         if (a.sample_grad_u)
                             for (int c = 0; c < D; ++c) a.sample_grad_u[kk * D + c] = 0.0f;
+                        if (a.sample_jvp) a.sample_jvp[kk] = 0.0f;
                         k = -1;
                     } else {
                         prim = ip;
@@ -1190,7 +1214,8 @@ int launch_adjoint_values(const SceneView &s, int n_points, int nwalks, const in
 }
 
 int launch_shape_gradient(const SceneView &s, const GradSettings &g, const float *points, int n_points, int32_t seed,
-                          const float *grad_out, float *grad_vertices, float *sample_grad_u, hipStream_t st) {
+                          const float *grad_out, float *grad_vertices, float *sample_grad_u, hipStream_t st,
+                          const float *velocity, float *sample_jvp) {
     if (n_points <= 0) return 0;
     if (!s.use_bvh) {
         set_error("the shape gradient needs the scene's BVH (use_bvh)");
@@ -1209,6 +1234,8 @@ int launch_shape_gradient(const SceneView &s, const GradSettings &g, const float
     a.grad_out = grad_out;
     a.grad_vertices = grad_vertices;
     a.sample_grad_u = sample_grad_u;
+    a.velocity = velocity;
+    a.sample_jvp = velocity ? sample_jvp : nullptr;
     long waves_needed = (long)((a.total + 63) / 64);
     int gr = (int)((waves_needed + 3) / 4);
     int cap = occupancy_grid();
@@ -1240,6 +1267,34 @@ int launch_shape_gradient(const SceneView &s, const GradSettings &g, const float
     if (order) (void)hipFreeAsync(order, st);
     KCHK();
     return 0;
+}
+
+// Forward mode of the shape-gradient integrand along a vertex velocity
+// field: per-sample -grad u . V(exit) into a scratch buffer, then the FP64
+// per-point mean / variance of k_reduce.
+int launch_shape_derivative(const SceneView &s, const GradSettings &g, const float *points, int n_points,
+                            int32_t seed, const float *velocity, float *mean, double *mean64, double *var64,
+                            hipStream_t st) {
+    if (n_points <= 0) return 0;
+    float *jvp = nullptr;
+    const size_t n = (size_t)n_points * (size_t)g.nwalks;
+    if (hipMallocAsync((void **)&jvp, sizeof(float) * n, st) != hipSuccess) {
+        set_error("cannot allocate %zu per-sample derivatives", n);
+        return -1;
+    }
+    int rc = launch_shape_gradient(s, g, points, n_points, seed, nullptr, nullptr, nullptr, st, velocity, jvp);
+    if (rc == 0 && (mean || mean64 || var64)) {
+        int gr = (int)(((long)n_points * 64 + kBlock - 1) / kBlock);
+        if (gr > 8192) gr = 8192;
+        k_reduce<<<gr, kBlock, 0, st>>>(jvp, n_points, g.nwalks, mean, mean64, var64);
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) {
+            set_error("kernel launch: %s", hipGetErrorString(e));
+            rc = -1;
+        }
+    }
+    (void)hipFreeAsync(jvp, st);
+    return rc;
 }
 
 }  // namespace wos

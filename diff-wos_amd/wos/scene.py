@@ -44,12 +44,17 @@ class Detector:
         return p + torch.stack([jx, jy], dim=1) * d
 
     def index(self, p):
+        """wos/scene.py:18-24: pixel of p, rows flipped.  The reference's
+        row (res_y - uv.y) puts uv.y = 0 one row past the image; such samples
+        are marked invalid here rather than scattered out of bounds."""
         vmin = torch.tensor(self.vmin, device=p.device)
         vmax = torch.tensor(self.vmax, device=p.device)
         uv = (p - vmin) / (vmax - vmin) * torch.tensor(self.res, device=p.device) + 0.5
         uv = uv.to(torch.int32)
         valid = (uv[:, 0] >= 0) & (uv[:, 0] < self.res[0]) & (uv[:, 1] >= 0) & (uv[:, 1] < self.res[1])
-        return valid, uv[:, 0] + (self.res[1] - uv[:, 1]) * self.res[0]
+        idx = uv[:, 0] + (self.res[1] - uv[:, 1]) * self.res[0]
+        valid = valid & (idx >= 0) & (idx < self.res[0] * self.res[1])
+        return valid, torch.where(valid, idx, torch.zeros_like(idx))
 
     def save(self, image, filename):
         np.save(str(filename), np.asarray(image))

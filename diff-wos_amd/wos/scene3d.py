@@ -24,6 +24,13 @@ class Detector3D(Detector):
     def size(self):
         return float(np.prod(np.subtract(self.vmax, self.vmin))) * self.l_z
 
+    def index(self, p):
+        """wos/scene3d.py:35-43: the 2D pixel of (x, y) for points within
+        half a pixel depth of the slab."""
+        inside_z = (p[:, 2] - float(self.z)).abs() < (self.l_z / 2.0)
+        valid, idx = super().index(p[:, :2])
+        return valid & inside_z, idx
+
     def make_points(self):
         p2 = super().make_points()
         return torch.cat([p2, torch.full((p2.shape[0], 1), float(self.z), device=p2.device)], dim=1)

@@ -37,6 +37,14 @@ def _requires(t):
     return isinstance(t, torch.Tensor) and t.requires_grad
 
 
+def _has_tangent(t):
+    """True for a forward-mode dual tensor (torch.autograd.forward_ad)."""
+    if not isinstance(t, torch.Tensor):
+        return False
+    import torch.autograd.forward_ad as fwAD
+    return fwAD.unpack_dual(t).tangent is not None
+
+
 def _source_tensor(scene):
     sp = getattr(scene, "source_params", None)
     return sp if isinstance(sp, torch.Tensor) else None
@@ -74,21 +82,30 @@ class _ValueWalk(torch.autograd.Function):
 
 class _ShapeWalk(torch.autograd.Function):
     """Zero-valued output whose gradient w.r.t. the vertices is the
-    OursCUDA / Ours3D shape-gradient estimate, walks recomputed from the key."""
+    OursCUDA / Ours3D shape-gradient estimate, walks recomputed from the key.
+    Forward mode (jvp) gives the derivative along a vertex velocity field
+    from the same samples -- what the reference's renderD takes with
+    dr.forward_to (diff_solve/runner.py:75-109)."""
 
     @staticmethod
-    def forward(ctx, solver, points, scene, seed, vertices):
-        ctx.solver, ctx.points, ctx.scene, ctx.seed = solver, points.detach(), scene, seed
+    def forward(ctx, solver, points, scene, seed, vertices, nwalks):
+        ctx.solver, ctx.points, ctx.scene, ctx.seed, ctx.nwalks = solver, points.detach(), scene, seed, nwalks
         return torch.zeros(points.shape[0], dtype=torch.float32, device=points.device)
 
     @staticmethod
     def backward(ctx, g):
         s = ctx.solver
-        gv, _ = wos_ext.shape_gradient(ctx.scene, ctx.points, ctx.seed, g.contiguous(), nwalks=s.nwalks,
+        gv, _ = wos_ext.shape_gradient(ctx.scene, ctx.points, ctx.seed, g.contiguous(), nwalks=ctx.nwalks,
                                        nsteps=s.nsteps, epsilon=s.epsilon, epsilon2=s.epsilon2,
                                        clamping=s.clamping, double_sided=s.double_sided,
                                        control_variates=getattr(s, "control_variates", True))
-        return None, None, None, None, gv
+        return None, None, None, None, gv, None
+
+    @staticmethod
+    def jvp(ctx, d_solver, d_points, d_scene, d_seed, d_vertices, d_nwalks):
+        if d_vertices is None:
+            return torch.zeros(ctx.points.shape[0], dtype=torch.float32, device=ctx.points.device)
+        return ctx.solver.walk_derivative(ctx.points, ctx.scene, d_vertices, seed=ctx.seed, nwalks=ctx.nwalks)
 
 
 @dataclass
@@ -241,11 +258,19 @@ class ShapeSolverMixin:
         return self._walk_with(p, scene, seed, nwalks, scene.vertices, scene.values, _source_tensor(scene))
 
     def _walk_with(self, p, scene, seed, nwalks, vertices, values, source_params):
-        if nwalks != self.nwalks:
-            raise ValueError("shape-gradient solvers walk nwalks per point")
-        if not (torch.is_grad_enabled() and _requires(vertices)):
+        if not ((torch.is_grad_enabled() and _requires(vertices)) or _has_tangent(vertices)):
             return torch.zeros(p.shape[0], dtype=torch.float32, device=p.device)
-        return _ShapeWalk.apply(self, p, scene, seed, vertices)
+        return _ShapeWalk.apply(self, p, scene, seed, vertices, int(nwalks))
+
+    def walk_derivative(self, p, scene, velocity, seed=0, stats=False, nwalks=None):
+        """Per point, the derivative of this estimator when vertex v moves
+        with velocity[v] (forward mode, wos_ext.shape_derivative); the
+        samples are those of the backward pass with the same seed."""
+        W = self.nwalks if nwalks is None else int(nwalks)
+        return wos_ext.shape_derivative(scene, p, seed, velocity, nwalks=W, nsteps=self.nsteps,
+                                        epsilon=self.epsilon, epsilon2=self.epsilon2, clamping=self.clamping,
+                                        double_sided=self.double_sided,
+                                        control_variates=getattr(self, "control_variates", True), stats=stats)
 
     def sample_grad_u(self, p, scene, seed=0):
         """Per-sample spatial gradient estimates (n_points*nwalks, dim)."""

@@ -102,13 +102,15 @@ _sig("wos_get_tuning", _vp, _vp, _vp, _vp)
 _sig("wos_set_spatial_order", _i)
 _sig("wos_adjoint_values", _vp, _i, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp)
 _sig("wos_shape_gradient", _vp, ctypes.POINTER(_GradParams), _vp, _i, _i, _vp, _vp, _vp, _vp)
+_sig("wos_shape_derivative", _vp, ctypes.POINTER(_GradParams), _vp, _i, _i, _vp, _vp, _vp, _vp, _vp)
 
 EXPORTED = ("wos_last_error", "wos_scene_create", "wos_scene_destroy", "wos_scene_set_values",
             "wos_scene_set_source", "wos_scene_set_use_bvh", "wos_scene_get_normals", "wos_scene_get_info",
             "wos_scene_get_bvh", "wos_scene_set_grid", "wos_scene_get_grid",
             "wos_scene_set_start_grid", "wos_scene_get_start_grid", "wos_closest_point_index", "wos_sdf", "wos_closest_point",
             "wos_largest_inscribed_ball", "wos_dirichlet", "wos_single_walk", "wos_walk_reduce",
-            "wos_walk_reduce_counts", "wos_adjoint_values", "wos_shape_gradient", "wos_set_tuning",
+            "wos_walk_reduce_counts", "wos_adjoint_values", "wos_shape_gradient", "wos_shape_derivative",
+            "wos_set_tuning",
             "wos_get_tuning", "wos_set_spatial_order", "wos_batch_create", "wos_batch_run", "wos_batch_destroy")
 
 
@@ -605,6 +607,27 @@ def shape_gradient(scene, points, seed, grad_out, *, nwalks, nsteps, epsilon, ep
     _check(lib.wos_shape_gradient(scene.handle, ctypes.byref(gp), _p(q), P, int(seed), _p(go),
                                   _p(gv if go is not None else None), _p(sg), _stream()))
     return gv, sg
+
+
+def shape_derivative(scene, points, seed, velocity, *, nwalks, nsteps, epsilon, epsilon2, clamping,
+                     double_sided=False, control_variates=True, stats=False):
+    """Forward mode of the OursCUDA / Ours3D integrand: per point, the
+    derivative of its estimate when vertex v moves with velocity[v]
+    (wos_shape_derivative).  Returns the float32 mean, and with stats=True
+    also (mean64, var64)."""
+    q = _dev(points, torch.float32, scene.dim)
+    P = q.shape[0]
+    vel = _dev(velocity, torch.float32, scene.dim)
+    if vel.shape[0] != scene.c_vertices.shape[0]:
+        raise ValueError(f"velocity needs one row per vertex ({scene.c_vertices.shape[0]}), got {vel.shape[0]}")
+    gp = _GradParams(int(nwalks), int(nsteps), float(epsilon), float(epsilon2), float(clamping),
+                     int(bool(double_sided)), int(bool(control_variates)))
+    mean = torch.empty(P, dtype=torch.float32, device=q.device)
+    m64 = torch.empty(P, dtype=torch.float64, device=q.device) if stats else None
+    v64 = torch.empty(P, dtype=torch.float64, device=q.device) if stats else None
+    _check(lib.wos_shape_derivative(scene.handle, ctypes.byref(gp), _p(q), P, int(seed), _p(vel), _p(mean),
+                                    _p(m64), _p(v64), _stream()))
+    return (mean, m64, v64) if stats else mean
 
 
 class Batch:

@@ -34,7 +34,7 @@ typedef struct {
     int   use_IS_for_greens;
 } wos_walk_params;
 
-/* OursCUDA / Ours3D settings (wos/wos_grad.py:425-440, wos/wos_grad_3d.py:283-293) */
+/* OursCUDA / Ours3D settings (wos/wos_grad.py:114-128, wos/wos_grad_3d.py:105-116) */
 typedef struct {
     int   nwalks;
     int   nsteps;
@@ -151,7 +151,7 @@ int wos_single_walk(const wos_scene *scene, const wos_walk_params *params, const
                     uint64_t *state, uint64_t *inc, const wos_record *out, void *stream);
 
 /* Solver.walk_reduce(p, scene, seed) (wos/solver.py:49-65) over the CUDA walk
- * value (wos/wos.py:171-179, wos/wos_with_source.py:257-265): W = nwalks
+ * value (wos/wos.py:46-54, wos/wos_with_source.py:257-265): W = nwalks
  * walks per point, walk k = i*W + w keyed by TEA(seed, i) and k.  One
  * persistent launch; outputs (all nullable except as noted):
  *   mean     n_points floats       (needs walk_value)
@@ -184,13 +184,26 @@ int wos_adjoint_values(const wos_scene *scene, int n_points, int nwalks, const i
                        void *stream);
 
 /* Shape gradient: CustomSolver backward (wos/solver.py:109-114) with
- * bwd_solver = OursCUDA (wos/wos_grad.py:442-463) in 2D or Ours3D
- * (wos/wos_grad_3d.py:295-315) in 3D, walks recomputed from (seed, point,
+ * bwd_solver = OursCUDA (wos/wos_grad.py:130-151) in 2D or Ours3D
+ * (wos/wos_grad_3d.py:117-137) in 3D, walks recomputed from (seed, point,
  * walk).  Adds dL/dvertices into grad_vertices (n_vertices*dim, nullable).
  * sample_grad_u (nullable): per-sample spatial gradient estimate,
  * n_points*nwalks*dim floats (0 for samples that do not contribute). */
 int wos_shape_gradient(const wos_scene *scene, const wos_grad_params *params, const float *points, int n_points,
                        int seed, const float *grad_out, float *grad_vertices, float *sample_grad_u, void *stream);
+
+/* Forward mode of the same integrand: the derivative of each point's
+ * OursCUDA / Ours3D estimate along a deformation that moves vertex v with
+ * velocity[v] (n_vertices*dim) -- what diff_solve/runner.py:75-109 (renderD)
+ * obtains with dr.forward_to on the image.  Per sample -grad u . V(exit),
+ * V interpolated like the exit point; per point the mean over the walks
+ * (out_mean f32, out_mean64 f64, out_var64 f64 variance of one sample; each
+ * nullable).  The samples are those of wos_shape_gradient with the same
+ * seed, so sum_i g_i out_mean[i] equals grad_vertices . velocity for
+ * grad_out = g up to float rounding. */
+int wos_shape_derivative(const wos_scene *scene, const wos_grad_params *params, const float *points, int n_points,
+                         int seed, const float *velocity, float *out_mean, double *out_mean64, double *out_var64,
+                         void *stream);
 
 /* ---- batches of scenes ----------------------------------------------------
  * No counterpart in the reference, which solves one scene per call: a batch

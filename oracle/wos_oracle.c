@@ -673,9 +673,9 @@ void or_adjoint_values(const or_scene *s, int n_points, int nwalks, const int *r
 
 /* ---- shape gradient ------------------------------------------------------- */
 
-/* WoS.single_walk at a secondary point (wos/wos.py:171-179 through the CUDA
+/* WoS.single_walk at a secondary point (wos/wos.py:46-54 through the CUDA
  * walk), used inside normal_derivative: |sdf| < eps2 -> boundary value of
- * the closest point (wos/wos.py:221-223).  The CUDA walk draws no random
+ * the closest point (wos/wos.py:96-98).  The CUDA walk draws no random
  * numbers in that case, so evaluating the walk unconditionally (as Dr.Jit's
  * select does) consumes the same stream. */
 static float secondary_u(const or_scene *s, const or_walk_params *w2, const float *y, or_pcg32 *rng) {
@@ -690,7 +690,7 @@ static float secondary_u(const or_scene *s, const or_walk_params *w2, const floa
     return or_walk_value(s, &rec);
 }
 
-/* wos/wos.py:184-232 (2D) / wos/wos3d.py:97-135 (3D) normal derivative, then
+/* wos/wos.py:59-107 (2D) / wos/wos3d.py:97-135 (3D) normal derivative, then
  * tangent derivative (wos/scene.py:451-460, wos/wos3d.py:137-156).
  * grad_u receives the spatial gradient estimate; returns 0 if the sample
  * does not contribute. */

@@ -89,7 +89,7 @@ float or_source(const or_scene *s, float x, float y);
  * the Fourier source, length n_source_params/4. */
 void  or_walk(const or_scene *s, const or_walk_params *w, const float *p,
               or_pcg32 *rng, or_record *rec, float *basis);
-/* wos/wos.py:171-179, wos/wos_with_source.py:257-265: per-walk value */
+/* wos/wos.py:46-54, wos/wos_with_source.py:257-265: per-walk value */
 float or_walk_value(const or_scene *s, const or_record *rec);
 
 /* wos/solver.py:49-65 with FP64 accumulation.  Per point: mean, variance of
@@ -115,7 +115,7 @@ typedef struct {
     int   double_sided, control_variates;
 } or_grad_params;
 
-/* wos/wos_grad.py:442-463 (OursCUDA) and wos/wos_grad_3d.py:295-315 (Ours3D),
+/* wos/wos_grad.py:130-151 (OursCUDA) and wos/wos_grad_3d.py:117-137 (Ours3D),
  * run through Solver.walk_reduce (wos/solver.py:49-65) and transposed:
  * grad_vertices[v] += grad_out[i]/W * d res / d v  (FP64). */
 void  or_shape_gradient(const or_scene *s, const or_grad_params *g, const float *points,

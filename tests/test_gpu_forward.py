@@ -1,0 +1,114 @@
+"""Forward-mode shape derivatives (the reference's renderD, which seeds a
+scene parameter's tangent and reads dr.grad of the image: diff_solve/
+runner.py:75-109): wos_shape_derivative against the reverse-mode shape
+gradient on the same samples (an exact identity up to float rounding), the
+torch forward-AD plumbing through the shape solvers, and a central
+difference of the primal."""
+import numpy as np
+import pytest
+import torch
+import torch.autograd.forward_ad as fwAD
+
+from wos import configs as C
+
+pytestmark = pytest.mark.gpu
+
+
+def _t(a):
+    return torch.from_numpy(np.ascontiguousarray(a)).cuda()
+
+
+def _scene(dim):
+    import wos_ext
+    if dim == 2:
+        v, ix = C.star_polyline(1024)
+        sc = wos_ext.Scene(_t(v), _t(ix), _t(C.g_saddle(v).astype(np.float32)), use_bvh=True)
+        rng = np.random.default_rng(5)
+        r, t = 0.55 * np.sqrt(rng.random(1500)), 2 * np.pi * rng.random(1500)
+        pts = np.stack([r * np.cos(t), r * np.sin(t)], 1).astype(np.float32)
+    else:
+        v, f = C.icosphere(3)
+        v, f = v.astype(np.float32), f.astype(np.int32)
+        sc = wos_ext.Scene3D(_t(v), _t(f), _t(C.g_harmonic3(v).astype(np.float32)), use_bvh=True)
+        rng = np.random.default_rng(6)
+        d = rng.normal(size=(800, 3))
+        pts = (0.6 * rng.random((800, 1)) ** (1 / 3) * d / np.linalg.norm(d, axis=1, keepdims=True)).astype(np.float32)
+    return sc, v, pts
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_shape_derivative_matches_reverse_mode(dim):
+    """sum_i g_i du_i/dV = grad_vertices(g) . V for the same seed: both sum
+    -grad u . V(exit) over the same samples."""
+    import wos_ext
+    sc, v, pts = _scene(dim)
+    rng = np.random.default_rng(dim)
+    g = rng.normal(size=pts.shape[0]).astype(np.float32)
+    V = rng.normal(size=v.shape).astype(np.float32)
+    kw = dict(nwalks=32, nsteps=128, epsilon=1e-3, epsilon2=1e-3, clamping=0.1)
+    mean, m64, var64 = wos_ext.shape_derivative(sc, _t(pts), 17, _t(V), stats=True, **kw)
+    gv, _ = wos_ext.shape_gradient(sc, _t(pts), 17, _t(g), **kw)
+    fwd = float((g.astype(np.float64) * m64.cpu().numpy()).sum())
+    rev = float((gv.cpu().numpy().astype(np.float64) * V).sum())
+    scale = float((np.abs(g) * np.abs(m64.cpu().numpy())).sum())
+    assert np.isfinite(fwd) and scale > 0
+    assert abs(fwd - rev) <= 1e-4 * scale, (fwd, rev, scale)
+    np.testing.assert_allclose(mean.cpu().numpy(), m64.cpu().numpy(), rtol=1e-5, atol=1e-6)
+    assert (var64.cpu().numpy() >= 0).all()
+
+
+def test_forward_ad_through_the_shape_solver_and_central_difference():
+    """renderD in torch: a dual deformation parameter s (uniform scaling of
+    the star, vertices = s * base) through OursCUDA gives per-point tangents
+    equal to wos_shape_derivative with V = base; d/ds of the mean square of
+    u then agrees with a central difference of the primal (common seeds)."""
+    import wos_ext
+    from wos.scene import Polyline
+    from wos.wos import WoSCUDA
+    from wos.wos_grad import OursCUDA
+    v, ix = C.star_polyline(1024)
+    vals = _t(C.g_saddle(v).astype(np.float32))
+    rng = np.random.default_rng(1)
+    r, t = 0.55 * np.sqrt(rng.random(2048)), 2 * np.pi * rng.random(2048)
+    pts = _t(np.stack([r * np.cos(t), r * np.sin(t)], 1).astype(np.float32))
+    base = _t(v)
+    ours = OursCUDA(nwalks=256, nsteps=256, epsilon=1e-4, epsilon2=1e-4, clamping=0.1)
+    with fwAD.dual_level():
+        s = fwAD.make_dual(torch.tensor(1.0, device="cuda"), torch.tensor(1.0, device="cuda"))
+        scene = Polyline(base * s, _t(ix), vals)
+        out = ours.walk(pts, scene, seed=11)
+        tangent = fwAD.unpack_dual(out).tangent
+    assert tangent is not None and tangent.shape == (pts.shape[0],)
+    direct = wos_ext.shape_derivative(Polyline(base, _t(ix), vals), pts, 11, base, nwalks=256, nsteps=256,
+                                      epsilon=1e-4, epsilon2=1e-4, clamping=0.1)
+    np.testing.assert_array_equal(tangent.cpu().numpy(), direct.cpu().numpy())
+
+    fwd = WoSCUDA(nwalks=256, nsteps=256, epsilon=1e-4)
+    u = fwd.walk(pts, Polyline(base, _t(ix), vals), seed=11)
+    dL = float((2.0 * u * tangent).mean())
+    h = 2e-2
+    L = lambda sc: float((fwd.walk(pts, Polyline(base * sc, _t(ix), vals), seed=11) ** 2).mean())
+    fd = (L(1 + h) - L(1 - h)) / (2 * h)
+    assert abs(dL - fd) < 0.1 * abs(fd) + 1e-3, (dL, fd)
+
+
+def test_derivative_image_on_a_detector():
+    """walk_detector with a shape solver under forward AD: one derivative
+    per pixel (nwalks jittered walks each), finite, and zero outside."""
+    from wos.scene import Detector, Polyline
+    from wos.wos_grad import OursCUDA
+    v, ix = C.star_polyline(512)
+    base = _t(v)
+    det = Detector(vmin=(-1.5, -1.5), vmax=(1.5, 1.5), res=(32, 32))
+    ours = OursCUDA(nwalks=16, nsteps=128, epsilon=1e-3, epsilon2=1e-3, clamping=0.1)
+    with fwAD.dual_level():
+        s = fwAD.make_dual(torch.tensor(0.0, device="cuda"), torch.tensor(1.0, device="cuda"))
+        shift = torch.stack([torch.zeros_like(s), s])
+        scene = Polyline(base + shift, _t(ix), _t(C.g_saddle(v).astype(np.float32)))
+        img = ours.walk_detector(scene, det, seed=3)
+        d_img = fwAD.unpack_dual(img).tangent
+    assert d_img.shape == (32 * 32,)
+    d = d_img.cpu().numpy()
+    assert np.isfinite(d).all() and np.abs(d).max() > 0
+    corner = d.reshape(32, 32)[[0, 0, -1, -1], [0, -1, 0, -1]]
+    np.testing.assert_array_equal(corner, 0.0)

@@ -113,3 +113,30 @@ def test_srgb_transfer_is_inverse():
     x = np.linspace(0, 1, 101)
     np.testing.assert_allclose(io.srgb_to_linear(io.linear_to_srgb(x)), x, atol=1e-12)
     assert abs(io.linear_to_srgb(0.0031308) - 0.0031308 * 12.92) < 1e-9
+
+
+def test_detector_index_stays_in_the_image():
+    """Detector.index (wos/scene.py:18-24) flips rows as res_y - uv.y, which
+    sends uv.y = 0 one row past the image: such points must come back
+    invalid, and every valid index must address a pixel."""
+    import torch
+    from wos.scene import Detector
+    from wos.scene3d import Detector3D
+    det = Detector(vmin=(-1.2, -1.2), vmax=(1.2, 1.2), res=(64, 48))
+    rng = np.random.default_rng(0)
+    p = torch.from_numpy(rng.uniform(-1.4, 1.4, (200000, 2)).astype(np.float32))
+    edge = torch.tensor([[0.0, -1.2], [0.0, -1.2 + 0.01], [-1.2, 0.0], [1.1, 1.1]], dtype=torch.float32)
+    valid, idx = det.index(torch.cat([p, edge]))
+    n = det.res[0] * det.res[1]
+    assert ((idx[valid] >= 0) & (idx[valid] < n)).all()
+    assert not bool(valid[-4]) and not bool(valid[-3])     # the bottom row (uv.y = 0)
+    assert bool(valid[-2]) and bool(valid[-1])
+    # the rows that remain are the reference's: uv.x + (res_y - uv.y) res_x
+    uv = ((torch.cat([p, edge]) + 1.2) / 2.4 * torch.tensor([64.0, 48.0]) + 0.5).to(torch.int32)
+    ref = uv[:, 0] + (48 - uv[:, 1]) * 64
+    assert torch.equal(idx[valid], ref[valid])
+    d3 = Detector3D(vmin=(-1.0, -1.0), vmax=(1.0, 1.0), res=(32, 32), z=0.1)
+    q = torch.tensor([[0.0, 0.0, 0.1], [0.0, 0.0, 0.5], [0.0, -1.0, 0.1]], dtype=torch.float32)
+    v3, i3 = d3.index(q)
+    assert v3.tolist() == [True, False, False]
+    assert 0 <= int(i3[0]) < 32 * 32
