@@ -112,3 +112,39 @@ def test_derivative_image_on_a_detector():
     assert np.isfinite(d).all() and np.abs(d).max() > 0
     corner = d.reshape(32, 32)[[0, 0, -1, -1], [0, -1, 0, -1]]
     np.testing.assert_array_equal(corner, 0.0)
+
+
+def test_runner_render_tasks(tmp_path):
+    """TestRunner (diff_solve/runner.py): the star moved up by delta.
+    renderD (OursCUDA + WoSBoundary, forward mode) correlates with renderFD
+    (difference of two primal images under common seeds) over the interior,
+    and every task writes its EXR image."""
+    from dataclasses import dataclass
+    from wos.io import read_exr
+    from wos.runner import RenderC, RenderD, RenderFD, TestRunner
+    from wos.scene import Detector, Polyline
+    from wos.wos import WoSCUDA
+    from wos.wos_boundary import WoSBoundary
+    from wos.wos_grad import OursCUDA
+    v, ix = C.star_polyline(1024)
+    base, I, vals = _t(v), _t(ix), _t(C.g_saddle(v).astype(np.float32))
+
+    @dataclass
+    class Star(TestRunner):
+        def make_scene(self, delta=0.0):
+            if not isinstance(delta, torch.Tensor):
+                delta = torch.tensor(float(delta), device="cuda")
+            return Polyline(base + torch.stack([torch.zeros_like(delta), delta]), I, vals)
+
+    r = Star(detector=Detector(vmin=(-1.2, -1.2), vmax=(1.2, 1.2), res=(64, 64)), npasses=4, out_dir=str(tmp_path),
+             delta=2e-3, wos_boundary=WoSBoundary(nwalks=16),
+             tasks=[RenderC(WoSCUDA(nwalks=256, nsteps=256, epsilon=1e-4)),
+                    RenderD(OursCUDA(nwalks=256, nsteps=256, epsilon=1e-4, epsilon2=1e-4, clamping=0.1)),
+                    RenderFD(WoSCUDA(nwalks=256, nsteps=256, epsilon=1e-4, prevent_fd_artifacts=True))])
+    c, d, fd = r.run()
+    for name in ("renderC.exr", "renderD.exr", "renderFD.exr"):
+        assert read_exr(str(tmp_path / name)).shape == (64, 64, 1)
+    inner = (r.make_scene().sdf(r.p).cpu().numpy() < -0.05)
+    a, b = d.reshape(-1)[inner], fd.reshape(-1)[inner]
+    assert np.isfinite(a).all() and np.isfinite(c).all()
+    assert np.corrcoef(a, b)[0, 1] > 0.4
