@@ -476,7 +476,7 @@ extern "C" int wos_batch_run(wos_batch *B, void *stream) {
         long gw = (g.walks / 64 + 4) / 4;
         int gwk = (int)std::min<long>(std::max<long>(gw, 1), cap);
         int chunk = g_tuning.chunk > 0 ? g_tuning.chunk : 256;
-        int window = g_tuning.window > 0 ? g_tuning.window : 8;
+        int window = window_for(d == 0 ? 2 : 3);
         if (d == 0) {
             k_batch_init<2><<<gi, kBlock, 0, st>>>(g.items, g.n_items, g.points, g.iprim, g.isd, spill, stride);
             k_batch_walk<2><<<gwk, kBlock, 0, st>>>(g.items, g.n_items, g.walks, B->counters + d, chunk, window,

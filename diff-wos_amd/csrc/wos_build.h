@@ -110,10 +110,14 @@ struct Tuning {
     int chunk = 128;        // walks a wave claims per queue visit
     int wgs_per_cu = 8;     // persistent workgroups per CU
     int top_levels = 6;     // BVH levels kept in LDS
-    int window = 8;         // node visits per lane between refills of the walk kernel
+    int window = 0;         // node visits per lane between refills; 0: 8 in 2D, 16 in 3D
     int spatial_order = 1;  // visit query points in Morton order (0: input order)
 };
 extern Tuning g_tuning;
+
+// 3D queries are longer (about 50 node visits against 12 in 2D), so a wider
+// window pays for its idle lanes by running the step phase fewer times.
+inline int window_for(int dim) { return g_tuning.window > 0 ? g_tuning.window : (dim == 2 ? 8 : 16); }
 
 // Grows (never shrinks) the traversal spill and the launch counters.
 int2 *spill_buffer(const SceneView &s, long lanes, long *stride, hipStream_t stream);

@@ -351,7 +351,7 @@ int wos_set_tuning(int chunk, int wgs_per_cu, int top_levels, int window) {
     REQUIRE(chunk >= 64 && chunk <= (1 << 20), "chunk must be in [64, 2^20] (got %d)", chunk);
     REQUIRE(wgs_per_cu >= 1 && wgs_per_cu <= 32, "wgs_per_cu must be in [1, 32] (got %d)", wgs_per_cu);
     REQUIRE(top_levels >= 0 && top_levels <= 16, "top_levels must be in [0, 16] (got %d)", top_levels);
-    REQUIRE(window >= 1 && window <= 4096, "window must be in [1, 4096] (got %d)", window);
+    REQUIRE(window >= 0 && window <= 4096, "window must be in [0, 4096] (0: per dimension; got %d)", window);
     g_tuning.window = window;
     g_tuning.chunk = chunk;
     g_tuning.wgs_per_cu = wgs_per_cu;

@@ -1139,7 +1139,7 @@ int launch_walk_reduce(const SceneView &s, const WalkSettings &w, const float *p
     a.counter = launch_counter(st);
     if (!a.counter) return -1;
     a.chunk = chunk > 0 ? chunk : (g_tuning.chunk > 0 ? g_tuning.chunk : 256);
-    a.window = g_tuning.window > 0 ? g_tuning.window : 16;
+    a.window = window_for(s.dim);
     a.walk_value = o.walk_value;
     a.rec_prim = o.rec_prim;
     a.rec_t = o.rec_t;
@@ -1245,7 +1245,7 @@ int launch_shape_gradient(const SceneView &s, const GradSettings &g, const float
     a.counter = launch_counter(st);
     if (!a.counter) return -1;
     a.chunk = g_tuning.chunk > 0 ? g_tuning.chunk : 128;
-    a.window = g_tuning.window > 0 ? g_tuning.window : 8;
+    a.window = window_for(s.dim);
     a.grad_out = grad_out;
     a.grad_vertices = grad_vertices;
     a.sample_grad_u = sample_grad_u;

@@ -116,11 +116,12 @@ EXPORTED = ("wos_last_error", "wos_scene_create", "wos_scene_destroy", "wos_scen
 
 def set_tuning(chunk=None, wgs_per_cu=None, top_levels=None, window=None):
     """Process-wide launch tuning (walks per queue claim, workgroups per CU,
-    BVH levels in LDS, node visits per lane between refills); unspecified
-    values keep their current setting."""
+    BVH levels in LDS, node visits per lane between refills; window 0 is the
+    per-dimension default, 8 in 2D and 16 in 3D); unspecified values keep
+    their current setting."""
     c, w, t, v = get_tuning()
     _check(lib.wos_set_tuning(int(chunk or c), int(wgs_per_cu or w), int(t if top_levels is None else top_levels),
-                              int(window or v)))
+                              int(v if window is None else window)))
 
 
 def set_spatial_order(enable=True):

@@ -75,8 +75,9 @@ int wos_scene_create(int dim, const float *vertices, int n_vertices, const int *
 /* Launch tuning of every walk launch (no counterpart in the reference, whose
  * thrust launches are untuned): walks a wave claims per queue visit (chunk),
  * persistent workgroups per CU, BVH levels kept in LDS, and node visits per
- * lane between refills of the walk kernel (window).  Process-wide; chosen by
- * the autotuner in the Python package. */
+ * lane between refills of the walk kernel (window; 0, the default, means 8
+ * for 2D scenes and 16 for 3D).  Process-wide; chosen by the autotuner in the
+ * Python package. */
 int wos_set_tuning(int chunk, int wgs_per_cu, int top_levels, int window);
 int wos_get_tuning(int *chunk, int *wgs_per_cu, int *top_levels, int *window);
 /* Whether walk launches visit the query points in Morton order within the
