@@ -617,13 +617,14 @@ __global__ void k_reorder_nodes(const float4 *src, const int *pos, int n_int, in
 }
 
 // 2D node layout used by the traversal (4 x float4):
-//   word k (k = 0, 1): bound of child k -- a leaf's segment {a.xy, b.xy}, or
-//                      for an internal child whichever of its capsule axis
-//                      {a.xy, b.xy} and its box {lo.xy, hi.xy} has the
+//   word k (k = 0, 1): bound of child k as {a.xy, e.xy} -- a leaf's segment
+//                      (its leaf word: a, e = b - a), or for an internal
+//                      child whichever of its capsule axis (a, e = b - a) and
+//                      its box (a = lo, e = hi - lo rounded up) has the
 //                      smaller area
 //   word 2:            {inv0, r0, inv1, r1}: r = 0 for a leaf (inv as
 //                      computed for the leaf), r > 0 a capsule radius (inv =
-//                      1/|b - a|^2, 0 for a point), r < 0 a box
+//                      1/|e|^2, 0 for a point), r = -0.0 a box
 //   word 3:            {id0, id1}: internal child index, or ~primitive
 // A leaf's distance comes from the same arithmetic as a brute-force scan, so
 // both children go through one code path and a node fetch carries everything
@@ -650,7 +651,7 @@ __global__ void k_pack_nodes_2d(const float4 *src, const float4 *leaves, int n_i
             float4 g = src[W * i + k];
             if (ch[k] < 0) {
                 float4 l1 = leaves[2 * (~ch[k]) + 1];
-                dst[4 * i + k] = g;
+                dst[4 * i + k] = leaves[2 * (~ch[k])];
                 aux[2 * k] = l1.x;
                 aux[2 * k + 1] = 0.0f;
                 ids[k] = ~__float_as_int(l1.y);
@@ -668,13 +669,14 @@ __global__ void k_pack_nodes_2d(const float4 *src, const float4 *leaves, int n_i
             if (cap_cost < bx_cost) {
                 float ex = g.z - g.x, ey = g.w - g.y;
                 float dd = dot2(ex, ey, ex, ey);
-                dst[4 * i + k] = g;
+                dst[4 * i + k] = make_float4(g.x, g.y, ex, ey);
                 aux[2 * k] = dd > 0.0f ? 1.0f / dd : 0.0f;
                 aux[2 * k + 1] = rr;
             } else {
-                dst[4 * i + k] = bx;
+                dst[4 * i + k] = make_float4(bx.x, bx.y, nextafterf(bw, __builtin_inff()),
+                                             nextafterf(bh, __builtin_inff()));
                 aux[2 * k] = 0.0f;
-                aux[2 * k + 1] = -1.0f;
+                aux[2 * k + 1] = -0.0f;
             }
             ids[k] = ch[k];
         }
@@ -994,7 +996,7 @@ __global__ void k_leaf_of_prim(const int *sorted_prim, int n, int *leaf_of_prim)
 // axis, and whether it is a leaf.
 __device__ __forceinline__ float child_axis_d2(float px, float py, float4 g, float inv) {
     float h, dx, dy;
-    return seg_d2(px, py, g.x, g.y, g.z - g.x, g.w - g.y, inv, h, dx, dy);
+    return seg_d2(px, py, g.x, g.y, g.z, g.w, inv, h, dx, dy);
 }
 
 // Squared distance from (px, py) to the boundary over the packed 2D nodes.

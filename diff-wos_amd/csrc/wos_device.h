@@ -319,12 +319,6 @@ __device__ __forceinline__ void load_top_nodes(const SceneView &s) {
     }
 }
 
-// Box distance^2 from p to [lo, hi].
-__device__ __forceinline__ float box_d2_2(float px, float py, float4 b) {
-    float dx = fmaxf(fmaxf(b.x - px, px - b.z), 0.0f);
-    float dy = fmaxf(fmaxf(b.y - py, py - b.w), 0.0f);
-    return fmaf(dx, dx, dy * dy);
-}
 __device__ __forceinline__ float box_d2_3(const float *p, float lx, float ly, float lz, float hx, float hy,
                                           float hz) {
     float dx = fmaxf(fmaxf(lx - p[0], p[0] - hx), 0.0f);
@@ -479,23 +473,31 @@ __device__ __forceinline__ bool stack_pop(Query &q, const TravStack &st, int slo
     return false;
 }
 
-// Distance lower bound of an internal child of a packed 2D node: r >= 0 is
-// a capsule (s_axis = squared distance to its axis), r < 0 a box {lo, hi}.
+// Distance lower bound of an internal child of a packed 2D node, whose word
+// g is {a.xy, e.xy}: r > 0 is a capsule of radius r around the axis a + t e
+// (s_axis = squared distance to the axis), r = -0.0 a box [a, a + e] (e
+// rounded up).  The box distance reuses p - a from the axis distance, and
+// subtracting r = -0.0 leaves it unchanged, so neither kind needs a max.
+// The box arithmetic rounds within the pruning margin tol.
 __device__ __forceinline__ float child_bound_2d(float px, float py, float4 g, float s_axis, float r) {
-    const float b = box_d2_2(px, py, g);
-    return __builtin_amdgcn_sqrtf(r < 0.0f ? b : s_axis) - fmaxf(r, 0.0f);
+    const float ax = px - g.x, ay = py - g.y;
+    const float dx = fmaxf(fmaxf(-ax, ax - g.z), 0.0f);
+    const float dy = fmaxf(fmaxf(-ay, ay - g.w), 0.0f);
+    const float b = fmaf(dx, dx, dy * dy);
+    return __builtin_amdgcn_sqrtf(__float_as_int(r) < 0 ? b : s_axis) - r;
 }
 
 // One node visit; true when the query is complete.
 //
-// 2D nodes (4 x float4): words 0 and 1 hold the bound of each child -- a
-// leaf's segment {a.xy, b.xy}, an internal child's capsule axis {a.xy, b.xy}
-// or box {lo.xy, hi.xy} --, word 2 holds {inv0, r0, inv1, r1} (r = 0 leaf,
-// r > 0 capsule radius, r < 0 box; inv = 1/|b - a|^2) and word 3 {id0, id1}
-// (child node index, or ~primitive for a leaf).  Both children take the same
-// arithmetic: the squared distance to the segment, which for a leaf is the
-// exact primitive distance of a brute-force scan, and the child's lower
-// bound, which for a capsule is |p - axis| - r.
+// 2D nodes (4 x float4): words 0 and 1 hold the bound of each child as
+// {a.xy, e.xy} -- a leaf's segment (e = b - a, the leaf's own words), an
+// internal child's capsule axis (e = b - a) or box (lo = a, hi <= a + e) --,
+// word 2 holds {inv0, r0, inv1, r1} (r = 0 leaf, r > 0 capsule radius, r =
+// -0.0 box; inv = 1/|e|^2) and word 3 {id0, id1} (child node index, or
+// ~primitive for a leaf).  Both children take the same arithmetic: the
+// squared distance to the segment, which for a leaf is the exact primitive
+// distance of a brute-force scan, and the child's lower bound, which for a
+// capsule is |p - axis| - r.
 template <int D, bool STATS>
 __device__ __forceinline__ bool query_step(const SceneView &s, const float *p, Query &q, const TravStack &st,
                                            int slot, long spill_lane, float tol) {
@@ -519,8 +521,8 @@ __device__ __forceinline__ bool query_step(const SceneView &s, const float *p, Q
         const int c0 = __float_as_int(n3.x), c1 = __float_as_int(n3.y);
         const bool leaf0 = c0 < 0, leaf1 = c1 < 0;
         float h, dx, dy;
-        const float s0 = seg_d2(p[0], p[1], n0.x, n0.y, n0.z - n0.x, n0.w - n0.y, n2.x, h, dx, dy);
-        const float s1 = seg_d2(p[0], p[1], n1.x, n1.y, n1.z - n1.x, n1.w - n1.y, n2.z, h, dx, dy);
+        const float s0 = seg_d2(p[0], p[1], n0.x, n0.y, n0.z, n0.w, n2.x, h, dx, dy);
+        const float s1 = seg_d2(p[0], p[1], n1.x, n1.y, n1.z, n1.w, n2.z, h, dx, dy);
         if constexpr (STATS) {
             q.nbox += (leaf0 ? 0u : 1u) + (leaf1 ? 0u : 1u);
             q.nprim += (leaf0 ? 1u : 0u) + (leaf1 ? 1u : 0u);
@@ -529,7 +531,9 @@ __device__ __forceinline__ bool query_step(const SceneView &s, const float *p, Q
         if (u0) { q.best = s0; q.best_id = ~c0; }
         bool u1 = leaf1 && better(s1, ~c1, q.best, q.best_id);
         if (u1) { q.best = s1; q.best_id = ~c1; }
-        if (u0 || u1) q.thr = fminf(q.thr, prune_for<2>(q.best, tol));
+        // a compare and select: fminf would first canonicalise the loaded thr
+        const float pr = prune_for<2>(q.best, tol);
+        if ((u0 || u1) && pr < q.thr) q.thr = pr;
         const float l0 = child_bound_2d(p[0], p[1], n0, s0, n2.y), l1 = child_bound_2d(p[0], p[1], n1, s1, n2.w);
         const bool g0 = !leaf0 && l0 <= q.thr, g1 = !leaf1 && l1 <= q.thr;
         if (g0 && g1) {
