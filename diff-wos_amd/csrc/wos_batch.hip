@@ -239,7 +239,7 @@ __global__ __launch_bounds__(kBlock) void k_batch_walk(const BatchItem *items, i
 #pragma unroll
                         for (int j = 0; j < kMaxBatchBasis; ++j) {
                             if (j < K) {
-                                const float *c = s.src + 4 * j;
+                                const const_f32 *c = as_constant(s.src) + 4 * j;
                                 float bq = sin_cw(fmaf(c[0], yx, fmaf(c[1], yy, c[2])));
                                 f = fmaf(c[3], bq, f);
                                 bas[j] = fmaf(bq * G, area, bas[j]);

@@ -21,6 +21,13 @@ constexpr int kNodeWords2 = 4;       // float4 words per 2D internal node
 constexpr int kNodeWords3 = 6;       // float4 words per 3D internal node
 __host__ __device__ constexpr int node_words(int dim) { return dim == 2 ? kNodeWords2 : kNodeWords3; }
 
+// Read-only kernel inputs seen through the constant address space: loads at
+// a wave-uniform address become scalar loads (scalar cache, waited on by
+// lgkmcnt), not vector loads whose wait would also wait for every vector
+// load in flight.  Only for buffers no kernel writes while it runs.
+typedef __attribute__((address_space(4))) const float const_f32;
+__device__ __forceinline__ const const_f32 *as_constant(const float *p) { return (const const_f32 *)p; }
+
 // ---------------------------------------------------------------------------
 // random numbers
 // ---------------------------------------------------------------------------
@@ -809,7 +816,7 @@ __device__ __forceinline__ float dirichlet_at(const SceneView &s, int f, float t
 
 // include/scene.cuh:278-301, plus type 3: sum_k c_k sin(ax_k x + ay_k y + ph_k)
 __device__ __forceinline__ float source_at(const SceneView &s, float x, float y) {
-    const float *q = s.src;
+    const const_f32 *q = as_constant(s.src);
     if (s.source_type == 1) {
         float ex = x - q[0], ey = y - q[1];
         float dx = ex * ex / (2.0f * q[2] * q[2]);

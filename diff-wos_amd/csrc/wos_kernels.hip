@@ -453,12 +453,13 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(D == 2 ?
 #pragma unroll
                         for (int j = 0; j < basis_cap(KB); ++j) {
                             if (KB > 0 || j < a.K) {
-                                const float *c = s.src + 4 * j;
+                                const const_f32 *c = as_constant(s.src) + 4 * j;
                                 float b = sin_cw(fmaf(c[0], yx, fmaf(c[1], yy, c[2])));
                                 f = fmaf(c[3], b, f);
                                 bas[j] = fmaf(b * G, area, bas[j]);
-                                // one term at a time: hoisting every term's
-                                // coefficient load costs registers (occupancy)
+                                // one term at a time: interleaving the terms'
+                                // arithmetic costs registers (occupancy); the
+                                // coefficients are scalar loads (as_constant)
                                 __builtin_amdgcn_sched_barrier(0);
                             }
                         }
