@@ -177,7 +177,7 @@ __global__ __launch_bounds__(kBlock) void k_batch_walk(const BatchItem *items, i
                     item = find_item(s_begin, n_items, gk);
                     const BatchItem &b = items[item];
                     long long kk = gk - s_begin[item];
-                    long long i = kk / b.w.nwalks;
+                    long long i = point_of_walk(kk, b.w.nwalks);
                     long long gp = b.point_begin + i;
                     int ip = iprim[gp];
                     float sd = isd[gp];
@@ -385,6 +385,7 @@ extern "C" int wos_batch_create(const wos_batch_item *items, int n_items, wos_ba
         BREQ(it.params.nwalks >= 1 && it.params.nsteps >= 0, "item %d: bad walk settings", j);
         BREQ(it.n_points >= 0 && (it.n_points == 0 || it.points), "item %d: bad points", j);
         BREQ(it.n_points == 0 || it.walk_value, "item %d: walk_value is required", j);
+        BREQ((long long)it.n_points * it.params.nwalks < (1LL << 32), "item %d: at most 2^32 - 1 walks", j);
         SceneView v = scene_view(it.scene);
         v.n_top = 0;
         BatchItem b;

@@ -28,6 +28,18 @@ __host__ __device__ constexpr int node_words(int dim) { return dim == 2 ? kNodeW
 typedef __attribute__((address_space(4))) const float const_f32;
 __device__ __forceinline__ const const_f32 *as_constant(const float *p) { return (const const_f32 *)p; }
 
+// base + off bytes, off an unsigned 32-bit offset (zero-extended, so the
+// address needs no 64-bit arithmetic per lane)
+template <typename T>
+__device__ __forceinline__ const T *at_byte_offset(const T *base, uint32_t off) {
+    return reinterpret_cast<const T *>(reinterpret_cast<const char *>(base) + off);
+}
+
+// Point of walk k under W walks per point.  Launches hold fewer than 2^32
+// walks (checked on the host), so this is a 32-bit unsigned division, not
+// the 64-bit routine.
+__device__ __forceinline__ uint32_t point_of_walk(long long k, int W) { return (uint32_t)k / (uint32_t)W; }
+
 // ---------------------------------------------------------------------------
 // random numbers
 // ---------------------------------------------------------------------------
@@ -346,8 +358,9 @@ __device__ __forceinline__ bool better(float d2, int id, float best_d2, int best
 template <int D>
 __device__ __forceinline__ float prim_d2(const SceneView &s, int f, const float *p) {
     if constexpr (D == 2) {
-        const float4 g = s.pleaves[2 * f];
-        const float inv = s.pleaves[2 * f + 1].x;
+        const float4 *lf = at_byte_offset(s.pleaves, (uint32_t)f << 5);
+        const float4 g = lf[0];
+        const float inv = lf[1].x;
         float h, dx, dy;
         return seg_d2(p[0], p[1], g.x, g.y, g.z, g.w, inv, h, dx, dy);
     } else {
@@ -408,13 +421,15 @@ __device__ __forceinline__ int4 start_nodes_of(const SceneView &s, const float *
     const float fy = (p[1] - s.st_oy) * s.st_inv_cell;
     // written so that NaN coordinates fail the test
     if (!(fx >= 0.0f && fy >= 0.0f && fx < (float)s.st_nx && fy < (float)s.st_ny)) return make_int4(0, -1, 0, 0);
-    long c = (long)(int)fy * s.st_nx + (int)fx;
+    // cell byte offsets fit in 32 bits: a grid holds at most (8192 + 2)^2
+    // cells in 2D and (256 + 2)^3 in 3D, 16 bytes each
+    uint32_t c = (uint32_t)(int)fy * (uint32_t)s.st_nx + (uint32_t)(int)fx;
     if constexpr (D == 3) {
         const float fz = (p[2] - s.st_oz) * s.st_inv_cell;
         if (!(fz >= 0.0f && fz < (float)s.st_nz)) return make_int4(0, -1, 0, 0);
-        c += (long)(int)fz * s.st_nx * s.st_ny;
+        c += (uint32_t)(int)fz * (uint32_t)(s.st_nx * s.st_ny);
     }
-    return reinterpret_cast<const int4 *>(s.start_node)[c];
+    return *at_byte_offset(reinterpret_cast<const int4 *>(s.start_node), c << 4);
 }
 
 // Starts a query whose answer lies within bound_d2, optionally seeded with a
@@ -520,10 +535,13 @@ __device__ __forceinline__ bool query_step(const SceneView &s, const float *p, Q
             // they become one generic (flat) load for both address spaces
             __asm__ volatile("" ::: "memory");
         } else {
-            n0 = s.nodes[4 * node];
-            n1 = s.nodes[4 * node + 1];
-            n2 = s.nodes[4 * node + 2];
-            n3 = s.nodes[4 * node + 3];
+            // a 32-bit byte offset from the uniform base: one shift, and the
+            // loads take the scalar-base + vector-offset form
+            const float4 *nd = at_byte_offset(s.nodes, (uint32_t)node << 6);
+            n0 = nd[0];
+            n1 = nd[1];
+            n2 = nd[2];
+            n3 = nd[3];
         }
         const int c0 = __float_as_int(n3.x), c1 = __float_as_int(n3.y);
         const bool leaf0 = c0 < 0, leaf1 = c1 < 0;

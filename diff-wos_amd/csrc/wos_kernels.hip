@@ -373,8 +373,9 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(D == 2 ?
                 else if (fresh && (r - avail) < end2 - base2) slot = base2 + (r - avail);
                 if (slot >= 0) {
                     // slot -> (point, walk): points in the visiting order
-                    const long long i = a.order ? (long long)a.order[slot / W] : slot / W;
-                    const long long kk = i * W + slot % W;
+                    const uint32_t pi = point_of_walk(slot, W);
+                    const long long i = a.order ? (long long)a.order[pi] : (long long)pi;
+                    const long long kk = i * W + ((uint32_t)slot - pi * (uint32_t)W);
                     int ip = a.init_prim[i];
                     float sd = a.init_sd[i];
                     load_point<D>(a.points, i, p);
@@ -432,8 +433,9 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(D == 2 ?
             if (s.use_bvh) {
                 start = start_nodes_of<D>(s, p);
                 if constexpr (D == 2) {
-                    sg = s.pleaves[2 * prim];
-                    sinv = s.pleaves[2 * prim + 1].x;
+                    const float4 *lf = at_byte_offset(s.pleaves, (uint32_t)prim << 5);
+                    sg = lf[0];
+                    sinv = lf[1].x;
                 }
             }
             if (SRC) {
@@ -559,7 +561,7 @@ __global__ __launch_bounds__(kBlock) void k_adjoint_values(SceneView s, int n_po
     for (long long k = (long long)blockIdx.x * kBlock + threadIdx.x; k < total; k += (long long)gridDim.x * kBlock) {
         int f = rec_prim[k];
         if (f < 0) continue;
-        float g = grad_out[k / W] / (float)W;
+        float g = grad_out[point_of_walk(k, W)] / (float)W;
         if (grad_values) {
             if constexpr (D == 2) {
                 float t = rec_t[k];
@@ -866,7 +868,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6))) voi
         if (__popcll(__ballot(mode == kDone || mode == kIdle)) >= kFinishBatch || exhausted ||
             __ballot(mode != kDone && mode != kIdle) == 0) {
             if (mode == kDone) {
-                const long long i = k / W;
+                const long long i = point_of_walk(k, W);
                 float gu[3] = {0.0f, 0.0f, 0.0f};
                 if (done_ok) {
                     const float n[3] = {-nb[0], -nb[1], D == 3 ? -nb[2] : 0.0f};
@@ -946,8 +948,9 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6))) voi
                 if (r < avail) slot = c_next + r;
                 else if (fresh && (r - avail) < end2 - base2) slot = base2 + (r - avail);
                 if (slot >= 0) {
-                    const long long i = a.order ? (long long)a.order[slot / W] : slot / W;
-                    const long long kk = i * W + slot % W;
+                    const uint32_t pi = point_of_walk(slot, W);
+                    const long long i = a.order ? (long long)a.order[pi] : (long long)pi;
+                    const long long kk = i * W + ((uint32_t)slot - pi * (uint32_t)W);
                     const int ip = a.init_prim[i];
                     const float sd = a.init_sd[i];
                     k = kk;
@@ -1122,10 +1125,18 @@ int launch_single_walk(const SceneView &s, const WalkSettings &w, const float *p
 
 __global__ void k_zero_u64(unsigned long long *p) { *p = 0ull; }
 
+// Walk indices are split into (point, walk) in 32-bit arithmetic on the device.
+static bool walk_count_fits(long long total) {
+    if (total < (1LL << 32)) return true;
+    set_error("at most 2^32 - 1 walks per launch (points x walks = %lld)", total);
+    return false;
+}
+
 int launch_walk_reduce(const SceneView &s, const WalkSettings &w, const float *points, int n_points, int32_t seed,
                        int chunk, const WalkOutputs &o, hipStream_t st) {
     if (n_points <= 0) return 0;
     long long total = (long long)n_points * w.nwalks;
+    if (!walk_count_fits(total)) return -1;
     int K = (s.source_type == 3) ? s.n_src / 4 : 0;
     if (o.basis && K > kMaxBasis) {
         set_error("Fourier source has %d terms; at most %d are supported", K, kMaxBasis);
@@ -1216,6 +1227,7 @@ int launch_adjoint_values(const SceneView &s, int n_points, int nwalks, const in
         return -1;
     }
     long long total = (long long)n_points * nwalks;
+    if (!walk_count_fits(total)) return -1;
     int g = grid_of((long)((total + 15) / 16), 2048);
     bool lds = s.n_vertices <= kLdsValues;
     if (s.dim == 2) {
@@ -1243,6 +1255,7 @@ int launch_shape_gradient(const SceneView &s, const GradSettings &g, const float
     a.g = g;
     a.seed = seed;
     a.total = (long long)n_points * g.nwalks;
+    if (!walk_count_fits(a.total)) return -1;
     a.counter = launch_counter(st);
     if (!a.counter) return -1;
     a.chunk = g_tuning.chunk > 0 ? g_tuning.chunk : 128;
