@@ -1554,14 +1554,18 @@ int sort_points_morton(const float *points, int n, int D, const float *amin, con
     for (int k = 0; k < 3; ++k) e[k] = amax[k] - amin[k];
     float3 inv = make_float3(e[0] > 0 ? 1.0f / e[0] : 0.0f, e[1] > 0 ? 1.0f / e[1] : 0.0f, e[2] > 0 ? 1.0f / e[2] : 0.0f);
     k_point_morton<<<grid_for(n), kBlock, 0, stream>>>(points, n, D, lo, inv, keys[0], vals[0]);
-    for (int pass = 0; pass < 4; ++pass) {
+    // the order only steers cache locality, so the sort stops at 8 bits per
+    // axis (256^2 or 256^3 cells; equal cells keep the input order): the top
+    // 16 bits of the 2D code, the top 24 of the 30-bit 3D code
+    const int passes = D == 2 ? 2 : 3, low = D == 2 ? 16 : 6;
+    for (int pass = 0; pass < passes; ++pass) {
         int src = pass & 1, dst = src ^ 1;
-        k_rs_hist<<<nblocks, kBlock, 0, stream>>>(keys[src], n, 8 * pass, nblocks, hist);
+        k_rs_hist<<<nblocks, kBlock, 0, stream>>>(keys[src], n, low + 8 * pass, nblocks, hist);
         k_rs_scan<<<1, 1024, 0, stream>>>(hist, 256L * nblocks);
-        k_rs_scatter<<<nblocks, kBlock, 0, stream>>>(keys[src], vals[src], n, 8 * pass, nblocks, hist, keys[dst],
-                                                     vals[dst]);
+        k_rs_scatter<<<nblocks, kBlock, 0, stream>>>(keys[src], vals[src], n, low + 8 * pass, nblocks, hist,
+                                                     keys[dst], vals[dst]);
     }
-    BCHK(hipMemcpyAsync(order, vals[0], sizeof(int) * (size_t)n, hipMemcpyDeviceToDevice, stream));
+    BCHK(hipMemcpyAsync(order, vals[passes & 1], sizeof(int) * (size_t)n, hipMemcpyDeviceToDevice, stream));
     for (int b = 0; b < 2; ++b) {
         (void)hipFreeAsync(keys[b], stream);
         (void)hipFreeAsync(vals[b], stream);

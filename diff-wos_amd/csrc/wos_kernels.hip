@@ -245,10 +245,13 @@ struct PersistArgs {
 // All walks of a point start at the same place, so the (unbounded) start
 // query runs once per point here instead of once per walk.
 template <int D>
-__global__ __launch_bounds__(kBlock) void k_point_init(SceneView s, const float *p, int n, int *prim, float *sd,
-                                                       int2 *spill, long stride) {
+__global__ __launch_bounds__(kBlock) void k_point_init(SceneView s, const float *p, int n, const int *order,
+                                                       int *prim, float *sd, int2 *spill, long stride) {
     DECLARE_STACK(spill, stride);
-    for (long i = lane_g; i < n; i += (long)gridDim.x * kBlock) {
+    for (long j = lane_g; j < n; j += (long)gridDim.x * kBlock) {
+        // neighbouring lanes take neighbouring points (in Morton order when
+        // given), so their traversals follow the same paths
+        const long i = order ? order[j] : j;
         float q[3];
         load_point<D>(p, i, q);
         float d2;
@@ -548,7 +551,9 @@ __global__ __launch_bounds__(kBlock) void k_adjoint_values(SceneView s, int n_po
                                                            const float *rec_t, const float *basis, int K,
                                                            const float *grad_out, float *grad_values,
                                                            float *grad_coeffs) {
-    __shared__ float h[LDSHIST ? kLdsValues : 1];
+    // the histogram is sized to the scene (dynamic LDS), so small meshes
+    // leave room for more resident workgroups
+    extern __shared__ float h[];
     __shared__ float cacc[kMaxBasis];
     if (LDSHIST)
         for (int v = threadIdx.x; v < s.n_vertices; v += kBlock) h[v] = 0.0f;
@@ -1024,7 +1029,7 @@ static size_t top_bytes(const SceneView &s) {
     return s.use_bvh ? (size_t)s.n_top * sizeof(float4) * node_words(s.dim) : 0;
 }
 
-static int occupancy_grid() {
+static int cu_count() {
     static int cus = 0;
     if (!cus) {
         int dev = 0;
@@ -1032,8 +1037,10 @@ static int occupancy_grid() {
         (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
         if (cus <= 0) cus = 256;
     }
-    return cus * (g_tuning.wgs_per_cu > 0 ? g_tuning.wgs_per_cu : 8);
+    return cus;
 }
+
+static int occupancy_grid() { return cu_count() * (g_tuning.wgs_per_cu > 0 ? g_tuning.wgs_per_cu : 8); }
 
 // Visiting order of the query points for the walk kernels: Morton order
 // within the scene's box for point sets large enough to pay for the sort
@@ -1178,8 +1185,8 @@ int launch_walk_reduce(const SceneView &s, const WalkSettings &w, const float *p
     if (spatial_order(s, points, n_points, &order, st) != 0) return -1;
     a.order = order;
     size_t tb = top_bytes(s);
-    if (s.dim == 2) k_point_init<2><<<gi, kBlock, tb, st>>>(s, points, n_points, iprim, isd, a.spill, a.spill_stride);
-    else k_point_init<3><<<gi, kBlock, tb, st>>>(s, points, n_points, iprim, isd, a.spill, a.spill_stride);
+    if (s.dim == 2) k_point_init<2><<<gi, kBlock, tb, st>>>(s, points, n_points, order, iprim, isd, a.spill, a.spill_stride);
+    else k_point_init<3><<<gi, kBlock, tb, st>>>(s, points, n_points, order, iprim, isd, a.spill, a.spill_stride);
     k_zero_u64<<<1, 1, 0, st>>>(a.counter);
     bool src = s.dim == 2 && s.source_type != 0;
     bool bas = src && o.basis != nullptr && K > 0;
@@ -1228,13 +1235,17 @@ int launch_adjoint_values(const SceneView &s, int n_points, int nwalks, const in
     }
     long long total = (long long)n_points * nwalks;
     if (!walk_count_fits(total)) return -1;
-    int g = grid_of((long)((total + 15) / 16), 2048);
+    // up to eight workgroups per CU, each walking a strided share of the
+    // records into its own LDS histogram (sized to the mesh: the 48 KB of a
+    // fixed-size one allowed three per CU, 0.48 -> 0.36 ms on the clover)
+    int g = grid_of((long)((total + 15) / 16), 8 * cu_count());
     bool lds = s.n_vertices <= kLdsValues;
+    const size_t hb = lds ? sizeof(float) * (size_t)s.n_vertices : 0;
     if (s.dim == 2) {
-        if (lds) k_adjoint_values<2, true><<<g, kBlock, 0, st>>>(s, n_points, nwalks, rec_prim, rec_t, basis, K, grad_out, grad_values, grad_coeffs);
+        if (lds) k_adjoint_values<2, true><<<g, kBlock, hb, st>>>(s, n_points, nwalks, rec_prim, rec_t, basis, K, grad_out, grad_values, grad_coeffs);
         else k_adjoint_values<2, false><<<g, kBlock, 0, st>>>(s, n_points, nwalks, rec_prim, rec_t, basis, K, grad_out, grad_values, grad_coeffs);
     } else {
-        if (lds) k_adjoint_values<3, true><<<g, kBlock, 0, st>>>(s, n_points, nwalks, rec_prim, rec_t, basis, K, grad_out, grad_values, grad_coeffs);
+        if (lds) k_adjoint_values<3, true><<<g, kBlock, hb, st>>>(s, n_points, nwalks, rec_prim, rec_t, basis, K, grad_out, grad_values, grad_coeffs);
         else k_adjoint_values<3, false><<<g, kBlock, 0, st>>>(s, n_points, nwalks, rec_prim, rec_t, basis, K, grad_out, grad_values, grad_coeffs);
     }
     KCHK();
@@ -1286,8 +1297,8 @@ int launch_shape_gradient(const SceneView &s, const GradSettings &g, const float
     if (spatial_order(s, points, n_points, &order, st) != 0) return -1;
     a.order = order;
     size_t tb = top_bytes(s);
-    if (s.dim == 2) k_point_init<2><<<gi, kBlock, tb, st>>>(s, points, n_points, iprim, isd, a.spill, a.spill_stride);
-    else k_point_init<3><<<gi, kBlock, tb, st>>>(s, points, n_points, iprim, isd, a.spill, a.spill_stride);
+    if (s.dim == 2) k_point_init<2><<<gi, kBlock, tb, st>>>(s, points, n_points, order, iprim, isd, a.spill, a.spill_stride);
+    else k_point_init<3><<<gi, kBlock, tb, st>>>(s, points, n_points, order, iprim, isd, a.spill, a.spill_stride);
     k_zero_u64<<<1, 1, 0, st>>>(a.counter);
     if (s.dim == 2) k_grad_persistent<2><<<gr, kBlock, tb, st>>>(s, a);
     else k_grad_persistent<3><<<gr, kBlock, tb, st>>>(s, a);
