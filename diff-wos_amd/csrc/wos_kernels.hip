@@ -371,14 +371,14 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(D == 2 ?
             }
             if (mode == kIdle) {
                 int r = rank_in_mask(idle);
-                long long slot = -1;
-                if (r < avail) slot = c_next + r;
-                else if (fresh && (r - avail) < end2 - base2) slot = base2 + (r - avail);
-                if (slot >= 0) {
-                    // slot -> (point, walk): points in the visiting order
-                    const uint32_t pi = point_of_walk(slot, W);
+                long long wslot = -1;   // walk slot (slot is the lane's LDS stack slot)
+                if (r < avail) wslot = c_next + r;
+                else if (fresh && (r - avail) < end2 - base2) wslot = base2 + (r - avail);
+                if (wslot >= 0) {
+                    // wslot -> (point, walk): points in the visiting order
+                    const uint32_t pi = point_of_walk(wslot, W);
                     const long long i = a.order ? (long long)a.order[pi] : (long long)pi;
-                    const long long kk = i * W + ((uint32_t)slot - pi * (uint32_t)W);
+                    const long long kk = i * W + ((uint32_t)wslot - pi * (uint32_t)W);
                     int ip = a.init_prim[i];
                     float sd = a.init_sd[i];
                     load_point<D>(a.points, i, p);
@@ -949,13 +949,13 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6))) voi
             }
             if (mode == kIdle) {
                 int r = rank_in_mask(idle);
-                long long slot = -1;
-                if (r < avail) slot = c_next + r;
-                else if (fresh && (r - avail) < end2 - base2) slot = base2 + (r - avail);
-                if (slot >= 0) {
-                    const uint32_t pi = point_of_walk(slot, W);
+                long long wslot = -1;   // walk slot (slot is the lane's LDS stack slot)
+                if (r < avail) wslot = c_next + r;
+                else if (fresh && (r - avail) < end2 - base2) wslot = base2 + (r - avail);
+                if (wslot >= 0) {
+                    const uint32_t pi = point_of_walk(wslot, W);
                     const long long i = a.order ? (long long)a.order[pi] : (long long)pi;
-                    const long long kk = i * W + ((uint32_t)slot - pi * (uint32_t)W);
+                    const long long kk = i * W + ((uint32_t)wslot - pi * (uint32_t)W);
                     const int ip = a.init_prim[i];
                     const float sd = a.init_sd[i];
                     k = kk;
