@@ -348,9 +348,13 @@ __device__ __forceinline__ float box_d2_3(const float *p, float lx, float ly, fl
 
 // (d2, prim) lexicographic minimum: equal squared distances keep the
 // smallest primitive index, which is what the in-order brute-force scan with
-// a strict '<' returns (include/scene.cuh:102-128).
+// a strict '<' returns (include/scene.cuh:102-128).  Squared distances are
+// >= 0 (or NaN) and ids >= 0, so the float bits order like the values and
+// the pair compares as one unsigned 64-bit key (one compare, no branch).
 __device__ __forceinline__ bool better(float d2, int id, float best_d2, int best_id) {
-    return d2 < best_d2 || (d2 == best_d2 && id < best_id);
+    const uint64_t a = ((uint64_t)__float_as_uint(d2) << 32) | (uint32_t)id;
+    const uint64_t b = ((uint64_t)__float_as_uint(best_d2) << 32) | (uint32_t)best_id;
+    return a < b;
 }
 
 // Squared distance from p to primitive f computed from the scene arrays
