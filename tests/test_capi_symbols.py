@@ -31,3 +31,20 @@ def test_error_path_without_device_work():
     rc = wos_ext.lib.wos_scene_create(4, None, 0, None, 0, None, 0, 0, None, 0, None, ctypes.byref(h))
     assert rc == -1
     assert b"dim must be 2 or 3" in wos_ext.lib.wos_last_error()
+
+
+def test_tuning_validation_and_window_default():
+    """wos_set_tuning touches no device: the window accepts 0 (per-dimension
+    default) and rejects negatives; other knobs keep their ranges."""
+    import wos_ext
+    before = wos_ext.get_tuning()
+    try:
+        assert wos_ext.lib.wos_set_tuning(128, 8, 6, 0) == 0
+        assert wos_ext.get_tuning() == (128, 8, 6, 0)
+        assert wos_ext.lib.wos_set_tuning(128, 8, 6, -1) == -1
+        assert b"window" in wos_ext.lib.wos_last_error()
+        assert wos_ext.lib.wos_set_tuning(32, 8, 6, 8) == -1
+        wos_ext.set_tuning(window=12)
+        assert wos_ext.get_tuning()[3] == 12
+    finally:
+        wos_ext.set_tuning(*before)

@@ -175,3 +175,35 @@ def test_spatial_order_leaves_results_unchanged(dim):
         wos_ext.set_spatial_order(True)
     for k in a:
         assert torch.equal(a[k], b[k]), k
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_query_window_leaves_results_unchanged(dim):
+    """The node visits per round (window; 0 = the per-dimension default)
+    only reschedule queries: walk values, exit records and means are
+    bit-identical for every setting."""
+    import wos_ext
+    t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).cuda()
+    if dim == 2:
+        c = C.config2(res=64)
+        sc = wos_ext.Scene(t(c["vertices"]), t(c["indices"]), t(c["values"]), use_bvh=True, source_type=3,
+                           source_params=t(c["source_params"]))
+        w = wos_ext.WoS(nwalks=16, nsteps=256, epsilon=1e-4)
+        pts = t(c["points"])
+    else:
+        v, f, noise = C.displaced_sphere(3)
+        sc = wos_ext.Scene3D(t(v), t(f), t(C.g_harmonic3(v).astype(np.float32)), use_bvh=True)
+        w = wos_ext.WoS3D(nwalks=16, nsteps=256, epsilon=1e-3)
+        pts = t(C.points_in_star(3000, noise))
+    before = wos_ext.get_tuning()
+    try:
+        outs = []
+        for win in (0, 1, 8, 16, 37):
+            wos_ext.set_tuning(window=win)
+            assert wos_ext.get_tuning()[3] == win
+            outs.append(w.walk_reduce(pts, sc, seed=9, records=True, basis=dim == 2))
+    finally:
+        wos_ext.set_tuning(*before)
+    for o in outs[1:]:
+        for k in outs[0]:
+            assert torch.equal(outs[0][k], o[k]), k
