@@ -230,8 +230,10 @@ int wos_scene_create(int dim, const float *vertices, int n_vertices, const int *
     *out = nullptr;
     REQUIRE(dim == 2 || dim == 3, "dim must be 2 or 3 (got %d)", dim);
     REQUIRE(n_vertices > 0 && n_prims > 0, "empty scene (%d vertices, %d primitives)", n_vertices, n_prims);
-    // node and leaf byte offsets are 32-bit in the traversal (64 B per node)
-    REQUIRE(n_prims <= (1 << 26), "at most 2^26 primitives per scene (got %d)", n_prims);
+    // node and leaf byte offsets are 32-bit in the traversal (64 B per 2D
+    // node, 96 B per 3D node)
+    REQUIRE(n_prims <= (dim == 2 ? (1 << 26) : (1 << 25)), "at most 2^%d primitives per %dD scene (got %d)",
+            dim == 2 ? 26 : 25, dim, n_prims);
     REQUIRE(vertices && indices && values, "vertices, indices and values are required");
     if (check_source(source_type, source_params, n_source_params)) return -1;
     hipStream_t st = S(stream);

@@ -584,8 +584,10 @@ __device__ __forceinline__ bool query_step(const SceneView &s, const float *p, Q
             for (int k = 0; k < 6; ++k) w[k] = g_top_nodes[W * node + k];
             __asm__ volatile("" ::: "memory");
         } else {
+            // 32-bit byte offset (96 B nodes; scenes hold at most 2^25 triangles)
+            const float4 *nd = at_byte_offset(s.nodes, (uint32_t)node * (uint32_t)(W * sizeof(float4)));
 #pragma unroll
-            for (int k = 0; k < 6; ++k) w[k] = s.nodes[W * node + k];
+            for (int k = 0; k < 6; ++k) w[k] = nd[k];
         }
         const int c0 = __float_as_int(w[2].w), c1 = __float_as_int(w[5].w);
         float lb[2];

@@ -69,7 +69,7 @@ const char *wos_last_error(void);
  * Builds vertex normals, the AABB and the LBVH on the device (always, as the
  * reference does, include/scene.cuh:437-452) and synchronises `stream`.
  * Rejects indices out of range, zero-length / zero-area primitives and scenes
- * of more than 2^26 primitives. */
+ * of more than 2^26 segments (2D) or 2^25 triangles (3D). */
 int wos_scene_create(int dim, const float *vertices, int n_vertices, const int *indices, int n_prims,
                      const float *values, int use_bvh, int source_type, const float *source_params,
                      int n_source_params, void *stream, wos_scene **out);
