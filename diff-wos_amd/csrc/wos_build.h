@@ -20,7 +20,7 @@ struct BuildInputs {
 
 struct BuildOutputs {
     float4 *nodes;          // (n_prims - 1) * 4 float4 (2D: capsule bounds with leaf segments inline)
-    float4 *leaves;         // n_prims * (dim == 2 ? 4 : 3) float4 (2D: Morton order, then primitive order)
+    float4 *leaves;         // n_prims * (dim == 2 ? 4 : 6) float4: Morton order, then primitive order
     float *normals;         // n_vertices * dim
     int *sorted_prim;       // n_prims
     uint32_t *sorted_codes; // n_prims

@@ -359,9 +359,14 @@ __global__ void k_leaves(const int *sorted_prim, int n, int D, const float *vtx,
             leaves[2 * (long)n + 2 * f + 1] = make_float4(inv, __int_as_float(j), 0.0f, 0.0f);
         } else {
             int i0 = idx[3 * f], i1 = idx[3 * f + 1], i2 = idx[3 * f + 2];
-            leaves[3 * j] = make_float4(vtx[3 * i0], vtx[3 * i0 + 1], vtx[3 * i0 + 2], vtx[3 * i1]);
-            leaves[3 * j + 1] = make_float4(vtx[3 * i1 + 1], vtx[3 * i1 + 2], vtx[3 * i2], vtx[3 * i2 + 1]);
+            const float4 w0 = make_float4(vtx[3 * i0], vtx[3 * i0 + 1], vtx[3 * i0 + 2], vtx[3 * i1]);
+            const float4 w1 = make_float4(vtx[3 * i1 + 1], vtx[3 * i1 + 2], vtx[3 * i2], vtx[3 * i2 + 1]);
+            leaves[3 * j] = w0;
+            leaves[3 * j + 1] = w1;
             leaves[3 * j + 2] = make_float4(vtx[3 * i2 + 2], __int_as_float(f), 0.0f, 0.0f);
+            leaves[3 * (long)n + 3 * f] = w0;
+            leaves[3 * (long)n + 3 * f + 1] = w1;
+            leaves[3 * (long)n + 3 * f + 2] = make_float4(vtx[3 * i2 + 2], __int_as_float(j), 0.0f, 0.0f);
         }
     }
 }

@@ -108,7 +108,7 @@ static SceneView view_of(const wos_scene *s) {
     v.src = s->src;
     v.nodes = s->nodes;
     v.leaves = s->leaves;
-    v.pleaves = s->dim == 2 ? s->leaves + 2 * (size_t)s->n_prims : nullptr;
+    v.pleaves = s->leaves + (s->dim == 2 ? 2 : 3) * (size_t)s->n_prims;
     v.tol = s->tol;
     v.max_depth = s->max_depth;
     v.stats = nullptr;
@@ -261,7 +261,7 @@ int wos_scene_create(int dim, const float *vertices, int n_vertices, const int *
     ACHK(hipMalloc((void **)&s->sorted_codes, sizeof(uint32_t) * (size_t)n_prims));
     ACHK(hipMalloc((void **)&s->children, sizeof(int) * 2 * (size_t)nn));
     ACHK(hipMalloc((void **)&s->nodes, sizeof(float4) * node_words(dim) * (size_t)nn));
-    ACHK(hipMalloc((void **)&s->leaves, sizeof(float4) * (dim == 2 ? 4 : 3) * (size_t)n_prims));
+    ACHK(hipMalloc((void **)&s->leaves, sizeof(float4) * (dim == 2 ? 4 : 6) * (size_t)n_prims));
     ACHK(hipMalloc((void **)&s->src, sizeof(float) * (size_t)std::max(1, s->n_src)));
     ACHK(hipMemcpyAsync(s->vtx, vertices, vb, hipMemcpyDefault, st));
     ACHK(hipMemcpyAsync(s->val, values, sizeof(float) * (size_t)n_vertices, hipMemcpyDefault, st));

@@ -283,7 +283,8 @@ __device__ __forceinline__ float tri_closest(const float *pa, const float *pb, c
 //   2D leaf: 2 x float4  {a.xy, (b-a).xy} {1/|b-a|^2, prim, -, -} in Morton
 //            order (leaves), and again in primitive order (pleaves, whose
 //            second word holds the Morton position instead of prim)
-//   3D leaf: 3 x float4  {a.xyz, b.x} {b.yz, c.xy} {c.z, prim, -, -}
+//   3D leaf: 3 x float4  {a.xyz, b.x} {b.yz, c.xy} {c.z, prim, -, -}, also
+//            twice (pleaves: primitive order, Morton position in word 2)
 struct SceneView {
     int dim, n_vertices, n_prims, use_bvh;
     const float *vtx;      // n_vertices * dim
@@ -295,7 +296,7 @@ struct SceneView {
     const float *src;      // source parameters (device)
     const float4 *nodes;   // n_prims - 1 internal nodes
     const float4 *leaves;  // n_prims leaves
-    const float4 *pleaves; // 2D: the same leaf payload in primitive order
+    const float4 *pleaves; // the same leaf payload in primitive order
     float tol;             // absolute pruning margin (scale-relative)
     int max_depth;
     unsigned long long *stats;  // null, or {box tests, primitive tests, queries, walk steps}
@@ -368,10 +369,10 @@ __device__ __forceinline__ float prim_d2(const SceneView &s, int f, const float 
         float h, dx, dy;
         return seg_d2(p[0], p[1], g.x, g.y, g.z, g.w, inv, h, dx, dy);
     } else {
-        int i0 = s.idx[3 * f], i1 = s.idx[3 * f + 1], i2 = s.idx[3 * f + 2];
-        float a[3] = {s.vtx[3 * i0], s.vtx[3 * i0 + 1], s.vtx[3 * i0 + 2]};
-        float b[3] = {s.vtx[3 * i1], s.vtx[3 * i1 + 1], s.vtx[3 * i1 + 2]};
-        float c[3] = {s.vtx[3 * i2], s.vtx[3 * i2 + 1], s.vtx[3 * i2 + 2]};
+        // the primitive-ordered copy of the leaf: three loads, no index chain
+        const float4 *lf = at_byte_offset(s.pleaves, (uint32_t)f * 48u);
+        const float4 w0 = lf[0], w1 = lf[1], w2 = lf[2];
+        float a[3] = {w0.x, w0.y, w0.z}, b[3] = {w0.w, w1.x, w1.y}, c[3] = {w1.z, w1.w, w2.x};
         float q[3], bary[2];
         return tri_closest(a, b, c, p, q, bary);
     }
