@@ -150,6 +150,52 @@ def primal_workload(args, torch, wos_ext, C):
                       "counts": cnt}))
 
 
+def shape2d_workload(args, torch, wos_ext, C):
+    """2D shape gradient on the clover (4,096 segments, 126,820 points, 16
+    walks, Laplace with g = x^2 - y^2): forward walk_reduce, loss L = mean
+    (u - u*)^2 against the solution u* for 1.1 g, OursCUDA backward onto all
+    vertex positions.  One "primal+grad walk" is a forward walk plus a
+    gradient sample (primary walk, LIB, two secondary walks)."""
+    c = C.config2()
+    dv = torch.device("cuda", 0)
+    t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dv)
+    V, F, G, pts = t(c["vertices"]), t(c["indices"]), t(c["values"]), t(c["points"])
+    P, W = pts.shape[0], 16
+    scene = wos_ext.Scene(V, F, G, use_bvh=True)
+    cw = wos_ext.WoS(nwalks=W, nsteps=c["nsteps"], epsilon=c["epsilon"])
+    kw = dict(nwalks=W, nsteps=c["nsteps"], epsilon=c["epsilon"], epsilon2=c["epsilon"], clamping=0.1)
+    target = cw.walk_reduce(pts, wos_ext.Scene(V, F, G * 1.1, use_bvh=True), seed=12345)["mean"]
+    torch.cuda.synchronize()
+
+    def step(seed):
+        ev = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
+        ev[0].record()
+        u = cw.walk_reduce(pts, scene, seed=seed)["mean"]
+        g = (2.0 / P) * (u - target)
+        ev[1].record()
+        gv, _ = wos_ext.shape_gradient(scene, pts, seed, g, **kw)
+        ev[2].record()
+        return ev, gv
+
+    for i in range(args.warmup):
+        step(i)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    evs = [step(100 + i)[0] for i in range(args.steps)]
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / args.steps
+    fwd = float(np.mean([e[0].elapsed_time(e[1]) for e in evs]))
+    bwd = float(np.mean([e[1].elapsed_time(e[2]) for e in evs]))
+    log(f"shape2d: {ms:.2f} ms/step (forward {fwd:.2f}, gradient {bwd:.2f}), {P * W / ms * 1e3:.3e} walks/s")
+    print(json.dumps({"metric": "primal+grad walks/s (2D shape gradient)", "value": P * W / ms * 1e3,
+                      "unit": "walks/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms,
+                      "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+                      "data": "synthetic",
+                      "config": {"workload": "2d-shape-gradient-clover-4096seg-126820pts-16walks", "points": P,
+                                 "walks_per_point": W, "segments": int(c["indices"].shape[0])},
+                      "forward_ms": fwd, "gradient_ms": bwd}))
+
+
 def shape3d_workload(args, torch, wos_ext, C):
     """3D shape gradient (200k-triangle decimated displaced sphere, 262,144
     points, 16 walks): forward walk_reduce, loss L = mean (u - g)^2, Ours3D
@@ -269,10 +315,12 @@ def main():
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run only a 1 GiB device-to-device copy (3 times), the reference kernel for "
                          "scaling FETCH_SIZE / WRITE_SIZE counters")
-    ap.add_argument("--workload", choices=["clover", "circle", "sphere", "shape3d", "batch"], default="clover",
+    ap.add_argument("--workload", choices=["clover", "circle", "sphere", "shape2d", "shape3d", "batch"],
+                    default="clover",
                     help="clover: the headline primal+adjoint workload; circle / sphere: primal-only "
                          "Laplace workloads (1024-segment circle, 81,920-triangle displaced icosphere); "
-                         "shape3d: 3D shape gradient (200k triangles); batch: 64 uneven scenes")
+                         "shape2d / shape3d: shape gradients (clover, 200k-triangle sphere); "
+                         "batch: 64 uneven scenes")
     ap.add_argument("--autotune", action="store_true",
                     help="batch workload: choose the launch tuning first (cached per device)")
     ap.add_argument("--batch-scale", type=float, default=0.1,
@@ -296,6 +344,8 @@ def main():
         torch.cuda.synchronize()
         print(json.dumps({"calibration": "copy", "bytes_read": a.numel() * 4, "bytes_written": b.numel() * 4}))
         return
+    if args.workload == "shape2d":
+        return shape2d_workload(args, torch, wos_ext, C)
     if args.workload == "shape3d":
         return shape3d_workload(args, torch, wos_ext, C)
     if args.workload == "batch":
