@@ -207,3 +207,26 @@ def test_query_window_leaves_results_unchanged(dim):
     for o in outs[1:]:
         for k in outs[0]:
             assert torch.equal(outs[0][k], o[k]), k
+
+
+def test_changed_values_match_a_fresh_scene():
+    """Scene.set_values(g') (wos_scene_set_values) gives the same walks as a
+    scene built with g': the leaf copies that carry the segment end-point
+    values follow the change."""
+    import wos_ext
+    t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).cuda()
+    c = C.config2(res=48)
+    v, ix = t(c["vertices"]), t(c["indices"])
+    g1 = t(c["values"])
+    g2 = t((c["values"] * 1.7 + 0.25).astype(np.float32))
+    w = wos_ext.WoS(nwalks=32, nsteps=256, epsilon=1e-4)
+    pts = t(c["points"])
+    sc = wos_ext.Scene(v, ix, g1, use_bvh=True)
+    a1 = w.walk_reduce(pts, sc, seed=5, records=True)
+    sc.set_values(g2)
+    a2 = w.walk_reduce(pts, sc, seed=5, records=True)
+    b2 = w.walk_reduce(pts, wos_ext.Scene(v, ix, g2, use_bvh=True), seed=5, records=True)
+    for k in a2:
+        assert torch.equal(a2[k], b2[k]), k
+    assert not torch.equal(a1["mean"], a2["mean"])
+    assert torch.equal(a1["rec_prim"], a2["rec_prim"])
