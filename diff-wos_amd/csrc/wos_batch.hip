@@ -1,15 +1,20 @@
 // wos_batch.hip -- many scenes of uneven size in one persistent launch.
 //
 // The reference solves one scene per call (Solver.walk_reduce,
-// wos/solver.py:49-65).  A batch concatenates the walk index spaces of all
+// wos/solver.py:49-65) and differentiates it per call (CustomSolver,
+// wos/solver.py:90-125).  A batch concatenates the walk index spaces of all
 // 2D items and of all 3D items; one persistent launch per dimension walks
 // them, each lane resolving its walk to (item, point, walk) by a binary
 // search over the item offsets kept in LDS.  Every item keeps its own scene,
 // walk settings, seed and outputs, and its walks draw exactly the streams of
 // a single-scene walk_reduce with the same seed, so batched and per-scene
-// results are identical.  All buffers are owned by the batch handle, so a
-// run performs no allocation and can be captured into a graph.
+// results are identical.  The reverse pass (wos_batch_adjoint) maps the
+// stored exit records of every item back onto its vertex values and Fourier
+// coefficients in one launch per dimension over fixed tiles of walks.  All
+// buffers are owned by the batch handle, so neither pass allocates and both
+// can be captured into one graph together with the caller's loss gradient.
 #include <algorithm>
+#include <mutex>
 #include <vector>
 
 #include "../../include/wos_amd.h"
@@ -19,6 +24,31 @@ namespace wos {
 
 constexpr int kMaxBatchItems = 1024;
 constexpr int kMaxBatchBasis = 16;
+constexpr long long kAdjointTile = 16384;   // walks of one item per adjoint tile
+constexpr int kBatchFinish = 8;             // finished walks written together (see k_walk_persistent)
+
+// This file's device copy of the tabulated Green's-function CDF.
+static __device__ float g_green_cdf[kGreenCdfSize];
+
+static int ensure_green_cdf_batch() {
+    static std::mutex lock;
+    static bool ready[64] = {false};
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) {
+        (void)hipGetLastError();
+        set_error("cannot query the current device");
+        return -1;
+    }
+    std::lock_guard<std::mutex> g(lock);
+    if (ready[dev]) return 0;
+    if (hipMemcpyToSymbol(HIP_SYMBOL(g_green_cdf), green_cdf_host(), sizeof(float) * kGreenCdfSize) != hipSuccess) {
+        (void)hipGetLastError();
+        set_error("cannot upload the Green's-function CDF table");
+        return -1;
+    }
+    ready[dev] = true;
+    return 0;
+}
 
 struct BatchItem {
     SceneView v;
@@ -32,6 +62,16 @@ struct BatchItem {
     float *walk_value, *rec_t, *basis, *mean;
     int *rec_prim;
     double *mean64, *var64;
+    const float *grad_out;      // reverse pass: dL/d mean per point (nullable)
+    float *grad_values;         // adds dL/d vertex values (nullable)
+    float *grad_coeffs;         // adds dL/d Fourier coefficients (nullable)
+};
+
+// A tile of the reverse pass: walks [begin, end) of one item (item-local
+// walk indices), so a workgroup reduces the coefficient sums of one item.
+struct AdjointTile {
+    int item;
+    long long begin, end;
 };
 
 __device__ __forceinline__ int find_item(const long long *begins, int n, long long x) {
@@ -96,6 +136,8 @@ __device__ __forceinline__ void batch_write(const BatchItem &b, long long k, boo
 // walk to its item by binary search, keeps the item's node pointer in
 // registers for its resumable queries and reads the rest of the item's
 // scene view from the item table (cached; items are few and shared).
+// Finished walks wait until kBatchFinish lanes of the wave are done or idle
+// and then write their outputs together.
 template <int D>
 __global__ __launch_bounds__(kBlock) void k_batch_walk(const BatchItem *items, int n_items, long long total,
                                                        unsigned long long *counter, int chunk, int window,
@@ -114,6 +156,7 @@ __global__ __launch_bounds__(kBlock) void k_batch_walk(const BatchItem *items, i
     float p[3] = {0.0f, 0.0f, 0.0f};
     float d = 0.0f, contrib = 0.0f;
     int step = 0, prim = -1;
+    bool done_valid = false;
     Pcg32 rng;
     rng.state = rng.inc = 0;
     Query q;
@@ -138,16 +181,20 @@ __global__ __launch_bounds__(kBlock) void k_batch_walk(const BatchItem *items, i
             const BatchItem &b = items[item];
             prim = q.best_id;
             d = sqrtf(q.best);
-            bool done_valid = d < b.w.epsilon && step < b.w.nsteps;
-            bool done_invalid = !done_valid && step >= b.w.nsteps;
-            if (done_valid || done_invalid) {
-                batch_write<D>(b, k, done_valid, prim, p, contrib, bas);
+            done_valid = d < b.w.epsilon && step < b.w.nsteps;
+            const bool done_invalid = !done_valid && step >= b.w.nsteps;
+            mode = (done_valid || done_invalid) ? kDone : kStart;
+        }
+        // 2. write finished walks once kBatchFinish lanes are done or idle
+        //    (or nothing else is left to run), then refill idle lanes
+        if (__popcll(__ballot(mode == kDone || mode == kIdle)) >= kBatchFinish || exhausted ||
+            __ballot(mode == kStart || mode == kQuery) == 0) {
+            if (mode == kDone) {
+                batch_write<D>(items[item], k, done_valid, prim, p, contrib, bas);
                 mode = kIdle;
-            } else {
-                mode = kStart;
+                k = -1;
             }
         }
-        // 2. refill idle lanes from the wave's chunk
         uint64_t idle = __ballot(mode == kIdle);
         if (idle != 0 && !exhausted) {
             int need = __popcll(idle);
@@ -220,19 +267,13 @@ __global__ __launch_bounds__(kBlock) void k_batch_walk(const BatchItem *items, i
 
         // 3. a new step: source sample, jump to the sphere, begin the query
         if (mode == kStart) {
-            const SceneView &s = items[item].v;
+            const BatchItem &b = items[item];
+            const SceneView &s = b.v;
             if (D == 2 && s.source_type != 0) {
-                const int K = items[item].K;
-                float u0 = rng.next_1d(), u1 = rng.next_1d();
-                float sn, cs;
-                sincos_turns(u0, sn, cs);
-                float r = sqrtf(u1);
-                float dyx = (r * cs) * d, dyy = (r * sn) * d;
-                float area = kPi * (d * d);
-                float pdf = 1.0f / area;
-                float rr = fmaxf(sqrtf(dot2(dyx, dyy, dyx, dyy)), 1e-4f);
-                float G = log_poly(d / rr) * kInv2Pi;
-                if (pdf > 1e-5f) {
+                const int K = b.K;
+                const float u0 = rng.next_1d(), u1 = rng.next_1d();
+                float dyx, dyy, G, w;
+                if (interior_sample(u0, u1, d, b.w.use_is != 0, g_green_cdf, dyx, dyy, G, w)) {
                     float yx = p[0] + dyx, yy = p[1] + dyy;
                     if (K > 0) {
                         float f = 0.0f;
@@ -242,12 +283,12 @@ __global__ __launch_bounds__(kBlock) void k_batch_walk(const BatchItem *items, i
                                 const const_f32 *c = as_constant(s.src) + 4 * j;
                                 float bq = sin_cw(fmaf(c[0], yx, fmaf(c[1], yy, c[2])));
                                 f = fmaf(c[3], bq, f);
-                                bas[j] = fmaf(bq * G, area, bas[j]);
+                                bas[j] = fmaf(bq * G, w, bas[j]);
                             }
                         }
-                        contrib = fmaf(f * G, area, contrib);
+                        contrib = fmaf(f * G, w, contrib);
                     } else {
-                        contrib = fmaf(source_at(s, yx, yy) * G, area, contrib);
+                        contrib = fmaf(source_at(s, yx, yy) * G, w, contrib);
                     }
                 }
             }
@@ -314,6 +355,66 @@ __global__ __launch_bounds__(kBlock) void k_batch_reduce(const BatchItem *items,
     }
 }
 
+// Reverse pass over the stored exit records, one tile of one item per
+// workgroup visit: a valid walk of point i adds g_i / W times the
+// interpolation weights of its exit primitive to the vertex values
+// (include/scene.cuh:251-264 transposed) and g_i / W times its Fourier basis
+// sums to the coefficients (include/wos.cuh:136-138 transposed).  Coefficient
+// sums reduce across the wave and the workgroup before one atomic per tile.
+template <int D>
+__global__ __launch_bounds__(kBlock) void k_batch_adjoint(const BatchItem *items, const AdjointTile *tiles,
+                                                          int n_tiles) {
+    __shared__ float cacc[kMaxBatchBasis];
+    for (int t = blockIdx.x; t < n_tiles; t += gridDim.x) {
+        const AdjointTile tl = tiles[t];
+        const BatchItem &b = items[tl.item];
+        if (threadIdx.x < kMaxBatchBasis) cacc[threadIdx.x] = 0.0f;
+        __syncthreads();
+        const int W = b.w.nwalks;
+        const long long total = (long long)b.n_points * W;
+        const bool want_c = b.grad_coeffs != nullptr && b.basis != nullptr && b.K > 0;
+        float cl[kMaxBatchBasis];
+#pragma unroll
+        for (int q = 0; q < kMaxBatchBasis; ++q) cl[q] = 0.0f;
+        for (long long k = tl.begin + threadIdx.x; k < tl.end; k += kBlock) {
+            const int f = b.rec_prim[k];
+            if (f < 0) continue;
+            const float g = b.grad_out[point_of_walk(k, W)] / (float)W;
+            if (b.grad_values) {
+                const int *ix = b.v.idx;
+                if constexpr (D == 2) {
+                    const float tt = b.rec_t[k];
+                    atomicAdd(&b.grad_values[ix[2 * f]], g * (1.0f - tt));
+                    atomicAdd(&b.grad_values[ix[2 * f + 1]], g * tt);
+                } else {
+                    const float u = b.rec_t[2 * k], v = b.rec_t[2 * k + 1];
+                    atomicAdd(&b.grad_values[ix[3 * f]], g * (1.0f - u - v));
+                    atomicAdd(&b.grad_values[ix[3 * f + 1]], g * u);
+                    atomicAdd(&b.grad_values[ix[3 * f + 2]], g * v);
+                }
+            }
+            if (want_c) {
+#pragma unroll
+                for (int q = 0; q < kMaxBatchBasis; ++q)
+                    if (q < b.K) cl[q] = fmaf(g, b.basis[(long long)q * total + k], cl[q]);
+            }
+        }
+        if (want_c) {
+#pragma unroll
+            for (int q = 0; q < kMaxBatchBasis; ++q) {
+                if (q < b.K) {
+                    float v = cl[q];
+                    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+                    if (__lane_id() == 0) atomicAdd(&cacc[q], v);
+                }
+            }
+        }
+        __syncthreads();
+        if (want_c && threadIdx.x < b.K) atomicAdd(&b.grad_coeffs[threadIdx.x], cacc[threadIdx.x]);
+        __syncthreads();
+    }
+}
+
 __global__ void k_batch_zero(unsigned long long *c, int n) {
     if (threadIdx.x < n) c[threadIdx.x] = 0ull;
 }
@@ -325,6 +426,8 @@ struct BatchGroup {
     int *iprim = nullptr;
     float *isd = nullptr;
     int max_depth = 0;
+    AdjointTile *tiles = nullptr; // device copy (null: no item has a reverse pass)
+    int n_tiles = 0;
 };
 
 }  // namespace wos
@@ -337,11 +440,16 @@ struct wos_batch {
     int2 *spill = nullptr;    // traversal spill owned by the batch
     long spill_lanes = 0;
     int device = 0;
+    // each item's scene and the scene's generation when the item table was
+    // built: a setter that frees or re-types what the table points at
+    // (source, grids, use_bvh) makes the next run fail instead of reading it
+    std::vector<std::pair<const wos_scene *, unsigned long long>> scenes;
 };
 
 // defined in wos_capi.hip
 namespace wos {
 SceneView scene_view(const wos_scene *s);
+unsigned long long scene_generation(const wos_scene *s);
 }
 
 #define BREQ(cond, ...)              \
@@ -358,6 +466,7 @@ static void free_batch(wos_batch *b) {
         (void)hipFree(g.items);
         (void)hipFree(g.iprim);
         (void)hipFree(g.isd);
+        (void)hipFree(g.tiles);
     }
     (void)hipFree(b->counters);
     (void)hipFree(b->spill);
@@ -375,10 +484,20 @@ static int occupancy_grid_b() {
     return cus * (g_tuning.wgs_per_cu > 0 ? g_tuning.wgs_per_cu : 8);
 }
 
+static int check_scenes(const wos_batch *B) {
+    for (size_t j = 0; j < B->scenes.size(); ++j)
+        BREQ(scene_generation(B->scenes[j].first) == B->scenes[j].second,
+             "the scene of batch item %zu changed after the batch was created (source, grid or use_bvh); "
+             "create the batch again", j);
+    return 0;
+}
+
 extern "C" int wos_batch_create(const wos_batch_item *items, int n_items, wos_batch **out) {
     BREQ(out && items && n_items > 0, "items and out are required");
     *out = nullptr;
     std::vector<BatchItem> grp[2];
+    std::vector<std::pair<const wos_scene *, unsigned long long>> scenes;
+    bool any_is = false;
     for (int j = 0; j < n_items; ++j) {
         const wos_batch_item &it = items[j];
         BREQ(it.scene, "item %d has no scene", j);
@@ -391,7 +510,8 @@ extern "C" int wos_batch_create(const wos_batch_item *items, int n_items, wos_ba
         BatchItem b;
         b.v = v;
         b.w = WalkSettings{it.params.nwalks, it.params.nsteps, it.params.epsilon, it.params.double_sided,
-                           it.params.prevent_fd_artifacts};
+                           it.params.prevent_fd_artifacts, it.params.use_IS_for_greens ? 1 : 0};
+        any_is = any_is || (b.w.use_is && v.source_type != 0);
         b.points = it.points;
         b.n_points = it.n_points;
         b.seed = it.seed;
@@ -408,38 +528,62 @@ extern "C" int wos_batch_create(const wos_batch_item *items, int n_items, wos_ba
         b.mean = it.mean;
         b.mean64 = it.mean64;
         b.var64 = it.var64;
+        b.grad_out = it.grad_out;
+        b.grad_values = it.grad_values;
+        b.grad_coeffs = it.grad_coeffs;
+        if (it.n_points > 0 && (it.grad_values || it.grad_coeffs)) {
+            BREQ(it.grad_out, "item %d: the reverse pass needs grad_out", j);
+            BREQ(it.rec_prim && it.rec_t, "item %d: the reverse pass needs the exit records (rec_prim, rec_t)", j);
+            BREQ(!it.grad_coeffs || it.basis, "item %d: coefficient gradients need the basis output", j);
+        }
         grp[v.dim == 2 ? 0 : 1].push_back(b);
+        scenes.emplace_back(it.scene, scene_generation(it.scene));
     }
+    for (int d = 0; d < 2; ++d)
+        BREQ((int)grp[d].size() <= kMaxBatchItems, "at most %d items per dimension", kMaxBatchItems);
+    if (any_is && ensure_green_cdf_batch() != 0) return -1;
     wos_batch *B = new wos_batch();
+    B->scenes = std::move(scenes);
     (void)hipGetDevice(&B->device);
     for (int d = 0; d < 2; ++d) {
         auto &v = grp[d];
-        BREQ((int)v.size() <= kMaxBatchItems, "at most %d items per dimension", kMaxBatchItems);
         long long wb = 0, pb = 0;
         int depth = 0;
-        for (auto &b : v) {
+        std::vector<AdjointTile> tiles;
+        for (int j = 0; j < (int)v.size(); ++j) {
+            BatchItem &b = v[j];
             b.walk_begin = wb;
             b.point_begin = pb;
-            wb += (long long)b.n_points * b.w.nwalks;
+            const long long nw = (long long)b.n_points * b.w.nwalks;
+            wb += nw;
             pb += b.n_points;
             depth = std::max(depth, b.v.use_bvh ? b.v.max_depth : 0);
+            if (b.n_points > 0 && (b.grad_values || b.grad_coeffs))
+                for (long long t = 0; t < nw; t += kAdjointTile) tiles.push_back(AdjointTile{j, t, std::min(nw, t + kAdjointTile)});
         }
         BatchGroup &g = B->g[d];
         g.n_items = (int)v.size();
         g.walks = wb;
         g.points = pb;
         g.max_depth = depth;
+        g.n_tiles = (int)tiles.size();
         if (!g.n_items) continue;
         if (hipMalloc((void **)&g.items, sizeof(BatchItem) * v.size()) != hipSuccess ||
             hipMalloc((void **)&g.iprim, sizeof(int) * (size_t)std::max(pb, 1LL)) != hipSuccess ||
             hipMalloc((void **)&g.isd, sizeof(float) * (size_t)std::max(pb, 1LL)) != hipSuccess ||
-            hipMemcpy(g.items, v.data(), sizeof(BatchItem) * v.size(), hipMemcpyHostToDevice) != hipSuccess) {
+            hipMemcpy(g.items, v.data(), sizeof(BatchItem) * v.size(), hipMemcpyHostToDevice) != hipSuccess ||
+            (g.n_tiles > 0 &&
+             (hipMalloc((void **)&g.tiles, sizeof(AdjointTile) * tiles.size()) != hipSuccess ||
+              hipMemcpy(g.tiles, tiles.data(), sizeof(AdjointTile) * tiles.size(), hipMemcpyHostToDevice) !=
+                  hipSuccess))) {
+            (void)hipGetLastError();
             free_batch(B);
             set_error("cannot allocate the batch tables");
             return -1;
         }
     }
     if (hipMalloc((void **)&B->counters, 2 * sizeof(unsigned long long)) != hipSuccess) {
+        (void)hipGetLastError();
         free_batch(B);
         set_error("cannot allocate the batch counters");
         return -1;
@@ -448,6 +592,7 @@ extern "C" int wos_batch_create(const wos_batch_item *items, int n_items, wos_ba
     long deep = (long)depth + 2 - kLdsStack;
     B->spill_lanes = (long)occupancy_grid_b() * kBlock;
     if (deep > 0 && hipMalloc((void **)&B->spill, sizeof(int2) * (size_t)deep * (size_t)B->spill_lanes) != hipSuccess) {
+        (void)hipGetLastError();
         free_batch(B);
         set_error("cannot allocate the batch traversal spill");
         return -1;
@@ -464,6 +609,7 @@ extern "C" int wos_batch_destroy(wos_batch *b) {
 
 extern "C" int wos_batch_run(wos_batch *B, void *stream) {
     BREQ(B, "batch is NULL");
+    if (check_scenes(B)) return -1;
     hipStream_t st = reinterpret_cast<hipStream_t>(stream);
     const int cap = (int)(B->spill_lanes / kBlock);
     k_batch_zero<<<1, 64, 0, st>>>(B->counters, 2);
@@ -492,5 +638,22 @@ extern "C" int wos_batch_run(wos_batch *B, void *stream) {
     }
     hipError_t e = hipGetLastError();
     BREQ(e == hipSuccess, "batch launch: %s", hipGetErrorString(e));
+    return 0;
+}
+
+extern "C" int wos_batch_adjoint(wos_batch *B, void *stream) {
+    BREQ(B, "batch is NULL");
+    if (check_scenes(B)) return -1;
+    hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+    const int cap = (int)(B->spill_lanes / kBlock);
+    for (int d = 0; d < 2; ++d) {
+        BatchGroup &g = B->g[d];
+        if (!g.n_tiles) continue;
+        const int grid = std::max(1, std::min(g.n_tiles, cap));
+        if (d == 0) k_batch_adjoint<2><<<grid, kBlock, 0, st>>>(g.items, g.tiles, g.n_tiles);
+        else k_batch_adjoint<3><<<grid, kBlock, 0, st>>>(g.items, g.tiles, g.n_tiles);
+    }
+    hipError_t e = hipGetLastError();
+    BREQ(e == hipSuccess, "batch adjoint launch: %s", hipGetErrorString(e));
     return 0;
 }

@@ -93,7 +93,13 @@ struct WalkSettings {
     int nwalks, nsteps;
     float epsilon;
     int double_sided, prevent_fd_artifacts;
+    int use_is;   // interior samples of a source from the tabulated Green's-function CDF
 };
+
+// Host copy of the tabulated Green's-function CDF (kGreenCdfSize floats,
+// entry i = x^2 (1 - 2 ln x) at x = (i + 1) / kGreenCdfSize, evaluated in
+// double and rounded to float).
+const float *green_cdf_host();
 
 struct WalkOutputs {
     float *walk_value;   // P*W, nullable
@@ -119,8 +125,11 @@ extern Tuning g_tuning;
 // window pays for its idle lanes by running the step phase fewer times.
 inline int window_for(int dim) { return g_tuning.window > 0 ? g_tuning.window : (dim == 2 ? 8 : 16); }
 
-// Grows (never shrinks) the traversal spill and the launch counters.
-int2 *spill_buffer(const SceneView &s, long lanes, long *stride, hipStream_t stream);
+// Traversal spill of `stream` (grown, never shrunk) for `lanes` lanes: 0 with
+// *out = null when the tree fits the LDS stack, -1 (error set) when it cannot
+// be allocated.  Launches on one stream share it; other streams get their own.
+int spill_buffer(const SceneView &s, long lanes, long *stride, int2 **out, hipStream_t stream);
+// Queue counter of `stream` (null, error set, when it cannot be allocated).
 unsigned long long *launch_counter(hipStream_t stream);
 
 int launch_closest_index(const SceneView &s, const float *p, int n, int *out_idx, float *out_d2, hipStream_t st);
@@ -139,6 +148,9 @@ int launch_walk_reduce(const SceneView &s, const WalkSettings &w, const float *p
 int launch_adjoint_values(const SceneView &s, int n_points, int nwalks, const int *rec_prim, const float *rec_t,
                           const float *basis, const float *grad_out, float *grad_values, float *grad_coeffs,
                           hipStream_t st);
+
+int launch_source_adjoint(const SceneView &s, const WalkSettings &w, const float *points, int n_points,
+                          int32_t seed, const float *grad_out, float *grad_src, hipStream_t st);
 
 struct GradSettings {
     int nwalks, nsteps;

@@ -1,9 +1,11 @@
 // wos_capi.hip -- the C ABI declared in include/wos_amd.h: scene ownership,
 // argument checks, workspace, and dispatch to the kernels.
+#include <math.h>
 #include <stdarg.h>
 #include <stdio.h>
 
 #include <algorithm>
+#include <map>
 #include <mutex>
 
 #include "../../include/wos_amd.h"
@@ -23,6 +25,9 @@ struct wos_scene {
     int top_counts[17] = {0};
     wos::GridOutputs grid;        // 2D near-boundary candidate grid (start == null: none)
     wos::StartGridOutputs sgrid;  // 2D start-node grid (start == null: none)
+    // bumped by every setter that frees a buffer or changes what a walk
+    // reads besides the vertex values (batches check it before each run)
+    unsigned long long generation = 0;
 };
 
 namespace wos {
@@ -36,61 +41,82 @@ void set_error(const char *fmt, ...) {
     va_end(ap);
 }
 
-// ---- workspace: traversal spill and queue counters (per process, device 0..7)
+// ---- workspace: traversal spill and queue counter, one per (device, stream).
+// Launches on one stream run in order, so they can share one spill buffer and
+// one counter; launches on different streams get different ones and never
+// overwrite each other's traversal stacks.
 namespace {
-struct DeviceWorkspace {
+struct StreamWorkspace {
     int2 *spill = nullptr;
     size_t spill_bytes = 0;
-    unsigned long long *counters = nullptr;
-    int next_counter = 0;
+    unsigned long long *counter = nullptr;
 };
-constexpr int kMaxDevices = 16;
-constexpr int kCounters = 64;
-DeviceWorkspace g_ws[kMaxDevices];
+std::map<std::pair<int, hipStream_t>, StreamWorkspace> g_ws;
 std::mutex g_ws_lock;
-}  // namespace
 
-int2 *spill_buffer(const SceneView &s, long lanes, long *stride, hipStream_t stream) {
-    *stride = lanes;
-    long deep = (long)s.max_depth + 2 - kLdsStack;
-    if (!s.use_bvh || deep <= 0) return nullptr;
-    size_t need = (size_t)deep * (size_t)lanes * sizeof(int2);
+StreamWorkspace &workspace_of(hipStream_t stream) {
     int dev = 0;
     (void)hipGetDevice(&dev);
+    return g_ws[std::make_pair(dev, stream)];
+}
+}  // namespace
+
+// Failure injection for the workspace error paths (set through
+// wos_debug_fail_workspace): the next n spill requests fail without
+// allocating, so tests can drive every launcher's error return.
+static int g_fail_workspace_allocs = 0;
+
+int spill_buffer(const SceneView &s, long lanes, long *stride, int2 **out, hipStream_t stream) {
+    *stride = lanes;
+    *out = nullptr;
+    {
+        std::lock_guard<std::mutex> g(g_ws_lock);
+        if (g_fail_workspace_allocs > 0) {
+            --g_fail_workspace_allocs;
+            set_error("traversal spill allocation failed (injected)");
+            return -1;
+        }
+    }
+    long deep = (long)s.max_depth + 2 - kLdsStack;
+    if (!s.use_bvh || deep <= 0) return 0;   // the LDS part holds every entry
+    size_t need = (size_t)deep * (size_t)lanes * sizeof(int2);
     std::lock_guard<std::mutex> g(g_ws_lock);
-    DeviceWorkspace &w = g_ws[dev % kMaxDevices];
+    StreamWorkspace &w = workspace_of(stream);
     if (w.spill_bytes < need) {
-        // the old buffer may still be read by queued kernels
-        (void)hipDeviceSynchronize();
-        if (w.spill) (void)hipFree(w.spill);
+        // the old buffer may still be read by kernels queued on this stream
+        if (w.spill) {
+            if (hipStreamSynchronize(stream) != hipSuccess) {
+                set_error("cannot synchronise the stream before growing the traversal spill");
+                return -1;
+            }
+            (void)hipFree(w.spill);
+        }
         w.spill = nullptr;
         w.spill_bytes = 0;
         if (hipMalloc((void **)&w.spill, need) != hipSuccess) {
+            (void)hipGetLastError();
             set_error("cannot allocate %zu bytes of traversal spill", need);
-            return nullptr;
+            return -1;
         }
         w.spill_bytes = need;
     }
-    (void)stream;
-    return w.spill;
+    *out = w.spill;
+    return 0;
 }
 
 unsigned long long *launch_counter(hipStream_t stream) {
-    (void)stream;
-    int dev = 0;
-    (void)hipGetDevice(&dev);
     std::lock_guard<std::mutex> g(g_ws_lock);
-    DeviceWorkspace &w = g_ws[dev % kMaxDevices];
-    if (!w.counters) {
-        if (hipMalloc((void **)&w.counters, kCounters * sizeof(unsigned long long)) != hipSuccess) {
-            set_error("cannot allocate queue counters");
+    StreamWorkspace &w = workspace_of(stream);
+    if (!w.counter) {
+        if (hipMalloc((void **)&w.counter, 4 * sizeof(unsigned long long)) != hipSuccess) {
+            (void)hipGetLastError();
+            set_error("cannot allocate the queue counter");
             return nullptr;
         }
     }
-    unsigned long long *c = w.counters + w.next_counter;
-    w.next_counter = (w.next_counter + 1) % kCounters;
-    return c;
+    return w.counter;
 }
+
 
 static SceneView view_of(const wos_scene *s) {
     SceneView v;
@@ -135,7 +161,20 @@ static SceneView view_of(const wos_scene *s) {
     return v;
 }
 
+const float *green_cdf_host() {
+    static float table[kGreenCdfSize];
+    static std::once_flag once;
+    std::call_once(once, [] {
+        for (int i = 0; i < kGreenCdfSize; ++i) {
+            const double x = (double)(i + 1) / (double)kGreenCdfSize;
+            table[i] = (float)(x * x * (1.0 - 2.0 * log(x)));
+        }
+    });
+    return table;
+}
+
 SceneView scene_view(const wos_scene *s) { return view_of(s); }
+unsigned long long scene_generation(const wos_scene *s) { return s->generation; }
 
 }  // namespace wos
 
@@ -221,6 +260,13 @@ static int check_source(int source_type, const float *params, int n) {
 extern "C" {
 
 const char *wos_last_error(void) { return g_err; }
+
+int wos_debug_fail_workspace(int n) {
+    REQUIRE(n >= 0, "n must be >= 0 (got %d)", n);
+    std::lock_guard<std::mutex> g(g_ws_lock);
+    g_fail_workspace_allocs = n;
+    return 0;
+}
 
 int wos_scene_create(int dim, const float *vertices, int n_vertices, const int *indices, int n_prims,
                      const float *values, int use_bvh, int source_type, const float *source_params,
@@ -313,6 +359,7 @@ int wos_scene_set_grid(wos_scene *s, float cell_factor, int band_cells, int max_
     REQUIRE(s->dim == 2, "the candidate grid is built for 2D scenes only");
     REQUIRE(band_cells >= 0 && max_cand >= 0, "band_cells and max_cand must be non-negative");
     CHK(hipDeviceSynchronize());   // kernels in flight may still read the old grid
+    ++s->generation;
     (void)hipFree(s->grid.start);
     (void)hipFree(s->grid.cand);
     s->grid = GridOutputs{};
@@ -329,6 +376,7 @@ int wos_scene_set_start_grid(wos_scene *s, int resolution, void *stream) {
     REQUIRE(resolution >= kStartGridAuto && resolution <= cap,
             "resolution must be in [0, %d] or %d (automatic) (got %d)", cap, kStartGridAuto, resolution);
     CHK(hipDeviceSynchronize());   // kernels in flight may still read the old grid
+    ++s->generation;
     (void)hipFree(s->sgrid.start);
     s->sgrid = StartGridOutputs{};
     return build_start_grid(s, resolution, S(stream));
@@ -393,6 +441,7 @@ int wos_scene_set_source(wos_scene *s, int source_type, const float *params, int
     REQUIRE(s, "scene is NULL");
     if (check_source(source_type, params, n)) return -1;
     int need = source_type ? n : 0;
+    ++s->generation;
     if (need > s->n_src) {
         CHK(hipStreamSynchronize(S(stream)));
         (void)hipFree(s->src);
@@ -407,6 +456,7 @@ int wos_scene_set_source(wos_scene *s, int source_type, const float *params, int
 
 int wos_scene_set_use_bvh(wos_scene *s, int use_bvh) {
     REQUIRE(s, "scene is NULL");
+    if ((use_bvh ? 1 : 0) != s->use_bvh) ++s->generation;
     s->use_bvh = use_bvh ? 1 : 0;
     return 0;
 }
@@ -484,7 +534,8 @@ int wos_single_walk(const wos_scene *s, const wos_walk_params *w, const float *p
     REQUIRE(s && o && (n == 0 || (p && state && inc)), "scene, p, sampler state/inc and out are required");
     REQUIRE(n >= 0, "negative count");
     if (check_walk(w)) return -1;
-    WalkSettings ws{w->nwalks, w->nsteps, w->epsilon, w->double_sided, w->prevent_fd_artifacts};
+    WalkSettings ws{w->nwalks, w->nsteps, w->epsilon, w->double_sided, w->prevent_fd_artifacts,
+                    w->use_IS_for_greens ? 1 : 0};
     return launch_single_walk(view_of(s), ws, p, n, state, inc, o->valid, o->p, o->n, o->t, o->prim_id, o->contrib,
                               S(stream));
 }
@@ -498,7 +549,8 @@ int wos_walk_reduce(const wos_scene *s, const wos_walk_params *w, const float *p
     REQUIRE((long long)n_points * w->nwalks < (1LL << 40), "too many walks");
     REQUIRE(!(rec_prim && !rec_t) && !(rec_t && !rec_prim), "rec_prim and rec_t go together");
     if (basis) REQUIRE(s->source_type == 3, "basis output needs a Fourier source (source_type 3)");
-    WalkSettings ws{w->nwalks, w->nsteps, w->epsilon, w->double_sided, w->prevent_fd_artifacts};
+    WalkSettings ws{w->nwalks, w->nsteps, w->epsilon, w->double_sided, w->prevent_fd_artifacts,
+                    w->use_IS_for_greens ? 1 : 0};
     WalkOutputs o{walk_value, rec_prim, rec_t, basis, mean, mean64, var64};
     return launch_walk_reduce(view_of(s), ws, points, n_points, (int32_t)seed, chunk, o, S(stream));
 }
@@ -510,7 +562,8 @@ int wos_walk_reduce_counts(const wos_scene *s, const wos_walk_params *w, const f
     if (n_points == 0) return 0;
     SceneView v = view_of(s);
     v.stats = counts;
-    WalkSettings ws{w->nwalks, w->nsteps, w->epsilon, w->double_sided, w->prevent_fd_artifacts};
+    WalkSettings ws{w->nwalks, w->nsteps, w->epsilon, w->double_sided, w->prevent_fd_artifacts,
+                    w->use_IS_for_greens ? 1 : 0};
     WalkOutputs o{walk_value, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
     float *basis = nullptr;
     if (with_basis && s->source_type == 3) {
@@ -531,6 +584,17 @@ int wos_adjoint_values(const wos_scene *s, int n_points, int nwalks, const int *
     if (basis) REQUIRE(s->source_type == 3, "basis input needs a Fourier source (source_type 3)");
     return launch_adjoint_values(view_of(s), n_points, nwalks, rec_prim, rec_t, basis, grad_out, grad_values,
                                  grad_coeffs, S(stream));
+}
+
+int wos_adjoint_source(const wos_scene *s, const wos_walk_params *w, const float *points, int n_points, int seed,
+                       const float *grad_out, float *grad_source, void *stream) {
+    REQUIRE(s && (n_points == 0 || (points && grad_out && grad_source)),
+            "scene, points, grad_out and grad_source are required");
+    REQUIRE(n_points >= 0, "negative count");
+    if (check_walk(w)) return -1;
+    WalkSettings ws{w->nwalks, w->nsteps, w->epsilon, w->double_sided, w->prevent_fd_artifacts,
+                    w->use_IS_for_greens ? 1 : 0};
+    return launch_source_adjoint(view_of(s), ws, points, n_points, (int32_t)seed, grad_out, grad_source, S(stream));
 }
 
 int wos_shape_gradient(const wos_scene *s, const wos_grad_params *g, const float *points, int n_points, int seed,

@@ -863,6 +863,92 @@ __device__ __forceinline__ float source_at(const SceneView &s, float x, float y)
 }
 
 // ---------------------------------------------------------------------------
+// interior samples of the source term (include/wos.cuh:44-88, 115-139)
+// ---------------------------------------------------------------------------
+
+// Tabulated CDF of the radial density r log(1/r) on [0, 1]: entry i is
+// F((i + 1) / kGreenCdfSize) with F(x) = x^2 (1 - 2 ln x), rounded to float
+// (the closed form of include/tabulated_G_cdf.cuh's table).  Every
+// translation unit whose kernels sample it holds its own copy, filled from
+// the host by green_cdf_host() before the first launch on a device.
+constexpr int kGreenCdfSize = 10000;
+
+// Bin of u in the table, as the reference's binary search returns it
+// (include/wos.cuh:49-58): the smallest l with u < cdf[l], or the last bin
+// when there is none.  Found without the search: u = F(x) is, in b = -2 ln x,
+// e^-b (1 + b) = u, whose series solution b ~ t + t^2/3 + t^3/36 (t =
+// sqrt(-2 ln u)) after one Newton step on ln(1 + b) - b - ln u gives x =
+// e^(-b/2) within one bin of the answer for every u that next_1d can draw;
+// the table entries either side confirm it (they are the pmf and the
+// previous CDF value the sample needs anyway), and a guess that misses by
+// more than two bins falls back to the binary search.  cdf_lo / cdf_hi:
+// cdf[l - 1] (0 for l = 0) and cdf[l].
+__device__ __forceinline__ int green_bin(const float *cdf, float u, float &cdf_lo, float &cdf_hi) {
+    const float lu = log_poly(fmaxf(u, 1e-30f));
+    const float t = sqrtf(-2.0f * lu);
+    float b = t * fmaf(t, fmaf(t, 0.02777777777777777778f, 0.33333333333333333333f), 1.0f);
+    const float g = log_poly(1.0f + b) - b - lu;
+    b = fmaf(g * (1.0f + b), 1.0f / fmaxf(b, 1e-30f), b);
+    const float x = exp_poly(-0.5f * b);
+    int l = (int)(x * (float)kGreenCdfSize);
+    l = l < 0 ? 0 : (l > kGreenCdfSize - 1 ? kGreenCdfSize - 1 : l);
+    for (int it = 0; it < 3; ++it) {
+        cdf_lo = l > 0 ? cdf[l - 1] : 0.0f;
+        cdf_hi = cdf[l];
+        if (l > 0 && u < cdf_lo) { --l; continue; }
+        if (!(u < cdf_hi) && l < kGreenCdfSize - 1) { ++l; continue; }
+        return l;
+    }
+    int lo = 0, hi = kGreenCdfSize - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi) / 2;
+        if (u < cdf[mid]) hi = mid; else lo = mid + 1;
+    }
+    cdf_lo = lo > 0 ? cdf[lo - 1] : 0.0f;
+    cdf_hi = cdf[lo];
+    return lo;
+}
+
+// One interior sample of the disk of radius d around the walk position from
+// the draws (u0, u1): the offset (dx, dy), the Green's function G of the
+// disk at it (include/wos.cuh:44-47) and the weight w = Jacobian / pdf, so
+// that the step adds f(x + offset) G w.  Uniform in the disk (w = pi d^2),
+// or, with `is`, radius from the tabulated CDF with the draw reused inside
+// its bin and angle 2 pi u1 (include/wos.cuh:49-88; w = |offset| / pdf).
+// Returns false where the reference adds nothing (pdf <= 1e-5).
+__device__ __forceinline__ bool interior_sample(float u0, float u1, float d, bool is, const float *cdf, float &dx,
+                                                float &dy, float &G, float &w) {
+    float pdf;
+    if (is) {
+        float lo, hi;
+        const int l = green_bin(cdf, u0, lo, hi);
+        const float pmf = hi - lo;
+        const float ur = (u0 - lo) / pmf;
+        const float rad = d * ((float)l + ur) / (float)kGreenCdfSize;
+        pdf = pmf * (float)kGreenCdfSize / d / (2.0f * kPi);
+        float sn, cs;
+        sincos_turns(u1, sn, cs);
+        dx = rad * cs;
+        dy = rad * sn;
+        const float r = sqrtf(dot2(dx, dy, dx, dy));
+        G = log_poly(d / fmaxf(r, 1e-4f)) * kInv2Pi;
+        w = r / pdf;
+    } else {
+        float sn, cs;
+        sincos_turns(u0, sn, cs);
+        const float r = sqrtf(u1);
+        dx = (r * cs) * d;
+        dy = (r * sn) * d;
+        const float area = kPi * (d * d);
+        pdf = 1.0f / area;
+        const float rr = fmaxf(sqrtf(dot2(dx, dy, dx, dy)), 1e-4f);
+        G = log_poly(d / rr) * kInv2Pi;
+        w = area;
+    }
+    return pdf > 1e-5f;
+}
+
+// ---------------------------------------------------------------------------
 // walk steps (include/wos.cuh:14-42, 115-142)
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ float bound_d2(const SceneView &s, float r) {
@@ -877,20 +963,14 @@ __device__ __forceinline__ void load_point(const float *p, long i, float *q) {
     q[2] = (D == 3) ? p[D * i + 2] : 0.0f;
 }
 
+// include/wos.cuh:115-139: one interior sample, f G w added to contrib
 template <int D>
-__device__ __forceinline__ void source_step(const SceneView &s, const float *p, float d, Pcg32 &rng,
-                                            float &contrib) {
-    // include/wos.cuh:115-139: uniform point in the ball, G_2D / pdf
-    float u0 = rng.next_1d(), u1 = rng.next_1d();
-    float sn, cs;
-    sincos_turns(u0, sn, cs);
-    float r = sqrtf(u1);
-    float dyx = (r * cs) * d, dyy = (r * sn) * d;
-    float area = kPi * (d * d);
-    float pdf = 1.0f / area;
-    float rr = fmaxf(sqrtf(dot2(dyx, dyy, dyx, dyy)), 1e-4f);
-    float G = log_poly(d / rr) * kInv2Pi;
-    if (pdf > 1e-5f) contrib = fmaf(source_at(s, p[0] + dyx, p[1] + dyy) * G, area, contrib);
+__device__ __forceinline__ void source_step(const SceneView &s, const float *p, float d, bool is, const float *cdf,
+                                            Pcg32 &rng, float &contrib) {
+    const float u0 = rng.next_1d(), u1 = rng.next_1d();
+    float dyx, dyy, G, w;
+    if (interior_sample(u0, u1, d, is, cdf, dyx, dyy, G, w))
+        contrib = fmaf(source_at(s, p[0] + dyx, p[1] + dyy) * G, w, contrib);
 }
 
 template <int D>

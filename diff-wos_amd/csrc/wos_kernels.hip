@@ -14,6 +14,7 @@
 //   k_shape_gradient   OursCUDA / Ours3D (wos/wos_grad.py:130-151,
 //                      wos/wos_grad_3d.py:117-137) transposed onto vertices
 #include <algorithm>
+#include <mutex>
 
 #include "wos_build.h"
 
@@ -31,6 +32,30 @@ __host__ __device__ constexpr int basis_cap(int KB) { return KB < 0 ? kMaxBasis 
 // instead of once per round for one or two.
 constexpr int kFinishBatch = 8;
 constexpr int kGridCap = 4096;  // workgroups of a grid-stride launch
+
+// This file's device copy of the tabulated Green's-function CDF, filled once
+// per device before the first launch that may sample it.
+static __device__ float g_green_cdf[kGreenCdfSize];
+
+static int ensure_green_cdf() {
+    static std::mutex lock;
+    static bool ready[64] = {false};
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) {
+        (void)hipGetLastError();
+        set_error("cannot query the current device");
+        return -1;
+    }
+    std::lock_guard<std::mutex> g(lock);
+    if (ready[dev]) return 0;
+    if (hipMemcpyToSymbol(HIP_SYMBOL(g_green_cdf), green_cdf_host(), sizeof(float) * kGreenCdfSize) != hipSuccess) {
+        (void)hipGetLastError();
+        set_error("cannot upload the Green's-function CDF table");
+        return -1;
+    }
+    ready[dev] = true;
+    return 0;
+}
 
 #define KCHK()                                                        \
     do {                                                              \
@@ -166,10 +191,55 @@ __global__ void k_dirichlet(SceneView s, const uint8_t *valid, const int *prim, 
 // ---------------------------------------------------------------------------
 // one walk on one lane (include/wos.cuh:97-145)
 // ---------------------------------------------------------------------------
+constexpr int kMaxSourceParams = 4 * kMaxBasis;   // Fourier: (ax, ay, phase, coeff) per term
+
+// Adds d f / d theta (x, y) * Gw to acc[theta] for every parameter theta of
+// the scene's source (include/scene.cuh:278-301: Gaussian (mu_x, mu_y,
+// sigma_x, sigma_y, A), sinusoid (A, B, C, amplitude, offset); Fourier
+// sum_k c_k sin(ax_k x + ay_k y + ph_k) as (ax, ay, ph, c) per term).
+__device__ __forceinline__ void add_source_param_grad(const SceneView &s, float x, float y, float Gw, float *acc) {
+    const float *q = s.src;
+    if (s.source_type == 1) {
+        const float ex = x - q[0], ey = y - q[1];
+        const float isx2 = 1.0f / (q[2] * q[2]), isy2 = 1.0f / (q[3] * q[3]);
+        const float e = exp_poly(-(ex * ex) * (0.5f * isx2) - (ey * ey) * (0.5f * isy2));
+        const float f = q[4] * e;
+        acc[0] = fmaf(f * ex * isx2, Gw, acc[0]);
+        acc[1] = fmaf(f * ey * isy2, Gw, acc[1]);
+        acc[2] = fmaf(f * (ex * ex) * isx2 / q[2], Gw, acc[2]);
+        acc[3] = fmaf(f * (ey * ey) * isy2 / q[3], Gw, acc[3]);
+        acc[4] = fmaf(e, Gw, acc[4]);
+    } else if (s.source_type == 2) {
+        const float phi = fmaf(q[0], x, fmaf(q[1], y, q[2]));
+        const float ac = q[3] * sin_cw(phi + 1.57079632679489661923f);
+        acc[0] = fmaf(ac * x, Gw, acc[0]);
+        acc[1] = fmaf(ac * y, Gw, acc[1]);
+        acc[2] = fmaf(ac, Gw, acc[2]);
+        acc[3] = fmaf(sin_cw(phi), Gw, acc[3]);
+        acc[4] += Gw;
+    } else if (s.source_type == 3) {
+        const int K = s.n_src / 4;
+#pragma unroll
+        for (int k = 0; k < kMaxBasis; ++k) {
+            if (k < K) {
+                const float phi = fmaf(q[4 * k], x, fmaf(q[4 * k + 1], y, q[4 * k + 2]));
+                const float ac = q[4 * k + 3] * sin_cw(phi + 1.57079632679489661923f);
+                acc[4 * k] = fmaf(ac * x, Gw, acc[4 * k]);
+                acc[4 * k + 1] = fmaf(ac * y, Gw, acc[4 * k + 1]);
+                acc[4 * k + 2] = fmaf(ac, Gw, acc[4 * k + 2]);
+                acc[4 * k + 3] = fmaf(sin_cw(phi), Gw, acc[4 * k + 3]);
+            }
+        }
+    }
+}
+
 // init_prim >= 0: the closest primitive of p0 is already known (init_d2).
-template <int D>
+// PGRAD: also accumulate d contrib / d theta for every source parameter
+// into pacc (kMaxSourceParams floats, zeroed by the caller).
+template <int D, bool PGRAD = false>
 __device__ void walk_one(const SceneView &s, const WalkSettings &w, const float *p0, Pcg32 &rng, int init_prim,
-                         float init_d2, const TravStack &st, int slot, long lane_g, Rec &rec, float &contrib) {
+                         float init_d2, const TravStack &st, int slot, long lane_g, Rec &rec, float &contrib,
+                         float *pacc = nullptr) {
     float p[3] = {p0[0], p0[1], D == 3 ? p0[2] : 0.0f};
     rec.valid = 0;
     rec.prim = -1;
@@ -189,7 +259,19 @@ __device__ void walk_one(const SceneView &s, const WalkSettings &w, const float 
             closest_record<D>(s, prim, p, rec);
             return;
         }
-        if (src) source_step<D>(s, p, d, rng, contrib);
+        if constexpr (PGRAD) {
+            if (src) {
+                const float u0 = rng.next_1d(), u1 = rng.next_1d();
+                float dyx, dyy, G, wt;
+                if (interior_sample(u0, u1, d, w.use_is != 0, g_green_cdf, dyx, dyy, G, wt)) {
+                    const float yx = p[0] + dyx, yy = p[1] + dyy;
+                    contrib = fmaf(source_at(s, yx, yy) * G, wt, contrib);
+                    add_source_param_grad(s, yx, yy, G * wt, pacc);
+                }
+            }
+        } else {
+            if (src) source_step<D>(s, p, d, w.use_is != 0, g_green_cdf, rng, contrib);
+        }
         move_point<D>(p, d, rng);
         if (s.stats) atomicAdd(&s.stats[3], 1ull);
         prim = closest_prim<D>(s, p, bound_d2(s, 2.0f * d), d2, st, slot, lane_g);
@@ -216,6 +298,53 @@ __global__ __launch_bounds__(kBlock) void k_single_walk(SceneView s, WalkSetting
         state[i] = rng.state;
         inc[i] = rng.inc;
     }
+}
+
+// Reverse mode of walk_reduce w.r.t. every source parameter: each walk k = i
+// W + w is re-run from its key (TEA(seed, i), sequence k -- the streams of
+// the forward launch, so the same paths), accumulating d contrib / d theta
+// along the way; a walk that ends on the boundary adds grad_out[i] / W times
+// it (an invalid walk's value, and so its derivative, is 0).  Loops are
+// wave-uniform so the per-walk sums reduce across the wave by shuffles, then
+// in LDS per workgroup, then once per workgroup into grad_src.
+__global__ __launch_bounds__(kBlock) void k_source_adjoint(SceneView s, WalkSettings w, const float *points,
+                                                           int n_points, int32_t seed, const float *grad_out,
+                                                           float *grad_src, int2 *spill, long stride) {
+    DECLARE_STACK(spill, stride);
+    __shared__ float sacc[kMaxSourceParams];
+    for (int q = threadIdx.x; q < kMaxSourceParams; q += kBlock) sacc[q] = 0.0f;
+    __syncthreads();
+    const int np = s.n_src;
+    const int W = w.nwalks;
+    const long long total = (long long)n_points * W;
+    const long long wave_base = (long long)blockIdx.x * kBlock + (threadIdx.x & ~63);
+    for (long long kb = wave_base; kb < total; kb += (long long)gridDim.x * kBlock) {
+        const long long k = kb + __lane_id();
+        float acc[kMaxSourceParams];
+#pragma unroll
+        for (int q = 0; q < kMaxSourceParams; ++q) acc[q] = 0.0f;
+        float gw = 0.0f;
+        if (k < total) {
+            const uint32_t i = point_of_walk(k, W);
+            float p[3];
+            load_point<2>(points, i, p);
+            Pcg32 rng = walk_stream(seed, i, k);
+            Rec r;
+            float contrib;
+            walk_one<2, true>(s, w, p, rng, -1, 0.0f, st, slot, lane_g, r, contrib, acc);
+            if (r.valid) gw = grad_out[i] / (float)W;
+        }
+#pragma unroll
+        for (int q = 0; q < kMaxSourceParams; ++q) {
+            if (q < np) {
+                float v = acc[q] * gw;
+                for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+                if (__lane_id() == 0) atomicAdd(&sacc[q], v);
+            }
+        }
+    }
+    __syncthreads();
+    for (int q = threadIdx.x; q < np; q += kBlock) atomicAdd(&grad_src[q], sacc[q]);
 }
 
 // ---------------------------------------------------------------------------
@@ -444,16 +573,9 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(D == 2 ?
                 }
             }
             if (SRC) {
-                // include/wos.cuh:115-139: uniform point in the ball, G_2D / pdf
-                float sn, cs;
-                sincos_turns(su0, sn, cs);
-                float r = sqrtf(su1);
-                float dyx = (r * cs) * d, dyy = (r * sn) * d;
-                float area = kPi * (d * d);
-                float pdf = 1.0f / area;
-                float rr = fmaxf(sqrtf(dot2(dyx, dyy, dyx, dyy)), 1e-4f);
-                float G = log_poly(d / rr) * kInv2Pi;
-                if (pdf > 1e-5f) {
+                // include/wos.cuh:115-139: one interior sample, f G w
+                float dyx, dyy, G, w;
+                if (interior_sample(su0, su1, d, a.w.use_is != 0, g_green_cdf, dyx, dyy, G, w)) {
                     float yx = px0 + dyx, yy = py0 + dyy;
                     if constexpr (KB != 0) {
                         float f = 0.0f;
@@ -463,16 +585,16 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(D == 2 ?
                                 const const_f32 *c = as_constant(s.src) + 4 * j;
                                 float b = sin_cw(fmaf(c[0], yx, fmaf(c[1], yy, c[2])));
                                 f = fmaf(c[3], b, f);
-                                bas[j] = fmaf(b * G, area, bas[j]);
+                                bas[j] = fmaf(b * G, w, bas[j]);
                                 // one term at a time: interleaving the terms'
                                 // arithmetic costs registers (occupancy); the
                                 // coefficients are scalar loads (as_constant)
                                 __builtin_amdgcn_sched_barrier(0);
                             }
                         }
-                        contrib = fmaf(f * G, area, contrib);
+                        contrib = fmaf(f * G, w, contrib);
                     } else {
-                        contrib = fmaf(source_at(s, yx, yy) * G, area, contrib);
+                        contrib = fmaf(source_at(s, yx, yy) * G, w, contrib);
                     }
                 }
             }
@@ -1062,7 +1184,8 @@ int launch_closest_index(const SceneView &s, const float *p, int n, int *out, fl
     if (n <= 0) return 0;
     int g = grid_of(n, occupancy_grid());
     long stride;
-    int2 *spill = spill_buffer(s, (long)g * kBlock, &stride, st);
+    int2 *spill = nullptr;
+    if (spill_buffer(s, (long)g * kBlock, &stride, &spill, st) != 0) return -1;
     if (s.dim == 2) k_closest_index<2><<<g, kBlock, top_bytes(s), st>>>(s, p, n, out, out_d2, spill, stride);
     else k_closest_index<3><<<g, kBlock, top_bytes(s), st>>>(s, p, n, out, out_d2, spill, stride);
     KCHK();
@@ -1073,7 +1196,8 @@ int launch_sdf(const SceneView &s, const float *p, int n, float *out, hipStream_
     if (n <= 0) return 0;
     int g = grid_of(n, occupancy_grid());
     long stride;
-    int2 *spill = spill_buffer(s, (long)g * kBlock, &stride, st);
+    int2 *spill = nullptr;
+    if (spill_buffer(s, (long)g * kBlock, &stride, &spill, st) != 0) return -1;
     if (s.dim == 2) k_sdf<2><<<g, kBlock, top_bytes(s), st>>>(s, p, n, out, spill, stride);
     else k_sdf<3><<<g, kBlock, top_bytes(s), st>>>(s, p, n, out, spill, stride);
     KCHK();
@@ -1085,7 +1209,8 @@ int launch_closest_point(const SceneView &s, const float *p, int n, uint8_t *val
     if (n <= 0) return 0;
     int g = grid_of(n, occupancy_grid());
     long stride;
-    int2 *spill = spill_buffer(s, (long)g * kBlock, &stride, st);
+    int2 *spill = nullptr;
+    if (spill_buffer(s, (long)g * kBlock, &stride, &spill, st) != 0) return -1;
     if (s.dim == 2)
         k_closest_point<2><<<g, kBlock, top_bytes(s), st>>>(s, p, n, valid, op, on, ot, oprim, ocontrib, spill, stride);
     else
@@ -1099,7 +1224,8 @@ int launch_lib(const SceneView &s, const uint8_t *valid, const float *p, const f
     if (n <= 0) return 0;
     int g = grid_of(n, occupancy_grid());
     long stride;
-    int2 *spill = spill_buffer(s, (long)g * kBlock, &stride, st);
+    int2 *spill = nullptr;
+    if (spill_buffer(s, (long)g * kBlock, &stride, &spill, st) != 0) return -1;
     if (s.dim == 2) k_lib<2><<<g, kBlock, top_bytes(s), st>>>(s, valid, p, nrm, n, eps, out, spill, stride);
     else k_lib<3><<<g, kBlock, top_bytes(s), st>>>(s, valid, p, nrm, n, eps, out, spill, stride);
     KCHK();
@@ -1119,9 +1245,11 @@ int launch_single_walk(const SceneView &s, const WalkSettings &w, const float *p
                        uint64_t *inc, uint8_t *valid, float *op, float *on, float *ot, int *oprim, float *ocontrib,
                        hipStream_t st) {
     if (n <= 0) return 0;
+    if (w.use_is && s.source_type != 0 && ensure_green_cdf() != 0) return -1;
     int g = grid_of(n, occupancy_grid());
     long stride;
-    int2 *spill = spill_buffer(s, (long)g * kBlock, &stride, st);
+    int2 *spill = nullptr;
+    if (spill_buffer(s, (long)g * kBlock, &stride, &spill, st) != 0) return -1;
     if (s.dim == 2)
         k_single_walk<2><<<g, kBlock, top_bytes(s), st>>>(s, w, p, n, state, inc, valid, op, on, ot, oprim, ocontrib, spill,
                                                stride);
@@ -1133,6 +1261,35 @@ int launch_single_walk(const SceneView &s, const WalkSettings &w, const float *p
 }
 
 __global__ void k_zero_u64(unsigned long long *p) { *p = 0ull; }
+
+// Per-launch start records of the persistent kernels (closest primitive and
+// signed distance of each query point) and the points' visiting order.  The
+// buffers are stream-ordered allocations released on the launch stream when
+// the guard leaves scope -- after the kernels that read them are queued, or
+// at once on an error return.
+struct StartRecords {
+    hipStream_t st;
+    int *prim = nullptr;
+    float *sd = nullptr;
+    int *order = nullptr;
+    explicit StartRecords(hipStream_t s) : st(s) {}
+    StartRecords(const StartRecords &) = delete;
+    StartRecords &operator=(const StartRecords &) = delete;
+    ~StartRecords() {
+        if (prim) (void)hipFreeAsync(prim, st);
+        if (sd) (void)hipFreeAsync(sd, st);
+        if (order) (void)hipFreeAsync(order, st);
+    }
+    int alloc(const SceneView &s, const float *points, int n_points) {
+        if (hipMallocAsync((void **)&prim, sizeof(int) * (size_t)n_points, st) != hipSuccess ||
+            hipMallocAsync((void **)&sd, sizeof(float) * (size_t)n_points, st) != hipSuccess) {
+            (void)hipGetLastError();
+            set_error("cannot allocate %d start records", n_points);
+            return -1;
+        }
+        return spatial_order(s, points, n_points, &order, st);
+    }
+};
 
 // Walk indices are split into (point, walk) in 32-bit arithmetic on the device.
 static bool walk_count_fits(long long total) {
@@ -1146,6 +1303,7 @@ int launch_walk_reduce(const SceneView &s, const WalkSettings &w, const float *p
     if (n_points <= 0) return 0;
     long long total = (long long)n_points * w.nwalks;
     if (!walk_count_fits(total)) return -1;
+    if (w.use_is && s.source_type != 0 && ensure_green_cdf() != 0) return -1;
     int K = (s.source_type == 3) ? s.n_src / 4 : 0;
     if (o.basis && K > kMaxBasis) {
         set_error("Fourier source has %d terms; at most %d are supported", K, kMaxBasis);
@@ -1173,22 +1331,15 @@ int launch_walk_reduce(const SceneView &s, const WalkSettings &w, const float *p
     if (g < 1) g = 1;
     int gi = grid_of(n_points, cap);
     long lanes = (long)std::max(g, gi) * kBlock;
-    a.spill = spill_buffer(s, lanes, &a.spill_stride, st);
-    int *iprim = nullptr;
-    float *isd = nullptr;
-    if (hipMallocAsync((void **)&iprim, sizeof(int) * (size_t)n_points, st) != hipSuccess ||
-        hipMallocAsync((void **)&isd, sizeof(float) * (size_t)n_points, st) != hipSuccess) {
-        set_error("cannot allocate %d start records", n_points);
-        return -1;
-    }
-    a.init_prim = iprim;
-    a.init_sd = isd;
-    int *order = nullptr;
-    if (spatial_order(s, points, n_points, &order, st) != 0) return -1;
-    a.order = order;
+    if (spill_buffer(s, lanes, &a.spill_stride, &a.spill, st) != 0) return -1;
+    StartRecords sr(st);
+    if (sr.alloc(s, points, n_points) != 0) return -1;
+    a.init_prim = sr.prim;
+    a.init_sd = sr.sd;
+    a.order = sr.order;
     size_t tb = top_bytes(s);
-    if (s.dim == 2) k_point_init<2><<<gi, kBlock, tb, st>>>(s, points, n_points, order, iprim, isd, a.spill, a.spill_stride);
-    else k_point_init<3><<<gi, kBlock, tb, st>>>(s, points, n_points, nullptr, iprim, isd, a.spill, a.spill_stride);
+    if (s.dim == 2) k_point_init<2><<<gi, kBlock, tb, st>>>(s, points, n_points, sr.order, sr.prim, sr.sd, a.spill, a.spill_stride);
+    else k_point_init<3><<<gi, kBlock, tb, st>>>(s, points, n_points, nullptr, sr.prim, sr.sd, a.spill, a.spill_stride);
     k_zero_u64<<<1, 1, 0, st>>>(a.counter);
     bool src = s.dim == 2 && s.source_type != 0;
     bool bas = src && o.basis != nullptr && K > 0;
@@ -1208,9 +1359,6 @@ int launch_walk_reduce(const SceneView &s, const WalkSettings &w, const float *p
         if (s.stats) k_walk_persistent<3, false, 0, true><<<g, kBlock, tb, st>>>(s, a);
         else k_walk_persistent<3, false, 0, false><<<g, kBlock, tb, st>>>(s, a);
     }
-    (void)hipFreeAsync(iprim, st);
-    (void)hipFreeAsync(isd, st);
-    if (order) (void)hipFreeAsync(order, st);
     KCHK();
     if (o.mean || o.mean64 || o.var64) {
         if (!o.walk_value) {
@@ -1254,6 +1402,29 @@ int launch_adjoint_values(const SceneView &s, int n_points, int nwalks, const in
     return 0;
 }
 
+int launch_source_adjoint(const SceneView &s, const WalkSettings &w, const float *points, int n_points, int32_t seed,
+                          const float *grad_out, float *grad_src, hipStream_t st) {
+    if (n_points <= 0 || s.source_type == 0 || s.n_src <= 0) return 0;
+    if (s.dim != 2) {
+        set_error("source terms exist for 2D scenes only");
+        return -1;
+    }
+    if (s.n_src > kMaxSourceParams) {
+        set_error("source has %d parameters; at most %d are supported", s.n_src, kMaxSourceParams);
+        return -1;
+    }
+    const long long total = (long long)n_points * w.nwalks;
+    if (!walk_count_fits(total)) return -1;
+    if (w.use_is && ensure_green_cdf() != 0) return -1;
+    int g = grid_of((long)std::min<long long>(total, 1LL << 30), occupancy_grid());
+    long stride;
+    int2 *spill = nullptr;
+    if (spill_buffer(s, (long)g * kBlock, &stride, &spill, st) != 0) return -1;
+    k_source_adjoint<<<g, kBlock, top_bytes(s), st>>>(s, w, points, n_points, seed, grad_out, grad_src, spill, stride);
+    KCHK();
+    return 0;
+}
+
 int launch_shape_gradient(const SceneView &s, const GradSettings &g, const float *points, int n_points, int32_t seed,
                           const float *grad_out, float *grad_vertices, float *sample_grad_u, hipStream_t st,
                           const float *velocity, float *sample_jvp) {
@@ -1285,28 +1456,18 @@ int launch_shape_gradient(const SceneView &s, const GradSettings &g, const float
     if (gr < 1) gr = 1;
     int gi = grid_of(n_points, cap);
     long lanes = (long)std::max(gr, gi) * kBlock;
-    a.spill = spill_buffer(s, lanes, &a.spill_stride, st);
-    int *iprim = nullptr;
-    float *isd = nullptr;
-    if (hipMallocAsync((void **)&iprim, sizeof(int) * (size_t)n_points, st) != hipSuccess ||
-        hipMallocAsync((void **)&isd, sizeof(float) * (size_t)n_points, st) != hipSuccess) {
-        set_error("cannot allocate %d start records", n_points);
-        return -1;
-    }
-    a.init_prim = iprim;
-    a.init_sd = isd;
-    int *order = nullptr;
-    if (spatial_order(s, points, n_points, &order, st) != 0) return -1;
-    a.order = order;
+    if (spill_buffer(s, lanes, &a.spill_stride, &a.spill, st) != 0) return -1;
+    StartRecords sr(st);
+    if (sr.alloc(s, points, n_points) != 0) return -1;
+    a.init_prim = sr.prim;
+    a.init_sd = sr.sd;
+    a.order = sr.order;
     size_t tb = top_bytes(s);
-    if (s.dim == 2) k_point_init<2><<<gi, kBlock, tb, st>>>(s, points, n_points, order, iprim, isd, a.spill, a.spill_stride);
-    else k_point_init<3><<<gi, kBlock, tb, st>>>(s, points, n_points, nullptr, iprim, isd, a.spill, a.spill_stride);
+    if (s.dim == 2) k_point_init<2><<<gi, kBlock, tb, st>>>(s, points, n_points, sr.order, sr.prim, sr.sd, a.spill, a.spill_stride);
+    else k_point_init<3><<<gi, kBlock, tb, st>>>(s, points, n_points, nullptr, sr.prim, sr.sd, a.spill, a.spill_stride);
     k_zero_u64<<<1, 1, 0, st>>>(a.counter);
     if (s.dim == 2) k_grad_persistent<2><<<gr, kBlock, tb, st>>>(s, a);
     else k_grad_persistent<3><<<gr, kBlock, tb, st>>>(s, a);
-    (void)hipFreeAsync(iprim, st);
-    (void)hipFreeAsync(isd, st);
-    if (order) (void)hipFreeAsync(order, st);
     KCHK();
     return 0;
 }

@@ -254,11 +254,20 @@ def random_star_mesh(n_tris, amp=0.15, seed=0):
     return (U * (1.0 + amp * noise(U))[:, None]).astype(np.float32), F.astype(np.int32), noise
 
 
+def icosphere_level(n_tris):
+    """Subdivision level (3..6) of the icosphere whose 20 * 4^s triangles are
+    nearest to n_tris on a log scale: 1,280 / 5,120 / 20,480 / 81,920."""
+    return int(np.clip(np.rint(np.log(n_tris / 20.0) / np.log(4.0)), 3, 6))
+
+
 def config5(scale=1.0, seed=0):
     """64 uneven scenes: 32 clover/star polylines with 256..16,384 segments
-    and 32 displaced spheres with 1k..100k triangles (log-uniform sizes),
-    4k..1M query points each (log-uniform, times `scale`), 16..128 walks,
-    half of the 2D scenes Poisson (Fourier source), the rest Laplace."""
+    and 32 displaced icospheres with 1k..100k triangles (log-uniform target
+    sizes; each 3D scene is the icosphere of the nearest subdivision level,
+    1,280..81,920 triangles, displaced radially by 1 + 0.15 n with its own
+    noise seed), 4k..1M query points each (log-uniform, times `scale`),
+    16..128 walks, half of the 2D scenes Poisson (Fourier source), the rest
+    Laplace."""
     rng = np.random.default_rng(seed)
     out = []
     for j in range(64):
@@ -280,7 +289,7 @@ def config5(scale=1.0, seed=0):
                 item.update(source_type=3, source_params=fourier_params())
         else:
             ntri = int(np.exp(rng.uniform(np.log(1e3), np.log(1e5))))
-            v, f, noise = random_star_mesh(ntri, seed=j)
+            v, f, noise = displaced_sphere(icosphere_level(ntri), seed=j)
             p = points_in_star(n_pts, noise, seed=j)
             item = dict(dim=3, vertices=v, indices=f, values=g_harmonic3(v).astype(np.float32), points=p, nwalks=W,
                         nsteps=256, epsilon=1e-3, seed=j)

@@ -89,7 +89,72 @@ class BoundarySamplingRecord:
     pdf: torch.Tensor = None
 
 
-class Polyline(wos_ext.Scene):
+class NativeFields:
+    """``vertices``, ``values`` and ``source_params`` of a scene class over
+    the native scene.  In the reference they are the native scene's
+    def_readwrite fields (src/python/scene.cu:35-47), so assigning one
+    changes what the walks use: here assigning ``values`` or
+    ``source_params`` copies the new tensor into the native scene
+    (wos_scene_set_values / wos_scene_set_source), and ``vertices``, from
+    which the BVH is built, cannot be reassigned (build a new scene).  The
+    tensors are kept as given, so solvers' gradients reach them."""
+
+    @property
+    def vertices(self):
+        return self.__dict__.get("_t_vertices")
+
+    @vertices.setter
+    def vertices(self, v):
+        if self.__dict__.get("_t_vertices") is not None:
+            raise wos_ext.WoSError("a scene's vertices are fixed when it is built (the BVH is built from them); "
+                                   "build a new scene for new vertices")
+        self.__dict__["_t_vertices"] = v
+
+    @property
+    def values(self):
+        return self.__dict__.get("_t_values")
+
+    @values.setter
+    def values(self, v):
+        if not isinstance(v, torch.Tensor):
+            v = Float(v)
+        self.__dict__["_t_values"] = v
+        h = self.__dict__.get("_h")
+        if h is not None and h.value:
+            self.set_values(v)
+
+    @property
+    def source_params(self):
+        return self.__dict__.get("_t_source_params")
+
+    @source_params.setter
+    def source_params(self, v):
+        if v is not None and not isinstance(v, torch.Tensor):
+            v = Float(v)
+        self.__dict__["_t_source_params"] = v
+        h = self.__dict__.get("_h")
+        if h is not None and h.value and self.source_type:
+            if v is None:
+                raise wos_ext.WoSError("a scene with a source needs source_params")
+            self.set_source(self.source_type, v)
+
+    def swap_fields(self, vertices, values, source_params):
+        """Puts tensors equal in value to the current fields in their place
+        (e.g. detached copies that require grad) without touching the native
+        scene; returns the previous ones for the matching restore."""
+        d = self.__dict__
+        saved = (d.get("_t_vertices"), d.get("_t_values"), d.get("_t_source_params"))
+        d["_t_vertices"], d["_t_values"] = vertices, values
+        if source_params is not None:
+            d["_t_source_params"] = source_params
+        return saved
+
+    def restore_fields(self, saved):
+        d = self.__dict__
+        d["_t_vertices"], d["_t_values"], d["_t_source_params"] = saved
+
+
+class Polyline(NativeFields, wos_ext.Scene):
     """wos/scene.py:192-475 over the native scene.  ``use_bvh`` defaults to
     True: the LBVH query returns the same primitive as the brute-force scan
     (equal squared distances resolve to the smallest index in both), so the

@@ -8,7 +8,7 @@ import torch
 
 import wos_ext
 from wos.fwd import Array3, Array3i, Float, device
-from wos.scene import BoundarySamplingRecord, Detector
+from wos.scene import BoundarySamplingRecord, Detector, NativeFields
 from wos.utils import DiscreteDistribution
 
 
@@ -59,7 +59,7 @@ class ClosestPointRecord3D:
         return wos_ext.ClosestPointRecord3D(self.valid, self.p, self.n, self.uv, self.prim_id, self.contrib)
 
 
-class Scene3D(wos_ext.Scene3D):
+class Scene3D(NativeFields, wos_ext.Scene3D):
     """wos/scene3d.py:106-274 over the native triangle-mesh scene (LBVH on by
     default; the flag selects speed only, see wos.scene.Polyline)."""
 

@@ -52,31 +52,33 @@ def _source_tensor(scene):
 
 class _ValueWalk(torch.autograd.Function):
     """mean_i = 1/W sum_w value(walk i,w); reverse mode through the exit
-    records (values) and the per-walk source basis sums (coefficients)."""
+    records stored by the forward launch (vertex values) and, for the source
+    parameters, by re-running the walks from their keys (every parameter of
+    a Gaussian, sinusoid or Fourier source; wos_ext.adjoint_source)."""
 
     @staticmethod
     def forward(ctx, cwos, nwalks, points, scene, seed, values, source_params):
         need_v = ctx.needs_input_grad[5]
-        need_c = ctx.needs_input_grad[6] and scene.source_type == 3
-        out = cwos.walk_reduce(points, scene, seed, nwalks=nwalks, records=need_v or need_c, basis=need_c)
-        ctx.scene, ctx.nwalks, ctx.P = scene, nwalks, out["mean"].shape[0]
-        ctx.need_v, ctx.need_c = need_v, need_c
+        need_s = ctx.needs_input_grad[6] and bool(scene.source_type)
+        out = cwos.walk_reduce(points, scene, seed, nwalks=nwalks, records=need_v)
+        ctx.scene, ctx.cwos, ctx.nwalks, ctx.seed, ctx.P = scene, cwos, nwalks, seed, out["mean"].shape[0]
+        ctx.points = points.detach() if need_s else None
+        ctx.need_v, ctx.need_s = need_v, need_s
         ctx.sp_shape = None if source_params is None else source_params.shape
-        ctx.out = {k: out[k] for k in ("rec_prim", "rec_t", "basis") if k in out}
+        ctx.out = {k: out[k] for k in ("rec_prim", "rec_t") if k in out}
         return out["mean"]
 
     @staticmethod
     def backward(ctx, g):
         o = ctx.out
-        gv = gc = None
-        if ctx.need_v or ctx.need_c:
-            gv, gc = wos_ext.adjoint_values(ctx.scene, ctx.P, ctx.nwalks, o["rec_prim"], o["rec_t"], g.contiguous(),
-                                            basis=o.get("basis"), want_values=ctx.need_v, want_coeffs=ctx.need_c)
-        gsp = None
-        if ctx.need_c:
-            gsp = torch.zeros(ctx.sp_shape, dtype=torch.float32, device=g.device).reshape(-1, 4)
-            gsp[:, 3] = gc
-            gsp = gsp.reshape(ctx.sp_shape)
+        g = g.contiguous()
+        gv = gsp = None
+        if ctx.need_v:
+            gv, _ = wos_ext.adjoint_values(ctx.scene, ctx.P, ctx.nwalks, o["rec_prim"], o["rec_t"], g,
+                                           want_values=True, want_coeffs=False)
+        if ctx.need_s:
+            gsp = wos_ext.adjoint_source(ctx.scene, ctx.cwos, ctx.points, ctx.seed, g,
+                                         nwalks=ctx.nwalks).reshape(ctx.sp_shape)
         return None, None, None, None, None, gv, gsp
 
 
@@ -178,14 +180,11 @@ class Solver:
     def _walk_with(self, p, scene, seed, nwalks, vertices, values, source_params):
         """_walk with the scene's vertex / value / source tensors replaced by
         the given ones for the duration of the call (CustomSolver backward)."""
-        saved = (scene.vertices, scene.values, getattr(scene, "source_params", None))
-        scene.vertices, scene.values = vertices, values
-        if source_params is not None:
-            scene.source_params = source_params
+        saved = scene.swap_fields(vertices, values, source_params)
         try:
             return self._walk(p, scene, seed, nwalks)
         finally:
-            scene.vertices, scene.values, scene.source_params = saved
+            scene.restore_fields(saved)
 
 
 class _CustomWalk(torch.autograd.Function):

@@ -91,12 +91,15 @@ _sig("wos_walk_reduce_counts", _vp, ctypes.POINTER(_WalkParams), _vp, _i, _i, _i
 class _BatchItem(ctypes.Structure):
     _fields_ = [("scene", _vp), ("params", _WalkParams), ("points", _vp), ("n_points", _i), ("seed", _i),
                 ("mean", _vp), ("mean64", _vp), ("var64", _vp), ("walk_value", _vp), ("rec_prim", _vp),
-                ("rec_t", _vp), ("basis", _vp)]
+                ("rec_t", _vp), ("basis", _vp), ("grad_out", _vp), ("grad_values", _vp), ("grad_coeffs", _vp)]
 
 
 _sig("wos_batch_create", ctypes.POINTER(_BatchItem), _i, ctypes.POINTER(_vp))
 _sig("wos_batch_run", _vp, _vp)
+_sig("wos_batch_adjoint", _vp, _vp)
 _sig("wos_batch_destroy", _vp)
+_sig("wos_adjoint_source", _vp, ctypes.POINTER(_WalkParams), _vp, _i, _i, _vp, _vp, _vp)
+_sig("wos_debug_fail_workspace", _i)
 _sig("wos_set_tuning", _i, _i, _i, _i)
 _sig("wos_get_tuning", _vp, _vp, _vp, _vp)
 _sig("wos_set_spatial_order", _i)
@@ -111,7 +114,8 @@ EXPORTED = ("wos_last_error", "wos_scene_create", "wos_scene_destroy", "wos_scen
             "wos_largest_inscribed_ball", "wos_dirichlet", "wos_single_walk", "wos_walk_reduce",
             "wos_walk_reduce_counts", "wos_adjoint_values", "wos_shape_gradient", "wos_shape_derivative",
             "wos_set_tuning",
-            "wos_get_tuning", "wos_set_spatial_order", "wos_batch_create", "wos_batch_run", "wos_batch_destroy")
+            "wos_get_tuning", "wos_set_spatial_order", "wos_batch_create", "wos_batch_run", "wos_batch_adjoint",
+            "wos_batch_destroy", "wos_adjoint_source", "wos_debug_fail_workspace")
 
 
 def set_tuning(chunk=None, wgs_per_cu=None, top_levels=None, window=None):
@@ -595,6 +599,31 @@ def adjoint_values(scene, n_points, nwalks, rec_prim, rec_t, grad_out, basis=Non
     return gv, gc
 
 
+def adjoint_source(scene, cwos, points, seed, grad_out, *, nwalks=None):
+    """Reverse mode of ``cwos.walk_reduce(points, scene, seed)`` w.r.t. every
+    source parameter (wos_adjoint_source): the walks are re-run from their
+    keys.  Returns a float32 tensor shaped like the scene's source
+    parameters (zeros for a scene without a source)."""
+    q = _dev(points, torch.float32, scene.dim)
+    P = q.shape[0]
+    sp = scene.c_source_params
+    n = 0 if (sp is None or not scene.source_type) else sp.numel()
+    out = torch.zeros(max(n, 1), dtype=torch.float32, device=q.device)
+    if n:
+        go = _dev(grad_out, torch.float32).reshape(-1)
+        if go.numel() != P:
+            raise WoSError(f"grad_out has {go.numel()} entries for {P} points")
+        wp = cwos._params(nwalks)
+        _check(lib.wos_adjoint_source(scene.handle, ctypes.byref(wp), _p(q), P, int(seed), _p(go), _p(out),
+                                      _stream()))
+    return out[:n].reshape(sp.shape) if n else out[:0]
+
+
+def debug_fail_workspace(n):
+    """Testing hook: the next n traversal-spill requests fail."""
+    _check(lib.wos_debug_fail_workspace(int(n)))
+
+
 def shape_gradient(scene, points, seed, grad_out, *, nwalks, nsteps, epsilon, epsilon2, clamping,
                    double_sided=False, control_variates=True, sample_grad_u=False):
     """OursCUDA / Ours3D backward onto vertex positions."""
@@ -636,8 +665,17 @@ class Batch:
     persistent launch per dimension (``wos_batch_run``).  Item j's outputs
     are ``outputs[j]`` (``mean``, ``mean64``, ``var64``, ``walk_value`` and,
     on request, ``rec_prim``/``rec_t``/``basis``); each equals what
-    ``WoS.walk_reduce`` returns for that item alone.  ``run()`` allocates
-    nothing, so it can be captured with ``torch.cuda.graph``."""
+    ``WoS.walk_reduce`` returns for that item alone.
+
+    Reverse pass: an item with ``grad_values=True`` (and/or, for a Fourier
+    source, ``grad_coeffs=True``) also gets ``grad_out`` (dL/d mean, filled
+    by the caller after ``run()``) and ``grad_values`` / ``grad_coeffs``
+    buffers that ``adjoint()`` adds into (``zero_grad()`` clears them); the
+    result equals ``adjoint_values`` on that item's records.  ``run()``,
+    ``zero_grad()`` and ``adjoint()`` allocate nothing, so a whole primal +
+    loss + adjoint step can be captured with ``torch.cuda.graph``.  The
+    batch holds its scenes' device buffers: changing an item's source, grids
+    or use_bvh afterwards makes ``run()`` raise (create the batch again)."""
 
     def __init__(self, items):
         self._h = _vp(None)
@@ -648,22 +686,33 @@ class Batch:
             q = _dev(it["points"], torch.float32, scene.dim)
             P, W = q.shape[0], cw.nwalks
             dv = q.device
+            want_v = bool(it.get("grad_values"))
+            want_c = bool(it.get("grad_coeffs"))
             o = {"mean": torch.empty(P, dtype=torch.float32, device=dv),
                  "mean64": torch.empty(P, dtype=torch.float64, device=dv),
                  "var64": torch.empty(P, dtype=torch.float64, device=dv),
                  "walk_value": torch.empty(P * W, dtype=torch.float32, device=dv)}
-            if it.get("records"):
+            if it.get("records") or want_v or want_c:
                 o["rec_prim"] = torch.empty(P * W, dtype=torch.int32, device=dv)
                 o["rec_t"] = torch.empty(P * W * (scene.dim - 1), dtype=torch.float32, device=dv)
-            if it.get("basis"):
+            if it.get("basis") or want_c:
+                if scene.source_type != 3:
+                    raise WoSError(f"item {j}: basis / coefficient gradients need a Fourier source (source_type 3)")
                 K = scene.c_source_params.numel() // 4
                 o["basis"] = torch.empty(K * P * W, dtype=torch.float32, device=dv)
+            if want_v or want_c:
+                o["grad_out"] = torch.zeros(P, dtype=torch.float32, device=dv)
+            if want_v:
+                o["grad_values"] = torch.zeros(scene.c_values.numel(), dtype=torch.float32, device=dv)
+            if want_c:
+                o["grad_coeffs"] = torch.zeros(scene.c_source_params.numel() // 4, dtype=torch.float32, device=dv)
             self.outputs.append(o)
             self._keep += [q, scene]
             g = o.get
             arr[j] = _BatchItem(scene.handle, cw._params(), q.data_ptr(), P, int(it.get("seed", 0)),
                                 g("mean").data_ptr(), g("mean64").data_ptr(), g("var64").data_ptr(),
-                                g("walk_value").data_ptr(), _p(g("rec_prim")), _p(g("rec_t")), _p(g("basis")))
+                                g("walk_value").data_ptr(), _p(g("rec_prim")), _p(g("rec_t")), _p(g("basis")),
+                                _p(g("grad_out")), _p(g("grad_values")), _p(g("grad_coeffs")))
         h = _vp()
         torch.cuda.synchronize()
         _check(lib.wos_batch_create(arr, len(items), ctypes.byref(h)))
@@ -671,6 +720,20 @@ class Batch:
 
     def run(self):
         _check(lib.wos_batch_run(self._h, _stream()))
+        return self.outputs
+
+    def zero_grad(self):
+        """Clears every item's grad_values / grad_coeffs (fill kernels, so it
+        can sit in a captured graph)."""
+        for o in self.outputs:
+            for k in ("grad_values", "grad_coeffs"):
+                if k in o:
+                    o[k].zero_()
+
+    def adjoint(self):
+        """Reverse pass of the last run: adds each item's dL/d values and dL/d
+        coefficients, from its ``grad_out``, into its gradient buffers."""
+        _check(lib.wos_batch_adjoint(self._h, _stream()))
         return self.outputs
 
     def autotune(self, key=None, reps=2):

@@ -23,8 +23,11 @@ typedef struct wos_scene wos_scene;
 /* WoS(nwalks, nsteps=32, epsilon=1e-3, double_sided=False,
  *     prevent_fd_artifacts=False, use_IS_for_greens=True)
  * -- src/python/wos.cu:9-12, fields include/wos.cuh:147-152.
- * use_IS_for_greens is accepted and ignored, as in the CUDA walk
- * (include/wos.cuh:121-134 is compiled out). */
+ * use_IS_for_greens selects how a walk with a source samples the interior
+ * term: the radius from the tabulated CDF of r log(R/r) with the draw reused
+ * inside its bin and pdf = pmf N / R / (2 pi) (include/wos.cuh:49-88, the
+ * branch the reference's source compiles out at include/wos.cuh:121-134), or
+ * uniformly in the ball (0; what the reference's CUDA walk executes). */
 typedef struct {
     int   nwalks;
     int   nsteps;
@@ -185,6 +188,16 @@ int wos_adjoint_values(const wos_scene *scene, int n_points, int nwalks, const i
                        const float *basis, const float *grad_out, float *grad_values, float *grad_coeffs,
                        void *stream);
 
+/* Reverse mode of walk_reduce with respect to every source parameter -- what
+ * Dr.Jit's AD takes through Scene.source_function (include/scene.cuh:278-301,
+ * wos/scene.py:359-381) for the Gaussian (5), sinusoid (5) and Fourier (4K:
+ * ax, ay, phase, coeff per term) sources.  Each walk is re-run from its key
+ * (the streams of a walk_reduce call with the same seed and settings), so
+ * nothing is stored.  grad_out: dL/d mean (n_points).  Adds into grad_source
+ * (n_source_params floats); a scene without a source adds nothing. */
+int wos_adjoint_source(const wos_scene *scene, const wos_walk_params *params, const float *points, int n_points,
+                       int seed, const float *grad_out, float *grad_source, void *stream);
+
 /* Shape gradient: CustomSolver backward (wos/solver.py:109-114) with
  * bwd_solver = OursCUDA (wos/wos_grad.py:130-151) in 2D or Ours3D
  * (wos/wos_grad_3d.py:117-137) in 3D, walks recomputed from (seed, point,
@@ -213,8 +226,13 @@ int wos_shape_derivative(const wos_scene *scene, const wos_grad_params *params, 
  * points, seed) items in one persistent launch per dimension.  Each item's
  * walks use exactly the streams of a single wos_walk_reduce call with the
  * same seed.  wos_batch_create uploads the item table and allocates every
- * buffer a run needs (synchronous); wos_batch_run allocates nothing and can
- * be captured into a graph; runs of one batch on concurrent streams race. */
+ * buffer a run needs (synchronous); wos_batch_run and wos_batch_adjoint
+ * allocate nothing and can be captured into one graph; runs of one batch on
+ * concurrent streams race.  The item table holds the scenes' device buffers:
+ * after wos_scene_set_source, wos_scene_set_grid, wos_scene_set_start_grid
+ * or a change of wos_scene_set_use_bvh on an item's scene, run and adjoint
+ * fail with an error until the batch is created again (new vertex values,
+ * wos_scene_set_values, are read by the next run). */
 typedef struct wos_batch wos_batch;
 typedef struct {
     const wos_scene *scene;
@@ -226,13 +244,30 @@ typedef struct {
     double          *mean64;      /* nullable */
     double          *var64;       /* nullable */
     float           *walk_value;  /* required: n_points * nwalks */
-    int             *rec_prim;    /* nullable */
-    float           *rec_t;       /* nullable */
+    int             *rec_prim;    /* nullable; required by the reverse pass */
+    float           *rec_t;       /* nullable; required by the reverse pass */
     float           *basis;       /* nullable: Fourier sources only */
+    /* reverse pass (wos_batch_adjoint), all nullable: dL/d mean per point,
+     * read when the pass runs; dL/d vertex values (n_vertices) and dL/d
+     * Fourier coefficients (K), added into (the caller zeroes them) */
+    const float     *grad_out;
+    float           *grad_values;
+    float           *grad_coeffs;
 } wos_batch_item;
 int wos_batch_create(const wos_batch_item *items, int n_items, wos_batch **out);
+/* Primal pass: walk_reduce of every item (wos/solver.py:49-65). */
 int wos_batch_run(wos_batch *batch, void *stream);
+/* Reverse pass of every item with grad_values or grad_coeffs, from the exit
+ * records of the last run: wos_adjoint_values per item (the transpose of
+ * include/scene.cuh:251-264 and of the contrib sum, include/wos.cuh:136-138). */
+int wos_batch_adjoint(wos_batch *batch, void *stream);
 int wos_batch_destroy(wos_batch *batch);
+
+/* ---- testing hook -----------------------------------------------------------
+ * No counterpart in the reference: the next n requests for a launch's
+ * traversal spill fail (without allocating), so tests can check that every
+ * entry point returns an error instead of launching. */
+int wos_debug_fail_workspace(int n);
 
 #ifdef __cplusplus
 }

@@ -41,7 +41,7 @@ class _Scene(ctypes.Structure):
 
 class _WalkParams(ctypes.Structure):
     _fields_ = [("nwalks", _i), ("nsteps", _i), ("epsilon", _f), ("double_sided", _i),
-                ("prevent_fd_artifacts", _i)]
+                ("prevent_fd_artifacts", _i), ("use_IS_for_greens", _i)]
 
 
 class _GradParams(ctypes.Structure):
@@ -72,6 +72,7 @@ lib.or_scene_init.argtypes = [ctypes.POINTER(_Scene)]
 lib.or_batch_closest.argtypes = [ctypes.POINTER(_Scene), _vp, _i, _vp, _vp]
 lib.or_batch_sdf.argtypes = [ctypes.POINTER(_Scene), _vp, _i, _vp]
 lib.or_batch_closest_point.argtypes = [ctypes.POINTER(_Scene), _vp, _i, _vp, _vp, _vp, _vp, _vp]
+lib.or_batch_dirichlet.argtypes = [ctypes.POINTER(_Scene), _vp, _vp, _vp, _i, _vp]
 lib.or_batch_lib.argtypes = [ctypes.POINTER(_Scene), _vp, _vp, _vp, _i, _f, _vp]
 lib.or_batch_single_walk.argtypes = [ctypes.POINTER(_Scene), ctypes.POINTER(_WalkParams), _vp, _i, _vp, _vp,
                                      _vp, _vp, _vp, _vp, _vp, _vp]
@@ -82,6 +83,12 @@ lib.or_shape_gradient.argtypes = [ctypes.POINTER(_Scene), ctypes.POINTER(_GradPa
                                   _vp, _i, _vp, _vp]
 lib.or_lbvh_build.argtypes = [ctypes.POINTER(_Scene), _vp, _vp, _vp, _vp, _vp, ctypes.POINTER(_i)]
 lib.or_lbvh_build.restype = _i
+lib.or_green_cdf.restype = ctypes.POINTER(_f)
+lib.or_green_cdf.argtypes = [ctypes.POINTER(_i)]
+lib.or_sample_green_radius.restype = _f
+lib.or_sample_green_radius.argtypes = [_f, _f, ctypes.POINTER(_f), ctypes.POINTER(_i)]
+lib.or_interior_sample.restype = _i
+lib.or_interior_sample.argtypes = [_f, _f, _f, _i] + [ctypes.POINTER(_f)] * 4
 lib.or_morton.restype = ctypes.c_uint32
 lib.or_morton.argtypes = [_i, _vp]
 
@@ -127,6 +134,40 @@ def elementary(fn, x):
     out = np.empty_like(x)
     lib.or_batch_elementary(code, _p(x), x.size, _p(out))
     return out
+
+
+def green_cdf():
+    """The tabulated CDF of r log(1/r) (float32 array)."""
+    n = _i()
+    ptr = lib.or_green_cdf(ctypes.byref(n))
+    return np.ctypeslib.as_array(ptr, shape=(n.value,)).copy()
+
+
+def sample_green_radius(u, R):
+    """(radius, radial pdf, bin) of include/wos.cuh:49-75 for each u."""
+    u = np.ascontiguousarray(u, dtype=np.float32).reshape(-1)
+    rad = np.empty_like(u)
+    pdf = np.empty_like(u)
+    b = np.empty(u.shape, np.int32)
+    pf, bi = _f(), _i()
+    for j, x in enumerate(u):
+        rad[j] = lib.or_sample_green_radius(float(x), float(R), ctypes.byref(pf), ctypes.byref(bi))
+        pdf[j], b[j] = pf.value, bi.value
+    return rad, pdf, b
+
+
+def interior_sample(u0, u1, d, use_is):
+    """(dx, dy, G, w, take) of one interior sample per (u0, u1)."""
+    u0 = np.ascontiguousarray(u0, dtype=np.float32).reshape(-1)
+    u1 = np.ascontiguousarray(u1, dtype=np.float32).reshape(-1)
+    out = np.empty((u0.size, 4), np.float32)
+    take = np.empty(u0.size, bool)
+    v = [_f() for _ in range(4)]
+    for j in range(u0.size):
+        take[j] = bool(lib.or_interior_sample(float(u0[j]), float(u1[j]), float(d), int(bool(use_is)),
+                                              *[ctypes.byref(x) for x in v]))
+        out[j] = [x.value for x in v]
+    return out[:, 0], out[:, 1], out[:, 2], out[:, 3], take
 
 
 def morton(dim, unit):
@@ -187,6 +228,15 @@ class Scene:
                                    _p(r["t"]), _p(r["prim"]))
         return r
 
+    def dirichlet(self, prim, t, valid=None):
+        """Scene.dirichlet over records (include/scene.cuh:251-264)."""
+        prim = np.ascontiguousarray(prim, dtype=np.int32).reshape(-1)
+        t = np.ascontiguousarray(t, dtype=np.float32)
+        v = None if valid is None else np.ascontiguousarray(valid, dtype=np.int32)
+        out = np.empty(prim.size, np.float32)
+        lib.or_batch_dirichlet(ctypes.byref(self.s), _p(v), _p(prim), _p(t), prim.size, _p(out))
+        return out
+
     def largest_inscribed_ball(self, valid, p, n, eps=1e-3):
         p = self._pts(p)
         nn = self._pts(n)
@@ -195,12 +245,13 @@ class Scene:
         lib.or_batch_lib(ctypes.byref(self.s), _p(v), _p(p), _p(nn), p.shape[0], float(eps), _p(out))
         return out
 
-    def single_walk(self, p, state, inc, nsteps=32, epsilon=1e-3, double_sided=False, prevent_fd_artifacts=False):
+    def single_walk(self, p, state, inc, nsteps=32, epsilon=1e-3, double_sided=False, prevent_fd_artifacts=False,
+                    use_is=False):
         p = self._pts(p)
         n, D = p.shape
         st = np.ascontiguousarray(state, dtype=np.uint64).copy()
         ic = np.ascontiguousarray(inc, dtype=np.uint64).copy()
-        w = _WalkParams(1, nsteps, epsilon, int(double_sided), int(prevent_fd_artifacts))
+        w = _WalkParams(1, nsteps, epsilon, int(double_sided), int(prevent_fd_artifacts), int(bool(use_is)))
         r = dict(valid=np.empty(n, np.int32), p=np.empty((n, D), np.float32), n=np.empty((n, D), np.float32),
                  t=np.empty(n if D == 2 else (n, 2), np.float32), prim=np.empty(n, np.int32),
                  contrib=np.empty(n, np.float32))
@@ -210,10 +261,12 @@ class Scene:
         return r
 
     def walk_reduce(self, points, nwalks, nsteps=32, epsilon=1e-3, seed=0, double_sided=False,
-                    prevent_fd_artifacts=False, threads=0, walk_values=False, records=False, basis=False):
+                    prevent_fd_artifacts=False, threads=0, walk_values=False, records=False, basis=False,
+                    use_is=False):
         p = self._pts(points)
         P = p.shape[0]
-        w = _WalkParams(int(nwalks), int(nsteps), float(epsilon), int(double_sided), int(prevent_fd_artifacts))
+        w = _WalkParams(int(nwalks), int(nsteps), float(epsilon), int(double_sided), int(prevent_fd_artifacts),
+                        int(bool(use_is)))
         out = dict(mean=np.empty(P, np.float64), var=np.empty(P, np.float64))
         wv = np.empty(P * nwalks, np.float32) if walk_values else None
         rp = np.empty(P * nwalks, np.int32) if records else None

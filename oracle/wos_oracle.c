@@ -489,6 +489,77 @@ float or_source(const or_scene *s, float x, float y) {
     }
 }
 
+/* Tabulated CDF of the radial density r log(1/r) on [0, 1]
+ * (include/tabulated_G_cdf.cuh:5-7): entry i = F((i + 1) / N) with the
+ * closed form F(x) = x^2 (1 - 2 ln x) = (integral_0^x r ln(1/r) dr) /
+ * (integral_0^1 r ln(1/r) dr), evaluated in double, rounded to float. */
+#define OR_GREEN_N 10000
+static float g_green_cdf[OR_GREEN_N];
+static pthread_once_t g_green_once = PTHREAD_ONCE_INIT;
+static void green_init(void) {
+    for (int i = 0; i < OR_GREEN_N; ++i) {
+        double x = (double)(i + 1) / (double)OR_GREEN_N;
+        g_green_cdf[i] = (float)(x * x * (1.0 - 2.0 * log(x)));
+    }
+}
+const float *or_green_cdf(int *n) {
+    pthread_once(&g_green_once, green_init);
+    if (n) *n = OR_GREEN_N;
+    return g_green_cdf;
+}
+
+/* WoSDevice::sample_G_radius (include/wos.cuh:49-75): binary search for the
+ * smallest bin l with u < cdf[l] (clamped to the last bin), the draw reused
+ * inside the bin, radius R (l + u') / N, radial pdf pmf N / R. */
+float or_sample_green_radius(float u, float R, float *pdf, int *bin) {
+    const float *cdf = or_green_cdf(NULL);
+    int l = 0, r = OR_GREEN_N - 1;
+    while (l < r) {
+        int mid = (l + r) / 2;
+        if (u < cdf[mid]) r = mid; else l = mid + 1;
+    }
+    float lo = l > 0 ? cdf[l - 1] : 0.0f;
+    float pmf = cdf[l] - lo;
+    float ur = (u - lo) / pmf;
+    if (bin) *bin = l;
+    *pdf = pmf * (float)OR_GREEN_N / R;
+    return R * ((float)l + ur) / (float)OR_GREEN_N;
+}
+
+/* One interior sample of the disk of radius d (include/wos.cuh:115-139):
+ * offset (dx, dy), G of the disk there (include/wos.cuh:44-47) and weight w
+ * = Jacobian / pdf, so the step adds f(p + offset) G w.  Uniform in the disk
+ * (include/wos.cuh:28-42), or, with is, by sample_G_2D (include/wos.cuh:
+ * 77-88: radius from the table, angle 2 pi u1, pdf / 2 pi, Jacobian |offset|).
+ * Returns 0 where the reference adds nothing (pdf <= 1e-5). */
+int or_interior_sample(float u0, float u1, float d, int is, float *dx, float *dy, float *G, float *w) {
+    float pdf;
+    if (is) {
+        float pr;
+        float rad = or_sample_green_radius(u0, d, &pr, NULL);
+        pdf = pr / (2.0f * OR_PI_F);
+        float sn, cs;
+        or_sincos_turns(u1, &sn, &cs);
+        *dx = rad * cs;
+        *dy = rad * sn;
+        float r = sqrtf(dot2(*dx, *dy, *dx, *dy));
+        *G = or_logf(d / fmaxf(r, 1e-4f)) * OR_INV2PI_F;
+        *w = r / pdf;
+    } else {
+        float sn, cs;
+        or_sincos_turns(u0, &sn, &cs);
+        float r = sqrtf(u1);
+        *dx = (r * cs) * d;
+        *dy = (r * sn) * d;
+        float area = OR_PI_F * (d * d);
+        pdf = 1.0f / area;
+        float rr = fmaxf(sqrtf(dot2(*dx, *dy, *dx, *dy)), 1e-4f);
+        *G = or_logf(d / rr) * OR_INV2PI_F;
+        *w = area;
+    }
+    return pdf > 1e-5f;
+}
+
 /* include/wos.cuh:97-145 */
 void or_walk(const or_scene *s, const or_walk_params *w, const float *p0, or_pcg32 *rng,
              or_record *rec, float *basis) {
@@ -512,24 +583,17 @@ void or_walk(const or_scene *s, const or_walk_params *w, const float *p0, or_pcg
             return;
         }
         if (has_source) {
-            /* include/wos.cuh:115-139: uniform point in the ball, G_2D / pdf */
+            /* include/wos.cuh:115-139: one interior sample, f G w */
             float u0 = or_pcg32_next_1d(rng), u1 = or_pcg32_next_1d(rng);
-            float sn, cs;
-            or_sincos_turns(u0, &sn, &cs);
-            float r = sqrtf(u1);
-            float dyx = (r * cs) * d, dyy = (r * sn) * d;
-            float area = OR_PI_F * (d * d);
-            float pdf = 1.0f / area;
-            float rr = fmaxf(sqrtf(dot2(dyx, dyy, dyx, dyy)), 1e-4f);
-            float G = or_logf(d / rr) * OR_INV2PI_F;
-            if (pdf > 1e-5f) {
+            float dyx, dyy, G, wt;
+            if (or_interior_sample(u0, u1, d, w->use_IS_for_greens, &dyx, &dyy, &G, &wt)) {
                 float yx = p[0] + dyx, yy = p[1] + dyy;
-                contrib = fmaf(or_source(s, yx, yy) * G, area, contrib);
+                contrib = fmaf(or_source(s, yx, yy) * G, wt, contrib);
                 if (basis) {
                     const float *q = s->source_params;
                     for (int k = 0; k < K; ++k) {
                         float phi = fmaf(q[4 * k], yx, fmaf(q[4 * k + 1], yy, q[4 * k + 2]));
-                        basis[k] = fmaf(or_sinf(phi) * G, area, basis[k]);
+                        basis[k] = fmaf(or_sinf(phi) * G, wt, basis[k]);
                     }
                 }
             }
@@ -697,8 +761,8 @@ static float secondary_u(const or_scene *s, const or_walk_params *w2, const floa
 static int shape_sample(const or_scene *s, const or_grad_params *g, const float *p,
                         or_pcg32 *rng, or_record *rec, float *grad_u) {
     int D = s->dim;
-    or_walk_params w1 = {g->nwalks, g->nsteps, g->epsilon, g->double_sided, 0};
-    or_walk_params w2 = {g->nwalks, g->nsteps, g->epsilon2, g->double_sided, 0};
+    or_walk_params w1 = {g->nwalks, g->nsteps, g->epsilon, g->double_sided, 0, 0};
+    or_walk_params w2 = {g->nwalks, g->nsteps, g->epsilon2, g->double_sided, 0, 0};
     float d0 = or_sdf(s, p);
     int inside = g->double_sided ? 1 : (d0 < 0.0f);
     or_walk(s, &w1, p, rng, rec, NULL);
@@ -1065,6 +1129,18 @@ void or_batch_lib(const or_scene *s, const int *valid, const float *p, const flo
         r.valid = valid ? valid[i] : 1;
         for (int k = 0; k < D; ++k) { r.p[k] = p[(size_t)i * D + k]; r.n[k] = nrm[(size_t)i * D + k]; }
         out[i] = or_largest_inscribed_ball(s, &r, eps);
+    }
+}
+
+void or_batch_dirichlet(const or_scene *s, const int *valid, const int *prim, const float *t, int n, float *out) {
+    for (int i = 0; i < n; ++i) {
+        or_record r;
+        memset(&r, 0, sizeof(r));
+        r.valid = valid ? valid[i] : 1;
+        r.prim_id = prim[i];
+        if (s->dim == 2) r.t = t[i];
+        else { r.uv[0] = t[2 * (size_t)i]; r.uv[1] = t[2 * (size_t)i + 1]; }
+        out[i] = (r.prim_id >= 0 && r.prim_id < s->n_prims) ? or_dirichlet(s, &r) : 0.0f;
     }
 }
 

@@ -68,7 +68,17 @@ typedef struct {
     int   nwalks, nsteps;
     float epsilon;
     int   double_sided, prevent_fd_artifacts;
+    int   use_IS_for_greens;   /* interior samples from the tabulated Green's-function CDF */
 } or_walk_params;
+
+/* Tabulated CDF of r log(1/r) on [0, 1] (include/tabulated_G_cdf.cuh:5-7,
+ * as its closed form): *n entries. */
+const float *or_green_cdf(int *n);
+/* include/wos.cuh:49-75: radius in [0, R] and its radial pdf; bin (nullable) */
+float or_sample_green_radius(float u, float R, float *pdf, int *bin);
+/* include/wos.cuh:115-139 (uniform) / 77-88 (is): offset, G and weight of one
+ * interior sample of the disk of radius d; 0 when the reference adds nothing */
+int   or_interior_sample(float u0, float u1, float d, int is, float *dx, float *dy, float *G, float *w);
 
 /* include/scene.cuh:368-380 (2D vertex normals) and 417-435 (AABB) */
 void  or_scene_init(or_scene *s);
@@ -131,6 +141,9 @@ void  or_batch_sdf(const or_scene *s, const float *p, int n, float *out);
 /* rec arrays: valid n, p n*dim, nrm n*dim, t n (2D) / 2n (3D), prim n */
 void  or_batch_closest_point(const or_scene *s, const float *p, int n, int *valid, float *rp, float *rn,
                              float *rt, int *prim);
+/* Scene.dirichlet over records (valid nullable; t: n (2D) or 2n (3D));
+ * 0 for invalid records and primitives out of range */
+void  or_batch_dirichlet(const or_scene *s, const int *valid, const int *prim, const float *t, int n, float *out);
 void  or_batch_lib(const or_scene *s, const int *valid, const float *p, const float *nrm, int n, float eps,
                    float *out);
 /* single walks from explicit PCG32 states (WoS::singleWalk semantics):

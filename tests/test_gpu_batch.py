@@ -85,3 +85,96 @@ def test_autotune_keeps_results_and_caches(tmp_path, monkeypatch):
             [t[k] for k in ("chunk", "wgs_per_cu", "top_levels", "window")]
     finally:
         wos_ext.set_tuning(*before)
+
+
+def _with_grads(items):
+    for it in items:
+        it["grad_values"] = True
+        it["grad_coeffs"] = it["basis"]
+    return items
+
+
+def test_batch_adjoint_matches_single_scene_adjoint():
+    """The batched reverse pass gives each item the vertex-value and
+    coefficient gradients wos_adjoint_values computes from the same records
+    (up to the order of float atomics)."""
+    import wos_ext
+    items = _with_grads(_build(0.01))
+    b = wos_ext.Batch(items)
+    b.run()
+    rng = np.random.default_rng(2)
+    gs = []
+    for o in b.outputs:
+        g = _t(rng.normal(size=o["mean"].shape[0]).astype(np.float32))
+        o["grad_out"].copy_(g)
+        gs.append(g)
+    b.zero_grad()
+    b.adjoint()
+    torch.cuda.synchronize()
+    for it, o, g in zip(items, b.outputs, gs):
+        P, W = o["mean"].shape[0], it["wos"].nwalks
+        gv, gc = wos_ext.adjoint_values(it["scene"], P, W, o["rec_prim"], o["rec_t"], g,
+                                        basis=o.get("basis"), want_values=True, want_coeffs=it["basis"])
+        scale = float(gv.abs().max()) + 1e-12
+        assert float((o["grad_values"] - gv).abs().max()) <= 1e-5 * scale
+        if it["basis"]:
+            np.testing.assert_allclose(o["grad_coeffs"].cpu().numpy(), gc.cpu().numpy(), rtol=1e-4, atol=1e-6)
+
+
+def test_batch_primal_loss_adjoint_in_one_graph():
+    """run + L2-loss gradient + zero_grad + adjoint captured together: every
+    replay reproduces the eager step."""
+    import wos_ext
+    items = _with_grads(_build(0.005))
+    b = wos_ext.Batch(items)
+    targets = [torch.zeros_like(o["mean"]) + 0.1 for o in b.outputs]
+
+    def step():
+        b.run()
+        for o, tg in zip(b.outputs, targets):
+            o["grad_out"].copy_((2.0 / tg.numel()) * (o["mean"] - tg))
+        b.zero_grad()
+        b.adjoint()
+
+    step()
+    torch.cuda.synchronize()
+    eager = [(o["mean"].clone(), o["grad_values"].clone()) for o in b.outputs]
+    s = torch.cuda.Stream()
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g, stream=s):
+        step()
+    for rep in range(3):
+        for o in b.outputs:
+            o["mean"].fill_(-1.0)
+            o["grad_values"].fill_(7.0)
+        torch.cuda.synchronize()
+        g.replay()
+        torch.cuda.synchronize()
+        for (m, gv), o in zip(eager, b.outputs):
+            assert torch.equal(m, o["mean"]), rep
+            scale = float(gv.abs().max()) + 1e-12
+            assert float((gv - o["grad_values"]).abs().max()) <= 1e-5 * scale, rep
+
+
+def test_batch_refuses_a_changed_scene():
+    """After an item's scene changes its source (the item table holds the
+    old buffer), run and adjoint raise until the batch is created again;
+    new vertex values are simply read by the next run."""
+    import wos_ext
+    items = _with_grads(_build(0.005))
+    b = wos_ext.Batch(items)
+    b.run()
+    j = next(i for i, it in enumerate(items) if it["basis"])
+    sc = items[j]["scene"]
+    sc.set_values(sc.c_values * 2.0)
+    b.run()   # values are read in place
+    sp = sc.c_source_params.clone()
+    sc.set_source(3, torch.cat([sp, sp]))   # more terms: a new buffer
+    with pytest.raises(wos_ext.WoSError, match="changed"):
+        b.run()
+    with pytest.raises(wos_ext.WoSError, match="changed"):
+        b.adjoint()
+    items[j]["scene"] = sc
+    b2 = wos_ext.Batch(items)
+    b2.run()
+    torch.cuda.synchronize()

@@ -147,8 +147,30 @@ def test_triangle_closest_point_against_dense_sampling():
     u, w = r["t"][:, 0], r["t"][:, 1]
     q = tri[1] * u[:, None] + tri[2] * w[:, None] + tri[0] * (1 - u - w)[:, None]
     np.testing.assert_allclose(q, r["p"], atol=1e-6)
-    # Dirichlet value of the record is the same interpolation of the values
-    assert np.all(np.isfinite(u))
+    # Dirichlet value of the record is the same interpolation of the values:
+    # for vertex values of a linear function the interpolant is that function
+    # itself, so the value at the record equals the function at the record point
+    lin = lambda x: 1.0 + 2.0 * x[:, 0] - 3.0 * x[:, 1] + 0.5 * x[:, 2]
+    sl = O.Scene(tri, np.array([[0, 1, 2]], np.int32), lin(tri.astype(np.float64)).astype(np.float32))
+    val = sl.dirichlet(r["prim"], r["t"], r["valid"])
+    np.testing.assert_allclose(val, lin(r["p"].astype(np.float64)), atol=2e-6)
+    # a wrong weight order (uv as the weights of (a, b)) would fail the same check
+    swapped = sl.dirichlet(r["prim"], r["t"][:, ::-1].copy(), r["valid"])
+    assert np.abs(swapped - lin(r["p"].astype(np.float64))).max() > 1e-2
+    # invalid records and out-of-range primitives give 0
+    assert (sl.dirichlet(np.array([0, -1, 5]), np.zeros((3, 2)), np.array([0, 1, 1])) == 0).all()
+
+
+def test_dirichlet_2d_linear_values_exact():
+    """2D records: t is the segment parameter, value = lerp of the end values,
+    which for linear vertex data equals the function at the record point."""
+    v, ix = C.circle_polyline(64)
+    lin = lambda x: 0.3 - 1.5 * x[:, 0] + 0.25 * x[:, 1]
+    s = O.Scene(v, ix, lin(v.astype(np.float64)).astype(np.float32))
+    p = np.random.default_rng(4).uniform(-0.9, 0.9, size=(500, 2)).astype(np.float32)
+    r = s.closest_point(p)
+    val = s.dirichlet(r["prim"], r["t"], r["valid"])
+    np.testing.assert_allclose(val, lin(r["p"].astype(np.float64)), atol=2e-6)
 
 
 def test_largest_inscribed_ball_circle_closed_form():
