@@ -1,22 +1,31 @@
 """Benchmark: primal + gradient walks/s of the 2D Poisson clover workload.
 
-Workload (one step): 126,820 query points on a 512x512 grid inside the clover
-r = 1 + 0.3 cos 4t (4,096 segments), 128 walks per point (16.2M walks),
-eps = 1e-4, 256 steps, source sin3x cos2y as 8 Fourier terms, Dirichlet data
-g = x^2 - y^2.  A step solves u, forms L = mean((u - u_target)^2) against a
-fixed target solved at 1.1 g, and back-propagates to the 4,096 vertex values
-and 8 source coefficients.  Inputs are resident in HBM before timing.
+Workload (one step, the default): 126,820 query points on a 512x512 grid
+inside the clover r = 1 + 0.3 cos 4t (4,096 segments), 128 walks per point
+(16.2M walks), eps = 1e-4, 256 steps, source sin3x cos2y as 8 Fourier terms
+sampled uniformly in each ball (the interior sampling the reference's CUDA
+walk executes), Dirichlet data g = x^2 - y^2.  A step solves u, forms L =
+mean((u - u_target)^2) against a fixed target solved at 1.1 g, and
+back-propagates to the 4,096 vertex values and 8 source coefficients.
+Inputs are resident in HBM before timing.
 
 Also reported: the same step through the reference-style path ("plain": one
 lane per walk, brute-force closest point, per-walk records, walks re-run in
-the backward pass), RMSE of both against the exact solution of the
-manufactured twin, the roofline of the dominant kernel and the CPU baseline.
+the backward pass) with the RMSE of both paths on the same points, the
+importance-sampled variant, the roofline of the dominant kernel and the CPU
+baseline (the oracle with its CPU LBVH on the box's 16-core share).
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-plain] [--no-cpu] [--no-counts]
+Other workloads (--workload): circle / sphere (primal Laplace, 1,024-segment
+circle and 81,920-triangle displaced icosphere with 1M points), shape2d /
+shape3d (shape gradients), batch (64 uneven scenes, primal + adjoint in one
+captured graph, against the plain path at equal RMSE).
 
-The HBM traffic of the walk kernel comes from the FETCH_SIZE / WRITE_SIZE
-counter runs summarised in profiles/walk_clover_hbm.md (rocprofv3 cannot run
-inside this process); it is null when that file is absent.
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload NAME] [--no-plain] [--no-cpu]
+       [--no-counts]
+
+The HBM traffic of a walk kernel comes from the FETCH_SIZE / WRITE_SIZE
+counter runs summarised under profiles/ (rocprofv3 cannot run inside this
+process); it is null when the summary is absent.
 """
 from __future__ import annotations
 
@@ -33,20 +42,20 @@ sys.path.insert(0, os.path.join(ROOT, "diff-wos_amd"))
 
 PEAK_FP32 = 157.3e12      # MI355X FP32 vector, FLOP/s
 PEAK_HBM = 8.0e12         # bytes/s
+CPU_THREADS = 16          # a one-GPU job's share of the host's cores
 
-# algorithmic flop model (FMA = 2): per AABB test, per segment test, per walk
-# step (direction, move, bound), per source sample and Fourier term, per walk
-C_BOX_2D, C_PRIM_2D = 11, 20
+# algorithmic flop model (FMA = 2): per bound test, per primitive test, per
+# walk step (direction, move, bound), per source sample and Fourier term,
+# per walk.  2D: segment 20, box 11; 3D: triangle 85, box 17.
+C_BOX = {2: 11, 3: 17}
+C_PRIM = {2: 20, 3: 85}
 C_STEP, C_SRC, C_TERM, C_WALK = 30, 25, 34, 15
 
 
-TRAFFIC_FILE = os.path.join(ROOT, "profiles", "walk_clover_hbm.md")
-
-
-def measured_traffic():
+def measured_traffic(name):
     """HBM bytes per walk launch recorded by the counter runs, or None."""
     try:
-        with open(TRAFFIC_FILE) as f:
+        with open(os.path.join(ROOT, "profiles", name)) as f:
             for line in f:
                 if line.startswith("hbm_bytes_per_launch:"):
                     return float(line.split(":", 1)[1])
@@ -59,29 +68,63 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def build_inputs(torch, C):
-    c = C.config2()
-    dv = torch.device("cuda", 0)
-    t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dv)
-    return c, dict(vertices=t(c["vertices"]), indices=t(c["indices"]), values=t(c["values"]),
-                   points=t(c["points"]), source=t(c["source_params"]))
+def emit(line):
+    print(json.dumps(line), flush=True)
 
 
-def fused_step(torch, wos_ext, scene, cwos, pts, target, seed):
-    out = cwos.walk_reduce(pts, scene, seed=seed, records=True, basis=True)
-    u = out["mean"]
-    P = u.shape[0]
-    g = (2.0 / P) * (u - target)
-    loss = torch.mean((u - target) ** 2)
-    gv, gc = wos_ext.adjoint_values(scene, P, cwos.nwalks, out["rec_prim"], out["rec_t"], g, basis=out["basis"],
-                                    want_values=True, want_coeffs=True)
-    return u, loss, gv, gc
+def synced_ms(torch, fn, steps):
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(steps):
+        fn(i)
+    torch.cuda.synchronize()
+    return (time.perf_counter() - t0) * 1e3 / steps
 
 
-def plain_step(torch, wos_ext, scene_bf, cwos, pts, target, seed, W):
+def roofline(flops, kernel_ms, kernel, traffic, extra=None):
+    s = kernel_ms * 1e-3
+    achieved = flops / s
+    r = {"bound": "valu", "achieved": round(achieved / 1e12, 3), "peak": PEAK_FP32 / 1e12, "unit": "TFLOP/s",
+         "frac": round(achieved / PEAK_FP32, 4), "traffic": traffic, "kernel": kernel,
+         "flops_per_launch": float(flops), "launch_ms": round(kernel_ms, 3)}
+    if traffic:
+        r["hbm_GBs"] = round(traffic / s / 1e9, 1)
+        r["hbm_frac"] = round(traffic / s / PEAK_HBM, 4)
+    if extra:
+        r.update(extra)
+    return r
+
+
+def walk_flops(counts, dim, walks, K=0, src=False):
+    per_step = C_STEP + ((C_SRC + C_TERM * K) if src else 0)
+    return (counts["box_tests"] * C_BOX[dim] + counts["prim_tests"] * C_PRIM[dim] + counts["steps"] * per_step
+            + walks * C_WALK)
+
+
+def cpu_leg(run_chunk, n_points, W, seconds, what):
+    """Times run_chunk(index array) over an evenly strided sample of the
+    points, growing the sample until about `seconds` of CPU work."""
+    done, tc = 0, 0.0
+    stride = max(1, n_points // 4096)
+    order = np.concatenate([np.arange(o, n_points, stride) for o in range(stride)])
+    blk = 64
+    while tc < seconds and done < n_points:
+        idx = order[done: done + blk]
+        ta = time.perf_counter()
+        run_chunk(idx)
+        tc += time.perf_counter() - ta
+        done += idx.size
+        blk = min(blk * 2, 4096)
+    return {"value": done * W / tc, "unit": "walks/s", "cores": CPU_THREADS, "kind": "port",
+            "sample": f"{done} of {n_points} points, evenly strided (stride {stride}) x {W} walks, {what}, "
+                      f"oracle with its CPU LBVH, {tc:.1f} s"}
+
+
+def plain_step(torch, scene_bf, cwos, pts, target, seed, W):
     """Reference data flow: PCG32 per walk (Dr.Jit PCG32 + TEA), one lane per
     walk with the brute-force scan, records to HBM, dirichlet + scatter mean;
-    backward re-runs the walks from the same seed and scatters g(1-t), g t."""
+    backward re-runs the walks from the same seed and scatters the
+    interpolation weights (2D: 1 - t, t; 3D: 1 - u - v, u, v)."""
     from wos.fwd import PCG32
     from wos.utils import sample_tea_32
     P = pts.shape[0]
@@ -101,13 +144,130 @@ def plain_step(torch, wos_ext, scene_bf, cwos, pts, target, seed, W):
     gw = torch.where(rec.valid, g[idx] / W, torch.zeros_like(rec.contrib))
     f = scene_bf.c_indices[rec.prim_id.clamp(min=0).long()].long()
     gv = torch.zeros(scene_bf.c_values.numel(), device=pts.device)
-    gv.index_add_(0, f[:, 0], gw * (1 - rec.t)).index_add_(0, f[:, 1], gw * rec.t)
+    if scene_bf.dim == 2:
+        t = rec.t
+        gv.index_add_(0, f[:, 0], gw * (1 - t)).index_add_(0, f[:, 1], gw * t)
+    else:
+        uv = rec.uv
+        gv.index_add_(0, f[:, 0], gw * (1 - uv[:, 0] - uv[:, 1])).index_add_(0, f[:, 1], gw * uv[:, 0])
+        gv.index_add_(0, f[:, 2], gw * uv[:, 1])
     return u, gv
 
 
+# ---------------------------------------------------------------------------
+# default: the 2D Poisson clover, primal + value / coefficient adjoint
+# ---------------------------------------------------------------------------
+def clover_workload(args, torch, wos_ext, C):
+    c = C.config2()
+    dv = torch.device("cuda", 0)
+    t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dv)
+    V, F, G, pts, SP = t(c["vertices"]), t(c["indices"]), t(c["values"]), t(c["points"]), t(c["source_params"])
+    P, W = pts.shape[0], c["nwalks"]
+    log(f"workload: {P} points x {W} walks, 4096 segments, eps 1e-4")
+    scene = wos_ext.Scene(V, F, G, use_bvh=True, source_type=3, source_params=SP)
+    cwos = wos_ext.WoS(nwalks=W, nsteps=c["nsteps"], epsilon=c["epsilon"], use_IS_for_greens=False)
+    tscene = wos_ext.Scene(V, F, G * 1.1, use_bvh=True, source_type=3, source_params=SP)
+    target = cwos.walk_reduce(pts, tscene, seed=12345)["mean"]
+    stream = torch.cuda.current_stream()
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(2 * max(args.steps, 1))]
+
+    def step(i, walker=cwos, timed=True):
+        if timed:
+            ev[2 * i].record(stream)
+        out = walker.walk_reduce(pts, scene, seed=100 + i, records=True, basis=True)
+        if timed:
+            ev[2 * i + 1].record(stream)
+        u = out["mean"]
+        g = (2.0 / P) * (u - target)
+        return wos_ext.adjoint_values(scene, P, W, out["rec_prim"], out["rec_t"], g, basis=out["basis"],
+                                      want_values=True, want_coeffs=True)
+
+    for i in range(args.warmup):
+        step(i, timed=False)
+    ms_per_step = synced_ms(torch, step, args.steps)
+    kernel_ms = float(np.mean([ev[2 * i].elapsed_time(ev[2 * i + 1]) for i in range(args.steps)]))
+    walks = P * W
+    value = walks / (ms_per_step * 1e-3)
+    log(f"fused: {ms_per_step:.2f} ms/step, {value:.3e} walks/s; walk launch {kernel_ms:.2f} ms")
+
+    rf = None
+    if not args.no_counts:
+        counts = cwos.walk_counts(pts, scene, seed=100)
+        K = c["source_params"].shape[0]
+        flops = walk_flops(counts, 2, walks, K=K, src=True)
+        rf = roofline(flops, kernel_ms, "k_walk_persistent<2,true,8,false>", measured_traffic("walk_clover_hbm.md"),
+                      {"per_query": {"box_tests": counts["box_tests"] / counts["queries"],
+                                     "prim_tests": counts["prim_tests"] / counts["queries"]},
+                       "mean_steps": counts["steps"] / walks})
+
+    # accuracy: the manufactured twin (exact u known), both interior samplers
+    ct = C.config2(twin=True)
+    tw = wos_ext.Scene(V, F, t(ct["values"]), use_bvh=True, source_type=3, source_params=SP)
+    o = cwos.walk_reduce(pts, tw, seed=7, stats=True)
+    m_fused = o["mean64"].cpu().numpy()
+    rmse = float(np.sqrt(np.mean((m_fused - ct["exact"]) ** 2)))
+    cis = wos_ext.WoS(nwalks=W, nsteps=c["nsteps"], epsilon=c["epsilon"], use_IS_for_greens=True)
+    step(0, walker=cis, timed=False)
+    ms_is = synced_ms(torch, lambda i: step(i, walker=cis, timed=False), max(2, args.steps // 2))
+    o_is = cis.walk_reduce(pts, tw, seed=7, stats=True)
+    rmse_is = float(np.sqrt(np.mean((o_is["mean64"].cpu().numpy() - ct["exact"]) ** 2)))
+    is_variant = {"ms_per_step": ms_is, "walks_per_s": walks / (ms_is * 1e-3), "rmse_twin": rmse_is,
+                  "interior_sampling": "tabulated Green's-function CDF"}
+    log(f"importance-sampled interior: {ms_is:.2f} ms/step, rmse {rmse_is:.4f} (uniform {rmse:.4f})")
+
+    plain = None
+    if not args.no_plain:
+        scene_bf = wos_ext.Scene(V, F, G, use_bvh=False, source_type=3, source_params=SP)
+        sub = slice(None, None, 8)   # every 8th grid point keeps the brute-force run short
+        psub, tsub = pts[sub].contiguous(), target[sub].contiguous()
+        plain_step(torch, scene_bf, cwos, psub, tsub, 0, W)
+        torch.cuda.synchronize()
+        ta = time.perf_counter()
+        plain_step(torch, scene_bf, cwos, psub, tsub, 1, W)
+        torch.cuda.synchronize()
+        tp = time.perf_counter() - ta
+        # equal RMSE: both paths on the twin, same points, same walks per point
+        tw_bf = wos_ext.Scene(V, F, t(ct["values"]), use_bvh=False, source_type=3, source_params=SP)
+        u_plain, _ = plain_step(torch, tw_bf, cwos, psub, tsub, 7, W)
+        ex = ct["exact"][sub]
+        rmse_plain = float(np.sqrt(np.mean((u_plain.double().cpu().numpy() - ex) ** 2)))
+        rmse_fused_sub = float(np.sqrt(np.mean((m_fused[sub] - ex) ** 2)))
+        plain = {"walks_per_s": psub.shape[0] * W / tp, "ms_per_step_sample": tp * 1e3,
+                 "sample": f"{psub.shape[0]} points (every 8th) x {W} walks, brute-force scan, walks re-run in "
+                           f"backward",
+                 "speedup_fused_vs_plain": value / (psub.shape[0] * W / tp),
+                 "rmse_twin_plain": rmse_plain, "rmse_twin_fused_same_points": rmse_fused_sub}
+        log(f"plain: {plain['walks_per_s']:.3e} walks/s, rmse {rmse_plain:.4f} vs fused {rmse_fused_sub:.4f}")
+
+    cpu = None
+    if not args.no_cpu:
+        import oracle as O
+        ref = O.Scene(c["vertices"], c["indices"], c["values"], 3, c["source_params"], use_bvh=True)
+        pn = c["points"]
+        gt = target.cpu().numpy().astype(np.float64)
+
+        def chunk(ix):
+            q = np.ascontiguousarray(pn[ix])
+            r = ref.walk_reduce(q, W, c["nsteps"], c["epsilon"], seed=0, threads=CPU_THREADS, records=True)
+            ref.adjoint_values(q.shape[0], W, r["rec_prim"], r["rec_t"], (2.0 / P) * (r["mean"] - gt[ix]))
+
+        cpu = cpu_leg(chunk, P, W, args.cpu_seconds, "primal + value adjoint")
+        log(f"cpu: {cpu['value']:.3e} walks/s on {CPU_THREADS} threads")
+
+    emit({"metric": "primal+grad walks/s at 1 GPU (2D Poisson clover, values + source-coefficient adjoint)",
+          "value": value, "unit": "walks/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+          "ms_per_step": ms_per_step, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+          "dtype": "f32", "data": "synthetic",
+          "config": {"workload": "2d-poisson-clover-4096seg-512grid-128walks-eps1e-4-fourier8",
+                     "points": P, "walks_per_point": W, "segments": 4096, "interior_sampling": "uniform",
+                     "parallelism": "single-gpu"},
+          "rmse_twin": rmse, "roofline": rf, "cpu_baseline": cpu, "plain": plain, "importance_sampled": is_variant})
+
+
+# ---------------------------------------------------------------------------
+# primal Laplace: circle (config 1) and the displaced icosphere (config 3)
+# ---------------------------------------------------------------------------
 def primal_workload(args, torch, wos_ext, C):
-    """Primal-only Laplace workloads: walks/s, closest-point queries/s, RMSE
-    against the exact harmonic solution, LBVH build time."""
     c = C.config1() if args.workload == "circle" else C.config3()
     dim = c["vertices"].shape[1]
     dv = torch.device("cuda", 0)
@@ -115,191 +275,239 @@ def primal_workload(args, torch, wos_ext, C):
     V, F, G, pts = t(c["vertices"]), t(c["indices"]), t(c["values"]), t(c["points"])
     Scene, WoS = (wos_ext.Scene, wos_ext.WoS) if dim == 2 else (wos_ext.Scene3D, wos_ext.WoS3D)
     Scene(V, F, G, use_bvh=True)   # warm the build kernels
-    torch.cuda.synchronize()
-    tb = time.perf_counter()
-    scene = Scene(V, F, G, use_bvh=True)
-    build_ms = (time.perf_counter() - tb) * 1e3
+    builds = []
+    for _ in range(3):
+        torch.cuda.synchronize()
+        tb = time.perf_counter()
+        scene = Scene(V, F, G, use_bvh=True)
+        torch.cuda.synchronize()
+        builds.append((time.perf_counter() - tb) * 1e3)
+    build_ms = float(np.median(builds))
     P, W = pts.shape[0], c["nwalks"]
     cw = WoS(nwalks=W, nsteps=c["nsteps"], epsilon=c["epsilon"])
+    stream = torch.cuda.current_stream()
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(2 * max(args.steps, 1))]
+    outs = {}
+
+    def step(i):
+        ev[2 * i].record(stream)
+        outs["o"] = cw.walk_reduce(pts, scene, seed=100 + i, stats=True)
+        ev[2 * i + 1].record(stream)
+
     for i in range(args.warmup):
         cw.walk_reduce(pts, scene, seed=i)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for i in range(args.steps):
-        out = cw.walk_reduce(pts, scene, seed=100 + i, stats=(i == args.steps - 1))
-    torch.cuda.synchronize()
-    ms = (time.perf_counter() - t0) * 1e3 / args.steps
-    rmse = float(np.sqrt(np.mean((out["mean64"].cpu().numpy() - c["exact"]) ** 2)))
-    # closest-point queries/s (LBVH traversal alone)
+    ms = synced_ms(torch, step, args.steps)
+    kernel_ms = float(np.mean([ev[2 * i].elapsed_time(ev[2 * i + 1]) for i in range(args.steps)]))
+    m, var = outs["o"]["mean64"].cpu().numpy(), outs["o"]["var64"].cpu().numpy()
+    rmse = float(np.sqrt(np.mean((m - c["exact"]) ** 2)))
+    mc_bound = float(np.sqrt(np.mean(var / W)))
+    # closest-point queries/s (LBVH traversal alone, unbounded queries from the points)
     scene.closest_point_preliminary(pts)
-    torch.cuda.synchronize()
-    tq = time.perf_counter()
-    for _ in range(3):
-        scene.closest_point_preliminary(pts)
-    torch.cuda.synchronize()
-    qps = 3 * P / (time.perf_counter() - tq)
-    cnt = None if args.no_counts else cw.walk_counts(pts, scene, seed=100, basis=False)
-    log(f"{args.workload}: {ms:.2f} ms/solve, {P * W / ms * 1e3:.3e} walks/s, rmse {rmse:.4f}, "
-        f"build {build_ms:.2f} ms, {qps:.3e} queries/s, counts {cnt}")
-    print(json.dumps({"metric": "walks/s (primal)", "value": P * W / ms * 1e3, "unit": "walks/s", "n_gpus": 1,
-                      "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True,
-                      "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
-                      "config": {"workload": args.workload, "points": P, "walks_per_point": W,
-                                 "primitives": int(c["indices"].shape[0])},
-                      "rmse_exact": rmse, "lbvh_build_ms": build_ms, "closest_point_queries_per_s": qps,
-                      "counts": cnt}))
+    qms = synced_ms(torch, lambda i: scene.closest_point_preliminary(pts), 3)
+    qps = P / (qms * 1e-3)
+    walks = P * W
+    rf = None
+    cnt = None
+    name = "walk_circle_hbm.md" if dim == 2 else "walk_sphere_hbm.md"
+    if not args.no_counts:
+        cnt = cw.walk_counts(pts, scene, seed=100, basis=False)
+        rf = roofline(walk_flops(cnt, dim, walks), kernel_ms, f"k_walk_persistent<{dim},false,0,false>",
+                      measured_traffic(name),
+                      {"per_query": {"box_tests": cnt["box_tests"] / cnt["queries"],
+                                     "prim_tests": cnt["prim_tests"] / cnt["queries"]},
+                       "mean_steps": cnt["steps"] / walks})
+    cpu = None
+    if not args.no_cpu:
+        import oracle as O
+        ref = O.Scene(c["vertices"], c["indices"], c["values"], use_bvh=True)
+        pn = c["points"]
+        cpu = cpu_leg(lambda ix: ref.walk_reduce(np.ascontiguousarray(pn[ix]), W, c["nsteps"], c["epsilon"], seed=0,
+                                                 threads=CPU_THREADS), P, W, args.cpu_seconds, "primal")
+    log(f"{args.workload}: {ms:.2f} ms/solve, {walks / ms * 1e3:.3e} walks/s, rmse {rmse:.4f} (Monte Carlo bound "
+        f"{mc_bound:.4f}), build {build_ms:.2f} ms, {qps:.3e} queries/s")
+    emit({"metric": "walks/s (primal)", "value": walks / ms * 1e3, "unit": "walks/s", "n_gpus": 1,
+          "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True,
+          "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+          "config": {"workload": ("2d-laplace-circle-1024seg-11428pts-64walks" if dim == 2 else
+                                  "3d-laplace-icosphere6-81920tris-1Mpts-32walks"),
+                     "points": P, "walks_per_point": W, "primitives": int(c["indices"].shape[0])},
+          "rmse_exact": rmse, "monte_carlo_bound": mc_bound, "lbvh_build_ms": build_ms,
+          "closest_point_queries_per_s": qps, "counts": cnt, "roofline": rf, "cpu_baseline": cpu})
 
 
-def shape2d_workload(args, torch, wos_ext, C):
-    """2D shape gradient on the clover (4,096 segments, 126,820 points, 16
-    walks, Laplace with g = x^2 - y^2): forward walk_reduce, loss L = mean
-    (u - u*)^2 against the solution u* for 1.1 g, OursCUDA backward onto all
-    vertex positions.  One "primal+grad walk" is a forward walk plus a
-    gradient sample (primary walk, LIB, two secondary walks)."""
-    c = C.config2()
+# ---------------------------------------------------------------------------
+# shape gradients: the clover (2D) and the 200k-triangle sphere (config 4)
+# ---------------------------------------------------------------------------
+def shape_workload(args, torch, wos_ext, C):
+    dim = 2 if args.workload == "shape2d" else 3
     dv = torch.device("cuda", 0)
     t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dv)
-    V, F, G, pts = t(c["vertices"]), t(c["indices"]), t(c["values"]), t(c["points"])
-    P, W = pts.shape[0], 16
-    scene = wos_ext.Scene(V, F, G, use_bvh=True)
-    cw = wos_ext.WoS(nwalks=W, nsteps=c["nsteps"], epsilon=c["epsilon"])
-    kw = dict(nwalks=W, nsteps=c["nsteps"], epsilon=c["epsilon"], epsilon2=c["epsilon"], clamping=0.1)
-    target = cw.walk_reduce(pts, wos_ext.Scene(V, F, G * 1.1, use_bvh=True), seed=12345)["mean"]
-    torch.cuda.synchronize()
+    if dim == 2:
+        c = C.config2()
+        V, F, G, pts = t(c["vertices"]), t(c["indices"]), t(c["values"]), t(c["points"])
+        P, W = pts.shape[0], 16
+        scene = wos_ext.Scene(V, F, G, use_bvh=True)
+        cw = wos_ext.WoS(nwalks=W, nsteps=c["nsteps"], epsilon=c["epsilon"])
+        kw = dict(nwalks=W, nsteps=c["nsteps"], epsilon=c["epsilon"], epsilon2=c["epsilon"], clamping=0.1)
+        target = cw.walk_reduce(pts, wos_ext.Scene(V, F, G * 1.1, use_bvh=True), seed=12345)["mean"]
+        wl = "2d-shape-gradient-clover-4096seg-126820pts-16walks"
+    else:
+        c = C.config4()
+        V, F, G, pts = t(c["vertices"]), t(c["indices"]), t(c["values"]), t(c["points"])
+        target = t(c["target"].astype(np.float32))
+        P, W = pts.shape[0], c["nwalks"]
+        scene = wos_ext.Scene3D(V, F, G, use_bvh=True)
+        cw = wos_ext.WoS3D(nwalks=W, nsteps=c["nsteps"], epsilon=c["epsilon"])
+        kw = dict(nwalks=W, nsteps=c["nsteps"], epsilon=c["epsilon"], epsilon2=c["epsilon2"], clamping=c["clamping"])
+        wl = "3d-shape-gradient-200k-tris-262144pts-16walks"
+    stream = torch.cuda.current_stream()
+    evs = []
 
-    def step(seed):
+    def step(i):
         ev = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
-        ev[0].record()
-        u = cw.walk_reduce(pts, scene, seed=seed)["mean"]
+        ev[0].record(stream)
+        u = cw.walk_reduce(pts, scene, seed=100 + i)["mean"]
         g = (2.0 / P) * (u - target)
-        ev[1].record()
-        gv, _ = wos_ext.shape_gradient(scene, pts, seed, g, **kw)
-        ev[2].record()
-        return ev, gv
+        ev[1].record(stream)
+        gv, _ = wos_ext.shape_gradient(scene, pts, 100 + i, g, **kw)
+        ev[2].record(stream)
+        evs.append(ev)
+        return gv
 
     for i in range(args.warmup):
         step(i)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    evs = [step(100 + i)[0] for i in range(args.steps)]
-    torch.cuda.synchronize()
-    ms = (time.perf_counter() - t0) * 1e3 / args.steps
+    evs.clear()
+    ms = synced_ms(torch, step, args.steps)
     fwd = float(np.mean([e[0].elapsed_time(e[1]) for e in evs]))
     bwd = float(np.mean([e[1].elapsed_time(e[2]) for e in evs]))
-    log(f"shape2d: {ms:.2f} ms/step (forward {fwd:.2f}, gradient {bwd:.2f}), {P * W / ms * 1e3:.3e} walks/s")
-    print(json.dumps({"metric": "primal+grad walks/s (2D shape gradient)", "value": P * W / ms * 1e3,
-                      "unit": "walks/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms,
-                      "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
-                      "data": "synthetic",
-                      "config": {"workload": "2d-shape-gradient-clover-4096seg-126820pts-16walks", "points": P,
-                                 "walks_per_point": W, "segments": int(c["indices"].shape[0])},
-                      "forward_ms": fwd, "gradient_ms": bwd}))
+    rf = None
+    if not args.no_counts:
+        cnt = wos_ext.shape_gradient_counts(scene, pts, 100, **kw)
+        flops = (cnt["box_tests"] * C_BOX[dim] + cnt["prim_tests"] * C_PRIM[dim] + cnt["steps"] * C_STEP
+                 + P * W * C_WALK * 3)
+        rf = roofline(flops, bwd, f"k_grad_persistent<{dim}>", None,
+                      {"per_query": {"box_tests": cnt["box_tests"] / max(cnt["queries"], 1),
+                                     "prim_tests": cnt["prim_tests"] / max(cnt["queries"], 1)},
+                       "steps_per_sample": cnt["steps"] / (P * W)})
+    cpu = None
+    if not args.no_cpu:
+        import oracle as O
+        ref = O.Scene(c["vertices"], c["indices"], c["values"], use_bvh=True)
+        pn = c["points"]
+        gt = target.cpu().numpy().astype(np.float64)
+
+        def chunk(ix):
+            q = np.ascontiguousarray(pn[ix])
+            r = ref.walk_reduce(q, W, kw["nsteps"], kw["epsilon"], seed=0, threads=CPU_THREADS)
+            ref.shape_gradient(q, W, kw["nsteps"], kw["epsilon"], kw["epsilon2"], kw["clamping"], seed=0,
+                               grad_out=(2.0 / P) * (r["mean"] - gt[ix]), threads=CPU_THREADS)
+
+        cpu = cpu_leg(chunk, P, W, args.cpu_seconds, "primal walk + shape-gradient sample")
+    log(f"{args.workload}: {ms:.2f} ms/step (forward {fwd:.2f}, gradient {bwd:.2f}), {P * W / ms * 1e3:.3e} walks/s")
+    emit({"metric": f"primal+grad walks/s ({dim}D shape gradient)", "value": P * W / ms * 1e3, "unit": "walks/s",
+          "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True,
+          "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+          "config": {"workload": wl, "points": P, "walks_per_point": W, "primitives": int(c["indices"].shape[0])},
+          "forward_ms": fwd, "gradient_ms": bwd, "roofline": rf, "cpu_baseline": cpu})
 
 
-def shape3d_workload(args, torch, wos_ext, C):
-    """3D shape gradient (200k-triangle decimated displaced sphere, 262,144
-    points, 16 walks): forward walk_reduce, loss L = mean (u - g)^2, Ours3D
-    backward onto all vertex positions.  One "primal+grad walk" is a forward
-    walk plus a gradient sample (primary walk, LIB, two secondary walks)."""
-    c = C.config4()
-    dv = torch.device("cuda", 0)
-    t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dv)
-    V, F, G, pts, target = t(c["vertices"]), t(c["indices"]), t(c["values"]), t(c["points"]), t(
-        c["target"].astype(np.float32))
-    P, W = pts.shape[0], c["nwalks"]
-    torch.cuda.synchronize()
-    tb = time.perf_counter()
-    scene = wos_ext.Scene3D(V, F, G, use_bvh=True)
-    torch.cuda.synchronize()
-    build_ms = (time.perf_counter() - tb) * 1e3
-    cw = wos_ext.WoS3D(nwalks=W, nsteps=c["nsteps"], epsilon=c["epsilon"])
-    kw = dict(nwalks=W, nsteps=c["nsteps"], epsilon=c["epsilon"], epsilon2=c["epsilon2"], clamping=c["clamping"])
-
-    def step(seed):
-        ev = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
-        ev[0].record()
-        u = cw.walk_reduce(pts, scene, seed=seed)["mean"]
-        g = (2.0 / P) * (u - target)
-        ev[1].record()
-        gv, _ = wos_ext.shape_gradient(scene, pts, seed, g, **kw)
-        ev[2].record()
-        return ev, gv
-
-    for i in range(args.warmup):
-        step(i)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    evs = [step(100 + i)[0] for i in range(args.steps)]
-    torch.cuda.synchronize()
-    ms = (time.perf_counter() - t0) * 1e3 / args.steps
-    fwd = float(np.mean([e[0].elapsed_time(e[1]) for e in evs]))
-    bwd = float(np.mean([e[1].elapsed_time(e[2]) for e in evs]))
-    log(f"shape3d: {ms:.2f} ms/step (forward {fwd:.2f}, gradient {bwd:.2f}), {P * W / ms * 1e3:.3e} walks/s")
-    print(json.dumps({"metric": "primal+grad walks/s (3D shape gradient)", "value": P * W / ms * 1e3,
-                      "unit": "walks/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms,
-                      "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
-                      "data": "synthetic",
-                      "config": {"workload": "3d-shape-gradient-200k-tris-262144pts-16walks", "points": P,
-                                 "walks_per_point": W, "triangles": int(c["indices"].shape[0])},
-                      "forward_ms": fwd, "gradient_ms": bwd, "scene_build_ms": build_ms}))
-
-
+# ---------------------------------------------------------------------------
+# 64 uneven scenes (config 5): batched primal + adjoint against the plain path
+# ---------------------------------------------------------------------------
 def batch_workload(args, torch, wos_ext, C):
-    """64 uneven scenes (32 2D, 32 3D; sizes log-uniform; points scaled by
-    --batch-scale): one batched launch per dimension, eager and replayed from
-    a captured graph, against the same items walked one call at a time."""
     items_np = C.config5(scale=args.batch_scale)
     dv = torch.device("cuda", 0)
     t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dv)
     items, total = [], 0
     for it in items_np:
+        src = "source_params" in it
         if it["dim"] == 2:
             sc = wos_ext.Scene(t(it["vertices"]), t(it["indices"]), t(it["values"]), use_bvh=True,
-                               source_type=it.get("source_type", 0),
-                               source_params=t(it["source_params"]) if "source_params" in it else None)
-            cw = wos_ext.WoS(nwalks=it["nwalks"], nsteps=it["nsteps"], epsilon=it["epsilon"])
+                               source_type=3 if src else 0, source_params=t(it["source_params"]) if src else None)
+            cw = wos_ext.WoS(nwalks=it["nwalks"], nsteps=it["nsteps"], epsilon=it["epsilon"], use_IS_for_greens=False)
         else:
             sc = wos_ext.Scene3D(t(it["vertices"]), t(it["indices"]), t(it["values"]), use_bvh=True)
             cw = wos_ext.WoS3D(nwalks=it["nwalks"], nsteps=it["nsteps"], epsilon=it["epsilon"])
-        items.append(dict(scene=sc, wos=cw, points=t(it["points"]), seed=it["seed"], records=True,
-                          basis="source_params" in it))
+        items.append(dict(scene=sc, wos=cw, points=t(it["points"]), seed=it["seed"], grad_values=True,
+                          grad_coeffs=src, basis=src))
         total += it["points"].shape[0] * it["nwalks"]
     batch = wos_ext.Batch(items)
     tuning = batch.autotune() if args.autotune else None
     if tuning:
         log(f"autotuned: {tuning}")
+    targets = [t((1.1 * it["exact"]).astype(np.float32)) for it in items_np]
 
-    def sequential():
-        for it in items:
-            it["wos"].walk_reduce(it["points"], it["scene"], seed=it["seed"], records=True, basis=it["basis"])
+    def step(_=0):
+        batch.run()
+        for o, tg in zip(batch.outputs, targets):
+            o["grad_out"].copy_((2.0 / tg.numel()) * (o["mean"] - tg))
+        batch.zero_grad()
+        batch.adjoint()
 
-    def timed(fn):
-        fn()
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        for _ in range(args.steps):
-            fn()
-        torch.cuda.synchronize()
-        return (time.perf_counter() - t0) * 1e3 / args.steps
-
-    seq_ms = timed(sequential)
-    eager_ms = timed(batch.run)
+    step()
+    eager_ms = synced_ms(torch, step, args.steps)
     s = torch.cuda.Stream()
     g = torch.cuda.CUDAGraph()
-    batch.run()
     torch.cuda.synchronize()
     with torch.cuda.graph(g, stream=s):
-        batch.run()
-    graph_ms = timed(g.replay)
-    log(f"batch: {total} walks; sequential {seq_ms:.2f} ms, batched {eager_ms:.2f} ms, graph {graph_ms:.2f} ms")
-    print(json.dumps({"metric": "walks/s (64 batched scenes)", "value": total / graph_ms * 1e3, "unit": "walks/s",
-                      "n_gpus": 1, "steps": args.steps, "warmup": 1, "ms_per_step": graph_ms,
-                      "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
-                      "data": "synthetic",
-                      "config": {"workload": f"64-scene-batch-scale-{args.batch_scale}", "walks": total},
-                      "sequential_ms": seq_ms, "batched_eager_ms": eager_ms, "graph_replay_ms": graph_ms,
-                      "tuning": tuning,
-                      "sequential_walks_per_s": total / seq_ms * 1e3}))
+        step()
+    graph_ms = synced_ms(torch, lambda i: g.replay(), args.steps)
+    # accuracy of the batched means (FP64) against each item's exact solution
+    err2, npts = 0.0, 0
+    for o, it in zip(batch.outputs, items_np):
+        e = o["mean64"].cpu().numpy() - it["exact"]
+        err2 += float((e * e).sum())
+        npts += e.size
+    rmse_batched = float(np.sqrt(err2 / npts))
+    log(f"batch: {total} walks; batched primal+adjoint {eager_ms:.1f} ms eager, {graph_ms:.1f} ms graph, "
+        f"rmse {rmse_batched:.4f}")
+
+    plain = None
+    if not args.no_plain:
+        # reference data flow per item (one lane per walk, brute-force scan, walks re-run in backward)
+        # on the same strided points of every item, and the batched means on those points
+        n_sample = args.plain_points
+        tp, pw, e_plain, e_bat, ns = 0.0, 0, 0.0, 0.0, 0
+        for j, (it, o, itn) in enumerate(zip(items, batch.outputs, items_np)):
+            sc = it["scene"]
+            P = it["points"].shape[0]
+            stride = max(1, P // n_sample)
+            sel = slice(None, None, stride)
+            psub = it["points"][sel].contiguous()
+            W = it["wos"].nwalks
+            if itn["dim"] == 2:
+                src = "source_params" in itn
+                bf = wos_ext.Scene(t(itn["vertices"]), t(itn["indices"]), t(itn["values"]), use_bvh=False,
+                                   source_type=3 if src else 0,
+                                   source_params=t(itn["source_params"]) if src else None)
+            else:
+                bf = wos_ext.Scene3D(t(itn["vertices"]), t(itn["indices"]), t(itn["values"]), use_bvh=False)
+            tsub = targets[j][sel].contiguous()
+            torch.cuda.synchronize()
+            ta = time.perf_counter()
+            u, _ = plain_step(torch, bf, it["wos"], psub, tsub, itn["seed"], W)
+            torch.cuda.synchronize()
+            tp += time.perf_counter() - ta
+            pw += psub.shape[0] * W
+            ex = itn["exact"][::stride]
+            e_plain += float(((u.double().cpu().numpy() - ex) ** 2).sum())
+            e_bat += float(((o["mean64"].cpu().numpy()[::stride] - ex) ** 2).sum())
+            ns += ex.size
+        plain = {"walks_per_s": pw / tp, "seconds": tp,
+                 "sample": f"{ns} points (about {n_sample} per item, evenly strided) x each item's walks, "
+                           f"brute-force scan, walks re-run in backward",
+                 "rmse_plain": float(np.sqrt(e_plain / ns)), "rmse_batched_same_points": float(np.sqrt(e_bat / ns)),
+                 "speedup_batched_vs_plain": (total / (graph_ms * 1e-3)) / (pw / tp)}
+        log(f"plain: {plain['walks_per_s']:.3e} walks/s; speedup {plain['speedup_batched_vs_plain']:.1f}x; rmse "
+            f"{plain['rmse_plain']:.4f} vs batched {plain['rmse_batched_same_points']:.4f}")
+    emit({"metric": "primal+grad walks/s at 1 GPU, batched vs plain (64 uneven scenes)",
+          "value": total / graph_ms * 1e3, "unit": "walks/s", "n_gpus": 1, "steps": args.steps, "warmup": 1,
+          "ms_per_step": graph_ms, "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+          "data": "synthetic",
+          "config": {"workload": f"64-scene-batch-primal-adjoint-scale-{args.batch_scale}", "walks": total,
+                     "scenes": len(items)},
+          "batched_eager_ms": eager_ms, "graph_replay_ms": graph_ms, "rmse_batched": rmse_batched,
+          "tuning": tuning, "plain": plain})
 
 
 def main():
@@ -320,11 +528,13 @@ def main():
                     help="clover: the headline primal+adjoint workload; circle / sphere: primal-only "
                          "Laplace workloads (1024-segment circle, 81,920-triangle displaced icosphere); "
                          "shape2d / shape3d: shape gradients (clover, 200k-triangle sphere); "
-                         "batch: 64 uneven scenes")
+                         "batch: 64 uneven scenes, primal + adjoint")
     ap.add_argument("--autotune", action="store_true",
                     help="batch workload: choose the launch tuning first (cached per device)")
-    ap.add_argument("--batch-scale", type=float, default=0.1,
+    ap.add_argument("--batch-scale", type=float, default=1.0,
                     help="query points per batched scene, as a fraction of the full workload's")
+    ap.add_argument("--plain-points", type=int, default=32,
+                    help="batch workload: points per item walked by the plain path")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU; multi-GPU runs are not part of it",
@@ -344,127 +554,15 @@ def main():
         torch.cuda.synchronize()
         print(json.dumps({"calibration": "copy", "bytes_read": a.numel() * 4, "bytes_written": b.numel() * 4}))
         return
-    if args.workload == "shape2d":
-        return shape2d_workload(args, torch, wos_ext, C)
-    if args.workload == "shape3d":
-        return shape3d_workload(args, torch, wos_ext, C)
-    if args.workload == "batch":
-        return batch_workload(args, torch, wos_ext, C)
-    if args.workload != "clover":
-        return primal_workload(args, torch, wos_ext, C)
-    c, d = build_inputs(torch, C)
-    P, W = c["points"].shape[0], c["nwalks"]
-    log(f"workload: {P} points x {W} walks, 4096 segments, eps 1e-4")
-    scene = wos_ext.Scene(d["vertices"], d["indices"], d["values"], use_bvh=True, source_type=3,
-                          source_params=d["source"])
-    cwos = wos_ext.WoS(nwalks=W, nsteps=c["nsteps"], epsilon=c["epsilon"])
-    # fixed target: the solution at 1.1 g (solved once, untimed)
-    tscene = wos_ext.Scene(d["vertices"], d["indices"], d["values"] * 1.1, use_bvh=True, source_type=3,
-                           source_params=d["source"])
-    target = cwos.walk_reduce(d["points"], tscene, seed=12345)["mean"]
-    torch.cuda.synchronize()
-
-    stream = torch.cuda.current_stream()
-    for i in range(args.warmup):
-        fused_step(torch, wos_ext, scene, cwos, d["points"], target, seed=i)
-    torch.cuda.synchronize()
-    log("warmup done")
-
-    # timed region: K full steps
-    ev = [torch.cuda.Event(enable_timing=True) for _ in range(2 * args.steps)]
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    kernel_ms = []
-    for i in range(args.steps):
-        # HIP events on the stream the walk kernel runs on, around the walk launch
-        ev[2 * i].record(stream)
-        out = cwos.walk_reduce(d["points"], scene, seed=100 + i, records=True, basis=True)
-        ev[2 * i + 1].record(stream)
-        u = out["mean"]
-        g = (2.0 / P) * (u - target)
-        gv, gc = wos_ext.adjoint_values(scene, P, W, out["rec_prim"], out["rec_t"], g, basis=out["basis"],
-                                        want_values=True, want_coeffs=True)
-    torch.cuda.synchronize()
-    t1 = time.perf_counter()
-    for i in range(args.steps):
-        kernel_ms.append(ev[2 * i].elapsed_time(ev[2 * i + 1]))
-    ms_per_step = (t1 - t0) * 1e3 / args.steps
-    walks = P * W
-    value = walks / (ms_per_step * 1e-3)
-    log(f"fused: {ms_per_step:.2f} ms/step, {value:.3e} walks/s; walk launch {np.mean(kernel_ms):.2f} ms")
-
-    # ---- algorithmic work of one walk launch (untimed counting run) ----------
-    counts = None if args.no_counts else cwos.walk_counts(d["points"], scene, seed=100)
-    roofline = None
-    if counts is not None:
-        K = c["source_params"].shape[0]
-        flops = (counts["box_tests"] * C_BOX_2D + counts["prim_tests"] * C_PRIM_2D
-                 + counts["steps"] * (C_STEP + C_SRC + C_TERM * K) + walks * C_WALK)
-        launch_s = np.mean(kernel_ms) * 1e-3
-        achieved = flops / launch_s / 1e12
-        roofline = {"bound": "valu", "achieved": round(achieved, 3), "peak": PEAK_FP32 / 1e12,
-                    "unit": "TFLOP/s", "frac": round(achieved * 1e12 / PEAK_FP32, 4), "traffic": measured_traffic(),
-                    "kernel": "k_walk_persistent<2,true,8,false>", "flops_per_launch": float(flops),
-                    "launch_ms": round(np.mean(kernel_ms), 3),
-                    "per_query": {"box_tests": counts["box_tests"] / counts["queries"],
-                                  "prim_tests": counts["prim_tests"] / counts["queries"]},
-                    "mean_steps": counts["steps"] / walks}
-        if roofline["traffic"]:
-            roofline["hbm_GBs"] = round(roofline["traffic"] / launch_s / 1e9, 1)
-            roofline["hbm_frac"] = round(roofline["traffic"] / launch_s / PEAK_HBM, 4)
-
-    # ---- accuracy: manufactured twin, RMSE vs exact ---------------------------
-    ct = C.config2(twin=True)
-    tw = wos_ext.Scene(d["vertices"], d["indices"], torch.from_numpy(ct["values"]).cuda(), use_bvh=True,
-                       source_type=3, source_params=d["source"])
-    o = cwos.walk_reduce(d["points"], tw, seed=7, stats=True)
-    rmse = float(np.sqrt(np.mean((o["mean64"].cpu().numpy() - ct["exact"]) ** 2)))
-
-    # ---- reference-style path ------------------------------------------------
-    plain = None
-    if not args.no_plain:
-        scene_bf = wos_ext.Scene(d["vertices"], d["indices"], d["values"], use_bvh=False, source_type=3,
-                                 source_params=d["source"])
-        sub = d["points"][:: 8].contiguous()   # 1/8 of the grid keeps the brute-force run short
-        tsub = target[:: 8].contiguous()
-        plain_step(torch, wos_ext, scene_bf, cwos, sub, tsub, 0, W)
-        torch.cuda.synchronize()
-        ta = time.perf_counter()
-        plain_step(torch, wos_ext, scene_bf, cwos, sub, tsub, 1, W)
-        torch.cuda.synchronize()
-        tp = time.perf_counter() - ta
-        plain = {"walks_per_s": sub.shape[0] * W / tp, "ms_per_step_sample": tp * 1e3,
-                 "sample": f"{sub.shape[0]} points (every 8th) x {W} walks, brute-force scan, walks re-run in backward",
-                 "speedup_fused_vs_plain": value / (sub.shape[0] * W / tp)}
-        log(f"plain: {plain['walks_per_s']:.3e} walks/s")
-
-    # ---- CPU baseline: the oracle on a bounded sample ------------------------
-    cpu = None
     if not args.no_cpu:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
-        import oracle as O
-        ref = O.Scene(c["vertices"], c["indices"], c["values"], 3, c["source_params"])
-        threads = min(os.cpu_count() or 1, 16)
-        done, tc, blk = 0, 0.0, 64
-        pts_np = c["points"]
-        while tc < args.cpu_seconds and done < P:
-            q = pts_np[done: done + blk]
-            ta = time.perf_counter()
-            ref.walk_reduce(q, W, c["nsteps"], c["epsilon"], seed=0, threads=threads)
-            tc += time.perf_counter() - ta
-            done += q.shape[0]
-        cpu = {"value": done * W / tc, "unit": "walks/s", "cores": threads, "kind": "port",
-               "sample": f"first {done} grid points x {W} walks, primal only, brute-force scan, {tc:.1f} s"}
-        log(f"cpu: {cpu['value']:.3e} walks/s on {threads} threads")
-
-    line = {"metric": "primal+grad walks/s at 1 GPU (2D Poisson clover, values + source-coefficient adjoint)",
-            "value": value, "unit": "walks/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
-            "ms_per_step": ms_per_step, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-            "dtype": "f32", "data": "synthetic",
-            "config": {"workload": "2d-poisson-clover-4096seg-512grid-128walks-eps1e-4-fourier8",
-                       "points": P, "walks_per_point": W, "segments": 4096, "parallelism": "single-gpu"},
-            "rmse_twin": rmse, "roofline": roofline, "cpu_baseline": cpu, "plain": plain}
-    print(json.dumps(line))
+    if args.workload in ("shape2d", "shape3d"):
+        return shape_workload(args, torch, wos_ext, C)
+    if args.workload == "batch":
+        return batch_workload(args, torch, wos_ext, C)
+    if args.workload in ("circle", "sphere"):
+        return primal_workload(args, torch, wos_ext, C)
+    return clover_workload(args, torch, wos_ext, C)
 
 
 if __name__ == "__main__":

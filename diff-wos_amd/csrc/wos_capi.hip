@@ -607,6 +607,16 @@ int wos_shape_gradient(const wos_scene *s, const wos_grad_params *g, const float
                                  sample_grad_u, S(stream));
 }
 
+int wos_shape_gradient_counts(const wos_scene *s, const wos_grad_params *g, const float *points, int n_points,
+                              int seed, unsigned long long *counts, void *stream) {
+    REQUIRE(s && g && counts && (n_points == 0 || points), "scene, params, points and counts are required");
+    REQUIRE(n_points >= 0 && g->nwalks >= 1 && g->nsteps >= 0, "bad sizes");
+    SceneView v = view_of(s);
+    v.stats = counts;
+    GradSettings gs{g->nwalks, g->nsteps, g->epsilon, g->epsilon2, g->clamping, g->double_sided, g->control_variates};
+    return launch_shape_gradient(v, gs, points, n_points, (int32_t)seed, nullptr, nullptr, nullptr, S(stream));
+}
+
 int wos_shape_derivative(const wos_scene *s, const wos_grad_params *g, const float *points, int n_points, int seed,
                          const float *velocity, float *out_mean, double *out_mean64, double *out_var64,
                          void *stream) {

@@ -846,8 +846,9 @@ __device__ __forceinline__ void add_tangent_derivative(const SceneView &s, int f
 // waves); at 80, with a few registers spilled, the 200k-triangle shape
 // gradient measured 7 % faster on one MI355X and the same on another.  The
 // 2D state fits.  Finished samples write in batches as in the walk kernel
-// (2D: 4 % faster).
-template <int D>
+// (2D: 4 % faster).  STATS: count bound tests, primitive tests, queries and
+// walk steps into s.stats (the instrumented launch of the measurements).
+template <int D, bool STATS>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6))) void k_grad_persistent(
     SceneView s, GradArgs a) {
     DECLARE_STACK(a.spill, a.spill_stride);
@@ -983,6 +984,10 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6))) voi
                     secondary_point<D>(xb, n, R, dr, side, p);
                 }
                 const float seed_d2 = prim_d2<D>(s, rprim, p);
+                if constexpr (STATS) {
+                    atomicAdd(&s.stats[1], 1ull);   // the seed primitive
+                    atomicAdd(&s.stats[2], 1ull);
+                }
                 if (s.n_prims == 1) {
                     q.best = seed_d2;
                     q.best_id = rprim;
@@ -1121,6 +1126,11 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6))) voi
             move_point<D>(p, d, rng);
             ++step;
             const float seed_d2 = prim_d2<D>(s, prim, p);
+            if constexpr (STATS) {
+                atomicAdd(&s.stats[1], 1ull);   // the seed primitive
+                atomicAdd(&s.stats[2], 1ull);
+                atomicAdd(&s.stats[3], 1ull);
+            }
             if (s.n_prims == 1) {
                 q.best = seed_d2;
                 q.best_id = prim;
@@ -1133,7 +1143,10 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6))) voi
         // 4. advance every pending query by up to `window` node visits
         for (int it = 0; it < a.window; ++it) {
             if (mode == kQuery) {
-                if (query_step<D, false>(s, p, q, st, slot, lane_g, s.tol)) mode = kReady;
+                if (query_step<D, STATS>(s, p, q, st, slot, lane_g, s.tol)) {
+                    mode = kReady;
+                    query_flush_stats<STATS>(s, q);
+                }
             }
             if (__ballot(mode == kQuery) == 0) break;
         }
@@ -1466,8 +1479,13 @@ int launch_shape_gradient(const SceneView &s, const GradSettings &g, const float
     if (s.dim == 2) k_point_init<2><<<gi, kBlock, tb, st>>>(s, points, n_points, sr.order, sr.prim, sr.sd, a.spill, a.spill_stride);
     else k_point_init<3><<<gi, kBlock, tb, st>>>(s, points, n_points, nullptr, sr.prim, sr.sd, a.spill, a.spill_stride);
     k_zero_u64<<<1, 1, 0, st>>>(a.counter);
-    if (s.dim == 2) k_grad_persistent<2><<<gr, kBlock, tb, st>>>(s, a);
-    else k_grad_persistent<3><<<gr, kBlock, tb, st>>>(s, a);
+    if (s.stats) {
+        if (s.dim == 2) k_grad_persistent<2, true><<<gr, kBlock, tb, st>>>(s, a);
+        else k_grad_persistent<3, true><<<gr, kBlock, tb, st>>>(s, a);
+    } else {
+        if (s.dim == 2) k_grad_persistent<2, false><<<gr, kBlock, tb, st>>>(s, a);
+        else k_grad_persistent<3, false><<<gr, kBlock, tb, st>>>(s, a);
+    }
     KCHK();
     return 0;
 }

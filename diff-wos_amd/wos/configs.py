@@ -284,15 +284,17 @@ def config5(scale=1.0, seed=0):
             t = rng.random(n_pts) * 2 * np.pi
             p = np.stack([r * np.cos(t), r * np.sin(t)], 1).astype(np.float32)
             item = dict(dim=2, vertices=v, indices=ix, values=vals, points=p, nwalks=W, nsteps=256, epsilon=1e-3,
-                        seed=j)
+                        seed=j, exact=g_saddle(p))
             if j % 2:
-                item.update(source_type=3, source_params=fourier_params())
+                # Poisson with the clover source; boundary data of its manufactured twin, so u is known
+                item.update(source_type=3, source_params=fourier_params(), values=u_clover_twin(v).astype(np.float32),
+                            exact=u_clover_twin(p))
         else:
             ntri = int(np.exp(rng.uniform(np.log(1e3), np.log(1e5))))
             v, f, noise = displaced_sphere(icosphere_level(ntri), seed=j)
             p = points_in_star(n_pts, noise, seed=j)
             item = dict(dim=3, vertices=v, indices=f, values=g_harmonic3(v).astype(np.float32), points=p, nwalks=W,
-                        nsteps=256, epsilon=1e-3, seed=j)
+                        nsteps=256, epsilon=1e-3, seed=j, exact=g_harmonic3(p))
         out.append(item)
     return out
 

@@ -100,6 +100,7 @@ _sig("wos_batch_adjoint", _vp, _vp)
 _sig("wos_batch_destroy", _vp)
 _sig("wos_adjoint_source", _vp, ctypes.POINTER(_WalkParams), _vp, _i, _i, _vp, _vp, _vp)
 _sig("wos_debug_fail_workspace", _i)
+_sig("wos_shape_gradient_counts", _vp, ctypes.POINTER(_GradParams), _vp, _i, _i, _vp, _vp)
 _sig("wos_set_tuning", _i, _i, _i, _i)
 _sig("wos_get_tuning", _vp, _vp, _vp, _vp)
 _sig("wos_set_spatial_order", _i)
@@ -115,7 +116,7 @@ EXPORTED = ("wos_last_error", "wos_scene_create", "wos_scene_destroy", "wos_scen
             "wos_walk_reduce_counts", "wos_adjoint_values", "wos_shape_gradient", "wos_shape_derivative",
             "wos_set_tuning",
             "wos_get_tuning", "wos_set_spatial_order", "wos_batch_create", "wos_batch_run", "wos_batch_adjoint",
-            "wos_batch_destroy", "wos_adjoint_source", "wos_debug_fail_workspace")
+            "wos_batch_destroy", "wos_adjoint_source", "wos_debug_fail_workspace", "wos_shape_gradient_counts")
 
 
 def set_tuning(chunk=None, wgs_per_cu=None, top_levels=None, window=None):
@@ -637,6 +638,20 @@ def shape_gradient(scene, points, seed, grad_out, *, nwalks, nsteps, epsilon, ep
     _check(lib.wos_shape_gradient(scene.handle, ctypes.byref(gp), _p(q), P, int(seed), _p(go),
                                   _p(gv if go is not None else None), _p(sg), _stream()))
     return gv, sg
+
+
+def shape_gradient_counts(scene, points, seed, *, nwalks, nsteps, epsilon, epsilon2, clamping, double_sided=False,
+                          control_variates=True):
+    """Instrumented shape-gradient launch: totals of bound tests, primitive
+    tests, closest-primitive queries and walk steps of its samples."""
+    q = _dev(points, torch.float32, scene.dim)
+    gp = _GradParams(int(nwalks), int(nsteps), float(epsilon), float(epsilon2), float(clamping),
+                     int(bool(double_sided)), int(bool(control_variates)))
+    cnt = torch.zeros(4, dtype=torch.int64, device=q.device)
+    _check(lib.wos_shape_gradient_counts(scene.handle, ctypes.byref(gp), _p(q), q.shape[0], int(seed), _p(cnt),
+                                         _stream()))
+    c = [int(x) for x in cnt.cpu()]
+    return dict(box_tests=c[0], prim_tests=c[1], queries=c[2], steps=c[3], samples=q.shape[0] * int(nwalks))
 
 
 def shape_derivative(scene, points, seed, velocity, *, nwalks, nsteps, epsilon, epsilon2, clamping,

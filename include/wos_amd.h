@@ -207,6 +207,14 @@ int wos_adjoint_source(const wos_scene *scene, const wos_walk_params *params, co
 int wos_shape_gradient(const wos_scene *scene, const wos_grad_params *params, const float *points, int n_points,
                        int seed, const float *grad_out, float *grad_vertices, float *sample_grad_u, void *stream);
 
+/* Instrumented wos_shape_gradient for measurement (no gradient output): adds
+ * into counts (4 device uint64, zeroed by the caller) the bound tests,
+ * primitive tests, closest-primitive queries and walk steps of the samples
+ * (primary walks, LIB queries, secondary walks).  Slower than the plain
+ * launch; no counterpart in the reference. */
+int wos_shape_gradient_counts(const wos_scene *scene, const wos_grad_params *params, const float *points,
+                              int n_points, int seed, unsigned long long *counts, void *stream);
+
 /* Forward mode of the same integrand: the derivative of each point's
  * OursCUDA / Ours3D estimate along a deformation that moves vertex v with
  * velocity[v] (n_vertices*dim) -- what diff_solve/runner.py:75-109 (renderD)

@@ -36,7 +36,7 @@ _f = ctypes.c_float
 class _Scene(ctypes.Structure):
     _fields_ = [("dim", _i), ("n_vertices", _i), ("n_prims", _i), ("vertices", _vp), ("indices", _vp),
                 ("values", _vp), ("normals", _vp), ("aabb_min", _f * 3), ("aabb_max", _f * 3),
-                ("source_type", _i), ("source_params", _vp), ("n_source_params", _i)]
+                ("source_type", _i), ("source_params", _vp), ("n_source_params", _i), ("bvh", _vp)]
 
 
 class _WalkParams(ctypes.Structure):
@@ -89,6 +89,9 @@ lib.or_sample_green_radius.restype = _f
 lib.or_sample_green_radius.argtypes = [_f, _f, ctypes.POINTER(_f), ctypes.POINTER(_i)]
 lib.or_interior_sample.restype = _i
 lib.or_interior_sample.argtypes = [_f, _f, _f, _i] + [ctypes.POINTER(_f)] * 4
+lib.or_bvh_create.restype = _vp
+lib.or_bvh_create.argtypes = [ctypes.POINTER(_Scene)]
+lib.or_bvh_destroy.argtypes = [_vp]
 lib.or_morton.restype = ctypes.c_uint32
 lib.or_morton.argtypes = [_i, _vp]
 
@@ -177,7 +180,11 @@ def morton(dim, unit):
 
 # ---- scenes ------------------------------------------------------------------
 class Scene:
-    def __init__(self, vertices, indices, values, source_type=0, source_params=None):
+    """use_bvh: answer closest-primitive queries through the CPU LBVH
+    (or_bvh_create) instead of the reference's brute-force scan; results are
+    identical, only the cost differs."""
+
+    def __init__(self, vertices, indices, values, source_type=0, source_params=None, use_bvh=False):
         self.vertices = np.ascontiguousarray(vertices, dtype=np.float32)
         self.dim = self.vertices.shape[1]
         self.indices = np.ascontiguousarray(indices, dtype=np.int32).reshape(-1, self.dim)
@@ -196,8 +203,31 @@ class Scene:
         s.source_type = int(source_type)
         s.source_params = _p(self.source_params)
         s.n_source_params = 0 if self.source_params is None else self.source_params.size
+        s.bvh = None
         lib.or_scene_init(ctypes.byref(s))
         self.s = s
+        self._bvh = None
+        if use_bvh:
+            self.set_bvh(True)
+
+    def set_bvh(self, enable):
+        if enable and not self._bvh:
+            self.s.bvh = None
+            h = lib.or_bvh_create(ctypes.byref(self.s))
+            if not h:
+                raise RuntimeError("or_bvh_create failed")
+            self._bvh = h
+            self.s.bvh = h
+        elif not enable and self._bvh:
+            self.s.bvh = None
+            lib.or_bvh_destroy(self._bvh)
+            self._bvh = None
+
+    def __del__(self):
+        try:
+            self.set_bvh(False)
+        except Exception:   # interpreter shutdown
+            pass
 
     @property
     def aabb(self):

@@ -341,8 +341,10 @@ static float prim_d2(const or_scene *s, int f, const float *p) {
 }
 
 /* include/scene.cuh:102-128.  Strict '<' over increasing index keeps the
- * smallest index among equal squared distances. */
+ * smallest index among equal squared distances.  With an LBVH attached the
+ * tree answers (include/scene.cuh:130-146) with the same result. */
 int or_closest_index(const or_scene *s, const float *p, float *out_d2) {
+    if (s->bvh) return or_bvh_closest(s, s->bvh, p, out_d2);
     float best = INFINITY;
     int id = -1;
     for (int f = 0; f < s->n_prims; ++f) {
@@ -1061,6 +1063,143 @@ int or_lbvh_build(const or_scene *s, uint32_t *codes_sorted, int *prim_sorted, i
     free(pairs);
     free(codes);
     return 0;
+}
+
+/* ---- CPU LBVH queries ----------------------------------------------------- */
+struct or_bvh {
+    int n;              /* primitives */
+    int *children;      /* 2 (n - 1): internal node, or ~sorted position of a leaf */
+    int *prim_sorted;   /* sorted position -> primitive */
+    float *box;         /* per internal node: lo[3], hi[3] */
+    float tol;          /* pruning margin (scale-relative) */
+};
+
+static void bvh_box_of(const or_scene *s, const or_bvh *b, int c, float *lo, float *hi) {
+    int D = s->dim;
+    for (int k = 0; k < 3; ++k) { lo[k] = 0.0f; hi[k] = 0.0f; }
+    if (c < 0) {
+        int f = b->prim_sorted[~c];
+        for (int k = 0; k < D; ++k) { lo[k] = FLT_MAX; hi[k] = -FLT_MAX; }
+        for (int v = 0; v < D; ++v)
+            for (int k = 0; k < D; ++k) {
+                float x = V(s, s->indices[(size_t)f * D + v])[k];
+                lo[k] = fminf(lo[k], x);
+                hi[k] = fmaxf(hi[k], x);
+            }
+    } else {
+        for (int k = 0; k < 3; ++k) { lo[k] = b->box[6 * (size_t)c + k]; hi[k] = b->box[6 * (size_t)c + 3 + k]; }
+    }
+}
+
+/* boxes bottom-up: children before parents in a post-order walk */
+static void bvh_fit(const or_scene *s, or_bvh *b, int node, int depth) {
+    float lo[3], hi[3], l2[3], h2[3];
+    for (int j = 0; j < 2; ++j) {
+        int c = b->children[2 * (size_t)node + j];
+        if (c >= 0 && depth < 4096) bvh_fit(s, b, c, depth + 1);
+    }
+    bvh_box_of(s, b, b->children[2 * (size_t)node], lo, hi);
+    bvh_box_of(s, b, b->children[2 * (size_t)node + 1], l2, h2);
+    for (int k = 0; k < 3; ++k) {
+        b->box[6 * (size_t)node + k] = fminf(lo[k], l2[k]);
+        b->box[6 * (size_t)node + 3 + k] = fmaxf(hi[k], h2[k]);
+    }
+}
+
+or_bvh *or_bvh_create(const or_scene *s) {
+    int n = s->n_prims;
+    if (n <= 0) return NULL;
+    or_bvh *b = (or_bvh *)calloc(1, sizeof(or_bvh));
+    b->n = n;
+    b->prim_sorted = (int *)malloc(sizeof(int) * (size_t)n);
+    int m = n > 1 ? n - 1 : 1;
+    b->children = (int *)malloc(sizeof(int) * 2 * (size_t)m);
+    b->box = (float *)calloc(6 * (size_t)m, sizeof(float));
+    int *pi = (int *)malloc(sizeof(int) * (size_t)m), *pl = (int *)malloc(sizeof(int) * (size_t)n);
+    int depth = 0;
+    if (or_lbvh_build(s, NULL, b->prim_sorted, b->children, pi, pl, &depth) != 0) {
+        free(pi); free(pl); or_bvh_destroy(b);
+        return NULL;
+    }
+    free(pi);
+    free(pl);
+    float scale = 1.0f;
+    for (int k = 0; k < s->dim; ++k) scale = fmaxf(scale, fmaxf(fabsf(s->aabb_min[k]), fabsf(s->aabb_max[k])));
+    b->tol = 1e-6f * scale;
+    if (n > 1) bvh_fit(s, b, 0, 0);
+    return b;
+}
+
+void or_bvh_destroy(or_bvh *b) {
+    if (!b) return;
+    free(b->children);
+    free(b->prim_sorted);
+    free(b->box);
+    free(b);
+}
+
+/* distance from p to a box, in double (a lower bound of every contained
+ * primitive's distance up to the pruning margin) */
+static double box_dist(const float *lo, const float *hi, const float *p, int D) {
+    double s2 = 0.0;
+    for (int k = 0; k < D; ++k) {
+        double d = fmax(fmax((double)lo[k] - p[k], (double)p[k] - hi[k]), 0.0);
+        s2 += d * d;
+    }
+    return sqrt(s2);
+}
+
+int or_bvh_closest(const or_scene *s, const or_bvh *b, const float *p, float *out_d2) {
+    float best = INFINITY;
+    int id = INT32_MAX;
+    if (b->n == 1) {
+        best = prim_d2(s, 0, p);
+        id = 0;
+    } else {
+        int stack_node[256];
+        double stack_key[256];
+        int sp = 0;
+        stack_node[sp] = 0;
+        stack_key[sp++] = 0.0;
+        while (sp > 0) {
+            --sp;
+            int node = stack_node[sp];
+            if (stack_key[sp] > sqrt((double)best) + b->tol) continue;
+            int cn[2];
+            double key[2];
+            for (int j = 0; j < 2; ++j) {
+                int c = b->children[2 * (size_t)node + j];
+                cn[j] = -1;
+                key[j] = INFINITY;
+                if (c < 0) {
+                    int f = b->prim_sorted[~c];
+                    float d2 = prim_d2(s, f, p);
+                    if (d2 < best || (d2 == best && f < id)) { best = d2; id = f; }
+                } else {
+                    float lo[3], hi[3];
+                    bvh_box_of(s, b, c, lo, hi);
+                    cn[j] = c;
+                    key[j] = box_dist(lo, hi, p, s->dim);
+                }
+            }
+            /* far child first, so the near one is popped next */
+            int near = key[0] <= key[1] ? 0 : 1;
+            for (int t = 0; t < 2; ++t) {
+                int j = t == 0 ? 1 - near : near;
+                if (cn[j] >= 0 && key[j] <= sqrt((double)best) + b->tol) {
+                    if (sp == 256) {   /* deeper than any tree of 32-bit codes: answer by the scan */
+                        const or_scene s2 = {s->dim, s->n_vertices, s->n_prims, s->vertices, s->indices,
+                                             s->values, s->normals, {0, 0, 0}, {0, 0, 0}, 0, NULL, 0, NULL};
+                        return or_closest_index(&s2, p, out_d2);
+                    }
+                    stack_node[sp] = cn[j];
+                    stack_key[sp++] = key[j];
+                }
+            }
+        }
+    }
+    if (out_d2) *out_d2 = best;
+    return id == INT32_MAX ? -1 : id;
 }
 
 /* ======================================================================= */

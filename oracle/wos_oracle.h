@@ -52,6 +52,9 @@ typedef struct {
     int source_type;         /* 0 none, 1 gaussian, 2 sinusoid, 3 fourier */
     const float *source_params;
     int n_source_params;
+    /* optional LBVH (or_bvh_create); when set, every closest-primitive query
+     * uses it, with the brute-force scan's result (same tie rule) */
+    const struct or_bvh *bvh;
 } or_scene;
 
 typedef struct {
@@ -160,6 +163,18 @@ int   or_lbvh_build(const or_scene *s, uint32_t *codes_sorted, int *prim_sorted,
                     int *children, int *parent_internal, int *parent_leaf, int *max_depth);
 /* Morton code of a point in [0,1]^dim (2D: 16 bits per axis, 3D: 10 bits) */
 uint32_t or_morton(int dim, const float *unit);
+
+/* CPU LBVH over the tree of or_lbvh_build with per-node boxes, for the
+ * closest-primitive query of include/scene.cuh:130-146: nearest-first
+ * traversal that prunes a box only when its distance exceeds the best
+ * distance so far plus a scale-relative margin, so the result is the
+ * brute-force scan's (smallest squared distance, then smallest index).
+ * Assign the result to or_scene.bvh; NULL on failure. */
+typedef struct or_bvh or_bvh;
+or_bvh *or_bvh_create(const or_scene *s);
+void    or_bvh_destroy(or_bvh *b);
+/* queries through the tree (whatever s->bvh holds) */
+int     or_bvh_closest(const or_scene *s, const or_bvh *b, const float *p, float *out_d2);
 
 #ifdef __cplusplus
 }

@@ -13,10 +13,15 @@ def _t(a):
     return torch.from_numpy(np.ascontiguousarray(a)).cuda()
 
 
+# 2D items with and without a Fourier source (odd items carry one) and 3D items
+PICK = [0, 1, 5, 8, 13, 16, 21, 24, 29, 32, 36, 40, 44, 48, 52, 56, 60]
+
+
 def _build(scale):
     import wos_ext
     items = []
-    for c in C.config5(scale=scale, seed=3)[::4]:
+    cfg = C.config5(scale=scale, seed=3)
+    for c in (cfg[j] for j in PICK):
         if c["dim"] == 2:
             kw = dict(source_type=3, source_params=_t(c["source_params"])) if "source_type" in c else {}
             sc = wos_ext.Scene(_t(c["vertices"]), _t(c["indices"]), _t(c["values"]), use_bvh=True, **kw)

@@ -353,3 +353,29 @@ def test_shape_sample_gradient_3d_linear_field():
     m = sg[ok].mean(0)
     se = sg[ok].std(0) / np.sqrt(ok.sum())
     assert (np.abs(m - gvec) < 4 * se + 0.02).all(), (m, se)
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_cpu_lbvh_equals_bruteforce(dim):
+    """The oracle's CPU LBVH (the CPU baseline's query) returns the scan's
+    primitive and squared distance everywhere, ties and far points included,
+    so walks through it are identical."""
+    rng = np.random.default_rng(dim + 10)
+    if dim == 2:
+        v, ix = C.star_polyline(2048, a=0.25, k=5)
+        p = rng.uniform(-1.6, 1.6, size=(20000, 2)).astype(np.float32)
+        p[:8] = 0.0   # equidistant from many segments
+    else:
+        v, ix, _ = C.displaced_sphere(3)
+        p = rng.uniform(-1.5, 1.5, size=(8000, 3)).astype(np.float32)
+        p[:8] = 0.0
+    vals = np.zeros(v.shape[0], np.float32)
+    a, b = O.Scene(v, ix, vals), O.Scene(v, ix, vals, use_bvh=True)
+    i1, d1 = a.closest_index(p)
+    i2, d2 = b.closest_index(p)
+    np.testing.assert_array_equal(i1, i2)
+    np.testing.assert_array_equal(d1, d2)
+    q = p[np.abs(a.sdf(p)) < 0.9][:400] * 0.9
+    r1 = a.walk_reduce(q, 8, 256, 1e-3, seed=3, walk_values=True)
+    r2 = b.walk_reduce(q, 8, 256, 1e-3, seed=3, walk_values=True)
+    np.testing.assert_array_equal(r1["walk_value"], r2["walk_value"])
