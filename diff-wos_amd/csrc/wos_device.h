@@ -779,9 +779,17 @@ __device__ __forceinline__ float signed_dist(const SceneView &s, int f, const fl
     }
 }
 
-// include/scene.cuh:180-223
+// include/scene.cuh:180-223.  side = 0: the normal points toward p's side of
+// the primitive (the sign of n.(p - a)), as the reference's closestPoint
+// computes it.  side = +-1: the normal points to the side of the boundary
+// where the sdf has that sign (the face orientation the sdf uses: (b-a).y,
+// -(b-a).x in 2D, (b-a) x (c-a) in 3D) -- the record of a walk that ends on
+// the boundary, whose final point can lie within float rounding of the
+// primitive (walk exits concentrate at depth 0), where the sign test is
+// arbitrary; a walk never leaves the side it started on.
 template <int D>
-__device__ __forceinline__ void closest_record(const SceneView &s, int f, const float *p, Rec &r) {
+__device__ __forceinline__ void closest_record(const SceneView &s, int f, const float *p, Rec &r,
+                                               float side = 0.0f) {
     r.valid = 1;
     r.prim = f;
     if constexpr (D == 2) {
@@ -794,7 +802,7 @@ __device__ __forceinline__ void closest_record(const SceneView &s, int f, const 
         float pax = p[0] - ax, pay = p[1] - ay;
         float l = sqrtf(dot2(bay, -bax, bay, -bax));
         float nx = bay / l, ny = -bax / l;
-        float sg = sgn(dot2(nx, ny, pax, pay));
+        float sg = side != 0.0f ? side : sgn(dot2(nx, ny, pax, pay));
         r.p[0] = fmaf(h, bax, ax);
         r.p[1] = fmaf(h, bay, ay);
         r.p[2] = 0.0f;
@@ -818,7 +826,7 @@ __device__ __forceinline__ void closest_record(const SceneView &s, int f, const 
         float l = sqrtf(dot3(n, n));
         n[0] = n[0] / l; n[1] = n[1] / l; n[2] = n[2] / l;
         float pa[3] = {p[0] - a[0], p[1] - a[1], p[2] - a[2]};
-        float sg = sgn(dot3(n, pa));
+        float sg = side != 0.0f ? side : sgn(dot3(n, pa));
         for (int k = 0; k < 3; ++k) { r.p[k] = q[k]; r.n[k] = sg * n[k]; }
         r.uv[0] = bary[1];
         r.uv[1] = 1.0f - bary[0] - bary[1];

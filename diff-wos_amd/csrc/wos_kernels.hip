@@ -256,7 +256,7 @@ __device__ void walk_one(const SceneView &s, const WalkSettings &w, const float 
     const bool src = (D == 2) && s.source_type != 0;
     for (int i = 0; i < w.nsteps; ++i) {
         if (d < w.epsilon) {
-            closest_record<D>(s, prim, p, rec);
+            closest_record<D>(s, prim, p, rec, sgn(sd));   // normal toward the walk's side
             return;
         }
         if constexpr (PGRAD) {
@@ -875,6 +875,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6))) voi
     float dr[3] = {0.0f, 0.0f, 0.0f};
     int lit = 0, side = 0;
     bool start_known = false, done_ok = false;
+    float wside = -1.0f;   // sign of the sdf at the sample's start point
     long long c_next = 0, c_end = 0;
     bool exhausted = false;
 
@@ -894,7 +895,9 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6))) voi
                 const float eps = phase == kPrimary ? g.epsilon : g.epsilon2;
                 if (d < eps && step < g.nsteps) {
                     Rec r;
-                    closest_record<D>(s, prim, p, r);
+                    // the primary exit's normal points to the side the walk
+                    // started on (its sign test is arbitrary at depth ~0)
+                    closest_record<D>(s, prim, p, r, wside);
                     if (phase == kPrimary) {
                         for (int c = 0; c < 3; ++c) { xb[c] = r.p[c]; nb[c] = r.n[c]; }
                         rprim = prim;
@@ -1103,6 +1106,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6))) voi
                         d = fabsf(sd);
                         step = 0;
                         start_known = true;   // the first test uses d = |sd| as is
+                        wside = sgn(sd);
                         mode = kReady;
                     }
                 }

@@ -382,7 +382,13 @@ float or_sdf(const or_scene *s, const float *p) {
     }
 }
 
-static void record_from_prim(const or_scene *s, int f, const float *p, or_record *rec) {
+/* include/scene.cuh:180-223.  side = 0: n toward p's side of the primitive
+ * (sign of n.(p - a)), as closestPoint computes it; side = +-1: n toward the
+ * side where the sdf has that sign (face orientation (b-a).y, -(b-a).x in 2D,
+ * (b-a) x (c-a) in 3D) -- the record that ends a walk, whose final point may
+ * lie within float rounding of the primitive, where the sign test is
+ * arbitrary (a walk stays on the side it started on). */
+static void record_from_prim_side(const or_scene *s, int f, const float *p, or_record *rec, float side) {
     memset(rec, 0, sizeof(*rec));
     rec->valid = 1;
     rec->prim_id = f;
@@ -395,7 +401,7 @@ static void record_from_prim(const or_scene *s, int f, const float *p, or_record
         float pax = p[0] - a[0], pay = p[1] - a[1];
         float l = sqrtf(dot2(bay, -bax, bay, -bax));
         float nx = bay / l, ny = -bax / l;
-        float sg = signf_(dot2(nx, ny, pax, pay));
+        float sg = side != 0.0f ? side : signf_(dot2(nx, ny, pax, pay));
         rec->p[0] = fmaf(h, bax, a[0]);
         rec->p[1] = fmaf(h, bay, a[1]);
         rec->n[0] = sg * nx;
@@ -414,12 +420,16 @@ static void record_from_prim(const or_scene *s, int f, const float *p, or_record
         float l = sqrtf(dot3(n, n));
         n[0] /= l; n[1] /= l; n[2] /= l;
         float pa[3] = {p[0] - a[0], p[1] - a[1], p[2] - a[2]};
-        float sg = signf_(dot3(n, pa));
+        float sg = side != 0.0f ? side : signf_(dot3(n, pa));
         for (int k = 0; k < 3; ++k) { rec->p[k] = pt[k]; rec->n[k] = sg * n[k]; }
         rec->uv[0] = bary[1];
         rec->uv[1] = 1.0f - bary[0] - bary[1];
         rec->d = sqrtf(d2);
     }
+}
+
+static void record_from_prim(const or_scene *s, int f, const float *p, or_record *rec) {
+    record_from_prim_side(s, f, p, rec, 0.0f);
 }
 
 /* include/scene.cuh:180-223 */
@@ -574,13 +584,14 @@ void or_walk(const or_scene *s, const or_walk_params *w, const float *p0, or_pcg
     rec->prim_id = -1;
     float d = or_sdf(s, p);
     if (d >= 0.0f && !w->double_sided) return;
+    const float side = signf_(d);   /* the exit record's normal points to this side */
     d = fabsf(d);
     if (w->prevent_fd_artifacts) d *= 0.5f;
     int has_source = (D == 2) && s->source_type != 0;
     for (int i = 0; i < w->nsteps; ++i) {
         if (d < w->epsilon) {
             int f = or_closest_index(s, p, NULL);
-            record_from_prim(s, f, p, rec);
+            record_from_prim_side(s, f, p, rec, side);
             rec->contrib = contrib;
             return;
         }

@@ -37,7 +37,8 @@ def test_shape_gradient_samples_match_oracle(dim):
     gv_ref, sg_ref = ref.shape_gradient(p, W, 128, 1e-3, 1e-3, 0.1, seed=3, grad_out=g, sample_grad_u=True)
     sg = sg.cpu().numpy()
     close = np.all(np.abs(sg - sg_ref) <= 1e-3 * (1 + np.abs(sg_ref)), axis=1)
-    assert close.mean() > 0.97, close.mean()
+    print(f"{dim}D gradient samples equal to the oracle's: {close.mean():.5f} of {close.size}")
+    assert close.mean() > 0.99, close.mean()
     gv = gv.cpu().numpy()
     scale = np.abs(gv_ref).max()
     assert np.abs(gv - gv_ref).max() < 0.1 * scale
@@ -82,8 +83,13 @@ def test_autograd_values_and_source_coefficients():
 
 
 def test_custom_solver_shape_gradient_directional_fd():
-    """d L / d s for a uniform scaling s of the boundary: adjoint (OursCUDA)
-    against a central difference of the primal under common seeds."""
+    """d L / d s for a uniform scaling s of the boundary: adjoint (OursCUDA
+    through CustomSolver) against the closed form -4 mean g^2 (g = x^2 - y^2
+    is harmonic on the plane, so u_s(x) = g(x / s)) and a central difference
+    of the primal under common seeds.  CustomSolver replays the forward seed
+    in the backward pass (wos/solver.py:105-110), so its gradient also
+    carries Cov(u, du)/W, a term that falls as 1/W: 2,048 walks keep it
+    near 1%."""
     from wos.scene import Polyline
     from wos.solver import CustomSolver
     from wos.wos import WoSCUDA
@@ -106,13 +112,16 @@ def test_custom_solver_shape_gradient_directional_fd():
             return L.item(), sv.grad.item()
         return L.item()
 
-    fwd = WoSCUDA(nwalks=256, nsteps=256, epsilon=1e-4)
-    bwd = OursCUDA(nwalks=256, nsteps=256, epsilon=1e-4, epsilon2=1e-4, clamping=0.1)
+    fwd = WoSCUDA(nwalks=2048, nsteps=256, epsilon=1e-4)
+    bwd = OursCUDA(nwalks=2048, nsteps=256, epsilon=1e-4, epsilon2=1e-4, clamping=0.1)
     _, dL = loss_at(1.0, CustomSolver(fwd, bwd), grad=True)
     h = 2e-2
     fd = (loss_at(1 + h, fwd) - loss_at(1 - h, fwd)) / (2 * h)
+    exact = -4.0 * float(np.mean(C.g_saddle(np.stack([r * np.cos(t), r * np.sin(t)], 1)) ** 2))
+    print(f"CustomSolver scaling: adjoint {dL:+.5e}, closed form {exact:+.5e}, central difference {fd:+.5e}")
     assert np.isfinite(dL)
-    assert abs(dL - fd) < 0.1 * abs(fd) + 1e-3, (dL, fd)
+    assert abs(dL - exact) < 0.05 * abs(exact), (dL, exact)
+    assert abs(fd - exact) < 0.03 * abs(exact), (fd, exact)
 
 
 def test_ours3d_directional_fd_on_64_vertices():

@@ -56,7 +56,7 @@ def test_source_walks_match_oracle_per_walk(stype, use_is):
     gv = out["walk_value"].cpu().numpy()
     same = np.abs(gv - r["walk_value"]) <= 1e-5 * (1 + np.abs(r["walk_value"]))
     print(f"source {stype} is={use_is}: {same.mean():.5f} of {gv.size} walks agree with the oracle")
-    assert same.mean() > 0.995
+    assert same.all()
     m, v = out["mean64"].cpu().numpy(), out["var64"].cpu().numpy()
     se = np.sqrt((v + r["var"]) / W) + 1e-9
     assert np.mean(np.abs(m - r["mean"]) / se > 3) < 0.01
@@ -117,7 +117,7 @@ def test_single_walk_3d_matches_oracle_per_walk():
     r = ref.single_walk(p, st0, inc0, nsteps=256, epsilon=1e-3)
     same = rec.prim_id.cpu().numpy() == r["prim"]
     print(f"3D single walks: {same.mean():.5f} of {n} exit on the oracle's triangle")
-    assert same.mean() > 0.995
+    assert same.all()
     np.testing.assert_array_equal(smp.state.cpu().numpy().view(np.uint64)[same], r["state"][same])
     np.testing.assert_allclose(rec.p.cpu().numpy()[same], r["p"][same], atol=1e-5)
     np.testing.assert_allclose(rec.uv.cpu().numpy()[same], r["t"][same], atol=1e-5)

@@ -38,8 +38,9 @@ def test_single_walk_matches_oracle_per_walk(source, use_is):
     rec = w.single_walk(_t(p), sc, smp)
     r = ref.single_walk(p, st0, inc0, nsteps=128, epsilon=1e-3, use_is=use_is)
     same = (rec.prim_id.cpu().numpy() == r["prim"])
+    # identical float expressions in the same order on both sides: every walk agrees
     print(f"per-walk exit agreement: {same.mean():.5f} of {n} walks")
-    assert same.mean() > 0.995
+    assert same.all()
     np.testing.assert_array_equal(smp.state.cpu().numpy().view(np.uint64)[same], r["state"][same])
     np.testing.assert_allclose(rec.p.cpu().numpy()[same], r["p"][same], atol=1e-5)
     if source:
