@@ -145,6 +145,8 @@ int launch_single_walk(const SceneView &s, const WalkSettings &w, const float *p
                        float *ocontrib, hipStream_t st);
 int launch_walk_reduce(const SceneView &s, const WalkSettings &w, const float *points, int n_points,
                        int32_t seed, int chunk, const WalkOutputs &o, hipStream_t st);
+int launch_walk_reduce_noalloc(const SceneView &s, const WalkSettings &w, const float *points, int n_points,
+                               int32_t seed, int chunk, const WalkOutputs &o, hipStream_t st);
 int launch_adjoint_values(const SceneView &s, int n_points, int nwalks, const int *rec_prim, const float *rec_t,
                           const float *basis, const float *grad_out, float *grad_values, float *grad_coeffs,
                           hipStream_t st);

@@ -555,6 +555,20 @@ int wos_walk_reduce(const wos_scene *s, const wos_walk_params *w, const float *p
     return launch_walk_reduce(view_of(s), ws, points, n_points, (int32_t)seed, chunk, o, S(stream));
 }
 
+int wos_walk_reduce_noalloc(const wos_scene *s, const wos_walk_params *w, const float *points, int n_points,
+                            int seed, int chunk, float *mean, double *mean64, double *var64, float *walk_value,
+                            int *rec_prim, float *rec_t, float *basis, void *stream) {
+    REQUIRE(s && (n_points == 0 || points), "scene and points are required");
+    REQUIRE(n_points >= 0, "negative count");
+    if (check_walk(w)) return -1;
+    REQUIRE(!(rec_prim && !rec_t) && !(rec_t && !rec_prim), "rec_prim and rec_t go together");
+    if (basis) REQUIRE(s->source_type == 3, "basis output needs a Fourier source (source_type 3)");
+    WalkSettings ws{w->nwalks, w->nsteps, w->epsilon, w->double_sided, w->prevent_fd_artifacts,
+                    w->use_IS_for_greens ? 1 : 0};
+    WalkOutputs o{walk_value, rec_prim, rec_t, basis, mean, mean64, var64};
+    return launch_walk_reduce_noalloc(view_of(s), ws, points, n_points, (int32_t)seed, chunk, o, S(stream));
+}
+
 int wos_walk_reduce_counts(const wos_scene *s, const wos_walk_params *w, const float *points, int n_points,
                            int seed, int with_basis, float *walk_value, unsigned long long *counts, void *stream) {
     REQUIRE(s && counts && (n_points == 0 || (points && walk_value)), "scene, points, walk_value and counts are required");

@@ -1315,48 +1315,70 @@ static bool walk_count_fits(long long total) {
     return false;
 }
 
-int launch_walk_reduce(const SceneView &s, const WalkSettings &w, const float *points, int n_points, int32_t seed,
-                       int chunk, const WalkOutputs &o, hipStream_t st) {
-    if (n_points <= 0) return 0;
-    long long total = (long long)n_points * w.nwalks;
-    if (!walk_count_fits(total)) return -1;
-    if (w.use_is && s.source_type != 0 && ensure_green_cdf() != 0) return -1;
-    int K = (s.source_type == 3) ? s.n_src / 4 : 0;
-    if (o.basis && K > kMaxBasis) {
-        set_error("Fourier source has %d terms; at most %d are supported", K, kMaxBasis);
+// Buffers of one walk_reduce launch beyond the caller's inputs and outputs.
+struct WalkScratch {
+    unsigned long long *counter = nullptr;   // the work queue's head, zeroed by the launch
+    int *init_prim = nullptr;                // n_points start records
+    float *init_sd = nullptr;
+    const int *order = nullptr;              // visiting order of the points (nullable: input order)
+    int2 *spill = nullptr;                   // traversal stack beyond the LDS part
+    long spill_stride = 0;
+    int grid = 1, grid_init = 1;             // persistent walk grid, start-query grid
+};
+
+// Checks shared by both walk_reduce launchers; *K: Fourier terms.
+static int walk_reduce_check(const SceneView &s, const WalkSettings &w, int n_points, const WalkOutputs &o, int *K) {
+    if (!walk_count_fits((long long)n_points * w.nwalks)) return -1;
+    *K = (s.source_type == 3) ? s.n_src / 4 : 0;
+    if (o.basis && *K > kMaxBasis) {
+        set_error("Fourier source has %d terms; at most %d are supported", *K, kMaxBasis);
         return -1;
     }
+    if ((o.mean || o.mean64 || o.var64) && !o.walk_value) {
+        set_error("per-point statistics need the per-walk value buffer");
+        return -1;
+    }
+    if (w.use_is && s.source_type != 0 && ensure_green_cdf() != 0) return -1;
+    return 0;
+}
+
+// Grids of the full-occupancy launch: one wave per 64 walks up to eight
+// workgroups per CU for the walks, one lane per point for the start queries.
+static void walk_reduce_grids(int n_points, int nwalks, int *g, int *gi) {
+    const long long total = (long long)n_points * nwalks;
+    long waves_needed = (long)((total + 63) / 64);
+    int cap = occupancy_grid();
+    long gg = (waves_needed + 3) / 4;
+    *g = (int)std::max(1L, std::min(gg, (long)cap));
+    *gi = grid_of(n_points, cap);
+}
+
+static int walk_reduce_launch(const SceneView &s, const WalkSettings &w, const float *points, int n_points,
+                              int32_t seed, int chunk, int K, const WalkOutputs &o, const WalkScratch &ws,
+                              hipStream_t st) {
     PersistArgs a;
     a.points = points;
     a.n_points = n_points;
     a.w = w;
     a.seed = seed;
-    a.total = total;
-    a.counter = launch_counter(st);
-    if (!a.counter) return -1;
+    a.total = (long long)n_points * w.nwalks;
+    a.counter = ws.counter;
     a.chunk = chunk > 0 ? chunk : (g_tuning.chunk > 0 ? g_tuning.chunk : 256);
     a.window = window_for(s.dim);
+    a.init_prim = ws.init_prim;
+    a.init_sd = ws.init_sd;
+    a.order = ws.order;
     a.walk_value = o.walk_value;
     a.rec_prim = o.rec_prim;
     a.rec_t = o.rec_t;
     a.basis = o.basis;
     a.K = K;
-    long waves_needed = (long)((total + 63) / 64);
-    int g = (int)((waves_needed + 3) / 4);
-    int cap = occupancy_grid();
-    if (g > cap) g = cap;
-    if (g < 1) g = 1;
-    int gi = grid_of(n_points, cap);
-    long lanes = (long)std::max(g, gi) * kBlock;
-    if (spill_buffer(s, lanes, &a.spill_stride, &a.spill, st) != 0) return -1;
-    StartRecords sr(st);
-    if (sr.alloc(s, points, n_points) != 0) return -1;
-    a.init_prim = sr.prim;
-    a.init_sd = sr.sd;
-    a.order = sr.order;
+    a.spill = ws.spill;
+    a.spill_stride = ws.spill_stride;
+    const int g = ws.grid, gi = ws.grid_init;
     size_t tb = top_bytes(s);
-    if (s.dim == 2) k_point_init<2><<<gi, kBlock, tb, st>>>(s, points, n_points, sr.order, sr.prim, sr.sd, a.spill, a.spill_stride);
-    else k_point_init<3><<<gi, kBlock, tb, st>>>(s, points, n_points, nullptr, sr.prim, sr.sd, a.spill, a.spill_stride);
+    if (s.dim == 2) k_point_init<2><<<gi, kBlock, tb, st>>>(s, points, n_points, ws.order, ws.init_prim, ws.init_sd, a.spill, a.spill_stride);
+    else k_point_init<3><<<gi, kBlock, tb, st>>>(s, points, n_points, nullptr, ws.init_prim, ws.init_sd, a.spill, a.spill_stride);
     k_zero_u64<<<1, 1, 0, st>>>(a.counter);
     bool src = s.dim == 2 && s.source_type != 0;
     bool bas = src && o.basis != nullptr && K > 0;
@@ -1378,10 +1400,6 @@ int launch_walk_reduce(const SceneView &s, const WalkSettings &w, const float *p
     }
     KCHK();
     if (o.mean || o.mean64 || o.var64) {
-        if (!o.walk_value) {
-            set_error("per-point statistics need the per-walk value buffer");
-            return -1;
-        }
         long waves = n_points;
         int gr = (int)((waves * 64 + kBlock - 1) / kBlock);
         if (gr > 8192) gr = 8192;
@@ -1389,6 +1407,101 @@ int launch_walk_reduce(const SceneView &s, const WalkSettings &w, const float *p
         KCHK();
     }
     return 0;
+}
+
+int launch_walk_reduce(const SceneView &s, const WalkSettings &w, const float *points, int n_points, int32_t seed,
+                       int chunk, const WalkOutputs &o, hipStream_t st) {
+    if (n_points <= 0) return 0;
+    int K = 0;
+    if (walk_reduce_check(s, w, n_points, o, &K) != 0) return -1;
+    WalkScratch ws;
+    ws.counter = launch_counter(st);
+    if (!ws.counter) return -1;
+    walk_reduce_grids(n_points, w.nwalks, &ws.grid, &ws.grid_init);
+    long lanes = (long)std::max(ws.grid, ws.grid_init) * kBlock;
+    if (spill_buffer(s, lanes, &ws.spill_stride, &ws.spill, st) != 0) return -1;
+    StartRecords sr(st);
+    if (sr.alloc(s, points, n_points) != 0) return -1;
+    ws.init_prim = sr.prim;
+    ws.init_sd = sr.sd;
+    ws.order = sr.order;
+    return walk_reduce_launch(s, w, points, n_points, seed, chunk, K, o, ws, st);
+}
+
+// ---- allocation-free walk_reduce -----------------------------------------
+// For callers that cannot allocate device memory: the queue head, the start
+// records and the traversal spill that launch_walk_reduce allocates are cut
+// from one file-scope device array instead.  The points keep their input
+// order (the Morton sort needs buffers of its own); that changes which lanes
+// walk which points, never a walk's value.  Every call on a device shares
+// that device's array, so calls that overlap on one device (other streams,
+// other host threads) race; calls on one stream run in order.
+constexpr size_t kScratchBytes = size_t(64) << 20;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+// Device address of `scratch` on the current device, looked up on the first
+// call there (each device holds its own copy of the array).
+static int scratch_address(unsigned char **out) {
+    static std::mutex lock;
+    static unsigned char *addr[64] = {nullptr};
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) {
+        (void)hipGetLastError();
+        set_error("cannot query the current device");
+        return -1;
+    }
+    std::lock_guard<std::mutex> g(lock);
+    if (!addr[dev]) {
+        void *ptr = nullptr;
+        hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr) {
+            (void)hipGetLastError();
+            set_error("cannot find the scratch array on device %d: %s", dev, hipGetErrorString(e));
+            return -1;
+        }
+        addr[dev] = (unsigned char *)ptr;
+    }
+    *out = addr[dev];
+    return 0;
+}
+
+static size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
+
+int launch_walk_reduce_noalloc(const SceneView &s, const WalkSettings &w, const float *points, int n_points,
+                               int32_t seed, int chunk, const WalkOutputs &o, hipStream_t st) {
+    if (n_points <= 0) return 0;
+    int K = 0;
+    if (walk_reduce_check(s, w, n_points, o, &K) != 0) return -1;
+    // layout: [queue head | start primitives | start distances | spill]
+    const size_t rec = align256(sizeof(int) * (size_t)n_points);
+    const size_t fixed = 256 + 2 * rec;
+    const long deep = s.use_bvh ? std::max(0L, (long)s.max_depth + 2 - kLdsStack) : 0L;
+    const size_t per_wg = (size_t)deep * sizeof(int2) * kBlock;   // spill of one workgroup
+    if (fixed + per_wg > kScratchBytes) {
+        long long most = (long long)((kScratchBytes - 256 - per_wg) / (2 * sizeof(int))) - 64;
+        set_error("%d points need %zu bytes of scratch; the allocation-free walk_reduce has %zu "
+                  "(at most about %lld points in this scene)", n_points, fixed + per_wg, kScratchBytes,
+                  std::max(0LL, most));
+        return -1;
+    }
+    unsigned char *base = nullptr;
+    if (scratch_address(&base) != 0) return -1;
+    WalkScratch ws;
+    walk_reduce_grids(n_points, w.nwalks, &ws.grid, &ws.grid_init);
+    if (per_wg > 0) {
+        // fewer workgroups when the spill of a full grid does not fit (the
+        // walk kernel is persistent and the start queries grid-stride)
+        const long fit = (long)((kScratchBytes - fixed) / per_wg);
+        ws.grid = (int)std::min((long)ws.grid, fit);
+        ws.grid_init = (int)std::min((long)ws.grid_init, fit);
+    }
+    ws.counter = (unsigned long long *)base;
+    ws.init_prim = (int *)(base + 256);
+    ws.init_sd = (float *)(base + 256 + rec);
+    ws.order = nullptr;
+    ws.spill_stride = (long)std::max(ws.grid, ws.grid_init) * kBlock;
+    ws.spill = per_wg > 0 ? (int2 *)(base + fixed) : nullptr;
+    return walk_reduce_launch(s, w, points, n_points, seed, chunk, K, o, ws, st);
 }
 
 int launch_adjoint_values(const SceneView &s, int n_points, int nwalks, const int *rec_prim, const float *rec_t,

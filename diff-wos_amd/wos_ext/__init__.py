@@ -87,6 +87,8 @@ _sig("wos_largest_inscribed_ball", _vp, ctypes.POINTER(_Record), _i, _f, _vp, _v
 _sig("wos_dirichlet", _vp, ctypes.POINTER(_Record), _i, _vp, _vp)
 _sig("wos_single_walk", _vp, ctypes.POINTER(_WalkParams), _vp, _i, _vp, _vp, ctypes.POINTER(_Record), _vp)
 _sig("wos_walk_reduce", _vp, ctypes.POINTER(_WalkParams), _vp, _i, _i, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp)
+_sig("wos_walk_reduce_noalloc", _vp, ctypes.POINTER(_WalkParams), _vp, _i, _i, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
+     _vp)
 _sig("wos_walk_reduce_counts", _vp, ctypes.POINTER(_WalkParams), _vp, _i, _i, _i, _vp, _vp, _vp)
 class _BatchItem(ctypes.Structure):
     _fields_ = [("scene", _vp), ("params", _WalkParams), ("points", _vp), ("n_points", _i), ("seed", _i),
@@ -116,7 +118,8 @@ EXPORTED = ("wos_last_error", "wos_scene_create", "wos_scene_destroy", "wos_scen
             "wos_walk_reduce_counts", "wos_adjoint_values", "wos_shape_gradient", "wos_shape_derivative",
             "wos_set_tuning",
             "wos_get_tuning", "wos_set_spatial_order", "wos_batch_create", "wos_batch_run", "wos_batch_adjoint",
-            "wos_batch_destroy", "wos_adjoint_source", "wos_debug_fail_workspace", "wos_shape_gradient_counts")
+            "wos_batch_destroy", "wos_adjoint_source", "wos_debug_fail_workspace", "wos_shape_gradient_counts",
+            "wos_walk_reduce_noalloc")
 
 
 def set_tuning(chunk=None, wgs_per_cu=None, top_levels=None, window=None):
@@ -528,11 +531,14 @@ class WoS:
                                    ctypes.byref(rec), _stream()))
         return _finish_record(out, valid)
 
-    def walk_reduce(self, p, scene, seed=0, *, nwalks=None, records=False, basis=False, stats=False, chunk=0):
+    def walk_reduce(self, p, scene, seed=0, *, nwalks=None, records=False, basis=False, stats=False, chunk=0,
+                    noalloc=False):
         """Solver.walk_reduce (wos/solver.py:49-65) over this walk, fused into
         one persistent launch.  Returns a dict with ``mean`` (float32) and, on
         request, ``mean64``/``var64`` (stats), ``rec_prim``/``rec_t``
-        (records), ``basis`` and ``walk_value``."""
+        (records), ``basis`` and ``walk_value``.  ``noalloc`` runs the launch
+        through wos_walk_reduce_noalloc, whose scratch is a static per-GPU
+        array (same results; the library allocates nothing)."""
         q = _dev(p, torch.float32, scene.dim)
         P = q.shape[0]
         W = self.nwalks if nwalks is None else int(nwalks)
@@ -552,9 +558,9 @@ class WoS:
             out["basis"] = torch.empty(K * P * W, dtype=torch.float32, device=dv)
         wp = self._params(W)
         g = out.get
-        _check(lib.wos_walk_reduce(scene.handle, ctypes.byref(wp), _p(q), P, int(seed), int(chunk), _p(g("mean")),
-                                   _p(g("mean64")), _p(g("var64")), _p(g("walk_value")), _p(g("rec_prim")),
-                                   _p(g("rec_t")), _p(g("basis")), _stream()))
+        fn = lib.wos_walk_reduce_noalloc if noalloc else lib.wos_walk_reduce
+        _check(fn(scene.handle, ctypes.byref(wp), _p(q), P, int(seed), int(chunk), _p(g("mean")), _p(g("mean64")),
+                  _p(g("var64")), _p(g("walk_value")), _p(g("rec_prim")), _p(g("rec_t")), _p(g("basis")), _stream()))
         return out
 
     def walk_counts(self, p, scene, seed=0, *, nwalks=None, basis=True):

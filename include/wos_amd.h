@@ -171,6 +171,21 @@ int wos_walk_reduce(const wos_scene *scene, const wos_walk_params *params, const
                     int seed, int chunk, float *mean, double *mean64, double *var64, float *walk_value,
                     int *rec_prim, float *rec_t, float *basis, void *stream);
 
+/* wos_walk_reduce for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * No counterpart in the reference.  Same arguments, checks and per-walk
+ * results as wos_walk_reduce; the queue head, start records and traversal
+ * spill live in one static 64 MiB device array per GPU (held until the
+ * process ends) instead of library allocations, so the call allocates
+ * nothing.  Limit: 8 bytes per point plus one workgroup's traversal spill
+ * must fit the array (about 8 million points); larger calls fail with an
+ * error.  The points are visited in input order.  Every call on a GPU
+ * shares that GPU's array: calls that overlap on one GPU (other streams,
+ * other threads) race; calls on one stream run in order. */
+int wos_walk_reduce_noalloc(const wos_scene *scene, const wos_walk_params *params, const float *points,
+                            int n_points, int seed, int chunk, float *mean, double *mean64, double *var64,
+                            float *walk_value, int *rec_prim, float *rec_t, float *basis, void *stream);
+
 /* Instrumented walk_reduce for measurement: same launch as wos_walk_reduce
  * (with the Fourier basis outputs when with_basis), and adds into counts
  * (4 device uint64, zeroed by the caller): AABB tests, primitive tests,
