@@ -43,6 +43,7 @@ sys.path.insert(0, os.path.join(ROOT, "diff-wos_amd"))
 PEAK_FP32 = 157.3e12      # MI355X FP32 vector, FLOP/s
 PEAK_HBM = 8.0e12         # bytes/s
 CPU_THREADS = 16          # a one-GPU job's share of the host's cores
+METRIC = "primal+grad walks/s at 1 GPU, batched vs plain, equal RMSE; % VALU/HBM peak"
 
 # algorithmic flop model (FMA = 2): per bound test, per primitive test, per
 # walk step (direction, move, bound), per source sample and Fourier term,
@@ -154,6 +155,35 @@ def plain_step(torch, scene_bf, cwos, pts, target, seed, W):
     return u, gv
 
 
+def fd_values_and_coefficients(step, wos_ext, t, cwos, V, F, G, SP, pts, target):
+    """The value / coefficient adjoint against central differences (h = 1e-3, the walks of the adjoint's
+    seed on both sides): a random direction in the vertex values and one in the source coefficients."""
+    gvals, gcoef = step(0, timed=False)
+    rng = np.random.default_rng(3)
+    dvals = t(rng.normal(size=G.shape[0]).astype(np.float32))
+    dcoef = t(rng.normal(size=SP.shape[0]).astype(np.float32))
+    h = 1e-3
+    t64 = target.double()
+    loss = lambda sc: float(((cwos.walk_reduce(pts, sc, seed=100, stats=True)["mean64"] - t64) ** 2).mean())
+    fd = {}
+    for name, dirn, grad in (("values", dvals, gvals), ("source_coefficients", dcoef, gcoef)):
+        diff = 0.0
+        for sgn in (1, -1):
+            if name == "values":
+                sc = wos_ext.Scene(V, F, G + sgn * h * dirn, use_bvh=True, source_type=3, source_params=SP)
+            else:
+                sp2 = SP.clone()
+                sp2[:, 3] += sgn * h * dirn
+                sc = wos_ext.Scene(V, F, G, use_bvh=True, source_type=3, source_params=sp2)
+            diff += sgn * loss(sc)
+        cd = diff / (2 * h)
+        a = float((grad.double() * dirn.double()).sum())
+        fd[name] = {"adjoint": a, "central_difference": cd, "relative_error": abs(a - cd) / abs(cd)}
+    log(f"fd check (h = 1e-3): values {fd['values']['relative_error']:.2e}, coefficients "
+        f"{fd['source_coefficients']['relative_error']:.2e}")
+    return fd
+
+
 # ---------------------------------------------------------------------------
 # default: the 2D Poisson clover, primal + value / coefficient adjoint
 # ---------------------------------------------------------------------------
@@ -190,6 +220,12 @@ def clover_workload(args, torch, wos_ext, C):
     value = walks / (ms_per_step * 1e-3)
     log(f"fused: {ms_per_step:.2f} ms/step, {value:.3e} walks/s; walk launch {kernel_ms:.2f} ms")
 
+    # the adjoint against central differences (h = 1e-3, the walks of the adjoint's seed on both sides):
+    # a random direction in the 4,096 vertex values and one in the 8 source coefficients
+    fd = None
+    if not args.no_fd:
+        fd = fd_values_and_coefficients(step, wos_ext, t, cwos, V, F, G, SP, pts, target)
+
     rf = None
     if not args.no_counts:
         counts = cwos.walk_counts(pts, scene, seed=100)
@@ -206,14 +242,16 @@ def clover_workload(args, torch, wos_ext, C):
     o = cwos.walk_reduce(pts, tw, seed=7, stats=True)
     m_fused = o["mean64"].cpu().numpy()
     rmse = float(np.sqrt(np.mean((m_fused - ct["exact"]) ** 2)))
-    cis = wos_ext.WoS(nwalks=W, nsteps=c["nsteps"], epsilon=c["epsilon"], use_IS_for_greens=True)
-    step(0, walker=cis, timed=False)
-    ms_is = synced_ms(torch, lambda i: step(i, walker=cis, timed=False), max(2, args.steps // 2))
-    o_is = cis.walk_reduce(pts, tw, seed=7, stats=True)
-    rmse_is = float(np.sqrt(np.mean((o_is["mean64"].cpu().numpy() - ct["exact"]) ** 2)))
-    is_variant = {"ms_per_step": ms_is, "walks_per_s": walks / (ms_is * 1e-3), "rmse_twin": rmse_is,
-                  "interior_sampling": "tabulated Green's-function CDF"}
-    log(f"importance-sampled interior: {ms_is:.2f} ms/step, rmse {rmse_is:.4f} (uniform {rmse:.4f})")
+    is_variant = None
+    if not args.no_variants:
+        cis = wos_ext.WoS(nwalks=W, nsteps=c["nsteps"], epsilon=c["epsilon"], use_IS_for_greens=True)
+        step(0, walker=cis, timed=False)
+        ms_is = synced_ms(torch, lambda i: step(i, walker=cis, timed=False), max(2, args.steps // 2))
+        o_is = cis.walk_reduce(pts, tw, seed=7, stats=True)
+        rmse_is = float(np.sqrt(np.mean((o_is["mean64"].cpu().numpy() - ct["exact"]) ** 2)))
+        is_variant = {"ms_per_step": ms_is, "walks_per_s": walks / (ms_is * 1e-3), "rmse_twin": rmse_is,
+                      "interior_sampling": "tabulated Green's-function CDF"}
+        log(f"importance-sampled interior: {ms_is:.2f} ms/step, rmse {rmse_is:.4f} (uniform {rmse:.4f})")
 
     plain = None
     if not args.no_plain:
@@ -226,6 +264,13 @@ def clover_workload(args, torch, wos_ext, C):
         plain_step(torch, scene_bf, cwos, psub, tsub, 1, W)
         torch.cuda.synchronize()
         tp = time.perf_counter() - ta
+        # the same data flow over this library's LBVH instead of the reference's brute-force scan
+        plain_step(torch, scene, cwos, psub, tsub, 2, W)
+        torch.cuda.synchronize()
+        ta = time.perf_counter()
+        plain_step(torch, scene, cwos, psub, tsub, 3, W)
+        torch.cuda.synchronize()
+        tpb = time.perf_counter() - ta
         # equal RMSE: both paths on the twin, same points, same walks per point
         tw_bf = wos_ext.Scene(V, F, t(ct["values"]), use_bvh=False, source_type=3, source_params=SP)
         u_plain, _ = plain_step(torch, tw_bf, cwos, psub, tsub, 7, W)
@@ -233,11 +278,14 @@ def clover_workload(args, torch, wos_ext, C):
         rmse_plain = float(np.sqrt(np.mean((u_plain.double().cpu().numpy() - ex) ** 2)))
         rmse_fused_sub = float(np.sqrt(np.mean((m_fused[sub] - ex) ** 2)))
         plain = {"walks_per_s": psub.shape[0] * W / tp, "ms_per_step_sample": tp * 1e3,
-                 "sample": f"{psub.shape[0]} points (every 8th) x {W} walks, brute-force scan, walks re-run in "
-                           f"backward",
+                 "sample": f"{psub.shape[0]} points (every 8th) x {W} walks, one lane per walk, walks re-run in "
+                           f"backward; brute-force scan (walks_per_s) and LBVH (walks_per_s_lbvh)",
                  "speedup_fused_vs_plain": value / (psub.shape[0] * W / tp),
+                 "walks_per_s_lbvh": psub.shape[0] * W / tpb,
+                 "speedup_fused_vs_plain_lbvh": value / (psub.shape[0] * W / tpb),
                  "rmse_twin_plain": rmse_plain, "rmse_twin_fused_same_points": rmse_fused_sub}
-        log(f"plain: {plain['walks_per_s']:.3e} walks/s, rmse {rmse_plain:.4f} vs fused {rmse_fused_sub:.4f}")
+        log(f"plain: {plain['walks_per_s']:.3e} walks/s (LBVH {plain['walks_per_s_lbvh']:.3e}), rmse "
+            f"{rmse_plain:.4f} vs fused {rmse_fused_sub:.4f}")
 
     cpu = None
     if not args.no_cpu:
@@ -254,14 +302,15 @@ def clover_workload(args, torch, wos_ext, C):
         cpu = cpu_leg(chunk, P, W, args.cpu_seconds, "primal + value adjoint")
         log(f"cpu: {cpu['value']:.3e} walks/s on {CPU_THREADS} threads")
 
-    emit({"metric": "primal+grad walks/s at 1 GPU (2D Poisson clover, values + source-coefficient adjoint)",
+    emit({"metric": METRIC,
           "value": value, "unit": "walks/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
           "ms_per_step": ms_per_step, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
           "dtype": "f32", "data": "synthetic",
           "config": {"workload": "2d-poisson-clover-4096seg-512grid-128walks-eps1e-4-fourier8",
                      "points": P, "walks_per_point": W, "segments": 4096, "interior_sampling": "uniform",
                      "parallelism": "single-gpu"},
-          "rmse_twin": rmse, "roofline": rf, "cpu_baseline": cpu, "plain": plain, "importance_sampled": is_variant})
+          "rmse_twin": rmse, "fd_check": fd, "roofline": rf, "cpu_baseline": cpu, "plain": plain,
+          "importance_sampled": is_variant})
 
 
 # ---------------------------------------------------------------------------
@@ -389,6 +438,42 @@ def shape_workload(args, torch, wos_ext, C):
                       {"per_query": {"box_tests": cnt["box_tests"] / max(cnt["queries"], 1),
                                      "prim_tests": cnt["prim_tests"] / max(cnt["queries"], 1)},
                        "steps_per_sample": cnt["steps"] / (P * W)})
+    fd = None
+    if not args.no_fd:
+        # the adjoint of a smooth anisotropic stretch (x += 0.5 x, last axis -= 0.3 z) against central
+        # differences of the loss; forward walks for u use other seeds than the gradient samples, the
+        # differences use common seeds on both sides
+        Sc = wos_ext.Scene if dim == 2 else wos_ext.Scene3D
+        Wc = wos_ext.WoS if dim == 2 else wos_ext.WoS3D
+        d = torch.zeros_like(V)
+        d[:, 0] = 0.5 * V[:, 0]
+        d[:, dim - 1] = -0.3 * V[:, dim - 1]
+        n_ad, n_fd, h, W_fd = (8, 4, 1e-2, 128) if dim == 2 else (16, 3, 1e-2, 128)
+        # points that stay inside under both perturbations (the clover's grid reaches the boundary)
+        inner = scene.sdf_native(pts) < -0.02
+        pf, tf = pts[inner].contiguous(), target[inner].contiguous()
+        Pf = pf.shape[0]
+        ad = []
+        for sd in range(n_ad):
+            u = cw.walk_reduce(pf, scene, seed=300 + sd)["mean"]
+            gv, _ = wos_ext.shape_gradient(scene, pf, 500 + sd, (2.0 / Pf) * (u - tf), **kw)
+            ad.append(float((gv.double() * d.double()).sum()))
+        cfd = Wc(nwalks=W_fd, nsteps=kw["nsteps"], epsilon=kw["epsilon"])
+        t64 = tf.double()
+        fds = []
+        for sd in range(n_fd):
+            up = cfd.walk_reduce(pf, Sc(V + h * d, F, G, use_bvh=True), seed=900 + sd, stats=True)["mean64"]
+            dn = cfd.walk_reduce(pf, Sc(V - h * d, F, G, use_bvh=True), seed=900 + sd, stats=True)["mean64"]
+            fds.append(float((((up - t64) ** 2).mean() - ((dn - t64) ** 2).mean()) / (2 * h)))
+        a, sa = float(np.mean(ad)), float(np.std(ad, ddof=1) / np.sqrt(n_ad))
+        b, sb = float(np.mean(fds)), float(np.std(fds, ddof=1) / np.sqrt(n_fd))
+        fd = {"direction": "stretch x by 0.5 x, last axis by -0.3 (coordinate)", "adjoint": a, "adjoint_se": sa,
+              "central_difference": b, "central_difference_se": sb, "relative_error": abs(a - b) / abs(b),
+              "sample": f"{Pf} of {P} points (at least 0.02 inside), adjoint over {n_ad} seeds x {W} walks, "
+                        f"differences h = {h} over {n_fd} seeds x {W_fd} walks"}
+        log(f"fd check: adjoint {a:+.5e} +- {sa:.1e}, central difference {b:+.5e} +- {sb:.1e}, relative error "
+            f"{fd['relative_error']:.4f}")
+
     cpu = None
     if not args.no_cpu:
         import oracle as O
@@ -408,7 +493,7 @@ def shape_workload(args, torch, wos_ext, C):
           "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True,
           "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
           "config": {"workload": wl, "points": P, "walks_per_point": W, "primitives": int(c["indices"].shape[0])},
-          "forward_ms": fwd, "gradient_ms": bwd, "roofline": rf, "cpu_baseline": cpu})
+          "forward_ms": fwd, "gradient_ms": bwd, "fd_check": fd, "roofline": rf, "cpu_baseline": cpu})
 
 
 # ---------------------------------------------------------------------------
@@ -446,6 +531,15 @@ def batch_workload(args, torch, wos_ext, C):
 
     step()
     eager_ms = synced_ms(torch, step, args.steps)
+    # the walk launches alone (both dimensions), HIP events on the launch stream
+    stream = torch.cuda.current_stream()
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(2 * args.steps)]
+    for i in range(args.steps):
+        ev[2 * i].record(stream)
+        batch.run()
+        ev[2 * i + 1].record(stream)
+    torch.cuda.synchronize()
+    run_ms = float(np.mean([ev[2 * i].elapsed_time(ev[2 * i + 1]) for i in range(args.steps)]))
     s = torch.cuda.Stream()
     g = torch.cuda.CUDAGraph()
     torch.cuda.synchronize()
@@ -462,52 +556,116 @@ def batch_workload(args, torch, wos_ext, C):
     log(f"batch: {total} walks; batched primal+adjoint {eager_ms:.1f} ms eager, {graph_ms:.1f} ms graph, "
         f"rmse {rmse_batched:.4f}")
 
+    rf = None
+    if not args.no_counts:
+        flops, cnt = 0.0, {"box_tests": 0, "prim_tests": 0, "queries": 0, "steps": 0}
+        for it, itn in zip(items, items_np):
+            src = "source_params" in itn
+            c_ = it["wos"].walk_counts(it["points"], it["scene"], seed=it["seed"], basis=src)
+            K = itn["source_params"].shape[0] if src else 0
+            flops += walk_flops(c_, itn["dim"], c_["walks"], K=K, src=src)
+            for k in cnt:
+                cnt[k] += c_[k]
+        log(f"counts: {cnt['box_tests']:.3e} bound tests, {cnt['prim_tests']:.3e} primitive tests")
+        rf = roofline(flops, run_ms, "k_batch_walk<2> + k_batch_walk<3> (one wos_batch_run)", None,
+                      {"per_query": {"box_tests": cnt["box_tests"] / cnt["queries"],
+                                     "prim_tests": cnt["prim_tests"] / cnt["queries"]},
+                       "mean_steps": cnt["steps"] / total})
+
     plain = None
     if not args.no_plain:
-        # reference data flow per item (one lane per walk, brute-force scan, walks re-run in backward)
-        # on the same strided points of every item, and the batched means on those points
-        n_sample = args.plain_points
-        tp, pw, e_plain, e_bat, ns = 0.0, 0, 0.0, 0.0, 0
+        # reference data flow per item (one lane per walk, walks re-run in backward) on the same strided
+        # points of every item, with the reference's brute-force scan and with this library's LBVH, and
+        # the batched means on those points
+        tp, tpb, pw, pwb, e_plain, e_bat, ns = 0.0, 0.0, 0, 0, 0.0, 0.0, 0
+        t_full = 0.0   # the LBVH plain path's time for every item's full point set, from its per-item rates
         for j, (it, o, itn) in enumerate(zip(items, batch.outputs, items_np)):
-            sc = it["scene"]
             P = it["points"].shape[0]
-            stride = max(1, P // n_sample)
-            sel = slice(None, None, stride)
-            psub = it["points"][sel].contiguous()
             W = it["wos"].nwalks
-            if itn["dim"] == 2:
-                src = "source_params" in itn
-                bf = wos_ext.Scene(t(itn["vertices"]), t(itn["indices"]), t(itn["values"]), use_bvh=False,
-                                   source_type=3 if src else 0,
-                                   source_params=t(itn["source_params"]) if src else None)
-            else:
-                bf = wos_ext.Scene3D(t(itn["vertices"]), t(itn["indices"]), t(itn["values"]), use_bvh=False)
-            tsub = targets[j][sel].contiguous()
-            torch.cuda.synchronize()
-            ta = time.perf_counter()
-            u, _ = plain_step(torch, bf, it["wos"], psub, tsub, itn["seed"], W)
-            torch.cuda.synchronize()
-            tp += time.perf_counter() - ta
-            pw += psub.shape[0] * W
-            ex = itn["exact"][::stride]
-            e_plain += float(((u.double().cpu().numpy() - ex) ** 2).sum())
-            e_bat += float(((o["mean64"].cpu().numpy()[::stride] - ex) ** 2).sum())
-            ns += ex.size
-        plain = {"walks_per_s": pw / tp, "seconds": tp,
-                 "sample": f"{ns} points (about {n_sample} per item, evenly strided) x each item's walks, "
-                           f"brute-force scan, walks re-run in backward",
+            for bvh in (False, True):
+                # the brute-force scan on a few points per item; the LBVH on enough that launch costs amortise
+                stride = max(1, P // (args.plain_lbvh_points if bvh else args.plain_points))
+                sel = slice(None, None, stride)
+                psub = it["points"][sel].contiguous()
+                tsub = targets[j][sel].contiguous()
+                if itn["dim"] == 2:
+                    src = "source_params" in itn
+                    sc_p = wos_ext.Scene(t(itn["vertices"]), t(itn["indices"]), t(itn["values"]), use_bvh=bvh,
+                                         source_type=3 if src else 0,
+                                         source_params=t(itn["source_params"]) if src else None)
+                else:
+                    sc_p = wos_ext.Scene3D(t(itn["vertices"]), t(itn["indices"]), t(itn["values"]), use_bvh=bvh)
+                if bvh:   # warm (workspace growth), then time
+                    plain_step(torch, sc_p, it["wos"], psub, tsub, itn["seed"], W)
+                torch.cuda.synchronize()
+                ta = time.perf_counter()
+                u, _ = plain_step(torch, sc_p, it["wos"], psub, tsub, itn["seed"], W)
+                torch.cuda.synchronize()
+                if bvh:
+                    dt = time.perf_counter() - ta
+                    tpb += dt
+                    pwb += psub.shape[0] * W
+                    t_full += dt * P / psub.shape[0]
+                    ex = itn["exact"][::stride]
+                    e_plain += float(((u.double().cpu().numpy() - ex) ** 2).sum())
+                    e_bat += float(((o["mean64"].cpu().numpy()[::stride] - ex) ** 2).sum())
+                    ns += ex.size
+                else:
+                    tp += time.perf_counter() - ta
+                    pw += psub.shape[0] * W
+            if j % 16 == 15:
+                log(f"plain: {j + 1} of {len(items)} items")
+        batched_rate = total / (graph_ms * 1e-3)
+        plain = {"walks_per_s": pw / tp, "seconds": tp, "walks_per_s_lbvh": total / t_full,
+                 "walks_per_s_lbvh_sample": pwb / tpb, "seconds_lbvh": tpb,
+                 "sample": f"each item's points evenly strided, about {args.plain_points} per item for the "
+                           f"brute-force scan (walks_per_s, {pw} walks) and {args.plain_lbvh_points} for the LBVH "
+                           f"({pwb} walks, RMSE on these {ns} points; walks_per_s_lbvh is the whole workload's "
+                           f"walks over the sum of each item's sample time scaled to its full point count); one "
+                           f"lane per walk, walks re-run in backward",
                  "rmse_plain": float(np.sqrt(e_plain / ns)), "rmse_batched_same_points": float(np.sqrt(e_bat / ns)),
-                 "speedup_batched_vs_plain": (total / (graph_ms * 1e-3)) / (pw / tp)}
-        log(f"plain: {plain['walks_per_s']:.3e} walks/s; speedup {plain['speedup_batched_vs_plain']:.1f}x; rmse "
+                 "speedup_batched_vs_plain": batched_rate / (pw / tp),
+                 "speedup_batched_vs_plain_lbvh": batched_rate * t_full / total}
+        log(f"plain: {plain['walks_per_s']:.3e} walks/s (LBVH {plain['walks_per_s_lbvh']:.3e}); speedup "
+            f"{plain['speedup_batched_vs_plain']:.1f}x ({plain['speedup_batched_vs_plain_lbvh']:.1f}x vs LBVH); rmse "
             f"{plain['rmse_plain']:.4f} vs batched {plain['rmse_batched_same_points']:.4f}")
+
+    cpu = None
+    if not args.no_cpu:
+        import oracle as O
+        refs = []
+        for itn in items_np:
+            src = "source_params" in itn
+            refs.append(O.Scene(itn["vertices"], itn["indices"], itn["values"], 3 if src else 0,
+                                itn.get("source_params"), use_bvh=True))
+        # every item's points thinned by the same stride (so the mix of items follows the workload's), the
+        # stride halved until about cpu_seconds of work
+        tc, done, stride = 0.0, 0, 1 << 14
+        while tc < args.cpu_seconds and stride >= 1:
+            for ref, itn, tg in zip(refs, items_np, targets):
+                q = np.ascontiguousarray(itn["points"][stride // 2::stride])
+                if q.shape[0] == 0:
+                    continue
+                W = itn["nwalks"]
+                gt = tg.cpu().numpy()[stride // 2::stride].astype(np.float64)
+                ta = time.perf_counter()
+                r = ref.walk_reduce(q, W, itn["nsteps"], itn["epsilon"], seed=0, threads=CPU_THREADS, records=True)
+                ref.adjoint_values(q.shape[0], W, r["rec_prim"], r["rec_t"], (2.0 / q.shape[0]) * (r["mean"] - gt))
+                tc += time.perf_counter() - ta
+                done += q.shape[0] * W
+            stride //= 2
+        cpu = {"value": done / tc, "unit": "walks/s", "cores": CPU_THREADS, "kind": "port",
+               "sample": f"every item's points at strides 16384 down to {2 * stride} ({done} walks), primal + value "
+                         f"adjoint, oracle with its CPU LBVH, {tc:.1f} s"}
+        log(f"cpu: {cpu['value']:.3e} walks/s on {CPU_THREADS} threads")
     emit({"metric": "primal+grad walks/s at 1 GPU, batched vs plain (64 uneven scenes)",
           "value": total / graph_ms * 1e3, "unit": "walks/s", "n_gpus": 1, "steps": args.steps, "warmup": 1,
           "ms_per_step": graph_ms, "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
           "data": "synthetic",
           "config": {"workload": f"64-scene-batch-primal-adjoint-scale-{args.batch_scale}", "walks": total,
                      "scenes": len(items)},
-          "batched_eager_ms": eager_ms, "graph_replay_ms": graph_ms, "rmse_batched": rmse_batched,
-          "tuning": tuning, "plain": plain})
+          "batched_eager_ms": eager_ms, "graph_replay_ms": graph_ms, "walk_launches_ms": run_ms,
+          "rmse_batched": rmse_batched, "tuning": tuning, "roofline": rf, "cpu_baseline": cpu, "plain": plain})
 
 
 def main():
@@ -518,6 +676,8 @@ def main():
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--no-fd", action="store_true", help="skip the finite-difference checks")
+    ap.add_argument("--no-variants", action="store_true", help="clover: skip the importance-sampled variant")
     ap.add_argument("--no-counts", action="store_true",
                     help="skip the instrumented counting launch (keeps profiler summaries to the timed kernels)")
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -534,7 +694,9 @@ def main():
     ap.add_argument("--batch-scale", type=float, default=1.0,
                     help="query points per batched scene, as a fraction of the full workload's")
     ap.add_argument("--plain-points", type=int, default=32,
-                    help="batch workload: points per item walked by the plain path")
+                    help="batch workload: points per item walked by the plain path with the brute-force scan")
+    ap.add_argument("--plain-lbvh-points", type=int, default=8192,
+                    help="batch workload: points per item walked by the plain path with the LBVH")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU; multi-GPU runs are not part of it",

@@ -24,6 +24,7 @@ namespace wos {
 
 constexpr int kMaxBatchItems = 1024;
 constexpr int kMaxBatchBasis = 16;
+constexpr int kBatchHist = 12288;   // vertex-value histogram of the reverse pass kept in LDS up to this size
 constexpr long long kAdjointTile = 16384;   // walks of one item per adjoint tile
 constexpr int kBatchFinish = 8;             // finished walks written together (see k_walk_persistent)
 
@@ -355,42 +356,88 @@ __global__ __launch_bounds__(kBlock) void k_batch_reduce(const BatchItem *items,
     }
 }
 
-// Reverse pass over the stored exit records, one tile of one item per
-// workgroup visit: a valid walk of point i adds g_i / W times the
-// interpolation weights of its exit primitive to the vertex values
-// (include/scene.cuh:251-264 transposed) and g_i / W times its Fourier basis
-// sums to the coefficients (include/wos.cuh:136-138 transposed).  Coefficient
-// sums reduce across the wave and the workgroup before one atomic per tile.
+// Reverse pass over the per-walk records (Solver.walk_reduce's adjoint,
+// wos/solver.py:90-125): each record adds g_i / W times the interpolation
+// weights of its exit primitive to the vertex values (include/scene.cuh:
+// 251-264 transposed) and g_i / W times its Fourier basis sums to the
+// coefficients (include/wos.cuh:136-138 transposed).  A workgroup takes a
+// contiguous run of tiles, so it mostly stays on one item: the value sums
+// go to a histogram in LDS (items with at most hist_size vertices; larger
+// meshes add to memory directly) and the coefficient sums stay in
+// registers, and both reach memory once per item the run touches.
 template <int D>
 __global__ __launch_bounds__(kBlock) void k_batch_adjoint(const BatchItem *items, const AdjointTile *tiles,
-                                                          int n_tiles) {
+                                                          int n_tiles, int hist_size) {
+    extern __shared__ float h[];
     __shared__ float cacc[kMaxBatchBasis];
-    for (int t = blockIdx.x; t < n_tiles; t += gridDim.x) {
+    for (int v = threadIdx.x; v < hist_size; v += kBlock) h[v] = 0.0f;
+    if (threadIdx.x < kMaxBatchBasis) cacc[threadIdx.x] = 0.0f;
+    const int per = (n_tiles + (int)gridDim.x - 1) / (int)gridDim.x;
+    const int t0 = (int)blockIdx.x * per, t1 = min(n_tiles, t0 + per);
+    float cl[kMaxBatchBasis];
+#pragma unroll
+    for (int q = 0; q < kMaxBatchBasis; ++q) cl[q] = 0.0f;
+    int cur = -1;
+    for (int t = t0; t <= t1; ++t) {
+        const int item = t < t1 ? tiles[t].item : -1;
+        if (item != cur) {
+            // flush the previous item's sums
+            if (cur >= 0) {
+                const BatchItem &b = items[cur];
+                const bool want_c = b.grad_coeffs != nullptr && b.basis != nullptr && b.K > 0;
+                if (want_c) {
+#pragma unroll
+                    for (int q = 0; q < kMaxBatchBasis; ++q) {
+                        if (q < b.K) {
+                            float v = cl[q];
+                            for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+                            if (__lane_id() == 0) atomicAdd(&cacc[q], v);
+                        }
+                        cl[q] = 0.0f;
+                    }
+                }
+                __syncthreads();
+                if (b.grad_values && b.v.n_vertices <= hist_size) {
+                    for (int v = threadIdx.x; v < b.v.n_vertices; v += kBlock) {
+                        const float x = h[v];
+                        if (x != 0.0f) atomicAdd(&b.grad_values[v], x);
+                        h[v] = 0.0f;
+                    }
+                }
+                if (want_c && threadIdx.x < b.K) {
+                    atomicAdd(&b.grad_coeffs[threadIdx.x], cacc[threadIdx.x]);
+                    cacc[threadIdx.x] = 0.0f;
+                }
+            }
+            __syncthreads();
+            cur = item;
+        }
+        if (t == t1) break;
         const AdjointTile tl = tiles[t];
         const BatchItem &b = items[tl.item];
-        if (threadIdx.x < kMaxBatchBasis) cacc[threadIdx.x] = 0.0f;
-        __syncthreads();
         const int W = b.w.nwalks;
         const long long total = (long long)b.n_points * W;
         const bool want_c = b.grad_coeffs != nullptr && b.basis != nullptr && b.K > 0;
-        float cl[kMaxBatchBasis];
-#pragma unroll
-        for (int q = 0; q < kMaxBatchBasis; ++q) cl[q] = 0.0f;
+        const bool lds = b.v.n_vertices <= hist_size;
+        float *gv = b.grad_values;
         for (long long k = tl.begin + threadIdx.x; k < tl.end; k += kBlock) {
             const int f = b.rec_prim[k];
             if (f < 0) continue;
             const float g = b.grad_out[point_of_walk(k, W)] / (float)W;
-            if (b.grad_values) {
+            if (gv) {
                 const int *ix = b.v.idx;
                 if constexpr (D == 2) {
                     const float tt = b.rec_t[k];
-                    atomicAdd(&b.grad_values[ix[2 * f]], g * (1.0f - tt));
-                    atomicAdd(&b.grad_values[ix[2 * f + 1]], g * tt);
+                    const int i0 = ix[2 * f], i1 = ix[2 * f + 1];
+                    const float g0 = g * (1.0f - tt), g1 = g * tt;
+                    if (lds) { atomicAdd(&h[i0], g0); atomicAdd(&h[i1], g1); }
+                    else { atomicAdd(&gv[i0], g0); atomicAdd(&gv[i1], g1); }
                 } else {
                     const float u = b.rec_t[2 * k], v = b.rec_t[2 * k + 1];
-                    atomicAdd(&b.grad_values[ix[3 * f]], g * (1.0f - u - v));
-                    atomicAdd(&b.grad_values[ix[3 * f + 1]], g * u);
-                    atomicAdd(&b.grad_values[ix[3 * f + 2]], g * v);
+                    const int i0 = ix[3 * f], i1 = ix[3 * f + 1], i2 = ix[3 * f + 2];
+                    const float g0 = g * (1.0f - u - v), g1 = g * u, g2 = g * v;
+                    if (lds) { atomicAdd(&h[i0], g0); atomicAdd(&h[i1], g1); atomicAdd(&h[i2], g2); }
+                    else { atomicAdd(&gv[i0], g0); atomicAdd(&gv[i1], g1); atomicAdd(&gv[i2], g2); }
                 }
             }
             if (want_c) {
@@ -399,19 +446,6 @@ __global__ __launch_bounds__(kBlock) void k_batch_adjoint(const BatchItem *items
                     if (q < b.K) cl[q] = fmaf(g, b.basis[(long long)q * total + k], cl[q]);
             }
         }
-        if (want_c) {
-#pragma unroll
-            for (int q = 0; q < kMaxBatchBasis; ++q) {
-                if (q < b.K) {
-                    float v = cl[q];
-                    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
-                    if (__lane_id() == 0) atomicAdd(&cacc[q], v);
-                }
-            }
-        }
-        __syncthreads();
-        if (want_c && threadIdx.x < b.K) atomicAdd(&b.grad_coeffs[threadIdx.x], cacc[threadIdx.x]);
-        __syncthreads();
     }
 }
 
@@ -428,6 +462,7 @@ struct BatchGroup {
     int max_depth = 0;
     AdjointTile *tiles = nullptr; // device copy (null: no item has a reverse pass)
     int n_tiles = 0;
+    int hist = 0;                 // LDS histogram entries of the reverse pass
 };
 
 }  // namespace wos
@@ -560,6 +595,8 @@ extern "C" int wos_batch_create(const wos_batch_item *items, int n_items, wos_ba
             depth = std::max(depth, b.v.use_bvh ? b.v.max_depth : 0);
             if (b.n_points > 0 && (b.grad_values || b.grad_coeffs))
                 for (long long t = 0; t < nw; t += kAdjointTile) tiles.push_back(AdjointTile{j, t, std::min(nw, t + kAdjointTile)});
+            if (b.n_points > 0 && b.grad_values && b.v.n_vertices <= kBatchHist)
+                B->g[d].hist = std::max(B->g[d].hist, b.v.n_vertices);
         }
         BatchGroup &g = B->g[d];
         g.n_items = (int)v.size();
@@ -650,8 +687,9 @@ extern "C" int wos_batch_adjoint(wos_batch *B, void *stream) {
         BatchGroup &g = B->g[d];
         if (!g.n_tiles) continue;
         const int grid = std::max(1, std::min(g.n_tiles, cap));
-        if (d == 0) k_batch_adjoint<2><<<grid, kBlock, 0, st>>>(g.items, g.tiles, g.n_tiles);
-        else k_batch_adjoint<3><<<grid, kBlock, 0, st>>>(g.items, g.tiles, g.n_tiles);
+        const size_t hb = sizeof(float) * (size_t)g.hist;
+        if (d == 0) k_batch_adjoint<2><<<grid, kBlock, hb, st>>>(g.items, g.tiles, g.n_tiles, g.hist);
+        else k_batch_adjoint<3><<<grid, kBlock, hb, st>>>(g.items, g.tiles, g.n_tiles, g.hist);
     }
     hipError_t e = hipGetLastError();
     BREQ(e == hipSuccess, "batch adjoint launch: %s", hipGetErrorString(e));

@@ -458,6 +458,11 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(D == 2 ?
     long long c_next = 0, c_end = 0;
     bool exhausted = false, done_valid = false;
     const int W = a.w.nwalks;
+    // instrumented launches (STATS): the lane's work tallies (bound tests,
+    // primitive tests, queries, steps) and the wave's lane-activity tallies,
+    // added to the launch counters once, when the wave ends
+    unsigned long long work[4] = {0ull, 0ull, 0ull, 0ull};
+    unsigned long long act[4] = {0ull, 0ull, 0ull, 0ull};
 
     for (;;) {
         // 1. completed queries: stop on the boundary (include/wos.cuh:108-113),
@@ -551,6 +556,14 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(D == 2 ?
         //    The random numbers are drawn in stream order (source sample,
         //    then direction); the loads for the new point are issued before
         //    the source arithmetic at the old point so that they overlap it.
+        if constexpr (STATS) {
+            // lane activity: lanes taking a step, per wave step phase
+            const uint64_t m = __ballot(mode == kStart);
+            if (m != 0) {
+                act[0] += (unsigned long long)__popcll(m);
+                act[1] += 1ull;
+            }
+        }
         if (mode == kStart) {
             float su0 = 0.0f, su1 = 0.0f;
             if (SRC) {
@@ -560,7 +573,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(D == 2 ?
             const float px0 = p[0], py0 = p[1];
             move_point<D>(p, d, rng);
             ++step;
-            if constexpr (STATS) atomicAdd(&s.stats[3], 1ull);
+            if constexpr (STATS) work[3] += 1ull;
             int4 start = make_int4(0, -1, 0, 0);
             float4 sg = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
             float sinv = 0.0f;
@@ -606,8 +619,8 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(D == 2 ?
                 mode = kReady;
             } else {
                 if constexpr (STATS) {
-                    atomicAdd(&s.stats[2], 1ull);
-                    atomicAdd(&s.stats[1], 1ull);   // the seed primitive
+                    work[2] += 1ull;
+                    work[1] += 1ull;   // the seed primitive
                 }
                 float seed_d2;
                 if constexpr (D == 2) {
@@ -628,14 +641,37 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(D == 2 ?
         }
         // 4. advance every pending query by up to `window` node visits
         for (int it = 0; it < a.window; ++it) {
+            if constexpr (STATS) {
+                // lane activity: querying lanes, per wave node-visit iteration
+                const uint64_t m = __ballot(mode == kQuery);
+                if (m != 0) {
+                    act[2] += (unsigned long long)__popcll(m);
+                    act[3] += 1ull;
+                }
+            }
             if (mode == kQuery) {
                 if (query_step<D, STATS>(s, p, q, st, slot, lane_g, s.tol)) {
                     mode = kReady;
-                    query_flush_stats<STATS>(s, q);
+                    if constexpr (STATS) {
+                        work[0] += q.nbox;
+                        work[1] += q.nprim;
+                    }
                 }
             }
             if (__ballot(mode == kQuery) == 0) break;
         }
+    }
+    if constexpr (STATS) {
+        for (int c = 0; c < 4; ++c) {
+            unsigned long long v = work[c];
+            for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+            work[c] = v;
+        }
+        if (lane == 0)
+            for (int c = 0; c < 4; ++c) {
+                atomicAdd(&s.stats[c], work[c]);
+                atomicAdd(&s.stats[4 + c], act[c]);
+            }
     }
 }
 

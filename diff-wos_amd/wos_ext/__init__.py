@@ -565,17 +565,21 @@ class WoS:
 
     def walk_counts(self, p, scene, seed=0, *, nwalks=None, basis=True):
         """Instrumented walk_reduce: totals of AABB tests, primitive tests,
-        closest-primitive queries and walk steps of one launch."""
+        closest-primitive queries and walk steps of one launch, and the mean
+        fraction of a wave's lanes at work in its step phases (step_lanes)
+        and node-visit iterations (query_lanes)."""
         q = _dev(p, torch.float32, scene.dim)
         P = q.shape[0]
         W = self.nwalks if nwalks is None else int(nwalks)
         wv = torch.empty(P * W, dtype=torch.float32, device=q.device)
-        cnt = torch.zeros(4, dtype=torch.int64, device=q.device)
+        cnt = torch.zeros(8, dtype=torch.int64, device=q.device)
         wp = self._params(W)
         _check(lib.wos_walk_reduce_counts(scene.handle, ctypes.byref(wp), _p(q), P, int(seed), int(bool(basis)),
                                           _p(wv), _p(cnt), _stream()))
         c = [int(x) for x in cnt.cpu()]
-        return dict(box_tests=c[0], prim_tests=c[1], queries=c[2], steps=c[3], walks=P * W)
+        return dict(box_tests=c[0], prim_tests=c[1], queries=c[2], steps=c[3], walks=P * W,
+                    step_lanes=c[4] / max(c[5], 1) / 64, query_lanes=c[6] / max(c[7], 1) / 64,
+                    step_phases=c[5], query_iterations=c[7])
 
     def solve(self, p, scene, seed=0):
         """WoS::solve (include/wos.cuh:320-364): mean of nwalks walks per point.

@@ -188,8 +188,12 @@ int wos_walk_reduce_noalloc(const wos_scene *scene, const wos_walk_params *param
 
 /* Instrumented walk_reduce for measurement: same launch as wos_walk_reduce
  * (with the Fourier basis outputs when with_basis), and adds into counts
- * (4 device uint64, zeroed by the caller): AABB tests, primitive tests,
- * closest-primitive queries, walk steps.  Slower than the plain launch. */
+ * (8 device uint64, zeroed by the caller): AABB tests, primitive tests,
+ * closest-primitive queries, walk steps, then the lane activity of the
+ * persistent kernel: lanes taking a step summed over the waves' step
+ * phases, the number of those phases, querying lanes summed over the waves'
+ * node-visit iterations, the number of those iterations.  Slower than the
+ * plain launch. */
 int wos_walk_reduce_counts(const wos_scene *scene, const wos_walk_params *params, const float *points, int n_points,
                            int seed, int with_basis, float *walk_value, unsigned long long *counts, void *stream);
 
