@@ -92,7 +92,10 @@ class Baseline2(Solver):
 class Baseline2CUDA(Solver):
     """wos/wos_grad.py:89-110: the native walk, the spatial gradient 5e-3
     inside the exit point, no inside test (the reference's class reads a
-    ``wos`` attribute it never sets; it is set here)."""
+    ``wos`` attribute it never sets; it is set here).  The reference steps
+    to p - 5e-3 n, taking the record normal for the outward one (its comment
+    at wos/wos_grad.py:101); exit records' normals point to the side the
+    walk came from, so the point inside is p + 5e-3 n."""
 
     def __post_init__(self):
         self.wos = WoSCUDA(nsteps=self.nsteps, nwalks=self.nwalks, epsilon=self.epsilon,
@@ -102,7 +105,7 @@ class Baseline2CUDA(Solver):
         with torch.no_grad():
             its = self.wos.single_walk_preliminary(p, scene, sampler)
         its = scene.get_point(its)
-        p_in_shell = its.p.detach() - its.n * 5e-3
+        p_in_shell = its.p.detach() + its.n * 5e-3
         with torch.no_grad():
             g = self.wos.grad(p_in_shell, scene, sampler)
         return _velocity_term(its, list(g), its.valid.bool())
