@@ -58,9 +58,11 @@ struct BatchItem {
     int n_points;
     int32_t seed;
     int K;                      // Fourier terms written to basis (0: none)
+    double basis_scale;         // 2^basis_exponent of the item's walk settings
     long long walk_begin;       // offset in the dimension group's walk space
     long long point_begin;      // offset in the dimension group's point space
-    float *walk_value, *rec_t, *basis, *mean;
+    float *walk_value, *rec_t, *mean;
+    unsigned long long *basis;  // K x n_points fixed-point sums, then doubles (the caller's buffer)
     int *rec_prim;
     double *mean64, *var64;
     const float *grad_out;      // reverse pass: dL/d mean per point (nullable)
@@ -124,11 +126,20 @@ __device__ __forceinline__ void batch_write(const BatchItem &b, long long k, boo
         if (D == 2) b.rec_t[k] = t0;
         else { b.rec_t[2 * k] = t0; b.rec_t[2 * k + 1] = t1; }
     }
-    if (b.basis) {
-        long long tot = (long long)b.n_points * b.w.nwalks;
-#pragma unroll
-        for (int q = 0; q < kMaxBatchBasis; ++q)
-            if (q < b.K) b.basis[(long long)q * tot + k] = valid ? bas[q] : 0.0f;
+    if (valid && b.basis)
+        add_basis<kMaxBatchBasis>(b.basis + point_of_walk(k, b.w.nwalks), b.n_points, b.K, b.basis_scale, bas);
+}
+
+// Per item (blockIdx.y): zeroes the fixed-point basis accumulators before the
+// walks (finish = 0), or turns them into doubles after them (finish = 1).
+__global__ void k_batch_basis(const BatchItem *items, int finish) {
+    const BatchItem &b = items[blockIdx.y];
+    if (!b.basis || b.K <= 0) return;
+    const long n = (long)b.K * b.n_points;
+    const double inv = 1.0 / b.basis_scale;
+    for (long j = (long)blockIdx.x * blockDim.x + threadIdx.x; j < n; j += (long)gridDim.x * blockDim.x) {
+        if (finish) b.basis[j] = (unsigned long long)__double_as_longlong((double)(long long)b.basis[j] * inv);
+        else b.basis[j] = 0ull;
     }
 }
 
@@ -359,24 +370,17 @@ __global__ __launch_bounds__(kBlock) void k_batch_reduce(const BatchItem *items,
 // Reverse pass over the per-walk records (Solver.walk_reduce's adjoint,
 // wos/solver.py:90-125): each record adds g_i / W times the interpolation
 // weights of its exit primitive to the vertex values (include/scene.cuh:
-// 251-264 transposed) and g_i / W times its Fourier basis sums to the
-// coefficients (include/wos.cuh:136-138 transposed).  A workgroup takes a
-// contiguous run of tiles, so it mostly stays on one item: the value sums
-// go to a histogram in LDS (items with at most hist_size vertices; larger
-// meshes add to memory directly) and the coefficient sums stay in
-// registers, and both reach memory once per item the run touches.
+// 251-264 transposed).  A workgroup takes a contiguous run of tiles, so it
+// mostly stays on one item: the sums go to a histogram in LDS (items with at
+// most hist_size vertices; larger meshes add to memory directly) and reach
+// memory once per item the run touches.
 template <int D>
 __global__ __launch_bounds__(kBlock) void k_batch_adjoint(const BatchItem *items, const AdjointTile *tiles,
                                                           int n_tiles, int hist_size) {
     extern __shared__ float h[];
-    __shared__ float cacc[kMaxBatchBasis];
     for (int v = threadIdx.x; v < hist_size; v += kBlock) h[v] = 0.0f;
-    if (threadIdx.x < kMaxBatchBasis) cacc[threadIdx.x] = 0.0f;
     const int per = (n_tiles + (int)gridDim.x - 1) / (int)gridDim.x;
     const int t0 = (int)blockIdx.x * per, t1 = min(n_tiles, t0 + per);
-    float cl[kMaxBatchBasis];
-#pragma unroll
-    for (int q = 0; q < kMaxBatchBasis; ++q) cl[q] = 0.0f;
     int cur = -1;
     for (int t = t0; t <= t1; ++t) {
         const int item = t < t1 ? tiles[t].item : -1;
@@ -384,18 +388,6 @@ __global__ __launch_bounds__(kBlock) void k_batch_adjoint(const BatchItem *items
             // flush the previous item's sums
             if (cur >= 0) {
                 const BatchItem &b = items[cur];
-                const bool want_c = b.grad_coeffs != nullptr && b.basis != nullptr && b.K > 0;
-                if (want_c) {
-#pragma unroll
-                    for (int q = 0; q < kMaxBatchBasis; ++q) {
-                        if (q < b.K) {
-                            float v = cl[q];
-                            for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
-                            if (__lane_id() == 0) atomicAdd(&cacc[q], v);
-                        }
-                        cl[q] = 0.0f;
-                    }
-                }
                 __syncthreads();
                 if (b.grad_values && b.v.n_vertices <= hist_size) {
                     for (int v = threadIdx.x; v < b.v.n_vertices; v += kBlock) {
@@ -403,10 +395,6 @@ __global__ __launch_bounds__(kBlock) void k_batch_adjoint(const BatchItem *items
                         if (x != 0.0f) atomicAdd(&b.grad_values[v], x);
                         h[v] = 0.0f;
                     }
-                }
-                if (want_c && threadIdx.x < b.K) {
-                    atomicAdd(&b.grad_coeffs[threadIdx.x], cacc[threadIdx.x]);
-                    cacc[threadIdx.x] = 0.0f;
                 }
             }
             __syncthreads();
@@ -416,8 +404,6 @@ __global__ __launch_bounds__(kBlock) void k_batch_adjoint(const BatchItem *items
         const AdjointTile tl = tiles[t];
         const BatchItem &b = items[tl.item];
         const int W = b.w.nwalks;
-        const long long total = (long long)b.n_points * W;
-        const bool want_c = b.grad_coeffs != nullptr && b.basis != nullptr && b.K > 0;
         const bool lds = b.v.n_vertices <= hist_size;
         float *gv = b.grad_values;
         for (long long k = tl.begin + threadIdx.x; k < tl.end; k += kBlock) {
@@ -440,13 +426,28 @@ __global__ __launch_bounds__(kBlock) void k_batch_adjoint(const BatchItem *items
                     else { atomicAdd(&gv[i0], g0); atomicAdd(&gv[i1], g1); atomicAdd(&gv[i2], g2); }
                 }
             }
-            if (want_c) {
-#pragma unroll
-                for (int q = 0; q < kMaxBatchBasis; ++q)
-                    if (q < b.K) cl[q] = fmaf(g, b.basis[(long long)q * total + k], cl[q]);
-            }
         }
     }
+}
+
+// Coefficient part of the reverse pass: dL/d coeff_q of item blockIdx.y (q =
+// blockIdx.x) = sum_i g_i / W * basis[q][i] over the item's per-point basis
+// sums, in FP64 with a fixed-order tree, so the result is bit-reproducible.
+__global__ __launch_bounds__(kBlock) void k_batch_adjoint_coeffs(const BatchItem *items) {
+    __shared__ double part[kBlock];
+    const BatchItem &b = items[blockIdx.y];
+    const int q = blockIdx.x;
+    if (!b.grad_coeffs || !b.basis || q >= b.K || b.n_points <= 0) return;
+    const double *bb = reinterpret_cast<const double *>(b.basis) + (long)q * b.n_points;
+    double acc = 0.0;
+    for (long i = threadIdx.x; i < b.n_points; i += kBlock) acc = fma((double)b.grad_out[i], bb[i], acc);
+    part[threadIdx.x] = acc;
+    __syncthreads();
+    for (int h = kBlock / 2; h > 0; h >>= 1) {
+        if (threadIdx.x < h) part[threadIdx.x] += part[threadIdx.x + h];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) b.grad_coeffs[q] += (float)(part[0] / b.w.nwalks);
 }
 
 __global__ void k_batch_zero(unsigned long long *c, int n) {
@@ -559,7 +560,8 @@ extern "C" int wos_batch_create(const wos_batch_item *items, int n_items, wos_ba
         b.walk_value = it.walk_value;
         b.rec_prim = it.rec_prim;
         b.rec_t = it.rec_t;
-        b.basis = it.basis;
+        b.basis = reinterpret_cast<unsigned long long *>(it.basis);
+        b.basis_scale = ldexp(1.0, basis_exponent(b.w.nwalks, b.w.nsteps));
         b.mean = it.mean;
         b.mean64 = it.mean64;
         b.var64 = it.var64;
@@ -593,7 +595,7 @@ extern "C" int wos_batch_create(const wos_batch_item *items, int n_items, wos_ba
             wb += nw;
             pb += b.n_points;
             depth = std::max(depth, b.v.use_bvh ? b.v.max_depth : 0);
-            if (b.n_points > 0 && (b.grad_values || b.grad_coeffs))
+            if (b.n_points > 0 && b.grad_values)
                 for (long long t = 0; t < nw; t += kAdjointTile) tiles.push_back(AdjointTile{j, t, std::min(nw, t + kAdjointTile)});
             if (b.n_points > 0 && b.grad_values && b.v.n_vertices <= kBatchHist)
                 B->g[d].hist = std::max(B->g[d].hist, b.v.n_vertices);
@@ -661,6 +663,7 @@ extern "C" int wos_batch_run(wos_batch *B, void *stream) {
         int gwk = (int)std::min<long>(std::max<long>(gw, 1), cap);
         int chunk = g_tuning.chunk > 0 ? g_tuning.chunk : 256;
         int window = window_for(d == 0 ? 2 : 3);
+        k_batch_basis<<<dim3(256, g.n_items), kBlock, 0, st>>>(g.items, 0);
         if (d == 0) {
             k_batch_init<2><<<gi, kBlock, 0, st>>>(g.items, g.n_items, g.points, g.iprim, g.isd, spill, stride);
             k_batch_walk<2><<<gwk, kBlock, 0, st>>>(g.items, g.n_items, g.walks, B->counters + d, chunk, window,
@@ -670,6 +673,7 @@ extern "C" int wos_batch_run(wos_batch *B, void *stream) {
             k_batch_walk<3><<<gwk, kBlock, 0, st>>>(g.items, g.n_items, g.walks, B->counters + d, chunk, window,
                                                     g.iprim, g.isd, spill, stride);
         }
+        k_batch_basis<<<dim3(256, g.n_items), kBlock, 0, st>>>(g.items, 1);
         long gr = std::min<long>((g.points * 64 + kBlock - 1) / kBlock, 8192);
         k_batch_reduce<<<(int)std::max<long>(gr, 1), kBlock, 0, st>>>(g.items, g.n_items, g.points);
     }
@@ -685,6 +689,7 @@ extern "C" int wos_batch_adjoint(wos_batch *B, void *stream) {
     const int cap = (int)(B->spill_lanes / kBlock);
     for (int d = 0; d < 2; ++d) {
         BatchGroup &g = B->g[d];
+        if (d == 0 && g.n_items) k_batch_adjoint_coeffs<<<dim3(kMaxBatchBasis, g.n_items), kBlock, 0, st>>>(g.items);
         if (!g.n_tiles) continue;
         const int grid = std::max(1, std::min(g.n_tiles, cap));
         const size_t hb = sizeof(float) * (size_t)g.hist;

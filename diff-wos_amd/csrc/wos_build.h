@@ -5,6 +5,8 @@
 #include <stdint.h>
 #include <string.h>
 
+#include <cmath>
+
 #include "wos_device.h"
 
 namespace wos {
@@ -105,11 +107,27 @@ struct WalkOutputs {
     float *walk_value;   // P*W, nullable
     int *rec_prim;       // P*W, nullable
     float *rec_t;        // P*W*(dim-1), nullable
-    float *basis;        // K*P*W (coefficient-major), nullable
+    double *basis;       // K*P (coefficient-major) per-point sums over the walks, nullable
     float *mean;         // P, nullable
     double *mean64;      // P, nullable
     double *var64;       // P, nullable
 };
+
+// Per-point Fourier basis sums are accumulated as 64-bit integers in units
+// of 2^-e, so their value does not depend on the order in which walks end.
+// One interior sample adds b G w with |b| <= 1 and |G w| <= R^2 (1 + ln(R /
+// 1e-4)) for a ball of radius R <= sqrt(1e5 / pi) (no sample is taken for a
+// larger ball: the reference's pdf > 1e-5 guard); a walk takes at most
+// nsteps samples, so the sum over nwalks walks stays below 2^61 / 2^e.
+// The same exponent is restated in the oracle.
+inline int basis_exponent(int nwalks, int nsteps) {
+    const double r = 178.5;                  // > sqrt(1e5 / pi) = 178.41
+    const double per_sample = r * r * 16.0;  // 1 + ln(178.5 / 1e-4) = 15.4 < 16
+    const double bound = (double)(nwalks > 1 ? nwalks : 1) * (double)(nsteps > 1 ? nsteps : 1) * per_sample;
+    int e = 0;
+    (void)frexp(bound, &e);   // bound < 2^e
+    return 61 - e;
+}
 
 // Launch tuning shared by all walk launches (set through wos_set_tuning).
 struct Tuning {
@@ -148,7 +166,7 @@ int launch_walk_reduce(const SceneView &s, const WalkSettings &w, const float *p
 int launch_walk_reduce_noalloc(const SceneView &s, const WalkSettings &w, const float *points, int n_points,
                                int32_t seed, int chunk, const WalkOutputs &o, hipStream_t st);
 int launch_adjoint_values(const SceneView &s, int n_points, int nwalks, const int *rec_prim, const float *rec_t,
-                          const float *basis, const float *grad_out, float *grad_values, float *grad_coeffs,
+                          const double *basis, const float *grad_out, float *grad_values, float *grad_coeffs,
                           hipStream_t st);
 
 int launch_source_adjoint(const SceneView &s, const WalkSettings &w, const float *points, int n_points,

@@ -542,7 +542,7 @@ int wos_single_walk(const wos_scene *s, const wos_walk_params *w, const float *p
 
 int wos_walk_reduce(const wos_scene *s, const wos_walk_params *w, const float *points, int n_points, int seed,
                     int chunk, float *mean, double *mean64, double *var64, float *walk_value, int *rec_prim,
-                    float *rec_t, float *basis, void *stream) {
+                    float *rec_t, double *basis, void *stream) {
     REQUIRE(s && (n_points == 0 || points), "scene and points are required");
     REQUIRE(n_points >= 0, "negative count");
     if (check_walk(w)) return -1;
@@ -557,7 +557,7 @@ int wos_walk_reduce(const wos_scene *s, const wos_walk_params *w, const float *p
 
 int wos_walk_reduce_noalloc(const wos_scene *s, const wos_walk_params *w, const float *points, int n_points,
                             int seed, int chunk, float *mean, double *mean64, double *var64, float *walk_value,
-                            int *rec_prim, float *rec_t, float *basis, void *stream) {
+                            int *rec_prim, float *rec_t, double *basis, void *stream) {
     REQUIRE(s && (n_points == 0 || points), "scene and points are required");
     REQUIRE(n_points >= 0, "negative count");
     if (check_walk(w)) return -1;
@@ -579,9 +579,9 @@ int wos_walk_reduce_counts(const wos_scene *s, const wos_walk_params *w, const f
     WalkSettings ws{w->nwalks, w->nsteps, w->epsilon, w->double_sided, w->prevent_fd_artifacts,
                     w->use_IS_for_greens ? 1 : 0};
     WalkOutputs o{walk_value, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
-    float *basis = nullptr;
+    double *basis = nullptr;
     if (with_basis && s->source_type == 3) {
-        size_t nb = (size_t)(s->n_src / 4) * (size_t)n_points * (size_t)w->nwalks * sizeof(float);
+        size_t nb = (size_t)(s->n_src / 4) * (size_t)n_points * sizeof(double);
         CHK(hipMallocAsync((void **)&basis, nb, S(stream)));
         o.basis = basis;
     }
@@ -591,7 +591,7 @@ int wos_walk_reduce_counts(const wos_scene *s, const wos_walk_params *w, const f
 }
 
 int wos_adjoint_values(const wos_scene *s, int n_points, int nwalks, const int *rec_prim, const float *rec_t,
-                       const float *basis, const float *grad_out, float *grad_values, float *grad_coeffs,
+                       const double *basis, const float *grad_out, float *grad_values, float *grad_coeffs,
                        void *stream) {
     REQUIRE(s && (n_points == 0 || (rec_prim && rec_t && grad_out)), "scene, records and grad_out are required");
     REQUIRE(n_points >= 0 && nwalks >= 1, "bad sizes");

@@ -1001,6 +1001,22 @@ __device__ __forceinline__ void move_point(float *p, float d, Pcg32 &rng) {
     }
 }
 
+// Adds a valid walk's Fourier basis sums bas[q] (q < K <= NMAX) to its
+// point's 64-bit fixed-point accumulators acc[q * stride], in units of
+// 1 / scale: integer additions, so the sums do not depend on the order in
+// which walks end.
+template <int NMAX>
+__device__ __forceinline__ void add_basis(unsigned long long *acc, long stride, int K, double scale,
+                                          const float *bas) {
+#pragma unroll
+    for (int q = 0; q < NMAX; ++q) {
+        if (q < K) {
+            const long long x = __double2ll_rn((double)bas[q] * scale);
+            if (x != 0) atomicAdd(acc + (long)q * stride, (unsigned long long)x);
+        }
+    }
+}
+
 // Lane modes of the persistent walk kernels: no work, a step to take, a
 // query in progress, a query just completed, a finished walk whose outputs
 // are not yet written.

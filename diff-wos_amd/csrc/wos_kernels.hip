@@ -365,7 +365,14 @@ struct PersistArgs {
     float *walk_value;
     int *rec_prim;
     float *rec_t;
-    float *basis;
+    unsigned long long *basis;       // K x n_points fixed-point sums (units of 1 / basis_scale), zeroed
+    double basis_scale;              // 2^basis_exponent
+    // staged outputs (nullable): per walk one 16-byte record {value, prim,
+    // t or u, v} and, with a basis, kq float4 of basis sums, written as whole
+    // vectors and split / reduced per point by k_reduce_staged
+    float4 *stage;
+    float4 *stage_basis;
+    int kq;
     int K;
     int2 *spill;
     long spill_stride;
@@ -393,6 +400,14 @@ __global__ __launch_bounds__(kBlock) void k_point_init(SceneView s, const float 
 }
 
 
+// Fixed-point basis accumulators -> doubles in place (x / scale).
+__global__ void k_basis_finish(unsigned long long *acc, long n, double inv_scale) {
+    for (long j = (long)blockIdx.x * blockDim.x + threadIdx.x; j < n; j += (long)gridDim.x * blockDim.x) {
+        const double v = (double)(long long)acc[j] * inv_scale;
+        acc[j] = (unsigned long long)__double_as_longlong(v);
+    }
+}
+
 template <int D, int KB>
 __device__ __forceinline__ void write_walk(const SceneView &s, const PersistArgs &a, long long k, bool valid,
                                            int prim, const float *p, float contrib, const float *bas) {
@@ -404,7 +419,25 @@ __device__ __forceinline__ void write_walk(const SceneView &s, const PersistArgs
         v = dirichlet_at<D>(s, prim, r.t, r.uv[0], r.uv[1]) + contrib;
         pr = prim;
         t0 = (D == 2) ? r.t : r.uv[0];
-        t1 = r.uv[1];
+        t1 = (D == 2) ? 0.0f : r.uv[1];
+    }
+    if (a.stage) {
+        // walks end out of order: a few whole-vector stores per walk instead
+        // of one 4-byte store into each of up to eleven arrays
+        a.stage[k] = make_float4(v, __int_as_float(pr), t0, t1);
+        if constexpr (KB != 0) {
+            float4 *bp = a.stage_basis + k * a.kq;
+#pragma unroll
+            for (int j = 0; j < (basis_cap(KB) + 3) / 4; ++j) {
+                if (KB > 0 || j < a.kq) {
+                    float x[4];
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) x[u] = (valid && 4 * j + u < basis_cap(KB)) ? bas[4 * j + u] : 0.0f;
+                    bp[j] = make_float4(x[0], x[1], x[2], x[3]);
+                }
+            }
+        }
+        return;
     }
     if (a.walk_value) a.walk_value[k] = v;
     if (a.rec_prim) a.rec_prim[k] = pr;
@@ -413,12 +446,7 @@ __device__ __forceinline__ void write_walk(const SceneView &s, const PersistArgs
         else { a.rec_t[2 * k] = t0; a.rec_t[2 * k + 1] = t1; }
     }
     if constexpr (KB != 0) {
-        float *bp = a.basis + k;
-#pragma unroll
-        for (int q = 0; q < basis_cap(KB); ++q) {
-            if (KB > 0 || q < a.K) *bp = valid ? bas[q] : 0.0f;
-            bp += a.total;
-        }
+        if (valid) add_basis<basis_cap(KB)>(a.basis + point_of_walk(k, a.w.nwalks), a.n_points, a.K, a.basis_scale, bas);
     }
 }
 
@@ -701,6 +729,75 @@ __global__ __launch_bounds__(kBlock) void k_reduce(const float *vals, int n_poin
     }
 }
 
+// One wave per point over staged walk outputs: splits the records into the
+// per-walk outputs (each nullable; whole lines, since a point's walks are
+// contiguous), forms the FP64 mean / variance like k_reduce (same order, so
+// identical results), and sums the basis (K <= kMaxBasis terms, kq float4
+// per walk; invalid walks carry zeros) in 64-bit fixed point, units of
+// 1 / scale, written as doubles basis[q * n_points + i].
+template <int D>
+__global__ __launch_bounds__(kBlock) void k_reduce_staged(const float4 *stage, const float4 *stage_basis, int kq,
+                                                          int K, int n_points, int W, double scale,
+                                                          float *walk_value, int *rec_prim, float *rec_t,
+                                                          float *mean, double *mean64, double *var64,
+                                                          double *basis) {
+    const int lane = __lane_id();
+    long wave = ((long)blockIdx.x * kBlock + threadIdx.x) >> 6;
+    long nwaves = ((long)gridDim.x * kBlock) >> 6;
+    for (long i = wave; i < n_points; i += nwaves) {
+        double sm = 0.0, s2 = 0.0;
+        long long bs[kMaxBasis];
+#pragma unroll
+        for (int q = 0; q < kMaxBasis; ++q) bs[q] = 0;
+        for (int w = lane; w < W; w += 64) {
+            const long long k = i * W + w;
+            const float4 r = stage[k];
+            if (walk_value) walk_value[k] = r.x;
+            if (rec_prim) rec_prim[k] = __float_as_int(r.y);
+            if (rec_t) {
+                if (D == 2) rec_t[k] = r.z;
+                else { rec_t[2 * k] = r.z; rec_t[2 * k + 1] = r.w; }
+            }
+            const double v = r.x;
+            sm += v;
+            s2 += v * v;
+            if (basis) {
+#pragma unroll
+                for (int j = 0; j < kMaxBasis / 4; ++j) {
+                    if (j < kq) {
+                        const float4 b = stage_basis[k * kq + j];
+                        const float x[4] = {b.x, b.y, b.z, b.w};
+#pragma unroll
+                        for (int u = 0; u < 4; ++u) bs[4 * j + u] += __double2ll_rn((double)x[u] * scale);
+                    }
+                }
+            }
+        }
+        for (int off = 32; off > 0; off >>= 1) {
+            sm += __shfl_xor(sm, off);
+            s2 += __shfl_xor(s2, off);
+        }
+        if (basis) {
+#pragma unroll
+            for (int q = 0; q < kMaxBasis; ++q) {
+                if (q < K) {
+                    long long x = bs[q];
+                    for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off);
+                    bs[q] = x;
+                }
+            }
+        }
+        if (lane == 0) {
+            double m = sm / W;
+            if (mean) mean[i] = (float)m;
+            if (mean64) mean64[i] = m;
+            if (var64) var64[i] = W > 1 ? (s2 - sm * m) / (W - 1) : 0.0;
+            if (basis)
+                for (int q = 0; q < K; ++q) basis[(long)q * n_points + i] = (double)bs[q] / scale;
+        }
+    }
+}
+
 // ---------------------------------------------------------------------------
 // adjoint of the walk value w.r.t. vertex values and Fourier coefficients
 // ---------------------------------------------------------------------------
@@ -708,26 +805,20 @@ constexpr int kLdsValues = 12288;   // vertex-value histogram kept in LDS up to 
 
 template <int D, bool LDSHIST>
 __global__ __launch_bounds__(kBlock) void k_adjoint_values(SceneView s, int n_points, int W, const int *rec_prim,
-                                                           const float *rec_t, const float *basis, int K,
-                                                           const float *grad_out, float *grad_values,
-                                                           float *grad_coeffs) {
+                                                           const float *rec_t, const float *grad_out,
+                                                           float *grad_values) {
     // the histogram is sized to the scene (dynamic LDS), so small meshes
     // leave room for more resident workgroups
     extern __shared__ float h[];
-    __shared__ float cacc[kMaxBasis];
     if (LDSHIST)
         for (int v = threadIdx.x; v < s.n_vertices; v += kBlock) h[v] = 0.0f;
-    if (threadIdx.x < kMaxBasis) cacc[threadIdx.x] = 0.0f;
     __syncthreads();
     long long total = (long long)n_points * W;
-    float cl[kMaxBasis];
-#pragma unroll
-    for (int q = 0; q < kMaxBasis; ++q) cl[q] = 0.0f;
     for (long long k = (long long)blockIdx.x * kBlock + threadIdx.x; k < total; k += (long long)gridDim.x * kBlock) {
         int f = rec_prim[k];
         if (f < 0) continue;
         float g = grad_out[point_of_walk(k, W)] / (float)W;
-        if (grad_values) {
+        {
             if constexpr (D == 2) {
                 float t = rec_t[k];
                 int i0 = s.idx[2 * f], i1 = s.idx[2 * f + 1];
@@ -742,27 +833,31 @@ __global__ __launch_bounds__(kBlock) void k_adjoint_values(SceneView s, int n_po
                 else { atomicAdd(&grad_values[i0], g0); atomicAdd(&grad_values[i1], g1); atomicAdd(&grad_values[i2], g2); }
             }
         }
-        if (basis) {
-#pragma unroll
-            for (int q = 0; q < kMaxBasis; ++q)
-                if (q < K) cl[q] = fmaf(g, basis[(long long)q * total + k], cl[q]);
-        }
-    }
-    if (basis && grad_coeffs) {
-#pragma unroll
-        for (int q = 0; q < kMaxBasis; ++q) {
-            if (q < K) {
-                float v = cl[q];
-                for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
-                if (__lane_id() == 0) atomicAdd(&cacc[q], v);
-            }
-        }
     }
     __syncthreads();
-    if (LDSHIST && grad_values)
+    if (LDSHIST)
         for (int v = threadIdx.x; v < s.n_vertices; v += kBlock)
             if (h[v] != 0.0f) atomicAdd(&grad_values[v], h[v]);
-    if (basis && grad_coeffs && threadIdx.x < K) atomicAdd(&grad_coeffs[threadIdx.x], cacc[threadIdx.x]);
+}
+
+// dL/d coeff_q = sum_i grad_out[i] / W * basis[q][i] (basis: per-point sums
+// over the walks): one workgroup per coefficient, FP64 partial sums per
+// thread over a fixed stride of points and a fixed-order tree in LDS, so the
+// result does not depend on scheduling.  Added into grad_coeffs[q].
+__global__ __launch_bounds__(kBlock) void k_adjoint_coeffs(int n_points, int W, const double *basis,
+                                                           const float *grad_out, float *grad_coeffs) {
+    __shared__ double part[kBlock];
+    const int q = blockIdx.x;
+    const double *b = basis + (long)q * n_points;
+    double acc = 0.0;
+    for (long i = threadIdx.x; i < n_points; i += kBlock) acc = fma((double)grad_out[i], b[i], acc);
+    part[threadIdx.x] = acc;
+    __syncthreads();
+    for (int h = kBlock / 2; h > 0; h >>= 1) {
+        if (threadIdx.x < h) part[threadIdx.x] += part[threadIdx.x + h];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) grad_coeffs[q] += (float)(part[0] / W);
 }
 
 // ---------------------------------------------------------------------------
@@ -1314,6 +1409,9 @@ int launch_single_walk(const SceneView &s, const WalkSettings &w, const float *p
 }
 
 __global__ void k_zero_u64(unsigned long long *p) { *p = 0ull; }
+__global__ void k_zero_u64_n(unsigned long long *p, long n) {
+    for (long j = (long)blockIdx.x * blockDim.x + threadIdx.x; j < n; j += (long)gridDim.x * blockDim.x) p[j] = 0ull;
+}
 
 // Per-launch start records of the persistent kernels (closest primitive and
 // signed distance of each query point) and the points' visiting order.  The
@@ -1360,6 +1458,9 @@ struct WalkScratch {
     int2 *spill = nullptr;                   // traversal stack beyond the LDS part
     long spill_stride = 0;
     int grid = 1, grid_init = 1;             // persistent walk grid, start-query grid
+    // staged outputs (nullable: the walk kernel then writes the caller's
+    // per-walk arrays directly and adds the basis with atomics)
+    float4 *stage = nullptr, *stage_basis = nullptr;
 };
 
 // Checks shared by both walk_reduce launchers; *K: Fourier terms.
@@ -1407,17 +1508,23 @@ static int walk_reduce_launch(const SceneView &s, const WalkSettings &w, const f
     a.walk_value = o.walk_value;
     a.rec_prim = o.rec_prim;
     a.rec_t = o.rec_t;
-    a.basis = o.basis;
+    a.basis = reinterpret_cast<unsigned long long *>(o.basis);
+    a.basis_scale = ldexp(1.0, basis_exponent(w.nwalks, w.nsteps));
+    a.stage = ws.stage;
+    a.stage_basis = ws.stage_basis;
+    a.kq = ws.stage_basis ? (K + 3) / 4 : 0;
     a.K = K;
     a.spill = ws.spill;
     a.spill_stride = ws.spill_stride;
     const int g = ws.grid, gi = ws.grid_init;
     size_t tb = top_bytes(s);
+    bool src = s.dim == 2 && s.source_type != 0;
+    bool bas = src && o.basis != nullptr && K > 0;
+    const long nbas = (long)K * n_points;
+    if (!ws.stage && o.basis && K > 0) k_zero_u64_n<<<grid_of(nbas), kBlock, 0, st>>>(a.basis, nbas);
     if (s.dim == 2) k_point_init<2><<<gi, kBlock, tb, st>>>(s, points, n_points, ws.order, ws.init_prim, ws.init_sd, a.spill, a.spill_stride);
     else k_point_init<3><<<gi, kBlock, tb, st>>>(s, points, n_points, nullptr, ws.init_prim, ws.init_sd, a.spill, a.spill_stride);
     k_zero_u64<<<1, 1, 0, st>>>(a.counter);
-    bool src = s.dim == 2 && s.source_type != 0;
-    bool bas = src && o.basis != nullptr && K > 0;
     if (s.dim == 2) {
         if (s.stats) {
             if (bas && a.K == 8) k_walk_persistent<2, true, 8, true><<<g, kBlock, tb, st>>>(s, a);
@@ -1435,6 +1542,26 @@ static int walk_reduce_launch(const SceneView &s, const WalkSettings &w, const f
         else k_walk_persistent<3, false, 0, false><<<g, kBlock, tb, st>>>(s, a);
     }
     KCHK();
+    if (ws.stage) {
+        long waves = n_points;
+        int gr = (int)((waves * 64 + kBlock - 1) / kBlock);
+        if (gr > 8192) gr = 8192;
+        double *bo = (o.basis && K > 0) ? o.basis : nullptr;
+        if (s.dim == 2)
+            k_reduce_staged<2><<<gr, kBlock, 0, st>>>(ws.stage, ws.stage_basis, a.kq, K, n_points, w.nwalks,
+                                                     a.basis_scale, o.walk_value, o.rec_prim, o.rec_t, o.mean,
+                                                     o.mean64, o.var64, bo);
+        else
+            k_reduce_staged<3><<<gr, kBlock, 0, st>>>(ws.stage, nullptr, 0, 0, n_points, w.nwalks, a.basis_scale,
+                                                     o.walk_value, o.rec_prim, o.rec_t, o.mean, o.mean64, o.var64,
+                                                     nullptr);
+        KCHK();
+        return 0;
+    }
+    if (o.basis && K > 0) {
+        k_basis_finish<<<grid_of(nbas), kBlock, 0, st>>>(a.basis, nbas, 1.0 / a.basis_scale);
+        KCHK();
+    }
     if (o.mean || o.mean64 || o.var64) {
         long waves = n_points;
         int gr = (int)((waves * 64 + kBlock - 1) / kBlock);
@@ -1461,7 +1588,18 @@ int launch_walk_reduce(const SceneView &s, const WalkSettings &w, const float *p
     ws.init_prim = sr.prim;
     ws.init_sd = sr.sd;
     ws.order = sr.order;
-    return walk_reduce_launch(s, w, points, n_points, seed, chunk, K, o, ws, st);
+    // staged outputs: 16 bytes per walk, plus the basis rounded up to float4
+    const size_t total = (size_t)n_points * (size_t)w.nwalks;
+    const int kq = (o.basis && K > 0) ? (K + 3) / 4 : 0;
+    if (hipMallocAsync((void **)&ws.stage, sizeof(float4) * total * (size_t)(1 + kq), st) != hipSuccess) {
+        (void)hipGetLastError();
+        set_error("cannot allocate the staged outputs of %zu walks", total);
+        return -1;
+    }
+    ws.stage_basis = kq ? ws.stage + total : nullptr;
+    const int rc = walk_reduce_launch(s, w, points, n_points, seed, chunk, K, o, ws, st);
+    (void)hipFreeAsync(ws.stage, st);
+    return rc;
 }
 
 // ---- allocation-free walk_reduce -----------------------------------------
@@ -1541,7 +1679,7 @@ int launch_walk_reduce_noalloc(const SceneView &s, const WalkSettings &w, const 
 }
 
 int launch_adjoint_values(const SceneView &s, int n_points, int nwalks, const int *rec_prim, const float *rec_t,
-                          const float *basis, const float *grad_out, float *grad_values, float *grad_coeffs,
+                          const double *basis, const float *grad_out, float *grad_values, float *grad_coeffs,
                           hipStream_t st) {
     if (n_points <= 0) return 0;
     int K = (s.source_type == 3) ? s.n_src / 4 : 0;
@@ -1557,14 +1695,20 @@ int launch_adjoint_values(const SceneView &s, int n_points, int nwalks, const in
     int g = grid_of((long)((total + 15) / 16), 8 * cu_count());
     bool lds = s.n_vertices <= kLdsValues;
     const size_t hb = lds ? sizeof(float) * (size_t)s.n_vertices : 0;
-    if (s.dim == 2) {
-        if (lds) k_adjoint_values<2, true><<<g, kBlock, hb, st>>>(s, n_points, nwalks, rec_prim, rec_t, basis, K, grad_out, grad_values, grad_coeffs);
-        else k_adjoint_values<2, false><<<g, kBlock, 0, st>>>(s, n_points, nwalks, rec_prim, rec_t, basis, K, grad_out, grad_values, grad_coeffs);
-    } else {
-        if (lds) k_adjoint_values<3, true><<<g, kBlock, hb, st>>>(s, n_points, nwalks, rec_prim, rec_t, basis, K, grad_out, grad_values, grad_coeffs);
-        else k_adjoint_values<3, false><<<g, kBlock, 0, st>>>(s, n_points, nwalks, rec_prim, rec_t, basis, K, grad_out, grad_values, grad_coeffs);
+    if (grad_values) {
+        if (s.dim == 2) {
+            if (lds) k_adjoint_values<2, true><<<g, kBlock, hb, st>>>(s, n_points, nwalks, rec_prim, rec_t, grad_out, grad_values);
+            else k_adjoint_values<2, false><<<g, kBlock, 0, st>>>(s, n_points, nwalks, rec_prim, rec_t, grad_out, grad_values);
+        } else {
+            if (lds) k_adjoint_values<3, true><<<g, kBlock, hb, st>>>(s, n_points, nwalks, rec_prim, rec_t, grad_out, grad_values);
+            else k_adjoint_values<3, false><<<g, kBlock, 0, st>>>(s, n_points, nwalks, rec_prim, rec_t, grad_out, grad_values);
+        }
+        KCHK();
     }
-    KCHK();
+    if (basis && grad_coeffs && K > 0) {
+        k_adjoint_coeffs<<<K, kBlock, 0, st>>>(n_points, nwalks, basis, grad_out, grad_coeffs);
+        KCHK();
+    }
     return 0;
 }
 

@@ -536,7 +536,8 @@ class WoS:
         """Solver.walk_reduce (wos/solver.py:49-65) over this walk, fused into
         one persistent launch.  Returns a dict with ``mean`` (float32) and, on
         request, ``mean64``/``var64`` (stats), ``rec_prim``/``rec_t``
-        (records), ``basis`` and ``walk_value``.  ``noalloc`` runs the launch
+        (records), ``basis`` (float64, K x P: per point the sum over its
+        walks of d value / d coeff_k, bit-reproducible) and ``walk_value``.  ``noalloc`` runs the launch
         through wos_walk_reduce_noalloc, whose scratch is a static per-GPU
         array (same results; the library allocates nothing)."""
         q = _dev(p, torch.float32, scene.dim)
@@ -555,7 +556,7 @@ class WoS:
             K = scene.c_source_params.numel() // 4 if scene.source_type == 3 else 0
             if K == 0:
                 raise WoSError("basis output needs a Fourier source (source_type 3)")
-            out["basis"] = torch.empty(K * P * W, dtype=torch.float32, device=dv)
+            out["basis"] = torch.empty(K * P, dtype=torch.float64, device=dv)
         wp = self._params(W)
         g = out.get
         fn = lib.wos_walk_reduce_noalloc if noalloc else lib.wos_walk_reduce
@@ -724,7 +725,7 @@ class Batch:
                 if scene.source_type != 3:
                     raise WoSError(f"item {j}: basis / coefficient gradients need a Fourier source (source_type 3)")
                 K = scene.c_source_params.numel() // 4
-                o["basis"] = torch.empty(K * P * W, dtype=torch.float32, device=dv)
+                o["basis"] = torch.empty(K * P, dtype=torch.float64, device=dv)
             if want_v or want_c:
                 o["grad_out"] = torch.zeros(P, dtype=torch.float32, device=dv)
             if want_v:

@@ -165,11 +165,15 @@ int wos_single_walk(const wos_scene *scene, const wos_walk_params *params, const
  *   walk_value  n_points*W floats  per-walk value
  *   rec_prim    n_points*W ints    exit primitive (-1 when the walk is invalid)
  *   rec_t       n_points*W floats (2D t) or 2*n_points*W (3D uv)
- *   basis       K*n_points*W floats, d value / d coeff_k of a Fourier source
+ *   basis       K*n_points doubles (coefficient-major, basis[k*n_points + i]):
+ *               sum over the W walks of point i of d value / d coeff_k of a
+ *               Fourier source (0 for invalid walks).  Accumulated in 64-bit
+ *               fixed point (units of 2^-e, e from nwalks and nsteps), so the
+ *               sums are bit-reproducible whatever order the walks end in.
  * chunk: walks a wave claims per queue visit (<= 0: default). */
 int wos_walk_reduce(const wos_scene *scene, const wos_walk_params *params, const float *points, int n_points,
                     int seed, int chunk, float *mean, double *mean64, double *var64, float *walk_value,
-                    int *rec_prim, float *rec_t, float *basis, void *stream);
+                    int *rec_prim, float *rec_t, double *basis, void *stream);
 
 /* wos_walk_reduce for callers that cannot allocate device memory (code under
  * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
@@ -184,7 +188,7 @@ int wos_walk_reduce(const wos_scene *scene, const wos_walk_params *params, const
  * other threads) race; calls on one stream run in order. */
 int wos_walk_reduce_noalloc(const wos_scene *scene, const wos_walk_params *params, const float *points,
                             int n_points, int seed, int chunk, float *mean, double *mean64, double *var64,
-                            float *walk_value, int *rec_prim, float *rec_t, float *basis, void *stream);
+                            float *walk_value, int *rec_prim, float *rec_t, double *basis, void *stream);
 
 /* Instrumented walk_reduce for measurement: same launch as wos_walk_reduce
  * (with the Fourier basis outputs when with_basis), and adds into counts
@@ -201,10 +205,12 @@ int wos_walk_reduce_counts(const wos_scene *scene, const wos_walk_params *params
  * Fourier coefficients: the gradient Dr.Jit's AD takes through
  * Scene.dirichlet's gathers (include/scene.cuh:455-468) and through the
  * contrib sum (include/wos.cuh:136-138), computed from the exit records of a
- * walk_reduce call with the same settings.  grad_out: dL/d mean (n_points).
- * Adds into grad_values (n_vertices, nullable) and grad_coeffs (K, nullable). */
+ * walk_reduce call with the same settings (basis: its per-point sums).
+ * grad_out: dL/d mean (n_points).  Adds into grad_values (n_vertices,
+ * nullable) and grad_coeffs (K, nullable); the coefficient gradients are
+ * reduced in a fixed order and so are bit-reproducible. */
 int wos_adjoint_values(const wos_scene *scene, int n_points, int nwalks, const int *rec_prim, const float *rec_t,
-                       const float *basis, const float *grad_out, float *grad_values, float *grad_coeffs,
+                       const double *basis, const float *grad_out, float *grad_values, float *grad_coeffs,
                        void *stream);
 
 /* Reverse mode of walk_reduce with respect to every source parameter -- what
@@ -273,7 +279,7 @@ typedef struct {
     float           *walk_value;  /* required: n_points * nwalks */
     int             *rec_prim;    /* nullable; required by the reverse pass */
     float           *rec_t;       /* nullable; required by the reverse pass */
-    float           *basis;       /* nullable: Fourier sources only */
+    double          *basis;       /* nullable: Fourier sources only; K * n_points per-point sums */
     /* reverse pass (wos_batch_adjoint), all nullable: dL/d mean per point,
      * read when the pass runs; dL/d vertex values (n_vertices) and dL/d
      * Fourier coefficients (K), added into (the caller zeroes them) */

@@ -58,6 +58,8 @@ def _p(a):
 
 
 lib.or_pcg32_next_u32.restype = ctypes.c_uint32
+lib.or_basis_exponent.restype = ctypes.c_int
+lib.or_basis_exponent.argtypes = [ctypes.c_int, ctypes.c_int]
 lib.or_pcg32_next_u32.argtypes = [ctypes.POINTER(_Pcg)]
 lib.or_pcg32_next_1d.restype = _f
 lib.or_pcg32_next_1d.argtypes = [ctypes.POINTER(_Pcg)]
@@ -293,6 +295,9 @@ class Scene:
     def walk_reduce(self, points, nwalks, nsteps=32, epsilon=1e-3, seed=0, double_sided=False,
                     prevent_fd_artifacts=False, threads=0, walk_values=False, records=False, basis=False,
                     use_is=False):
+        """or_walk_reduce: per-point FP64 mean and variance; on request the
+        per-walk values, exit records and ``basis`` (P x K: per point the sum
+        over its walks of d value / d coeff_k, in 64-bit fixed point)."""
         p = self._pts(points)
         P = p.shape[0]
         w = _WalkParams(int(nwalks), int(nsteps), float(epsilon), int(double_sided), int(prevent_fd_artifacts),

@@ -655,13 +655,29 @@ typedef struct {
     pthread_mutex_t lock;
 } reduce_job;
 
+/* Exponent e of the per-point basis sums, kept as 64-bit integers in units of
+ * 2^-e: the same bound as the library's (one interior sample adds b G w with
+ * |b| <= 1, |G w| <= R^2 (1 + ln(R / 1e-4)) < 16 R^2 for R <= 178.5, the
+ * largest ball the pdf > 1e-5 guard of include/wos.cuh:136 samples; at most
+ * nsteps samples per walk, nwalks walks), so the integer sum stays below 2^61
+ * and does not depend on the order of the additions. */
+int or_basis_exponent(int nwalks, int nsteps) {
+    const double r = 178.5;
+    const double per_sample = r * r * 16.0;
+    const double bound = (double)(nwalks > 1 ? nwalks : 1) * (double)(nsteps > 1 ? nsteps : 1) * per_sample;
+    int e = 0;
+    (void)frexp(bound, &e);
+    return 61 - e;
+}
+
 static void reduce_point(reduce_job *J, int i) {
     const or_scene *s = J->s;
     int W = J->w->nwalks, D = s->dim;
     int K = (s->source_type == 3) ? s->n_source_params / 4 : 0;
     float basis[64];
-    double bsum[64];
-    for (int k = 0; k < K; ++k) bsum[k] = 0.0;
+    long long bsum[64];
+    const double bscale = ldexp(1.0, or_basis_exponent(W, J->w->nsteps));
+    for (int k = 0; k < K; ++k) bsum[k] = 0;
     double sum = 0.0, sum2 = 0.0;
     for (int w = 0; w < W; ++w) {
         int64_t k = (int64_t)i * W + w;
@@ -678,12 +694,12 @@ static void reduce_point(reduce_job *J, int i) {
             if (D == 2) J->rec_t[k] = rec.t;
             else { J->rec_t[2 * k] = rec.uv[0]; J->rec_t[2 * k + 1] = rec.uv[1]; }
         }
-        if (rec.valid) for (int q = 0; q < K; ++q) bsum[q] += basis[q];
+        if (rec.valid) for (int q = 0; q < K; ++q) bsum[q] += llrint((double)basis[q] * bscale);
     }
     double m = sum / W;
     if (J->mean) J->mean[i] = m;
     if (J->var) J->var[i] = W > 1 ? (sum2 - sum * m) / (W - 1) : 0.0;
-    if (J->basis) for (int q = 0; q < K; ++q) J->basis[(size_t)i * K + q] = bsum[q] / W;
+    if (J->basis) for (int q = 0; q < K; ++q) J->basis[(size_t)i * K + q] = (double)bsum[q] / bscale;
 }
 
 static void *reduce_worker(void *arg) {

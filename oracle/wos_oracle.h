@@ -110,6 +110,11 @@ float or_walk_value(const or_scene *s, const or_record *rec);
  * rec_t (nullable): per-walk exit primitive (-1 invalid) and t (2D) / uv (3D,
  * 2 floats per walk).  basis (nullable): per point mean of d u / d c_k (P*K).
  * nthreads <= 0: one thread. */
+/* Fixed-point exponent of the per-point Fourier basis sums (see the C file). */
+int   or_basis_exponent(int nwalks, int nsteps);
+/* basis (nullable): n_points x K, per point the sum over its walks of
+ * d value / d coeff_k, summed as 64-bit integers in units of
+ * 2^-or_basis_exponent(nwalks, nsteps). */
 void  or_walk_reduce(const or_scene *s, const or_walk_params *w, const float *points,
                      int n_points, int64_t seed, int nthreads,
                      double *mean, double *var, float *walk_val,

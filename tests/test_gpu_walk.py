@@ -151,6 +151,50 @@ def test_adjoint_values_and_coeffs():
     assert gc.shape[0] == K
 
 
+def test_basis_sums_match_oracle_and_coefficient_gradients_are_reproducible():
+    """Per-point Fourier basis sums are 64-bit fixed-point sums: equal to the
+    oracle's (same walks, same arithmetic, same fixed-point units) at every
+    point whose walks all end on the oracle's records, and the coefficient
+    gradient built from them is bit-identical across runs and launch tunings
+    (the walks finish in a different order)."""
+    import wos_ext
+    c = C.config2(res=40)
+    sp = c["source_params"]
+    K = sp.shape[0]
+    sc = wos_ext.Scene(_t(c["vertices"]), _t(c["indices"]), _t(c["values"]), use_bvh=True, source_type=3,
+                       source_params=_t(sp))
+    W = 32
+    w = wos_ext.WoS(nwalks=W, nsteps=256, epsilon=1e-4, use_IS_for_greens=False)
+    P = c["points"].shape[0]
+    out = w.walk_reduce(_t(c["points"]), sc, seed=5, records=True, basis=True)
+    ref = O.Scene(c["vertices"], c["indices"], c["values"], 3, sp).walk_reduce(
+        c["points"], W, 256, 1e-4, seed=5, records=True, basis=True, use_is=False)
+    assert out["basis"].dtype == torch.float64 and out["basis"].numel() == K * P
+    gb = out["basis"].cpu().numpy().reshape(K, P).T
+    same = (out["rec_prim"].cpu().numpy() == ref["rec_prim"]).reshape(P, W).all(axis=1)
+    err = np.abs(gb[same] - ref["basis"][same]).max()
+    print(f"points whose walks all exit where the oracle's do: {same.mean():.4f} of {P}; "
+          f"max basis difference there {err:.3g} (scale {np.abs(ref['basis']).max():.3g})")
+    assert same.mean() > 0.97
+    np.testing.assert_array_equal(gb[same], ref["basis"][same])
+    g = _t(np.random.default_rng(1).normal(size=P).astype(np.float32))
+    _, gc1 = wos_ext.adjoint_values(sc, P, W, out["rec_prim"], out["rec_t"], g, basis=out["basis"],
+                                    want_values=False, want_coeffs=True)
+    before = wos_ext.get_tuning()
+    try:
+        wos_ext.set_tuning(chunk=96, wgs_per_cu=5, window=5)
+        out2 = w.walk_reduce(_t(c["points"]), sc, seed=5, records=True, basis=True)
+    finally:
+        wos_ext.set_tuning(*before)
+    _, gc2 = wos_ext.adjoint_values(sc, P, W, out2["rec_prim"], out2["rec_t"], g, basis=out2["basis"],
+                                    want_values=False, want_coeffs=True)
+    assert torch.equal(out["basis"], out2["basis"])
+    assert torch.equal(gc1, gc2)
+    # and against the oracle's basis in FP64: sum_i g_i / W * basis[i]
+    gd = g.cpu().numpy().astype(np.float64)
+    np.testing.assert_allclose(gc1.cpu().numpy(), (gd @ ref["basis"]) / W, rtol=2e-3, atol=1e-6)
+
+
 @pytest.mark.parametrize("dim", [2, 3])
 def test_spatial_order_leaves_results_unchanged(dim):
     """Visiting the points in Morton order only reschedules walks: every

@@ -148,3 +148,35 @@ def test_importance_sampled_radii_follow_r_log_R_over_r():
     assert np.abs(emp - F).max() < 1.63 / np.sqrt(x.size)   # KS at the 1% level
     # the uniform-area density (x^2) is rejected by the same test
     assert np.abs(emp - x * x).max() > 0.05
+
+
+def test_fourier_basis_sums_are_the_coefficient_derivative():
+    """The per-point basis sums (64-bit fixed point, units of 2^-e) are the
+    derivative of W times the point's mean with respect to each Fourier
+    coefficient: the walk is linear in the coefficients under common random
+    numbers, so central differences of the mean reproduce basis / W."""
+    c = C.config2(res=24)
+    sp = c["source_params"].astype(np.float32)
+    pts = c["points"][::3]
+    W, h = 16, 1e-2
+    sc = O.Scene(c["vertices"], c["indices"], c["values"], 3, sp)
+    r = sc.walk_reduce(pts, W, 256, 1e-4, seed=3, basis=True)
+    assert r["basis"].shape == (pts.shape[0], sp.shape[0])
+    for k in (0, 3, 7):
+        up, dn = sp.copy(), sp.copy()
+        up[k, 3] += h
+        dn[k, 3] -= h
+        mu = O.Scene(c["vertices"], c["indices"], c["values"], 3, up).walk_reduce(pts, W, 256, 1e-4, seed=3)["mean"]
+        md = O.Scene(c["vertices"], c["indices"], c["values"], 3, dn).walk_reduce(pts, W, 256, 1e-4, seed=3)["mean"]
+        fd = (mu - md) / (2 * h)
+        np.testing.assert_allclose(r["basis"][:, k] / W, fd, rtol=1e-3, atol=2e-5)
+
+
+def test_basis_exponent_bounds_the_fixed_point_sums():
+    """2^e times the bound on a point's basis sum (nwalks walks of at most
+    nsteps samples, each below 16 R^2 for R = 178.5) stays below 2^61, and
+    e is the largest such exponent."""
+    for W, n in ((1, 1), (16, 256), (128, 256), (4096, 1024), (1 << 20, 4096)):
+        e = O.lib.or_basis_exponent(W, n)
+        bound = W * n * 178.5 ** 2 * 16.0
+        assert bound * 2.0 ** e < 2.0 ** 61 <= bound * 2.0 ** (e + 1)
