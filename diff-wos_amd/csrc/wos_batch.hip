@@ -19,6 +19,7 @@
 
 #include "../../include/wos_amd.h"
 #include "wos_build.h"
+#include "wos_grad.h"
 
 namespace wos {
 
@@ -86,8 +87,9 @@ __device__ __forceinline__ int find_item(const long long *begins, int n, long lo
     return lo;
 }
 
-template <int D>
-__global__ __launch_bounds__(kBlock) void k_batch_init(const BatchItem *items, int n_items, long long n_total_pts,
+// Start query of every point of a group's items (BatchItem or GradItem).
+template <int D, class Item>
+__global__ __launch_bounds__(kBlock) void k_batch_init(const Item *items, int n_items, long long n_total_pts,
                                                        int *iprim, float *isd, int2 *spill, long stride) {
     __shared__ long long s_begin[kMaxBatchItems];
     for (int i = threadIdx.x; i < n_items; i += kBlock) s_begin[i] = items[i].point_begin;
@@ -454,6 +456,87 @@ __global__ void k_batch_zero(unsigned long long *c, int n) {
     if (threadIdx.x < n) c[threadIdx.x] = 0ull;
 }
 
+// A shape-gradient item: CustomSolver backward with OursCUDA / Ours3D
+// (wos/solver.py:109-114, wos/wos_grad.py:130-151, wos/wos_grad_3d.py:
+// 117-137) on one scene, its own sample settings and the batch item's seed.
+struct GradItem {
+    SceneView v;
+    GradSettings g;
+    const float *points;
+    int n_points;
+    int32_t seed;
+    long long walk_begin;       // offset in the group's sample space
+    long long point_begin;      // offset in the group's point space (start records)
+    const float *grad_out;      // dL/d mean per point
+    float *grad_vertices;       // adds dL/d vertex positions (n_vertices * dim)
+};
+
+// Sample source of the batched shape-gradient launch (grad_samples in
+// wos_grad.h): a claimed slot of the group's sample space is resolved to its
+// item by binary search over the item offsets in LDS; the item's scene and
+// settings are read from the item table (cached; items are few and shared),
+// except the node pointer and pruning margin of the traversal, which the
+// lane keeps in registers.
+struct GradBatch {
+    const GradItem *items;
+    int n_items;
+    const long long *begins;    // LDS: walk_begin of each item
+    long long n_samples;
+    unsigned long long *queue;
+    int claim, visits;
+    const int *iprim;
+    const float *isd;
+    __device__ __forceinline__ long long total() const { return n_samples; }
+    __device__ __forceinline__ unsigned long long *counter() const { return queue; }
+    __device__ __forceinline__ int chunk() const { return claim; }
+    __device__ __forceinline__ int window() const { return visits; }
+    __device__ __forceinline__ const SceneView &scene(int it) const { return items[it].v; }
+    __device__ __forceinline__ const GradSettings &settings(int it) const { return items[it].g; }
+    __device__ __forceinline__ const SceneView &trav(int, const SceneView &tv) const { return tv; }
+    __device__ __forceinline__ void bind(int it, SceneView &tv) const {
+        tv.nodes = items[it].v.nodes;
+        tv.tol = items[it].v.tol;
+        tv.n_top = 0;
+    }
+    __device__ __forceinline__ int take(long long wslot, long long &i, long long &kk) const {
+        const int it = find_item(begins, n_items, wslot);
+        kk = wslot - begins[it];
+        i = point_of_walk(kk, items[it].g.nwalks);
+        return it;
+    }
+    __device__ __forceinline__ int start_prim(int it, long long i) const { return iprim[items[it].point_begin + i]; }
+    __device__ __forceinline__ float start_sd(int it, long long i) const { return isd[items[it].point_begin + i]; }
+    template <int D>
+    __device__ __forceinline__ void load(int it, long long i, float *p) const { load_point<D>(items[it].points, i, p); }
+    __device__ __forceinline__ int32_t seed(int it) const { return items[it].seed; }
+    template <int D>
+    __device__ __forceinline__ void skip(int, long long) const {}
+    template <int D>
+    __device__ __forceinline__ void finish(int it, long long k, bool ok, int rprim, float rt0, float rt1,
+                                           const float *gu) const {
+        const GradItem &b = items[it];
+        if (ok && b.grad_vertices) {
+            const int W = b.g.nwalks;
+            scatter_vertex_grad<D>(b.v, b.grad_out[point_of_walk(k, W)] / (float)W, rprim, rt0, rt1, gu,
+                                   b.grad_vertices);
+        }
+    }
+};
+
+// Shape-gradient samples of every gradient item of a dimension group in one
+// persistent launch; 80 VGPRs as the single-scene kernel.
+template <int D>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6))) void k_batch_grad(
+    const GradItem *items, int n_items, long long n_samples, unsigned long long *queue, int claim, int visits,
+    const int *iprim, const float *isd, int2 *spill, long stride) {
+    __shared__ long long s_begin[kMaxBatchItems];
+    for (int i = threadIdx.x; i < n_items; i += kBlock) s_begin[i] = items[i].walk_begin;
+    __syncthreads();
+    const TravStack st{spill, stride};
+    const GradBatch src{items, n_items, s_begin, n_samples, queue, claim, visits, iprim, isd};
+    grad_samples<D, false>(src, st, threadIdx.x, (long)blockIdx.x * kBlock + threadIdx.x);
+}
+
 struct BatchGroup {
     BatchItem *items = nullptr;   // device copy
     int n_items = 0;
@@ -464,6 +547,12 @@ struct BatchGroup {
     AdjointTile *tiles = nullptr; // device copy (null: no item has a reverse pass)
     int n_tiles = 0;
     int hist = 0;                 // LDS histogram entries of the reverse pass
+    // shape-gradient items (wos_batch_shape_gradient)
+    GradItem *gitems = nullptr;   // device copy
+    int n_gitems = 0;
+    long long gsamples = 0, gpoints = 0;
+    int *giprim = nullptr;
+    float *gisd = nullptr;
 };
 
 }  // namespace wos
@@ -503,6 +592,9 @@ static void free_batch(wos_batch *b) {
         (void)hipFree(g.iprim);
         (void)hipFree(g.isd);
         (void)hipFree(g.tiles);
+        (void)hipFree(g.gitems);
+        (void)hipFree(g.giprim);
+        (void)hipFree(g.gisd);
     }
     (void)hipFree(b->counters);
     (void)hipFree(b->spill);
@@ -532,6 +624,7 @@ extern "C" int wos_batch_create(const wos_batch_item *items, int n_items, wos_ba
     BREQ(out && items && n_items > 0, "items and out are required");
     *out = nullptr;
     std::vector<BatchItem> grp[2];
+    std::vector<GradItem> ggrp[2];
     std::vector<std::pair<const wos_scene *, unsigned long long>> scenes;
     bool any_is = false;
     for (int j = 0; j < n_items; ++j) {
@@ -573,6 +666,24 @@ extern "C" int wos_batch_create(const wos_batch_item *items, int n_items, wos_ba
             BREQ(it.rec_prim && it.rec_t, "item %d: the reverse pass needs the exit records (rec_prim, rec_t)", j);
             BREQ(!it.grad_coeffs || it.basis, "item %d: coefficient gradients need the basis output", j);
         }
+        if (it.n_points > 0 && it.grad_vertices) {
+            const wos_grad_params &gp = it.grad_params;
+            BREQ(it.grad_out, "item %d: the shape gradient needs grad_out", j);
+            BREQ(v.use_bvh, "item %d: the shape gradient needs the scene's BVH (use_bvh)", j);
+            BREQ(gp.nwalks >= 1 && gp.nsteps >= 0 && gp.epsilon > 0.0f && gp.epsilon2 > 0.0f && gp.clamping >= 0.0f,
+                 "item %d: bad shape-gradient settings", j);
+            BREQ((long long)it.n_points * gp.nwalks < (1LL << 32), "item %d: at most 2^32 - 1 samples", j);
+            GradItem gi;
+            gi.v = v;
+            gi.g = GradSettings{gp.nwalks, gp.nsteps, gp.epsilon, gp.epsilon2, gp.clamping, gp.double_sided,
+                                gp.control_variates};
+            gi.points = it.points;
+            gi.n_points = it.n_points;
+            gi.seed = it.seed;
+            gi.grad_out = it.grad_out;
+            gi.grad_vertices = it.grad_vertices;
+            ggrp[v.dim == 2 ? 0 : 1].push_back(gi);
+        }
         grp[v.dim == 2 ? 0 : 1].push_back(b);
         scenes.emplace_back(it.scene, scene_generation(it.scene));
     }
@@ -606,6 +717,29 @@ extern "C" int wos_batch_create(const wos_batch_item *items, int n_items, wos_ba
         g.points = pb;
         g.max_depth = depth;
         g.n_tiles = (int)tiles.size();
+        {
+            auto &gv = ggrp[d];
+            long long sb = 0, gb = 0;
+            for (GradItem &gi : gv) {
+                gi.walk_begin = sb;
+                gi.point_begin = gb;
+                sb += (long long)gi.n_points * gi.g.nwalks;
+                gb += gi.n_points;
+            }
+            g.n_gitems = (int)gv.size();
+            g.gsamples = sb;
+            g.gpoints = gb;
+            if (g.n_gitems &&
+                (hipMalloc((void **)&g.gitems, sizeof(GradItem) * gv.size()) != hipSuccess ||
+                 hipMalloc((void **)&g.giprim, sizeof(int) * (size_t)std::max(gb, 1LL)) != hipSuccess ||
+                 hipMalloc((void **)&g.gisd, sizeof(float) * (size_t)std::max(gb, 1LL)) != hipSuccess ||
+                 hipMemcpy(g.gitems, gv.data(), sizeof(GradItem) * gv.size(), hipMemcpyHostToDevice) != hipSuccess)) {
+                (void)hipGetLastError();
+                free_batch(B);
+                set_error("cannot allocate the batch shape-gradient tables");
+                return -1;
+            }
+        }
         if (!g.n_items) continue;
         if (hipMalloc((void **)&g.items, sizeof(BatchItem) * v.size()) != hipSuccess ||
             hipMalloc((void **)&g.iprim, sizeof(int) * (size_t)std::max(pb, 1LL)) != hipSuccess ||
@@ -621,7 +755,8 @@ extern "C" int wos_batch_create(const wos_batch_item *items, int n_items, wos_ba
             return -1;
         }
     }
-    if (hipMalloc((void **)&B->counters, 2 * sizeof(unsigned long long)) != hipSuccess) {
+    // queue heads: primal walks of each group, then shape-gradient samples
+    if (hipMalloc((void **)&B->counters, 4 * sizeof(unsigned long long)) != hipSuccess) {
         (void)hipGetLastError();
         free_batch(B);
         set_error("cannot allocate the batch counters");
@@ -665,11 +800,11 @@ extern "C" int wos_batch_run(wos_batch *B, void *stream) {
         int window = window_for(d == 0 ? 2 : 3);
         k_batch_basis<<<dim3(256, g.n_items), kBlock, 0, st>>>(g.items, 0);
         if (d == 0) {
-            k_batch_init<2><<<gi, kBlock, 0, st>>>(g.items, g.n_items, g.points, g.iprim, g.isd, spill, stride);
+            k_batch_init<2, BatchItem><<<gi, kBlock, 0, st>>>(g.items, g.n_items, g.points, g.iprim, g.isd, spill, stride);
             k_batch_walk<2><<<gwk, kBlock, 0, st>>>(g.items, g.n_items, g.walks, B->counters + d, chunk, window,
                                                     g.iprim, g.isd, spill, stride);
         } else {
-            k_batch_init<3><<<gi, kBlock, 0, st>>>(g.items, g.n_items, g.points, g.iprim, g.isd, spill, stride);
+            k_batch_init<3, BatchItem><<<gi, kBlock, 0, st>>>(g.items, g.n_items, g.points, g.iprim, g.isd, spill, stride);
             k_batch_walk<3><<<gwk, kBlock, 0, st>>>(g.items, g.n_items, g.walks, B->counters + d, chunk, window,
                                                     g.iprim, g.isd, spill, stride);
         }
@@ -679,6 +814,37 @@ extern "C" int wos_batch_run(wos_batch *B, void *stream) {
     }
     hipError_t e = hipGetLastError();
     BREQ(e == hipSuccess, "batch launch: %s", hipGetErrorString(e));
+    return 0;
+}
+
+extern "C" int wos_batch_shape_gradient(wos_batch *B, void *stream) {
+    BREQ(B, "batch is NULL");
+    if (check_scenes(B)) return -1;
+    hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+    const int cap = (int)(B->spill_lanes / kBlock);
+    k_batch_zero<<<1, 64, 0, st>>>(B->counters + 2, 2);
+    for (int d = 0; d < 2; ++d) {
+        BatchGroup &g = B->g[d];
+        if (!g.n_gitems || !g.gsamples) continue;
+        const long stride = B->spill_lanes;
+        const int gi = (int)std::min<long>(std::max<long>((g.gpoints + kBlock - 1) / kBlock, 1), cap);
+        const int gw = (int)std::min<long>(std::max<long>((g.gsamples / 64 + 4) / 4, 1), cap);
+        const int chunk = g_tuning.chunk > 0 ? g_tuning.chunk : 128;
+        const int window = window_for(d == 0 ? 2 : 3);
+        if (d == 0) {
+            k_batch_init<2, GradItem><<<gi, kBlock, 0, st>>>(g.gitems, g.n_gitems, g.gpoints, g.giprim, g.gisd,
+                                                             B->spill, stride);
+            k_batch_grad<2><<<gw, kBlock, 0, st>>>(g.gitems, g.n_gitems, g.gsamples, B->counters + 2 + d, chunk,
+                                                   window, g.giprim, g.gisd, B->spill, stride);
+        } else {
+            k_batch_init<3, GradItem><<<gi, kBlock, 0, st>>>(g.gitems, g.n_gitems, g.gpoints, g.giprim, g.gisd,
+                                                             B->spill, stride);
+            k_batch_grad<3><<<gw, kBlock, 0, st>>>(g.gitems, g.n_gitems, g.gsamples, B->counters + 2 + d, chunk,
+                                                   window, g.giprim, g.gisd, B->spill, stride);
+        }
+    }
+    hipError_t e = hipGetLastError();
+    BREQ(e == hipSuccess, "batch shape-gradient launch: %s", hipGetErrorString(e));
     return 0;
 }
 

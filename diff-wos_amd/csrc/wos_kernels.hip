@@ -17,6 +17,7 @@
 #include <mutex>
 
 #include "wos_build.h"
+#include "wos_grad.h"
 
 namespace wos {
 
@@ -883,409 +884,87 @@ struct GradArgs {
     long spill_stride;
 };
 
-// Phases of a shape-gradient sample (wos/wos_grad.py:130-151 with
-// wos/wos.py:59-129; wos/wos_grad_3d.py:117-137 with wos/wos3d.py:97-156).
-enum : int { kPrimary = 0, kLib = 1, kSecQuery = 2, kSecWalk = 3 };
-
-// Half the smallest bounding-box extent: the upper end of the LIB bisection
-// (include/scene.cuh:225-249).
-template <int D>
-__device__ __forceinline__ float lib_upper(const SceneView &s) {
-    float b = fminf(fabsf(s.amax[0] - s.amin[0]), fabsf(s.amax[1] - s.amin[1]));
-    if (D == 3) b = fminf(b, fabsf(s.amax[2] - s.amin[2]));
-    return b / 2.0f;
-}
-
-// Point y of secondary evaluation `side` on the sphere of radius R around
-// c = x_b - n R (n = inward normal), from the sample's angle draws.
-template <int D>
-__device__ __forceinline__ void secondary_point(const float *xb, const float *n, float R, const float *dr, int side,
-                                                float *y) {
-    float c[3];
-    for (int q = 0; q < 3; ++q) c[q] = fmaf(-n[q], R, xb[q]);
-    if constexpr (D == 2) {
-        const float sn = dr[0], cs = dr[1];
-        const float pd[2] = {-n[1], n[0]};
-        const float ss = side ? -sn : sn;
-        y[0] = fmaf(R, fmaf(n[0], cs, pd[0] * ss), c[0]);
-        y[1] = fmaf(R, fmaf(n[1], cs, pd[1] * ss), c[1]);
-        y[2] = 0.0f;
-    } else {
-        const float ct = dr[0], sn = dr[1], cs = dr[2];
-        float up[3] = {0.0f, 0.0f, 1.0f};
-        if (!(fabsf(n[2]) < 0.9f)) { up[0] = 1.0f; up[2] = 0.0f; }
-        float pd[3];
-        cross3(n, up, pd);
-        float l = sqrtf(dot3(pd, pd));
-        pd[0] = pd[0] / l; pd[1] = pd[1] / l; pd[2] = pd[2] / l;
-        float qd[3];
-        cross3(n, pd, qd);
-        const float r = sqrtf(fmaxf(0.0f, 1.0f - ct * ct));
-        const float ss = side ? -sn : sn, cc = side ? -cs : cs;
-        for (int q = 0; q < 3; ++q) y[q] = fmaf(R, fmaf(ct, n[q], r * fmaf(cc, pd[q], ss * qd[q])), c[q]);
+// Sample source of the single-scene shape-gradient launch (grad_samples in
+// wos_grad.h): one scene and the launch arguments, uniform across the grid.
+struct GradOne {
+    const SceneView &s;
+    const GradArgs &a;
+    __device__ __forceinline__ long long total() const { return a.total; }
+    __device__ __forceinline__ unsigned long long *counter() const { return a.counter; }
+    __device__ __forceinline__ int chunk() const { return a.chunk; }
+    __device__ __forceinline__ int window() const { return a.window; }
+    __device__ __forceinline__ const SceneView &scene(int) const { return s; }
+    __device__ __forceinline__ const GradSettings &settings(int) const { return a.g; }
+    __device__ __forceinline__ const SceneView &trav(int, const SceneView &) const { return s; }
+    __device__ __forceinline__ void bind(int, SceneView &) const {}
+    // slot -> (point, sample): points in the visiting order
+    __device__ __forceinline__ int take(long long wslot, long long &i, long long &kk) const {
+        const int W = a.g.nwalks;
+        const uint32_t pi = point_of_walk(wslot, W);
+        i = a.order ? (long long)a.order[pi] : (long long)pi;
+        kk = i * W + ((uint32_t)wslot - pi * (uint32_t)W);
+        return 0;
     }
-}
-
-// Tangential derivative of the boundary data on primitive f (wos/scene.py:
-// 451-460, wos/wos3d.py:137-156), added to gu.
-template <int D>
-__device__ __forceinline__ void add_tangent_derivative(const SceneView &s, int f, float *gu) {
-    if constexpr (D == 2) {
-        int i0 = s.idx[2 * f], i1 = s.idx[2 * f + 1];
-        float ex = s.vtx[2 * i1] - s.vtx[2 * i0], ey = s.vtx[2 * i1 + 1] - s.vtx[2 * i0 + 1];
-        float len = sqrtf(dot2(ex, ey, ex, ey));
-        float dv = (s.val[i1] - s.val[i0]) / len;
-        gu[0] += dv * (ex / len);
-        gu[1] += dv * (ey / len);
-    } else {
-        int i0 = s.idx[3 * f], i1 = s.idx[3 * f + 1], i2 = s.idx[3 * f + 2];
-        float a[3], ab[3], ac[3];
-        for (int q = 0; q < 3; ++q) {
-            a[q] = s.vtx[3 * i0 + q];
-            ab[q] = s.vtx[3 * i1 + q] - a[q];
-            ac[q] = s.vtx[3 * i2 + q] - a[q];
+    __device__ __forceinline__ int start_prim(int, long long i) const { return a.init_prim[i]; }
+    __device__ __forceinline__ float start_sd(int, long long i) const { return a.init_sd[i]; }
+    template <int D>
+    __device__ __forceinline__ void load(int, long long i, float *p) const { load_point<D>(a.points, i, p); }
+    __device__ __forceinline__ int32_t seed(int) const { return a.seed; }
+    template <int D>
+    __device__ __forceinline__ void skip(int, long long kk) const {
+        if (a.sample_grad_u)
+            for (int c = 0; c < D; ++c) a.sample_grad_u[kk * D + c] = 0.0f;
+        if (a.sample_jvp) a.sample_jvp[kk] = 0.0f;
+    }
+    // A finished sample: its spatial gradient, its forward-mode derivative
+    // and the scatter -g_i / W w grad u onto the exit primitive's vertices.
+    template <int D>
+    __device__ __forceinline__ void finish(int, long long k, bool ok, int rprim, float rt0, float rt1,
+                                           const float *gu) const {
+        const int W = a.g.nwalks;
+        if (a.sample_grad_u)
+            for (int c = 0; c < D; ++c) a.sample_grad_u[k * D + c] = gu[c];
+        if (a.sample_jvp) {
+            // forward mode of the same integrand: the exit point moves with
+            // the interpolated vertex velocity
+            float jv = 0.0f;
+            if (ok) {
+                if constexpr (D == 2) {
+                    const int i0 = s.idx[2 * rprim], i1 = s.idx[2 * rprim + 1];
+                    for (int c = 0; c < 2; ++c)
+                        jv = fmaf(-gu[c], fmaf(rt0, a.velocity[2 * i1 + c], (1.0f - rt0) * a.velocity[2 * i0 + c]), jv);
+                } else {
+                    const int i0 = s.idx[3 * rprim], i1 = s.idx[3 * rprim + 1], i2 = s.idx[3 * rprim + 2];
+                    const float wa = 1.0f - rt0 - rt1;
+                    for (int c = 0; c < 3; ++c) {
+                        const float vc = fmaf(rt1, a.velocity[3 * i2 + c],
+                                              fmaf(rt0, a.velocity[3 * i1 + c], wa * a.velocity[3 * i0 + c]));
+                        jv = fmaf(-gu[c], vc, jv);
+                    }
+                }
+            }
+            a.sample_jvp[k] = jv;
         }
-        float lab = sqrtf(dot3(ab, ab));
-        float t[3] = {ab[0] / lab, ab[1] / lab, ab[2] / lab};
-        float fn[3];
-        cross3(ab, ac, fn);
-        float lf = sqrtf(dot3(fn, fn));
-        fn[0] = fn[0] / lf; fn[1] = fn[1] / lf; fn[2] = fn[2] / lf;
-        float nn[3];
-        cross3(fn, t, nn);
-        float ln = sqrtf(dot3(nn, nn));
-        nn[0] = nn[0] / ln; nn[1] = nn[1] / ln; nn[2] = nn[2] / ln;
-        float va = s.val[i0], vb = s.val[i1], vc = s.val[i2];
-        float gt = (vb - va) / lab;
-        float gn = ((vc - va) - gt * dot3(ac, t)) / dot3(ac, nn);
-        for (int q = 0; q < 3; ++q) gu[q] += fmaf(gt, t[q], gn * nn[q]);
+        if (ok && a.grad_vertices && a.grad_out)
+            scatter_vertex_grad<D>(s, a.grad_out[point_of_walk(k, W)] / (float)W, rprim, rt0, rt1, gu,
+                                   a.grad_vertices);
     }
-}
+};
 
-// Shape-gradient samples (OursCUDA / Ours3D), one per (point, walk), on the
-// same persistent machinery as the walk kernel: every closest-primitive
-// query of a sample -- the primary walk's steps, the ten LIB bisection
-// queries, the two secondary starts and their walks' steps -- is a
-// resumable query advanced `window` node visits per round, and lanes take
-// new samples from the wave's chunk as theirs finish.  Queries are seeded
-// with a primitive known to be near (the previous step's, or the primary
-// exit primitive, which is within m of a LIB point and 2R of a secondary
-// start), so they need no distance bound; the answer is the exact closest
-// primitive either way, and the random draws follow the single-lane order
-// (primary walk, angle draws, secondary walk 0, secondary walk 1).
-// Held to 80 VGPRs (6 waves per SIMD): the 3D sample state needs ~106 (4
-// waves); at 80, with a few registers spilled, the 200k-triangle shape
-// gradient measured 7 % faster on one MI355X and the same on another.  The
-// 2D state fits.  Finished samples write in batches as in the walk kernel
-// (2D: 4 % faster).  STATS: count bound tests, primitive tests, queries and
-// walk steps into s.stats (the instrumented launch of the measurements).
+// Shape-gradient samples (OursCUDA / Ours3D) of one scene: grad_samples
+// (wos_grad.h) over GradOne.  Held to 80 VGPRs (6 waves per SIMD): the 3D
+// sample state needs ~106 (4 waves); at 80, with a few registers spilled,
+// the 200k-triangle shape gradient measured 7 % faster on one MI355X and the
+// same on another.  The 2D state fits.  Finished samples write in batches as
+// in the walk kernel (2D: 4 % faster).  STATS: count bound tests, primitive
+// tests, queries and walk steps into s.stats (the instrumented launch of
+// the measurements).
 template <int D, bool STATS>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6))) void k_grad_persistent(
     SceneView s, GradArgs a) {
     DECLARE_STACK(a.spill, a.spill_stride);
-    const int lane = __lane_id();
-    const GradSettings g = a.g;
-    const int W = g.nwalks;
-    long long k = -1;
-    int mode = kIdle, phase = kPrimary;
-    float p[3] = {0.0f, 0.0f, 0.0f};
-    float d = 0.0f;
-    int step = 0, prim = -1;
-    Pcg32 rng;
-    rng.state = rng.inc = 0;
-    Query q;
-    q.node = q.sp = 0;
-    q.best = q.thr = 0.0f;
-    q.best_id = -1;
-    q.nbox = q.nprim = 0;
-    // primary exit record and the sample's derived quantities
-    float xb[3] = {0.0f, 0.0f, 0.0f}, nb[3] = {0.0f, 0.0f, 0.0f};   // exit point, rec.n
-    int rprim = -1;
-    float rt0 = 0.0f, rt1 = 0.0f, u_ref = 0.0f;
-    float la = 0.0f, lb = 0.0f, R = 0.0f, grad = 0.0f;
-    float dr[3] = {0.0f, 0.0f, 0.0f};
-    int lit = 0, side = 0;
-    bool start_known = false, done_ok = false;
-    float wside = -1.0f;   // sign of the sdf at the sample's start point
-    long long c_next = 0, c_end = 0;
-    bool exhausted = false;
-
-    for (;;) {
-        // 1. completed queries
-        if (mode == kReady) {
-            const int found = q.best_id;
-            bool finish = false, ok = false;
-            float u_side = 0.0f;
-            bool have_u = false;
-            if (phase == kPrimary || phase == kSecWalk) {
-                if (!start_known) {
-                    prim = found;
-                    d = sqrtf(q.best);
-                }
-                start_known = false;
-                const float eps = phase == kPrimary ? g.epsilon : g.epsilon2;
-                if (d < eps && step < g.nsteps) {
-                    Rec r;
-                    // the primary exit's normal points to the side the walk
-                    // started on (its sign test is arbitrary at depth ~0)
-                    closest_record<D>(s, prim, p, r, wside);
-                    if (phase == kPrimary) {
-                        for (int c = 0; c < 3; ++c) { xb[c] = r.p[c]; nb[c] = r.n[c]; }
-                        rprim = prim;
-                        rt0 = r.t;
-                        if (D == 3) { rt0 = r.uv[0]; rt1 = r.uv[1]; }
-                        u_ref = dirichlet_at<D>(s, prim, r.t, r.uv[0], r.uv[1]);
-                        la = 1e-3f;
-                        lb = lib_upper<D>(s);
-                        lit = 0;
-                        phase = kLib;
-                    } else {
-                        u_side = dirichlet_at<D>(s, prim, r.t, r.uv[0], r.uv[1]);
-                        have_u = true;
-                    }
-                } else if (step >= g.nsteps) {
-                    if (phase == kPrimary) finish = true;
-                    else { u_side = 0.0f; have_u = true; }
-                } else {
-                    mode = kStart;
-                }
-            } else if (phase == kLib) {
-                const float m = (la + lb) / 2.0f;
-                const float dd = fabsf(signed_dist<D>(s, found, p));
-                if (fabsf(dd - m) < 1e-5f) la = m; else lb = m;
-                if (++lit == 10) {
-                    R = la;
-                    if constexpr (D == 2) {
-                        float t = (float)rng.next_f64();
-                        const float lo = g.clamping * kInv2Pi;
-                        t = fminf(fmaxf(t, lo), 1.0f - lo);
-                        sincos_turns(t, dr[0], dr[1]);
-                    } else {
-                        const double z64 = 1.0 - 2.0 * rng.next_f64();
-                        const double ph64 = rng.next_f64();
-                        dr[0] = fminf(fmaxf((float)z64, -1.0f), 1.0f - g.clamping);
-                        sincos_turns((float)ph64, dr[1], dr[2]);
-                    }
-                    side = 0;
-                    grad = 0.0f;
-                    phase = kSecQuery;
-                }
-            } else {   // kSecQuery: the start of secondary walk `side`
-                const float sd = signed_dist<D>(s, found, p);
-                if (fabsf(sd) < g.epsilon2) {
-                    // wos/wos.py:96-98: the walk would stop at once
-                    Rec r;
-                    closest_record<D>(s, found, p, r);
-                    u_side = dirichlet_at<D>(s, found, r.t, r.uv[0], r.uv[1]);
-                    have_u = true;
-                } else if ((sd >= 0.0f && !g.double_sided) || g.nsteps <= 0) {
-                    u_side = 0.0f;   // invalid walk (include/wos.cuh:102-103, 144)
-                    have_u = true;
-                } else {
-                    prim = found;
-                    d = fabsf(sd);
-                    step = 0;
-                    phase = kSecWalk;
-                    mode = kStart;
-                }
-            }
-            if (have_u) {
-                float P;
-                if constexpr (D == 2) P = 1.0f / (dr[1] - 1.0f);
-                else {
-                    const float om = 1.0f - dr[0];
-                    P = -1.0f / (1.41421356237309504880f * (om * sqrtf(om)));
-                }
-                const float num = (D == 3 || g.control_variates) ? (u_side - u_ref) : u_side;
-                grad = grad + P * num / R;
-                if (++side == 2) {
-                    finish = true;
-                    ok = true;
-                } else {
-                    phase = kSecQuery;
-                }
-            }
-            if (finish) {
-                done_ok = ok;
-                mode = kDone;
-            } else if (mode == kReady) {
-                // a LIB or secondary-start query comes next: p and the seed
-                if (phase == kLib) {
-                    const float m = (la + lb) / 2.0f;
-                    for (int c = 0; c < D; ++c) p[c] = fmaf(nb[c], m, xb[c]);
-                } else {
-                    const float n[3] = {-nb[0], -nb[1], D == 3 ? -nb[2] : 0.0f};
-                    secondary_point<D>(xb, n, R, dr, side, p);
-                }
-                const float seed_d2 = prim_d2<D>(s, rprim, p);
-                if constexpr (STATS) {
-                    atomicAdd(&s.stats[1], 1ull);   // the seed primitive
-                    atomicAdd(&s.stats[2], 1ull);
-                }
-                if (s.n_prims == 1) {
-                    q.best = seed_d2;
-                    q.best_id = rprim;
-                } else {
-                    query_begin_at<D>(q, s, p, __builtin_inff(), rprim, seed_d2, st, slot, lane_g);
-                    mode = kQuery;
-                }
-            }
-        }
-        // finished samples write their outputs once kFinishBatch lanes of
-        // the wave are done or idle (or nothing else is left to run)
-        if (__popcll(__ballot(mode == kDone || mode == kIdle)) >= kFinishBatch || exhausted ||
-            __ballot(mode != kDone && mode != kIdle) == 0) {
-            if (mode == kDone) {
-                const long long i = point_of_walk(k, W);
-                float gu[3] = {0.0f, 0.0f, 0.0f};
-                if (done_ok) {
-                    const float n[3] = {-nb[0], -nb[1], D == 3 ? -nb[2] : 0.0f};
-                    const float gr = grad / 2.0f;
-                    for (int c = 0; c < D; ++c) gu[c] = gr * n[c];
-                    add_tangent_derivative<D>(s, rprim, gu);
-                }
-                if (a.sample_grad_u)
-                    for (int c = 0; c < D; ++c) a.sample_grad_u[k * D + c] = gu[c];
-                if (a.sample_jvp) {
-                    // forward mode of the same integrand: the exit point moves
-                    // with the interpolated vertex velocity
-                    float jv = 0.0f;
-                    if (done_ok) {
-                        if constexpr (D == 2) {
-                            const int i0 = s.idx[2 * rprim], i1 = s.idx[2 * rprim + 1];
-                            for (int c = 0; c < 2; ++c)
-                                jv = fmaf(-gu[c], fmaf(rt0, a.velocity[2 * i1 + c], (1.0f - rt0) * a.velocity[2 * i0 + c]), jv);
-                        } else {
-                            const int i0 = s.idx[3 * rprim], i1 = s.idx[3 * rprim + 1], i2 = s.idx[3 * rprim + 2];
-                            const float wa = 1.0f - rt0 - rt1;
-                            for (int c = 0; c < 3; ++c) {
-                                const float vc = fmaf(rt1, a.velocity[3 * i2 + c],
-                                                      fmaf(rt0, a.velocity[3 * i1 + c], wa * a.velocity[3 * i0 + c]));
-                                jv = fmaf(-gu[c], vc, jv);
-                            }
-                        }
-                    }
-                    a.sample_jvp[k] = jv;
-                }
-                if (done_ok && a.grad_vertices && a.grad_out) {
-                    const float gi = a.grad_out[i] / (float)W;
-                    if constexpr (D == 2) {
-                        int i0 = s.idx[2 * rprim], i1 = s.idx[2 * rprim + 1];
-                        for (int c = 0; c < 2; ++c) {
-                            atomicAdd(&a.grad_vertices[2 * i0 + c], -gi * (1.0f - rt0) * gu[c]);
-                            atomicAdd(&a.grad_vertices[2 * i1 + c], -gi * rt0 * gu[c]);
-                        }
-                    } else {
-                        int i0 = s.idx[3 * rprim], i1 = s.idx[3 * rprim + 1], i2 = s.idx[3 * rprim + 2];
-                        const float wa = 1.0f - rt0 - rt1;
-                        for (int c = 0; c < 3; ++c) {
-                            atomicAdd(&a.grad_vertices[3 * i0 + c], -gi * wa * gu[c]);
-                            atomicAdd(&a.grad_vertices[3 * i1 + c], -gi * rt0 * gu[c]);
-                            atomicAdd(&a.grad_vertices[3 * i2 + c], -gi * rt1 * gu[c]);
-                        }
-                    }
-                }
-                mode = kIdle;
-                k = -1;
-            }
-        }
-        // 2. refill idle lanes from the wave's chunk
-        uint64_t idle = __ballot(mode == kIdle);
-        if (idle != 0 && !exhausted) {
-            int need = __popcll(idle);
-            long long avail = c_end - c_next;
-            long long base2 = 0, end2 = 0;
-            bool fresh = false;
-            if (avail < need) {
-                unsigned long long b = 0;
-                if (lane == 0) b = atomicAdd(a.counter, (unsigned long long)a.chunk);
-                uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)b);
-                uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(b >> 32));
-                long long bb = (long long)(((uint64_t)hi << 32) | lo);
-                if (bb >= a.total) {
-                    exhausted = true;
-                } else {
-                    fresh = true;
-                    base2 = bb;
-                    end2 = bb + a.chunk < a.total ? bb + a.chunk : a.total;
-                }
-            }
-            if (mode == kIdle) {
-                int r = rank_in_mask(idle);
-                long long wslot = -1;   // walk slot (slot is the lane's LDS stack slot)
-                if (r < avail) wslot = c_next + r;
-                else if (fresh && (r - avail) < end2 - base2) wslot = base2 + (r - avail);
-                if (wslot >= 0) {
-                    const uint32_t pi = point_of_walk(wslot, W);
-                    const long long i = a.order ? (long long)a.order[pi] : (long long)pi;
-                    const long long kk = i * W + ((uint32_t)wslot - pi * (uint32_t)W);
-                    const int ip = a.init_prim[i];
-                    const float sd = a.init_sd[i];
-                    k = kk;
-                    load_point<D>(a.points, i, p);
-                    rng = walk_stream(a.seed, i, kk);
-                    phase = kPrimary;
-                    // include/wos.cuh:100-106 (no first-step halving here)
-                    const bool outside = ip < 0 || (sd >= 0.0f && !g.double_sided);
-                    if (outside || g.nsteps <= 0) {
-                        if (a.sample_grad_u)
-                            for (int c = 0; c < D; ++c) a.sample_grad_u[kk * D + c] = 0.0f;
-                        if (a.sample_jvp) a.sample_jvp[kk] = 0.0f;
-                        k = -1;
-                    } else {
-                        prim = ip;
-                        d = fabsf(sd);
-                        step = 0;
-                        start_known = true;   // the first test uses d = |sd| as is
-                        wside = sgn(sd);
-                        mode = kReady;
-                    }
-                }
-            }
-            if (avail >= need) {
-                c_next += need;
-            } else if (fresh) {
-                long long used2 = need - avail;
-                c_next = base2 + used2 < end2 ? base2 + used2 : end2;
-                c_end = end2;
-            } else {
-                c_next = c_end;
-            }
-        }
-        if (exhausted && __ballot(mode != kIdle) == 0) break;
-
-        // 3. a walk step (primary or secondary): jump to the sphere and begin
-        //    the query within 2d of the new point, seeded with the previous
-        //    primitive (Laplace: no source term)
-        if (mode == kStart) {
-            move_point<D>(p, d, rng);
-            ++step;
-            const float seed_d2 = prim_d2<D>(s, prim, p);
-            if constexpr (STATS) {
-                atomicAdd(&s.stats[1], 1ull);   // the seed primitive
-                atomicAdd(&s.stats[2], 1ull);
-                atomicAdd(&s.stats[3], 1ull);
-            }
-            if (s.n_prims == 1) {
-                q.best = seed_d2;
-                q.best_id = prim;
-                mode = kReady;
-            } else {
-                query_begin_at<D>(q, s, p, bound_d2(s, 2.0f * d), prim, seed_d2, st, slot, lane_g);
-                mode = kQuery;
-            }
-        }
-        // 4. advance every pending query by up to `window` node visits
-        for (int it = 0; it < a.window; ++it) {
-            if (mode == kQuery) {
-                if (query_step<D, STATS>(s, p, q, st, slot, lane_g, s.tol)) {
-                    mode = kReady;
-                    query_flush_stats<STATS>(s, q);
-                }
-            }
-            if (__ballot(mode == kQuery) == 0) break;
-        }
-    }
+    const GradOne src{s, a};
+    grad_samples<D, STATS>(src, st, slot, lane_g);
 }
 
 // ---------------------------------------------------------------------------

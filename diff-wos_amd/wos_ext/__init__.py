@@ -93,12 +93,14 @@ _sig("wos_walk_reduce_counts", _vp, ctypes.POINTER(_WalkParams), _vp, _i, _i, _i
 class _BatchItem(ctypes.Structure):
     _fields_ = [("scene", _vp), ("params", _WalkParams), ("points", _vp), ("n_points", _i), ("seed", _i),
                 ("mean", _vp), ("mean64", _vp), ("var64", _vp), ("walk_value", _vp), ("rec_prim", _vp),
-                ("rec_t", _vp), ("basis", _vp), ("grad_out", _vp), ("grad_values", _vp), ("grad_coeffs", _vp)]
+                ("rec_t", _vp), ("basis", _vp), ("grad_out", _vp), ("grad_values", _vp), ("grad_coeffs", _vp),
+                ("grad_params", _GradParams), ("grad_vertices", _vp)]
 
 
 _sig("wos_batch_create", ctypes.POINTER(_BatchItem), _i, ctypes.POINTER(_vp))
 _sig("wos_batch_run", _vp, _vp)
 _sig("wos_batch_adjoint", _vp, _vp)
+_sig("wos_batch_shape_gradient", _vp, _vp)
 _sig("wos_batch_destroy", _vp)
 _sig("wos_adjoint_source", _vp, ctypes.POINTER(_WalkParams), _vp, _i, _i, _vp, _vp, _vp)
 _sig("wos_debug_fail_workspace", _i)
@@ -119,7 +121,7 @@ EXPORTED = ("wos_last_error", "wos_scene_create", "wos_scene_destroy", "wos_scen
             "wos_set_tuning",
             "wos_get_tuning", "wos_set_spatial_order", "wos_batch_create", "wos_batch_run", "wos_batch_adjoint",
             "wos_batch_destroy", "wos_adjoint_source", "wos_debug_fail_workspace", "wos_shape_gradient_counts",
-            "wos_walk_reduce_noalloc")
+            "wos_walk_reduce_noalloc", "wos_batch_shape_gradient")
 
 
 def set_tuning(chunk=None, wgs_per_cu=None, top_levels=None, window=None):
@@ -697,9 +699,15 @@ class Batch:
     source, ``grad_coeffs=True``) also gets ``grad_out`` (dL/d mean, filled
     by the caller after ``run()``) and ``grad_values`` / ``grad_coeffs``
     buffers that ``adjoint()`` adds into (``zero_grad()`` clears them); the
-    result equals ``adjoint_values`` on that item's records.  ``run()``,
-    ``zero_grad()`` and ``adjoint()`` allocate nothing, so a whole primal +
-    loss + adjoint step can be captured with ``torch.cuda.graph``.  The
+    result equals ``adjoint_values`` on that item's records.  An item with
+    ``grad_vertices=True`` and ``grad_params`` (the OursCUDA / Ours3D
+    settings: nwalks, nsteps, epsilon, epsilon2, clamping, double_sided,
+    control_variates) gets a ``grad_vertices`` buffer that
+    ``shape_gradient()`` adds dL/d vertex positions into from the same
+    ``grad_out``, equal to ``shape_gradient`` on that item alone up to the
+    order of the float additions.  ``run()``, ``zero_grad()``, ``adjoint()``
+    and ``shape_gradient()`` allocate nothing, so a whole primal + loss +
+    adjoint step can be captured with ``torch.cuda.graph``.  The
     batch holds its scenes' device buffers: changing an item's source, grids
     or use_bvh afterwards makes ``run()`` raise (create the batch again)."""
 
@@ -714,6 +722,13 @@ class Batch:
             dv = q.device
             want_v = bool(it.get("grad_values"))
             want_c = bool(it.get("grad_coeffs"))
+            want_s = bool(it.get("grad_vertices"))
+            gpar = dict(it.get("grad_params") or {})
+            if want_s and "nwalks" not in gpar:
+                raise WoSError(f"item {j}: grad_vertices needs grad_params with at least nwalks")
+            gp = _GradParams(int(gpar.get("nwalks", 0)), int(gpar.get("nsteps", 0)), float(gpar.get("epsilon", 0.0)),
+                             float(gpar.get("epsilon2", 0.0)), float(gpar.get("clamping", 0.0)),
+                             int(bool(gpar.get("double_sided", False))), int(bool(gpar.get("control_variates", True))))
             o = {"mean": torch.empty(P, dtype=torch.float32, device=dv),
                  "mean64": torch.empty(P, dtype=torch.float64, device=dv),
                  "var64": torch.empty(P, dtype=torch.float64, device=dv),
@@ -726,8 +741,10 @@ class Batch:
                     raise WoSError(f"item {j}: basis / coefficient gradients need a Fourier source (source_type 3)")
                 K = scene.c_source_params.numel() // 4
                 o["basis"] = torch.empty(K * P, dtype=torch.float64, device=dv)
-            if want_v or want_c:
+            if want_v or want_c or want_s:
                 o["grad_out"] = torch.zeros(P, dtype=torch.float32, device=dv)
+            if want_s:
+                o["grad_vertices"] = torch.zeros(scene.c_vertices.shape, dtype=torch.float32, device=dv)
             if want_v:
                 o["grad_values"] = torch.zeros(scene.c_values.numel(), dtype=torch.float32, device=dv)
             if want_c:
@@ -738,7 +755,8 @@ class Batch:
             arr[j] = _BatchItem(scene.handle, cw._params(), q.data_ptr(), P, int(it.get("seed", 0)),
                                 g("mean").data_ptr(), g("mean64").data_ptr(), g("var64").data_ptr(),
                                 g("walk_value").data_ptr(), _p(g("rec_prim")), _p(g("rec_t")), _p(g("basis")),
-                                _p(g("grad_out")), _p(g("grad_values")), _p(g("grad_coeffs")))
+                                _p(g("grad_out")), _p(g("grad_values")), _p(g("grad_coeffs")), gp,
+                                _p(g("grad_vertices")))
         h = _vp()
         torch.cuda.synchronize()
         _check(lib.wos_batch_create(arr, len(items), ctypes.byref(h)))
@@ -752,7 +770,7 @@ class Batch:
         """Clears every item's grad_values / grad_coeffs (fill kernels, so it
         can sit in a captured graph)."""
         for o in self.outputs:
-            for k in ("grad_values", "grad_coeffs"):
+            for k in ("grad_values", "grad_coeffs", "grad_vertices"):
                 if k in o:
                     o[k].zero_()
 
@@ -760,6 +778,13 @@ class Batch:
         """Reverse pass of the last run: adds each item's dL/d values and dL/d
         coefficients, from its ``grad_out``, into its gradient buffers."""
         _check(lib.wos_batch_adjoint(self._h, _stream()))
+        return self.outputs
+
+    def shape_gradient(self):
+        """Shape-gradient pass: adds each item's dL/d vertex positions, from
+        its ``grad_out``, into its ``grad_vertices`` (one persistent launch
+        per dimension)."""
+        _check(lib.wos_batch_shape_gradient(self._h, _stream()))
         return self.outputs
 
     def autotune(self, key=None, reps=2):

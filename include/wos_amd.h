@@ -286,6 +286,12 @@ typedef struct {
     const float     *grad_out;
     float           *grad_values;
     float           *grad_coeffs;
+    /* shape gradient (wos_batch_shape_gradient), nullable: dL/d vertex
+     * positions (n_vertices * dim), added into, by the OursCUDA / Ours3D
+     * samples of grad_params (the bwd_solver of CustomSolver) with this
+     * item's seed and grad_out; needs the scene's BVH */
+    wos_grad_params  grad_params;
+    float           *grad_vertices;
 } wos_batch_item;
 int wos_batch_create(const wos_batch_item *items, int n_items, wos_batch **out);
 /* Primal pass: walk_reduce of every item (wos/solver.py:49-65). */
@@ -294,6 +300,14 @@ int wos_batch_run(wos_batch *batch, void *stream);
  * records of the last run: wos_adjoint_values per item (the transpose of
  * include/scene.cuh:251-264 and of the contrib sum, include/wos.cuh:136-138). */
 int wos_batch_adjoint(wos_batch *batch, void *stream);
+/* Shape-gradient pass of every item with grad_vertices: wos_shape_gradient
+ * per item (CustomSolver backward, wos/solver.py:109-114, with OursCUDA /
+ * Ours3D, wos/wos_grad.py:130-151, wos/wos_grad_3d.py:117-137), all of a
+ * dimension's items in one persistent launch.  Reads each item's grad_out;
+ * the samples are those of a single-scene wos_shape_gradient with the same
+ * seed and settings, so each item's result equals it up to the order of the
+ * float additions. */
+int wos_batch_shape_gradient(wos_batch *batch, void *stream);
 int wos_batch_destroy(wos_batch *batch);
 
 /* ---- testing hook -----------------------------------------------------------

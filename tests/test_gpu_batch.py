@@ -183,3 +183,78 @@ def test_batch_refuses_a_changed_scene():
     b2 = wos_ext.Batch(items)
     b2.run()
     torch.cuda.synchronize()
+
+
+GRAD2 = dict(nwalks=4, nsteps=128, epsilon=1e-3, epsilon2=5e-4, clamping=1e-1)
+GRAD3 = dict(nwalks=4, nsteps=128, epsilon=1e-3, epsilon2=1e-3, clamping=1e-1)
+
+
+def _with_shape_grads(items):
+    for it in items:
+        it["grad_vertices"] = True
+        it["grad_params"] = GRAD2 if it["scene"].dim == 2 else GRAD3
+    return items
+
+
+def test_batch_shape_gradient_matches_single_scene():
+    """One persistent launch per dimension computes every item's OursCUDA /
+    Ours3D vertex gradient: the samples are the single-scene launch's (same
+    seed, same streams), so each item equals wos_shape_gradient on it alone
+    up to the order of the float additions."""
+    import wos_ext
+    items = _with_shape_grads(_build(0.01))
+    b = wos_ext.Batch(items)
+    rng = np.random.default_rng(4)
+    gs = []
+    for o in b.outputs:
+        g = _t(rng.normal(size=o["mean"].shape[0]).astype(np.float32))
+        o["grad_out"].copy_(g)
+        gs.append(g)
+    b.zero_grad()
+    b.shape_gradient()
+    torch.cuda.synchronize()
+    for it, o, g in zip(items, b.outputs, gs):
+        ref, _ = wos_ext.shape_gradient(it["scene"], it["points"], it["seed"], g, **it["grad_params"])
+        scale = float(ref.abs().max()) + 1e-12
+        assert scale > 1e-9
+        err = float((o["grad_vertices"] - ref).abs().max())
+        assert err <= 1e-4 * scale, (it["scene"].dim, err, scale)
+
+
+def test_batch_primal_adjoint_and_shape_gradient_in_one_graph():
+    """run + loss gradient + zero_grad + adjoint + shape_gradient captured as
+    one graph: every replay reproduces the eager step (means bit for bit,
+    gradients up to the order of float additions)."""
+    import wos_ext
+    items = _with_shape_grads(_with_grads(_build(0.005)))
+    b = wos_ext.Batch(items)
+    targets = [torch.zeros_like(o["mean"]) + 0.1 for o in b.outputs]
+
+    def step():
+        b.run()
+        for o, tg in zip(b.outputs, targets):
+            o["grad_out"].copy_((2.0 / tg.numel()) * (o["mean"] - tg))
+        b.zero_grad()
+        b.adjoint()
+        b.shape_gradient()
+
+    step()
+    torch.cuda.synchronize()
+    eager = [(o["mean"].clone(), o["grad_values"].clone(), o["grad_vertices"].clone()) for o in b.outputs]
+    s = torch.cuda.Stream()
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g, stream=s):
+        step()
+    for rep in range(3):
+        for o in b.outputs:
+            o["mean"].fill_(-1.0)
+            o["grad_values"].fill_(7.0)
+            o["grad_vertices"].fill_(7.0)
+        torch.cuda.synchronize()
+        g.replay()
+        torch.cuda.synchronize()
+        for (m, gv, gx), o in zip(eager, b.outputs):
+            assert torch.equal(m, o["mean"]), rep
+            for ref, got in ((gv, o["grad_values"]), (gx, o["grad_vertices"])):
+                scale = float(ref.abs().max()) + 1e-12
+                assert float((ref - got).abs().max()) <= 1e-4 * scale, rep
