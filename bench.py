@@ -513,9 +513,12 @@ def batch_workload(args, torch, wos_ext, C):
         else:
             sc = wos_ext.Scene3D(t(it["vertices"]), t(it["indices"]), t(it["values"]), use_bvh=True)
             cw = wos_ext.WoS3D(nwalks=it["nwalks"], nsteps=it["nsteps"], epsilon=it["epsilon"])
+        gpar = dict(nwalks=args.batch_grad_walks, nsteps=it["nsteps"], epsilon=it["epsilon"],
+                    epsilon2=5e-4 if it["dim"] == 2 else 1e-3, clamping=0.1)
         items.append(dict(scene=sc, wos=cw, points=t(it["points"]), seed=it["seed"], grad_values=True,
-                          grad_coeffs=src, basis=src))
+                          grad_coeffs=src, basis=src, grad_vertices=args.batch_grad_walks > 0, grad_params=gpar))
         total += it["points"].shape[0] * it["nwalks"]
+    samples = sum(it["points"].shape[0] for it in items_np) * args.batch_grad_walks
     batch = wos_ext.Batch(items)
     tuning = batch.autotune() if args.autotune else None
     if tuning:
@@ -555,6 +558,32 @@ def batch_workload(args, torch, wos_ext, C):
     rmse_batched = float(np.sqrt(err2 / npts))
     log(f"batch: {total} walks; batched primal+adjoint {eager_ms:.1f} ms eager, {graph_ms:.1f} ms graph, "
         f"rmse {rmse_batched:.4f}")
+
+    # the same step with every item's shape gradient (OursCUDA / Ours3D samples) in the graph as well
+    shape = None
+    if samples:
+        def step_shape(_=0):
+            step()
+            batch.shape_gradient()
+
+        step_shape()
+        g2 = torch.cuda.CUDAGraph()
+        torch.cuda.synchronize()
+        with torch.cuda.graph(g2, stream=s):
+            step_shape()
+        graph2_ms = synced_ms(torch, lambda i: g2.replay(), args.steps)
+        for i in range(args.steps):
+            ev[2 * i].record(stream)
+            batch.shape_gradient()
+            ev[2 * i + 1].record(stream)
+        torch.cuda.synchronize()
+        sg_ms = float(np.mean([ev[2 * i].elapsed_time(ev[2 * i + 1]) for i in range(args.steps)]))
+        shape = {"samples": samples, "samples_per_point": args.batch_grad_walks,
+                 "graph_replay_ms_with_shape_gradient": graph2_ms, "shape_gradient_launches_ms": sg_ms,
+                 "samples_per_s": samples / (sg_ms * 1e-3),
+                 "primal_walks_plus_samples_per_s": (total + samples) / (graph2_ms * 1e-3)}
+        log(f"batched shape gradient: {samples} samples in {sg_ms:.1f} ms; primal + adjoint + shape gradient "
+            f"{graph2_ms:.1f} ms graph")
 
     rf = None
     if not args.no_counts:
@@ -665,7 +694,8 @@ def batch_workload(args, torch, wos_ext, C):
           "config": {"workload": f"64-scene-batch-primal-adjoint-scale-{args.batch_scale}", "walks": total,
                      "scenes": len(items)},
           "batched_eager_ms": eager_ms, "graph_replay_ms": graph_ms, "walk_launches_ms": run_ms,
-          "rmse_batched": rmse_batched, "tuning": tuning, "roofline": rf, "cpu_baseline": cpu, "plain": plain})
+          "rmse_batched": rmse_batched, "tuning": tuning, "roofline": rf, "cpu_baseline": cpu, "plain": plain,
+          "shape_gradient": shape})
 
 
 def main():
@@ -674,6 +704,9 @@ def main():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--batch-grad-walks", type=int, default=4,
+                    help="batch workload: OursCUDA / Ours3D samples per point of every item's shape gradient "
+                         "(0: none)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-fd", action="store_true", help="skip the finite-difference checks")
