@@ -25,7 +25,7 @@ namespace wos {
 
 constexpr int kMaxBatchItems = 1024;
 constexpr int kMaxBatchBasis = 16;
-constexpr int kBatchHist = 12288;   // vertex-value histogram of the reverse pass kept in LDS up to this size
+constexpr int kBatchHist = 6144;    // vertex-value histogram (64-bit entries) of the reverse pass kept in LDS up to this size
 constexpr long long kAdjointTile = 16384;   // walks of one item per adjoint tile
 constexpr int kBatchFinish = 8;             // finished walks written together (see k_walk_persistent)
 
@@ -68,6 +68,7 @@ struct BatchItem {
     double *mean64, *var64;
     const float *grad_out;      // reverse pass: dL/d mean per point (nullable)
     float *grad_values;         // adds dL/d vertex values (nullable)
+    unsigned long long *vacc;   // with grad_values: n_vertices fixed-point sums, then their scale (a double)
     float *grad_coeffs;         // adds dL/d Fourier coefficients (nullable)
 };
 
@@ -369,18 +370,37 @@ __global__ __launch_bounds__(kBlock) void k_batch_reduce(const BatchItem *items,
     }
 }
 
+// Per item with grad_values (blockIdx.x): zeroes its fixed-point value sums
+// and sets their scale from its grad_out (value_scale_block).
+__global__ __launch_bounds__(kBlock) void k_batch_value_prep(const BatchItem *items) {
+    const BatchItem &b = items[blockIdx.x];
+    if (!b.grad_values || b.n_points <= 0) return;
+    for (int v = threadIdx.x; v < b.v.n_vertices; v += kBlock) b.vacc[v] = 0ull;
+    value_scale_block(b.grad_out, b.n_points, reinterpret_cast<double *>(b.vacc + b.v.n_vertices));
+}
+
+// Per item (blockIdx.y): fixed-point value sums -> grad_values (added into).
+__global__ void k_batch_value_finish(const BatchItem *items) {
+    const BatchItem &b = items[blockIdx.y];
+    if (!b.grad_values || b.n_points <= 0) return;
+    const double inv = 1.0 / reinterpret_cast<const double *>(b.vacc + b.v.n_vertices)[0];
+    for (int v = blockIdx.x * blockDim.x + threadIdx.x; v < b.v.n_vertices; v += gridDim.x * blockDim.x)
+        if (b.vacc[v] != 0ull) b.grad_values[v] += (float)((double)(long long)b.vacc[v] * inv);
+}
+
 // Reverse pass over the per-walk records (Solver.walk_reduce's adjoint,
 // wos/solver.py:90-125): each record adds g_i / W times the interpolation
 // weights of its exit primitive to the vertex values (include/scene.cuh:
-// 251-264 transposed).  A workgroup takes a contiguous run of tiles, so it
-// mostly stays on one item: the sums go to a histogram in LDS (items with at
-// most hist_size vertices; larger meshes add to memory directly) and reach
-// memory once per item the run touches.
+// 251-264 transposed), rounded to the item's fixed-point unit and summed as
+// 64-bit integers (bit-reproducible).  A workgroup takes a contiguous run of
+// tiles, so it mostly stays on one item: the sums go to a histogram in LDS
+// (items with at most hist_size vertices; larger meshes add to the item's
+// sums directly) and reach memory once per item the run touches.
 template <int D>
 __global__ __launch_bounds__(kBlock) void k_batch_adjoint(const BatchItem *items, const AdjointTile *tiles,
                                                           int n_tiles, int hist_size) {
-    extern __shared__ float h[];
-    for (int v = threadIdx.x; v < hist_size; v += kBlock) h[v] = 0.0f;
+    extern __shared__ unsigned long long h[];
+    for (int v = threadIdx.x; v < hist_size; v += kBlock) h[v] = 0ull;
     const int per = (n_tiles + (int)gridDim.x - 1) / (int)gridDim.x;
     const int t0 = (int)blockIdx.x * per, t1 = min(n_tiles, t0 + per);
     int cur = -1;
@@ -393,9 +413,9 @@ __global__ __launch_bounds__(kBlock) void k_batch_adjoint(const BatchItem *items
                 __syncthreads();
                 if (b.grad_values && b.v.n_vertices <= hist_size) {
                     for (int v = threadIdx.x; v < b.v.n_vertices; v += kBlock) {
-                        const float x = h[v];
-                        if (x != 0.0f) atomicAdd(&b.grad_values[v], x);
-                        h[v] = 0.0f;
+                        const unsigned long long x = h[v];
+                        if (x != 0ull) atomicAdd(&b.vacc[v], x);
+                        h[v] = 0ull;
                     }
                 }
             }
@@ -407,26 +427,25 @@ __global__ __launch_bounds__(kBlock) void k_batch_adjoint(const BatchItem *items
         const BatchItem &b = items[tl.item];
         const int W = b.w.nwalks;
         const bool lds = b.v.n_vertices <= hist_size;
-        float *gv = b.grad_values;
+        if (!b.grad_values) continue;
+        unsigned long long *dst = lds ? h : b.vacc;
+        const double sc = reinterpret_cast<const double *>(b.vacc + b.v.n_vertices)[0];
         for (long long k = tl.begin + threadIdx.x; k < tl.end; k += kBlock) {
             const int f = b.rec_prim[k];
             if (f < 0) continue;
             const float g = b.grad_out[point_of_walk(k, W)] / (float)W;
-            if (gv) {
-                const int *ix = b.v.idx;
-                if constexpr (D == 2) {
-                    const float tt = b.rec_t[k];
-                    const int i0 = ix[2 * f], i1 = ix[2 * f + 1];
-                    const float g0 = g * (1.0f - tt), g1 = g * tt;
-                    if (lds) { atomicAdd(&h[i0], g0); atomicAdd(&h[i1], g1); }
-                    else { atomicAdd(&gv[i0], g0); atomicAdd(&gv[i1], g1); }
-                } else {
-                    const float u = b.rec_t[2 * k], v = b.rec_t[2 * k + 1];
-                    const int i0 = ix[3 * f], i1 = ix[3 * f + 1], i2 = ix[3 * f + 2];
-                    const float g0 = g * (1.0f - u - v), g1 = g * u, g2 = g * v;
-                    if (lds) { atomicAdd(&h[i0], g0); atomicAdd(&h[i1], g1); atomicAdd(&h[i2], g2); }
-                    else { atomicAdd(&gv[i0], g0); atomicAdd(&gv[i1], g1); atomicAdd(&gv[i2], g2); }
-                }
+            const int *ix = b.v.idx;
+            if constexpr (D == 2) {
+                const float tt = b.rec_t[k];
+                const int i0 = ix[2 * f], i1 = ix[2 * f + 1];
+                atomicAdd(&dst[i0], value_term(g * (1.0f - tt), sc));
+                atomicAdd(&dst[i1], value_term(g * tt, sc));
+            } else {
+                const float u = b.rec_t[2 * k], v = b.rec_t[2 * k + 1];
+                const int i0 = ix[3 * f], i1 = ix[3 * f + 1], i2 = ix[3 * f + 2];
+                atomicAdd(&dst[i0], value_term(g * (1.0f - u - v), sc));
+                atomicAdd(&dst[i1], value_term(g * u, sc));
+                atomicAdd(&dst[i2], value_term(g * v, sc));
             }
         }
     }
@@ -569,6 +588,7 @@ struct wos_batch {
     // built: a setter that frees or re-types what the table points at
     // (source, grids, use_bvh) makes the next run fail instead of reading it
     std::vector<std::pair<const wos_scene *, unsigned long long>> scenes;
+    std::vector<unsigned long long *> value_sums;   // per item with grad_values: its fixed-point sums
 };
 
 // defined in wos_capi.hip
@@ -598,6 +618,7 @@ static void free_batch(wos_batch *b) {
     }
     (void)hipFree(b->counters);
     (void)hipFree(b->spill);
+    for (unsigned long long *p : b->value_sums) (void)hipFree(p);
     delete b;
 }
 
@@ -692,6 +713,21 @@ extern "C" int wos_batch_create(const wos_batch_item *items, int n_items, wos_ba
     if (any_is && ensure_green_cdf_batch() != 0) return -1;
     wos_batch *B = new wos_batch();
     B->scenes = std::move(scenes);
+    for (auto &gv : grp)
+        for (BatchItem &b : gv) {
+            b.vacc = nullptr;
+            if (b.n_points > 0 && b.grad_values) {
+                unsigned long long *p = nullptr;
+                if (hipMalloc((void **)&p, sizeof(unsigned long long) * ((size_t)b.v.n_vertices + 1)) != hipSuccess) {
+                    (void)hipGetLastError();
+                    free_batch(B);
+                    set_error("cannot allocate the batch value-adjoint sums");
+                    return -1;
+                }
+                B->value_sums.push_back(p);
+                b.vacc = p;
+            }
+        }
     (void)hipGetDevice(&B->device);
     for (int d = 0; d < 2; ++d) {
         auto &v = grp[d];
@@ -858,9 +894,11 @@ extern "C" int wos_batch_adjoint(wos_batch *B, void *stream) {
         if (d == 0 && g.n_items) k_batch_adjoint_coeffs<<<dim3(kMaxBatchBasis, g.n_items), kBlock, 0, st>>>(g.items);
         if (!g.n_tiles) continue;
         const int grid = std::max(1, std::min(g.n_tiles, cap));
-        const size_t hb = sizeof(float) * (size_t)g.hist;
+        const size_t hb = sizeof(unsigned long long) * (size_t)g.hist;
+        k_batch_value_prep<<<g.n_items, kBlock, 0, st>>>(g.items);
         if (d == 0) k_batch_adjoint<2><<<grid, kBlock, hb, st>>>(g.items, g.tiles, g.n_tiles, g.hist);
         else k_batch_adjoint<3><<<grid, kBlock, hb, st>>>(g.items, g.tiles, g.n_tiles, g.hist);
+        k_batch_value_finish<<<dim3(64, g.n_items), kBlock, 0, st>>>(g.items);
     }
     hipError_t e = hipGetLastError();
     BREQ(e == hipSuccess, "batch adjoint launch: %s", hipGetErrorString(e));

@@ -1017,6 +1017,40 @@ __device__ __forceinline__ void add_basis(unsigned long long *acc, long stride, 
     }
 }
 
+// ---------------------------------------------------------------------------
+// vertex-value adjoint in 64-bit fixed point
+// ---------------------------------------------------------------------------
+// A walk that ends on primitive f adds g_i / W times its interpolation
+// weights (non-negative, summing to 1) to f's vertex values, so no vertex
+// gathers more than sum_i |g_i|.  The per-walk float terms are rounded to
+// integer multiples of 1 / scale, scale = 2^(61 - ceil(log2 sum |g|)), and
+// summed as 64-bit integers: the sums do not depend on the order of the
+// additions, so the gradient is bit-reproducible.
+
+// scale_out[0] = that scale for grad_out[0..n) (one workgroup; FP64 sum in
+// a fixed order).
+__device__ __forceinline__ void value_scale_block(const float *grad_out, long n, double *scale_out) {
+    __shared__ double part[256];
+    double acc = 0.0;
+    for (long i = threadIdx.x; i < n; i += blockDim.x) acc += fabs((double)grad_out[i]);
+    part[threadIdx.x] = acc;
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+        if ((int)threadIdx.x < h) part[threadIdx.x] += part[threadIdx.x + h];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        int e = 0;
+        (void)frexp(part[0], &e);   // sum < 2^e
+        scale_out[0] = part[0] > 0.0 ? ldexp(1.0, 61 - e) : 1.0;
+    }
+}
+
+// One term of the value adjoint, rounded to the fixed-point unit.
+__device__ __forceinline__ unsigned long long value_term(float x, double scale) {
+    return (unsigned long long)__double2ll_rn((double)x * scale);
+}
+
 // Lane modes of the persistent walk kernels: no work, a step to take, a
 // query in progress, a query just completed, a finished walk whose outputs
 // are not yet written.

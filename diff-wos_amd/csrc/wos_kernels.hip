@@ -802,18 +802,64 @@ __global__ __launch_bounds__(kBlock) void k_reduce_staged(const float4 *stage, c
 // ---------------------------------------------------------------------------
 // adjoint of the walk value w.r.t. vertex values and Fourier coefficients
 // ---------------------------------------------------------------------------
-constexpr int kLdsValues = 12288;   // vertex-value histogram kept in LDS up to this size
+constexpr int kLdsValues = 6144;   // vertex-value histogram (64-bit entries) kept in LDS up to this size
 
+// Deterministic FP64 sums over the points: kSumSlices fixed slices per
+// row, each reduced by one workgroup in a fixed order (thread strides, then
+// a tree in LDS), then the slices of a row added in order.
+constexpr int kSumSlices = 64;
+__device__ __forceinline__ double block_sum(double acc) {
+    __shared__ double part[kBlock];
+    part[threadIdx.x] = acc;
+    __syncthreads();
+    for (int h = kBlock / 2; h > 0; h >>= 1) {
+        if ((int)threadIdx.x < h) part[threadIdx.x] += part[threadIdx.x + h];
+        __syncthreads();
+    }
+    return part[0];
+}
+
+// part[blockIdx.x] = sum of |grad_out| over slice blockIdx.x.
+__global__ __launch_bounds__(kBlock) void k_abs_slices(const float *grad_out, int n, double *part) {
+    const long per = ((long)n + kSumSlices - 1) / kSumSlices;
+    const long b = blockIdx.x * per, e = min((long)n, b + per);
+    double acc = 0.0;
+    for (long i = b + threadIdx.x; i < e; i += kBlock) acc += fabs((double)grad_out[i]);
+    const double t = block_sum(acc);
+    if (threadIdx.x == 0) part[blockIdx.x] = t;
+}
+
+// scale[0] = the value adjoint's fixed-point scale (value_scale_block) from
+// the slice sums of |grad_out|.
+__global__ void k_value_scale_final(const double *part, double *scale) {
+    if (threadIdx.x != 0) return;
+    double sum = 0.0;
+    for (int b = 0; b < kSumSlices; ++b) sum += part[b];
+    int e = 0;
+    (void)frexp(sum, &e);
+    scale[0] = sum > 0.0 ? ldexp(1.0, 61 - e) : 1.0;
+}
+
+// Fixed-point value sums -> grad_values (added into).
+__global__ void k_value_finish(const unsigned long long *acc, int n, const double *scale, float *grad_values) {
+    const double inv = 1.0 / scale[0];
+    for (int v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += gridDim.x * blockDim.x)
+        if (acc[v] != 0ull) grad_values[v] += (float)((double)(long long)acc[v] * inv);
+}
+
+// Value part of the reverse pass: every walk's g_i / W times its exit
+// primitive's interpolation weights, summed per vertex in 64-bit fixed
+// point (units of 1 / scale[0]) into acc, through an LDS histogram sized to
+// the mesh (small meshes leave room for more resident workgroups).
 template <int D, bool LDSHIST>
 __global__ __launch_bounds__(kBlock) void k_adjoint_values(SceneView s, int n_points, int W, const int *rec_prim,
                                                            const float *rec_t, const float *grad_out,
-                                                           float *grad_values) {
-    // the histogram is sized to the scene (dynamic LDS), so small meshes
-    // leave room for more resident workgroups
-    extern __shared__ float h[];
+                                                           const double *scale, unsigned long long *acc) {
+    extern __shared__ unsigned long long h[];
     if (LDSHIST)
-        for (int v = threadIdx.x; v < s.n_vertices; v += kBlock) h[v] = 0.0f;
+        for (int v = threadIdx.x; v < s.n_vertices; v += kBlock) h[v] = 0ull;
     __syncthreads();
+    const double sc = scale[0];
     long long total = (long long)n_points * W;
     for (long long k = (long long)blockIdx.x * kBlock + threadIdx.x; k < total; k += (long long)gridDim.x * kBlock) {
         int f = rec_prim[k];
@@ -823,42 +869,50 @@ __global__ __launch_bounds__(kBlock) void k_adjoint_values(SceneView s, int n_po
             if constexpr (D == 2) {
                 float t = rec_t[k];
                 int i0 = s.idx[2 * f], i1 = s.idx[2 * f + 1];
-                float g0 = g * (1.0f - t), g1 = g * t;
-                if (LDSHIST) { atomicAdd(&h[i0], g0); atomicAdd(&h[i1], g1); }
-                else { atomicAdd(&grad_values[i0], g0); atomicAdd(&grad_values[i1], g1); }
+                const unsigned long long g0 = value_term(g * (1.0f - t), sc), g1 = value_term(g * t, sc);
+                unsigned long long *dst = LDSHIST ? h : acc;
+                atomicAdd(&dst[i0], g0);
+                atomicAdd(&dst[i1], g1);
             } else {
                 float u = rec_t[2 * k], v = rec_t[2 * k + 1];
                 int i0 = s.idx[3 * f], i1 = s.idx[3 * f + 1], i2 = s.idx[3 * f + 2];
-                float g0 = g * (1.0f - u - v), g1 = g * u, g2 = g * v;
-                if (LDSHIST) { atomicAdd(&h[i0], g0); atomicAdd(&h[i1], g1); atomicAdd(&h[i2], g2); }
-                else { atomicAdd(&grad_values[i0], g0); atomicAdd(&grad_values[i1], g1); atomicAdd(&grad_values[i2], g2); }
+                const unsigned long long g0 = value_term(g * (1.0f - u - v), sc), g1 = value_term(g * u, sc),
+                                         g2 = value_term(g * v, sc);
+                unsigned long long *dst = LDSHIST ? h : acc;
+                atomicAdd(&dst[i0], g0);
+                atomicAdd(&dst[i1], g1);
+                atomicAdd(&dst[i2], g2);
             }
         }
     }
     __syncthreads();
     if (LDSHIST)
         for (int v = threadIdx.x; v < s.n_vertices; v += kBlock)
-            if (h[v] != 0.0f) atomicAdd(&grad_values[v], h[v]);
+            if (h[v] != 0ull) atomicAdd(&acc[v], h[v]);
 }
 
 // dL/d coeff_q = sum_i grad_out[i] / W * basis[q][i] (basis: per-point sums
-// over the walks): one workgroup per coefficient, FP64 partial sums per
-// thread over a fixed stride of points and a fixed-order tree in LDS, so the
-// result does not depend on scheduling.  Added into grad_coeffs[q].
-__global__ __launch_bounds__(kBlock) void k_adjoint_coeffs(int n_points, int W, const double *basis,
-                                                           const float *grad_out, float *grad_coeffs) {
-    __shared__ double part[kBlock];
-    const int q = blockIdx.x;
-    const double *b = basis + (long)q * n_points;
+// over the walks) as deterministic FP64 slice sums: part[q * kSumSlices +
+// slice] for q = blockIdx.y, slice = blockIdx.x; k_adjoint_coeffs_final adds
+// each coefficient's slices in order into grad_coeffs[q].
+__global__ __launch_bounds__(kBlock) void k_adjoint_coeffs(int n_points, const double *basis,
+                                                           const float *grad_out, double *part) {
+    const int q = blockIdx.y;
+    const double *bq = basis + (long)q * n_points;
+    const long per = ((long)n_points + kSumSlices - 1) / kSumSlices;
+    const long b = blockIdx.x * per, e = min((long)n_points, b + per);
     double acc = 0.0;
-    for (long i = threadIdx.x; i < n_points; i += kBlock) acc = fma((double)grad_out[i], b[i], acc);
-    part[threadIdx.x] = acc;
-    __syncthreads();
-    for (int h = kBlock / 2; h > 0; h >>= 1) {
-        if (threadIdx.x < h) part[threadIdx.x] += part[threadIdx.x + h];
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) grad_coeffs[q] += (float)(part[0] / W);
+    for (long i = b + threadIdx.x; i < e; i += kBlock) acc = fma((double)grad_out[i], bq[i], acc);
+    const double t = block_sum(acc);
+    if (threadIdx.x == 0) part[q * kSumSlices + blockIdx.x] = t;
+}
+
+__global__ void k_adjoint_coeffs_final(const double *part, int K, int W, float *grad_coeffs) {
+    const int q = threadIdx.x;
+    if (q >= K) return;
+    double sum = 0.0;
+    for (int b = 0; b < kSumSlices; ++b) sum += part[q * kSumSlices + b];
+    grad_coeffs[q] += (float)(sum / W);
 }
 
 // ---------------------------------------------------------------------------
@@ -1373,21 +1427,38 @@ int launch_adjoint_values(const SceneView &s, int n_points, int nwalks, const in
     // fixed-size one allowed three per CU, 0.48 -> 0.36 ms on the clover)
     int g = grid_of((long)((total + 15) / 16), 8 * cu_count());
     bool lds = s.n_vertices <= kLdsValues;
-    const size_t hb = lds ? sizeof(float) * (size_t)s.n_vertices : 0;
+    const size_t hb = lds ? sizeof(unsigned long long) * (size_t)s.n_vertices : 0;
+    // scratch: fixed-point value sums (n_vertices), their scale, slice sums
+    const bool want_c = basis && grad_coeffs && K > 0;
+    if (!grad_values && !want_c) return 0;
+    unsigned long long *acc = nullptr;
+    const size_t nb = sizeof(unsigned long long) * ((size_t)s.n_vertices + 1 + (size_t)kSumSlices * (K + 1));
+    if (hipMallocAsync((void **)&acc, nb, st) != hipSuccess) {
+        (void)hipGetLastError();
+        set_error("cannot allocate the adjoint scratch (%d vertices)", s.n_vertices);
+        return -1;
+    }
+    double *scale = reinterpret_cast<double *>(acc + s.n_vertices);
+    double *part = scale + 1;
     if (grad_values) {
+        k_zero_u64_n<<<grid_of(s.n_vertices), kBlock, 0, st>>>(acc, s.n_vertices);
+        k_abs_slices<<<kSumSlices, kBlock, 0, st>>>(grad_out, n_points, part);
+        k_value_scale_final<<<1, 64, 0, st>>>(part, scale);
         if (s.dim == 2) {
-            if (lds) k_adjoint_values<2, true><<<g, kBlock, hb, st>>>(s, n_points, nwalks, rec_prim, rec_t, grad_out, grad_values);
-            else k_adjoint_values<2, false><<<g, kBlock, 0, st>>>(s, n_points, nwalks, rec_prim, rec_t, grad_out, grad_values);
+            if (lds) k_adjoint_values<2, true><<<g, kBlock, hb, st>>>(s, n_points, nwalks, rec_prim, rec_t, grad_out, scale, acc);
+            else k_adjoint_values<2, false><<<g, kBlock, 0, st>>>(s, n_points, nwalks, rec_prim, rec_t, grad_out, scale, acc);
         } else {
-            if (lds) k_adjoint_values<3, true><<<g, kBlock, hb, st>>>(s, n_points, nwalks, rec_prim, rec_t, grad_out, grad_values);
-            else k_adjoint_values<3, false><<<g, kBlock, 0, st>>>(s, n_points, nwalks, rec_prim, rec_t, grad_out, grad_values);
+            if (lds) k_adjoint_values<3, true><<<g, kBlock, hb, st>>>(s, n_points, nwalks, rec_prim, rec_t, grad_out, scale, acc);
+            else k_adjoint_values<3, false><<<g, kBlock, 0, st>>>(s, n_points, nwalks, rec_prim, rec_t, grad_out, scale, acc);
         }
-        KCHK();
+        k_value_finish<<<grid_of(s.n_vertices), kBlock, 0, st>>>(acc, s.n_vertices, scale, grad_values);
     }
-    if (basis && grad_coeffs && K > 0) {
-        k_adjoint_coeffs<<<K, kBlock, 0, st>>>(n_points, nwalks, basis, grad_out, grad_coeffs);
-        KCHK();
+    if (want_c) {
+        k_adjoint_coeffs<<<dim3(kSumSlices, K), kBlock, 0, st>>>(n_points, basis, grad_out, part + kSumSlices);
+        k_adjoint_coeffs_final<<<1, 64, 0, st>>>(part + kSumSlices, K, nwalks, grad_coeffs);
     }
+    (void)hipFreeAsync(acc, st);
+    KCHK();
     return 0;
 }
 

@@ -135,6 +135,11 @@ def test_adjoint_values_and_coeffs():
     ref = O.Scene(c["vertices"], c["indices"], c["values"], 3, c["source_params"])
     gv_ref = ref.adjoint_values(P, W, out["rec_prim"].cpu().numpy(), out["rec_t"].cpu().numpy(), g)
     np.testing.assert_allclose(gv.cpu().numpy(), gv_ref, rtol=1e-4, atol=1e-6)
+    # the value sums are 64-bit fixed point: repeated launches agree bit for bit
+    for _ in range(3):
+        gv2, gc2 = wos_ext.adjoint_values(sc, P, W, out["rec_prim"], out["rec_t"], _t(g), basis=out["basis"],
+                                          want_values=True, want_coeffs=True)
+        assert torch.equal(gv, gv2) and torch.equal(gc, gc2)
     # coefficients: d mean / d c_k by central differences under common random numbers (exactly linear)
     K = c["source_params"].shape[0]
     for k in (0, 2, 5):
