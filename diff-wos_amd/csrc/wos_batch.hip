@@ -488,6 +488,7 @@ struct GradItem {
     long long point_begin;      // offset in the group's point space (start records)
     const float *grad_out;      // dL/d mean per point
     float *grad_vertices;       // adds dL/d vertex positions (n_vertices * dim)
+    float4 *vstage;             // the group's staged gradient terms (dim - 1 per sample, at walk_begin + sample)
 };
 
 // Sample source of the batched shape-gradient launch (grad_samples in
@@ -529,16 +530,17 @@ struct GradBatch {
     __device__ __forceinline__ void load(int it, long long i, float *p) const { load_point<D>(items[it].points, i, p); }
     __device__ __forceinline__ int32_t seed(int it) const { return items[it].seed; }
     template <int D>
-    __device__ __forceinline__ void skip(int, long long) const {}
+    __device__ __forceinline__ void skip(int it, long long kk) const {
+        const float zero[3] = {0.0f, 0.0f, 0.0f};
+        stage_vertex_grad<D>(items[it].vstage, items[it].walk_begin + kk, false, 0, 0.0f, 0.0f, 0.0f, zero);
+    }
     template <int D>
     __device__ __forceinline__ void finish(int it, long long k, bool ok, int rprim, float rt0, float rt1,
                                            const float *gu) const {
         const GradItem &b = items[it];
-        if (ok && b.grad_vertices) {
-            const int W = b.g.nwalks;
-            scatter_vertex_grad<D>(b.v, b.grad_out[point_of_walk(k, W)] / (float)W, rprim, rt0, rt1, gu,
-                                   b.grad_vertices);
-        }
+        const int W = b.g.nwalks;
+        stage_vertex_grad<D>(b.vstage, b.walk_begin + k, ok, rprim, rt0, rt1, b.grad_out[point_of_walk(k, W)] / (float)W,
+                             gu);
     }
 };
 
@@ -572,6 +574,10 @@ struct BatchGroup {
     long long gsamples = 0, gpoints = 0;
     int *giprim = nullptr;
     float *gisd = nullptr;
+    float4 *vstage = nullptr;     // staged gradient terms of every sample
+    VgradRange *vranges = nullptr; // per item: its samples, vertex sums and scale
+    double *vpart = nullptr;      // slice sums (kVgradSlices per item)
+    int vwin = 1;                 // LDS windows of the vertex scatter (the largest item's)
 };
 
 }  // namespace wos
@@ -615,6 +621,9 @@ static void free_batch(wos_batch *b) {
         (void)hipFree(g.gitems);
         (void)hipFree(g.giprim);
         (void)hipFree(g.gisd);
+        (void)hipFree(g.vstage);
+        (void)hipFree(g.vranges);
+        (void)hipFree(g.vpart);
     }
     (void)hipFree(b->counters);
     (void)hipFree(b->spill);
@@ -765,15 +774,38 @@ extern "C" int wos_batch_create(const wos_batch_item *items, int n_items, wos_ba
             g.n_gitems = (int)gv.size();
             g.gsamples = sb;
             g.gpoints = gb;
-            if (g.n_gitems &&
-                (hipMalloc((void **)&g.gitems, sizeof(GradItem) * gv.size()) != hipSuccess ||
-                 hipMalloc((void **)&g.giprim, sizeof(int) * (size_t)std::max(gb, 1LL)) != hipSuccess ||
-                 hipMalloc((void **)&g.gisd, sizeof(float) * (size_t)std::max(gb, 1LL)) != hipSuccess ||
-                 hipMemcpy(g.gitems, gv.data(), sizeof(GradItem) * gv.size(), hipMemcpyHostToDevice) != hipSuccess)) {
-                (void)hipGetLastError();
-                free_batch(B);
-                set_error("cannot allocate the batch shape-gradient tables");
-                return -1;
+            if (g.n_gitems) {
+                // staged terms of every sample, and per item its vertex sums
+                std::vector<VgradRange> rv(gv.size());
+                bool ok = hipMalloc((void **)&g.vstage, sizeof(float4) * (d == 0 ? 1 : 2) * (size_t)std::max(sb, 1LL)) ==
+                              hipSuccess &&
+                          hipMalloc((void **)&g.vranges, sizeof(VgradRange) * gv.size()) == hipSuccess &&
+                          hipMalloc((void **)&g.vpart, sizeof(double) * kVgradSlices * gv.size()) == hipSuccess;
+                for (size_t j = 0; ok && j < gv.size(); ++j) {
+                    GradItem &gi = gv[j];
+                    gi.vstage = g.vstage;
+                    unsigned long long *acc = nullptr;
+                    ok = hipMalloc((void **)&acc, sizeof(unsigned long long) * (size_t)gi.v.n_vertices * gi.v.dim) ==
+                         hipSuccess;
+                    if (ok) B->value_sums.push_back(acc);
+                    g.vwin = std::max(g.vwin, vgrad_windows((long)gi.v.n_vertices * gi.v.dim));
+                    rv[j] = VgradRange{gi.walk_begin, (long long)gi.n_points * gi.g.nwalks, gi.v.n_vertices, gi.v.idx,
+                                       acc, gi.grad_vertices, 1.0};
+                }
+                ok = ok &&
+                     hipMalloc((void **)&g.gitems, sizeof(GradItem) * gv.size()) == hipSuccess &&
+                     hipMalloc((void **)&g.giprim, sizeof(int) * (size_t)std::max(gb, 1LL)) == hipSuccess &&
+                     hipMalloc((void **)&g.gisd, sizeof(float) * (size_t)std::max(gb, 1LL)) == hipSuccess &&
+                     hipMemcpy(g.gitems, gv.data(), sizeof(GradItem) * gv.size(), hipMemcpyHostToDevice) ==
+                         hipSuccess &&
+                     hipMemcpy(g.vranges, rv.data(), sizeof(VgradRange) * rv.size(), hipMemcpyHostToDevice) ==
+                         hipSuccess;
+                if (!ok) {
+                    (void)hipGetLastError();
+                    free_batch(B);
+                    set_error("cannot allocate the batch shape-gradient tables");
+                    return -1;
+                }
             }
         }
         if (!g.n_items) continue;
@@ -872,11 +904,21 @@ extern "C" int wos_batch_shape_gradient(wos_batch *B, void *stream) {
                                                              B->spill, stride);
             k_batch_grad<2><<<gw, kBlock, 0, st>>>(g.gitems, g.n_gitems, g.gsamples, B->counters + 2 + d, chunk,
                                                    window, g.giprim, g.gisd, B->spill, stride);
+            k_vgrad_bound<2><<<dim3(kVgradSlices, g.n_gitems), kBlock, 0, st>>>(g.vstage, g.vranges, g.vpart);
+            k_vgrad_scale<<<g.n_gitems, 64, 0, st>>>(g.vranges, g.vpart);
+            k_vgrad_scatter<2><<<dim3(64, g.n_gitems, g.vwin), kBlock, sizeof(unsigned long long) * kVgradLds, st>>>(
+                g.vstage, g.vranges, kVgradLds, nullptr);
+            k_vgrad_finish<2><<<dim3(64, g.n_gitems), kBlock, 0, st>>>(g.vranges);
         } else {
             k_batch_init<3, GradItem><<<gi, kBlock, 0, st>>>(g.gitems, g.n_gitems, g.gpoints, g.giprim, g.gisd,
                                                              B->spill, stride);
             k_batch_grad<3><<<gw, kBlock, 0, st>>>(g.gitems, g.n_gitems, g.gsamples, B->counters + 2 + d, chunk,
                                                    window, g.giprim, g.gisd, B->spill, stride);
+            k_vgrad_bound<3><<<dim3(kVgradSlices, g.n_gitems), kBlock, 0, st>>>(g.vstage, g.vranges, g.vpart);
+            k_vgrad_scale<<<g.n_gitems, 64, 0, st>>>(g.vranges, g.vpart);
+            k_vgrad_scatter<3><<<dim3(64, g.n_gitems, g.vwin), kBlock, sizeof(unsigned long long) * kVgradLds, st>>>(
+                g.vstage, g.vranges, kVgradLds, nullptr);
+            k_vgrad_finish<3><<<dim3(64, g.n_gitems), kBlock, 0, st>>>(g.vranges);
         }
     }
     hipError_t e = hipGetLastError();

@@ -94,28 +94,210 @@ __device__ __forceinline__ void add_tangent_derivative(const SceneView &s, int f
 }
 
 
-// dL/d vertices of one finished sample: -gi w grad u added at the exit
-// primitive's vertices, w the exit point's interpolation weights (the
-// transpose of Polyline.get_point / Scene3D.get_point, wos/scene.py:416-433,
-// wos/scene3d.py:225-239).  gi = dL/d mean of the point over its walks.
+// ---------------------------------------------------------------------------
+// vertex gradient in 64-bit fixed point
+// ---------------------------------------------------------------------------
+// A finished sample adds -w_j c to vertex j of its exit primitive, with w the
+// exit point's interpolation weights (non-negative, summing to 1) and c = g
+// grad u (g = dL/d mean of its point over W).  The samples stage their exit
+// primitive (-1: the sample adds nothing), its t or (u, v) and c -- one
+// float4 per sample in 2D, two in 3D.  The sums are then formed over each
+// range of samples (a scene's launch, or one batch item) as 64-bit integers
+// in units of 2^-e, e = 61 - ceil(log2 sum over the samples of max_k |c_k|)
+// (no vertex component gathers more), so the gradient does not depend on
+// the order of the additions.
+struct VgradRange {
+    long long begin, n;          // samples [begin, begin + n) of the staging buffer
+    int n_vertices;
+    const int *idx;              // the scene's primitive vertex indices
+    unsigned long long *acc;     // n_vertices * D sums
+    float *grad_vertices;        // added into
+    double scale;                // set by k_vgrad_scale
+};
+constexpr int kVgradSlices = 64;
+
 template <int D>
-__device__ __forceinline__ void scatter_vertex_grad(const SceneView &s, float gi, int rprim, float rt0, float rt1,
-                                                    const float *gu, float *gv) {
+__device__ __forceinline__ void stage_vertex_grad(float4 *stage, long long slot, bool ok, int rprim, float rt0,
+                                                  float rt1, float g, const float *gu) {
+    const float pr = __int_as_float(ok ? rprim : -1);
     if constexpr (D == 2) {
-        const int i0 = s.idx[2 * rprim], i1 = s.idx[2 * rprim + 1];
-        for (int c = 0; c < 2; ++c) {
-            atomicAdd(&gv[2 * i0 + c], -gi * (1.0f - rt0) * gu[c]);
-            atomicAdd(&gv[2 * i1 + c], -gi * rt0 * gu[c]);
-        }
+        stage[slot] = make_float4(pr, rt0, g * gu[0], g * gu[1]);
     } else {
-        const int i0 = s.idx[3 * rprim], i1 = s.idx[3 * rprim + 1], i2 = s.idx[3 * rprim + 2];
-        const float wa = 1.0f - rt0 - rt1;
-        for (int c = 0; c < 3; ++c) {
-            atomicAdd(&gv[3 * i0 + c], -gi * wa * gu[c]);
-            atomicAdd(&gv[3 * i1 + c], -gi * rt0 * gu[c]);
-            atomicAdd(&gv[3 * i2 + c], -gi * rt1 * gu[c]);
+        stage[2 * slot] = make_float4(pr, rt0, rt1, g * gu[0]);
+        stage[2 * slot + 1] = make_float4(g * gu[1], g * gu[2], 0.0f, 0.0f);
+    }
+}
+
+// A staged sample: exit primitive (-1: none), weights of its vertices, c.
+template <int D>
+__device__ __forceinline__ int staged_sample(const float4 *stage, long long k, float *wt, float *c) {
+    if constexpr (D == 2) {
+        const float4 w0 = stage[k];
+        wt[0] = 1.0f - w0.y;
+        wt[1] = w0.y;
+        c[0] = w0.z;
+        c[1] = w0.w;
+        return __float_as_int(w0.x);
+    } else {
+        const float4 w0 = stage[2 * k], w1 = stage[2 * k + 1];
+        wt[0] = 1.0f - w0.y - w0.z;
+        wt[1] = w0.y;
+        wt[2] = w0.z;
+        c[0] = w0.w;
+        c[1] = w1.x;
+        c[2] = w1.y;
+        return __float_as_int(w0.x);
+    }
+}
+
+__device__ __forceinline__ double vgrad_block_sum(double acc) {
+    __shared__ double part[256];
+    part[threadIdx.x] = acc;
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+        if ((int)threadIdx.x < h) part[threadIdx.x] += part[threadIdx.x + h];
+        __syncthreads();
+    }
+    return part[0];
+}
+
+// One range written from the host's values (no host buffer to keep alive).
+static __global__ void k_vgrad_setup(VgradRange *r, long long begin, long long n, int n_vertices, const int *idx,
+                                     unsigned long long *acc, float *grad_vertices) {
+    r->begin = begin;
+    r->n = n;
+    r->n_vertices = n_vertices;
+    r->idx = idx;
+    r->acc = acc;
+    r->grad_vertices = grad_vertices;
+    r->scale = 1.0;
+}
+
+// Range blockIdx.y, slice blockIdx.x: sum of max_k |c_k| over the slice's
+// samples (part[range * kVgradSlices + slice]); also zeroes the range's sums.
+template <int D>
+__global__ __launch_bounds__(256) void k_vgrad_bound(const float4 *stage, VgradRange *ranges, double *part) {
+    VgradRange &r = ranges[blockIdx.y];
+    for (long j = (long)blockIdx.x * 256 + threadIdx.x; j < (long)r.n_vertices * D; j += (long)gridDim.x * 256)
+        r.acc[j] = 0ull;
+    const long long per = (r.n + kVgradSlices - 1) / kVgradSlices;
+    const long long b = r.begin + blockIdx.x * per, e = min(r.begin + r.n, b + per);
+    double acc = 0.0;
+    for (long long k = b + threadIdx.x; k < e; k += 256) {
+        float wt[3], c[3];
+        if (staged_sample<D>(stage, k, wt, c) < 0) continue;
+        float m = 0.0f;
+        for (int q = 0; q < D; ++q) m = fmaxf(m, fabsf(c[q]));
+        acc += (double)m;
+    }
+    const double t = vgrad_block_sum(acc);
+    if (threadIdx.x == 0) part[blockIdx.y * kVgradSlices + blockIdx.x] = t;
+}
+
+// Per range (blockIdx.x): the slices added in order, then the scale.
+static __global__ void k_vgrad_scale(VgradRange *ranges, const double *part) {
+    if (threadIdx.x != 0) return;
+    double sum = 0.0;
+    for (int b = 0; b < kVgradSlices; ++b) sum += part[blockIdx.x * kVgradSlices + b];
+    int e = 0;
+    (void)frexp(sum, &e);
+    ranges[blockIdx.x].scale = sum > 0.0 ? ldexp(1.0, 61 - e) : 1.0;
+}
+
+// Range blockIdx.y: every staged sample adds its rounded terms, through an
+// LDS histogram of the sums [blockIdx.z * lds_cap, +lds_cap) when the range's
+// n_vertices * D sums fit gridDim.z such windows (dynamic LDS; terms outside
+// the window are left to the other blockIdx.z), else straight to the range's
+// sums (blockIdx.z 0 only).  Each thread takes one contiguous span of samples
+// and sums the terms of consecutive samples on one primitive in registers
+// before adding them.  A histogram is then written whole to
+// partial[blockIdx.x * m + window] when partial is given (summed by
+// k_vgrad_partials), else added to the sums entry by entry.
+constexpr int kVgradLds = 6144;
+// Windows (blockIdx.z) of kVgradLds sums at most: larger meshes add their
+// terms to the sums in memory.
+constexpr int kVgradWindows = 4;
+inline int vgrad_windows(long m) {
+    const long w = (m + kVgradLds - 1) / kVgradLds;
+    return w <= kVgradWindows ? (int)std::max(w, 1L) : 1;
+}
+template <int D>
+__device__ __forceinline__ void vgrad_flush(unsigned long long *dst, long lo, long hi, const int *idx, int f,
+                                            const unsigned long long *acc) {
+    if (f < 0) return;
+    for (int j = 0; j < D; ++j) {
+        const long v = idx[(long)D * f + j];
+        for (int q = 0; q < D; ++q) {
+            const long e = v * D + q;
+            if (e >= lo && e < hi && acc[j * D + q] != 0ull) atomicAdd(&dst[e - lo], acc[j * D + q]);
         }
     }
+}
+template <int D>
+__global__ __launch_bounds__(256) void k_vgrad_scatter(const float4 *stage, const VgradRange *ranges, int lds_cap,
+                                                       unsigned long long *partial) {
+    extern __shared__ unsigned long long hv[];
+    const VgradRange &r = ranges[blockIdx.y];
+    const long m = (long)r.n_vertices * D;
+    const bool lds = m <= (long)lds_cap * gridDim.z;
+    const long lo = lds ? (long)blockIdx.z * lds_cap : 0;
+    const long hi = lds ? (lo + lds_cap < m ? lo + lds_cap : m) : m;
+    if (lo >= hi || (!lds && blockIdx.z > 0)) return;  // uniform over the workgroup
+    if (lds)
+        for (long j = threadIdx.x; j < hi - lo; j += 256) hv[j] = 0ull;
+    __syncthreads();
+    unsigned long long *dst = lds ? hv : r.acc;
+    const double sc = r.scale;
+    const long long nthr = (long long)gridDim.x * 256;
+    const long long span = (r.n + nthr - 1) / nthr;
+    const long long k0 = r.begin + ((long long)blockIdx.x * 256 + threadIdx.x) * span;
+    const long long k1 = k0 + span < r.begin + r.n ? k0 + span : r.begin + r.n;
+    int cur = -1;
+    unsigned long long acc[D * D];
+    for (int j = 0; j < D * D; ++j) acc[j] = 0ull;
+    for (long long k = k0; k < k1; ++k) {
+        float wt[3], c[3];
+        const int f = staged_sample<D>(stage, k, wt, c);
+        if (f < 0) continue;
+        if (f != cur) {
+            vgrad_flush<D>(dst, lo, hi, r.idx, cur, acc);
+            cur = f;
+            for (int j = 0; j < D * D; ++j) acc[j] = 0ull;
+        }
+        for (int j = 0; j < D; ++j)
+            for (int q = 0; q < D; ++q)
+                acc[j * D + q] += (unsigned long long)__double2ll_rn((double)(-(wt[j] * c[q])) * sc);
+    }
+    vgrad_flush<D>(dst, lo, hi, r.idx, cur, acc);
+    __syncthreads();
+    if (lds && partial)
+        for (long j = threadIdx.x; j < hi - lo; j += 256) partial[(long)blockIdx.x * m + lo + j] = hv[j];
+    else if (lds)
+        for (long j = threadIdx.x; j < hi - lo; j += 256)
+            if (hv[j] != 0ull) atomicAdd(&r.acc[lo + j], hv[j]);
+}
+
+// Range 0's sums from the nb histograms of k_vgrad_scatter (integer sums:
+// any order gives the same bits).
+template <int D>
+__global__ void k_vgrad_partials(const VgradRange *ranges, const unsigned long long *partial, int nb) {
+    const VgradRange &r = ranges[0];
+    const long m = (long)r.n_vertices * D;
+    for (long j = (long)blockIdx.x * blockDim.x + threadIdx.x; j < m; j += (long)gridDim.x * blockDim.x) {
+        unsigned long long x = 0ull;
+        for (int b = 0; b < nb; ++b) x += partial[(long)b * m + j];
+        r.acc[j] = x;
+    }
+}
+
+// Range blockIdx.y: sums -> grad_vertices (added into).
+template <int D>
+__global__ void k_vgrad_finish(const VgradRange *ranges) {
+    const VgradRange &r = ranges[blockIdx.y];
+    const double inv = 1.0 / r.scale;
+    for (long j = (long)blockIdx.x * blockDim.x + threadIdx.x; j < (long)r.n_vertices * D;
+         j += (long)gridDim.x * blockDim.x)
+        if (r.acc[j] != 0ull) r.grad_vertices[j] += (float)((double)(long long)r.acc[j] * inv);
 }
 
 // Shape-gradient samples, one per (point, walk), on the persistent

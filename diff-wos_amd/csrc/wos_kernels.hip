@@ -934,9 +934,14 @@ struct GradArgs {
     float *sample_grad_u;
     const float *velocity;           // forward mode: per-vertex velocity (n_vertices * D), nullable
     float *sample_jvp;               // forward mode: per-sample -grad u . V(exit) (n_points * nwalks)
+    float4 *vstage;                  // reverse mode: staged vertex-gradient terms (D - 1 per sample), nullable
     int2 *spill;
     long spill_stride;
 };
+
+// Workgroups of the fixed-point vertex scatter of a small mesh (each keeps an
+// LDS histogram of the sums, written whole and summed in a second pass).
+constexpr int kVgradBlocks = 256;
 
 // Sample source of the single-scene shape-gradient launch (grad_samples in
 // wos_grad.h): one scene and the launch arguments, uniform across the grid.
@@ -969,6 +974,10 @@ struct GradOne {
         if (a.sample_grad_u)
             for (int c = 0; c < D; ++c) a.sample_grad_u[kk * D + c] = 0.0f;
         if (a.sample_jvp) a.sample_jvp[kk] = 0.0f;
+        if (a.vstage) {
+            const float zero[3] = {0.0f, 0.0f, 0.0f};
+            stage_vertex_grad<D>(a.vstage, kk, false, 0, 0.0f, 0.0f, 0.0f, zero);
+        }
     }
     // A finished sample: its spatial gradient, its forward-mode derivative
     // and the scatter -g_i / W w grad u onto the exit primitive's vertices.
@@ -999,9 +1008,8 @@ struct GradOne {
             }
             a.sample_jvp[k] = jv;
         }
-        if (ok && a.grad_vertices && a.grad_out)
-            scatter_vertex_grad<D>(s, a.grad_out[point_of_walk(k, W)] / (float)W, rprim, rt0, rt1, gu,
-                                   a.grad_vertices);
+        if (a.vstage)
+            stage_vertex_grad<D>(a.vstage, k, ok, rprim, rt0, rt1, a.grad_out[point_of_walk(k, W)] / (float)W, gu);
     }
 };
 
@@ -1506,6 +1514,7 @@ int launch_shape_gradient(const SceneView &s, const GradSettings &g, const float
     a.window = window_for(s.dim);
     a.grad_out = grad_out;
     a.grad_vertices = grad_vertices;
+    a.vstage = nullptr;
     a.sample_grad_u = sample_grad_u;
     a.velocity = velocity;
     a.sample_jvp = velocity ? sample_jvp : nullptr;
@@ -1522,6 +1531,33 @@ int launch_shape_gradient(const SceneView &s, const GradSettings &g, const float
     a.init_prim = sr.prim;
     a.init_sd = sr.sd;
     a.order = sr.order;
+    // reverse mode: staged terms (32 B per sample), the vertex sums, their range and slice sums
+    unsigned char *vbuf = nullptr;
+    VgradRange *range = nullptr;
+    unsigned long long *vacc = nullptr, *vhist = nullptr;
+    double *vpart = nullptr;
+    const bool rev = grad_vertices && grad_out;
+    if (rev) {
+        const size_t stage_b = sizeof(float4) * (size_t)(s.dim - 1) * (size_t)a.total;
+        const size_t acc_b = sizeof(unsigned long long) * (size_t)s.n_vertices * s.dim;
+        // with LDS histograms: kVgradBlocks histograms of the sums, summed in a second pass
+        const bool small = (long)s.n_vertices * s.dim <= (long)kVgradLds * kVgradWindows;
+        const size_t part_b = small ? acc_b * kVgradBlocks : 0;
+        if (hipMallocAsync((void **)&vbuf, stage_b + acc_b + 256 + sizeof(double) * kVgradSlices + part_b, st) !=
+            hipSuccess) {
+            (void)hipGetLastError();
+            set_error("cannot allocate the staged gradient terms of %lld samples", a.total);
+            return -1;
+        }
+        a.vstage = reinterpret_cast<float4 *>(vbuf);
+        vacc = reinterpret_cast<unsigned long long *>(vbuf + stage_b);
+        range = reinterpret_cast<VgradRange *>(vbuf + stage_b + acc_b);
+        vpart = reinterpret_cast<double *>(vbuf + stage_b + acc_b + 256);
+        vhist = small ? reinterpret_cast<unsigned long long *>(vbuf + stage_b + acc_b + 256 +
+                                                               sizeof(double) * kVgradSlices)
+                      : nullptr;
+        k_vgrad_setup<<<1, 1, 0, st>>>(range, 0, a.total, s.n_vertices, s.idx, vacc, grad_vertices);
+    }
     size_t tb = top_bytes(s);
     if (s.dim == 2) k_point_init<2><<<gi, kBlock, tb, st>>>(s, points, n_points, sr.order, sr.prim, sr.sd, a.spill, a.spill_stride);
     else k_point_init<3><<<gi, kBlock, tb, st>>>(s, points, n_points, nullptr, sr.prim, sr.sd, a.spill, a.spill_stride);
@@ -1532,6 +1568,30 @@ int launch_shape_gradient(const SceneView &s, const GradSettings &g, const float
     } else {
         if (s.dim == 2) k_grad_persistent<2, false><<<gr, kBlock, tb, st>>>(s, a);
         else k_grad_persistent<3, false><<<gr, kBlock, tb, st>>>(s, a);
+    }
+    if (rev) {
+        // small meshes: kVgradBlocks histograms in LDS per window of the sums,
+        // then summed; large ones: atomics into the sums in memory
+        const long m = (long)s.n_vertices * s.dim;
+        const bool lds = vhist != nullptr;
+        const int gs = lds ? kVgradBlocks : grid_of((long)std::min<long long>(a.total, 1LL << 30), occupancy_grid());
+        const dim3 gsz(gs, 1, lds ? vgrad_windows(m) : 1);
+        const size_t hb = lds ? sizeof(unsigned long long) * (size_t)std::min<long>(m, kVgradLds) : 0;
+        const int gv = grid_of(m);
+        if (s.dim == 2) {
+            k_vgrad_bound<2><<<dim3(kVgradSlices, 1), kBlock, 0, st>>>(a.vstage, range, vpart);
+            k_vgrad_scale<<<1, 64, 0, st>>>(range, vpart);
+            k_vgrad_scatter<2><<<gsz, kBlock, hb, st>>>(a.vstage, range, lds ? kVgradLds : 0, vhist);
+            if (lds) k_vgrad_partials<2><<<gv, kBlock, 0, st>>>(range, vhist, kVgradBlocks);
+            k_vgrad_finish<2><<<dim3(gv, 1), kBlock, 0, st>>>(range);
+        } else {
+            k_vgrad_bound<3><<<dim3(kVgradSlices, 1), kBlock, 0, st>>>(a.vstage, range, vpart);
+            k_vgrad_scale<<<1, 64, 0, st>>>(range, vpart);
+            k_vgrad_scatter<3><<<gsz, kBlock, hb, st>>>(a.vstage, range, lds ? kVgradLds : 0, vhist);
+            if (lds) k_vgrad_partials<3><<<gv, kBlock, 0, st>>>(range, vhist, kVgradBlocks);
+            k_vgrad_finish<3><<<dim3(gv, 1), kBlock, 0, st>>>(range);
+        }
+        (void)hipFreeAsync(vbuf, st);
     }
     KCHK();
     return 0;

@@ -226,7 +226,9 @@ int wos_adjoint_source(const wos_scene *scene, const wos_walk_params *params, co
 /* Shape gradient: CustomSolver backward (wos/solver.py:109-114) with
  * bwd_solver = OursCUDA (wos/wos_grad.py:130-151) in 2D or Ours3D
  * (wos/wos_grad_3d.py:117-137) in 3D, walks recomputed from (seed, point,
- * walk).  Adds dL/dvertices into grad_vertices (n_vertices*dim, nullable).
+ * walk).  Adds dL/dvertices into grad_vertices (n_vertices*dim, nullable),
+ * summed in 64-bit fixed point (bit-reproducible; allocates a 16*(dim-1)-byte
+ * staging record per sample on the stream).
  * sample_grad_u (nullable): per-sample spatial gradient estimate,
  * n_points*nwalks*dim floats (0 for samples that do not contribute). */
 int wos_shape_gradient(const wos_scene *scene, const wos_grad_params *params, const float *points, int n_points,
@@ -305,8 +307,8 @@ int wos_batch_adjoint(wos_batch *batch, void *stream);
  * Ours3D, wos/wos_grad.py:130-151, wos/wos_grad_3d.py:117-137), all of a
  * dimension's items in one persistent launch.  Reads each item's grad_out;
  * the samples are those of a single-scene wos_shape_gradient with the same
- * seed and settings, so each item's result equals it up to the order of the
- * float additions. */
+ * seed and settings, and both sum in the same fixed point, so each item's
+ * grad_vertices equals the single-scene result bit for bit. */
 int wos_batch_shape_gradient(wos_batch *batch, void *stream);
 int wos_batch_destroy(wos_batch *batch);
 
