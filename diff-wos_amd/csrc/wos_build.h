@@ -71,9 +71,14 @@ constexpr int kStartGridAuto3 = 96;
 constexpr int kStartGridResolution2 = kStartGridAuto;
 constexpr int kStartGridResolution3 = 0;   // measured no faster on the displaced spheres; opt-in
 struct StartGridOutputs {
-    int *start = nullptr;   // per cell int4 {first node, second or -1, key bits, 0} (hipMalloc, owned by the caller)
+    // per cell {first node, second or -1, key bits} (hipMalloc, owned by the
+    // caller): packed = 1, 8 bytes {first | second << 16 (0: none), key}
+    // when every internal node id fits 16 bits (2D), else int4 {first, second,
+    // key, 0}
+    int *start = nullptr;
     float ox = 0, oy = 0, oz = 0, inv_cell = 0;
     int nx = 0, ny = 0, nz = 0;
+    int packed = 0;
 };
 int build_start_grid_2d(const GridInputs &in, int resolution, hipStream_t stream, StartGridOutputs &out);
 struct GridInputs3 {

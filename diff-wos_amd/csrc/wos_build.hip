@@ -1272,7 +1272,8 @@ __device__ int descend_single(const float4 *nodes, int node, float x, float y, f
 // only one side can hold its answer.  key is a lower bound of b's bound over
 // the whole cell (the bound is 1-Lipschitz: its value at the centre minus
 // rho, less the float margin), with which a query can prune b unvisited.
-__global__ void k_start_cells_2d(const float4 *nodes, const float4 *leaves, int n_prims, GridGeom g, int4 *start) {
+__global__ void k_start_cells_2d(const float4 *nodes, const float4 *leaves, int n_prims, GridGeom g, int4 *start,
+                                 uint2 *packed) {
     const float two_rho = 1.41421356f * g.cell;
     for (long c = blockIdx.x * (long)blockDim.x + threadIdx.x; c < (long)g.nx * g.ny; c += (long)gridDim.x * blockDim.x) {
         float x, y;
@@ -1295,7 +1296,10 @@ __global__ void k_start_cells_2d(const float4 *nodes, const float4 *leaves, int 
                 out.z = __float_as_int(fmaxf(0.0f, (lb1 - 0.5f * two_rho) * 0.99999f - g.margin));
             }
         }
-        start[c] = out;
+        if (packed)
+            packed[c] = make_uint2((uint32_t)out.x | ((out.y < 0 ? 0u : (uint32_t)out.y) << 16), (uint32_t)out.z);
+        else
+            start[c] = out;
     }
 }
 
@@ -1326,10 +1330,14 @@ int build_start_grid_2d(const GridInputs &in, int resolution, hipStream_t stream
     g.band = 0.0f;
     g.margin = 8.0f * in.tol;
     g.cap = 0;
+    // 8-byte cells when every internal node id (0 .. n_prims - 2) fits 16
+    // bits; a second node is never the root, so 0 marks "none"
+    const bool packed = in.n_prims - 2 <= 0xffff;
     int *start = nullptr;
-    BCHK(hipMalloc((void **)&start, sizeof(int4) * (size_t)g.nx * g.ny));
-    k_start_cells_2d<<<grid_for((long)g.nx * g.ny), kBlock, 0, stream>>>(in.nodes, in.leaves, in.n_prims, g,
-                                                                         reinterpret_cast<int4 *>(start));
+    BCHK(hipMalloc((void **)&start, (packed ? sizeof(uint2) : sizeof(int4)) * (size_t)g.nx * g.ny));
+    k_start_cells_2d<<<grid_for((long)g.nx * g.ny), kBlock, 0, stream>>>(
+        in.nodes, in.leaves, in.n_prims, g, packed ? nullptr : reinterpret_cast<int4 *>(start),
+        packed ? reinterpret_cast<uint2 *>(start) : nullptr);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) {
         (void)hipFree(start);
@@ -1343,6 +1351,7 @@ int build_start_grid_2d(const GridInputs &in, int resolution, hipStream_t stream
     out.nx = g.nx;
     out.ny = g.ny;
     out.nz = 1;
+    out.packed = packed ? 1 : 0;
     return 0;
 }
 

@@ -158,6 +158,7 @@ static SceneView view_of(const wos_scene *s) {
     v.st_nx = s->sgrid.nx;
     v.st_ny = s->sgrid.ny;
     v.st_nz = s->sgrid.nz;
+    v.st_packed = s->sgrid.packed;
     return v;
 }
 

@@ -313,6 +313,7 @@ struct SceneView {
     const int *start_node;        // st_nx * st_ny * st_nz, null when absent
     float st_ox, st_oy, st_oz, st_inv_cell;
     int st_nx, st_ny, st_nz;
+    int st_packed;                // 8-byte cells (16-bit node ids, second 0: none), else int4 cells
 };
 
 // Per-lane traversal stack: the first kLdsStack entries live in LDS (lane
@@ -434,6 +435,10 @@ __device__ __forceinline__ int4 start_nodes_of(const SceneView &s, const float *
         if (!(fz >= 0.0f && fz < (float)s.st_nz)) return make_int4(0, -1, 0, 0);
         c += (uint32_t)(int)fz * (uint32_t)(s.st_nx * s.st_ny);
     }
+    if (s.st_packed) {
+        const uint2 w = *at_byte_offset(reinterpret_cast<const uint2 *>(s.start_node), c << 3);
+        return make_int4((int)(w.x & 0xffffu), (int)(w.x >> 16), (int)w.y, 0);
+    }
     return *at_byte_offset(reinterpret_cast<const int4 *>(s.start_node), c << 4);
 }
 
@@ -469,7 +474,8 @@ template <int D>
 __device__ __forceinline__ void query_begin_from(Query &q, int4 start, float bound_d2, int seed_id, float seed_d2,
                                                  float tol, const TravStack &st, int slot, long spill_lane) {
     query_begin<D>(q, bound_d2, seed_id, seed_d2, tol, start.x);
-    if (start.y >= 0) stack_push(q, st, slot, spill_lane, start.y, __int_as_float(start.z));
+    // no second node: -1, or 0 in packed cells (the root is never a second node)
+    if (start.y > 0) stack_push(q, st, slot, spill_lane, start.y, __int_as_float(start.z));
 }
 template <int D>
 __device__ __forceinline__ void query_begin_at(Query &q, const SceneView &s, const float *p, float bound_d2,
