@@ -110,7 +110,7 @@ __global__ __launch_bounds__(kBlock) void k_batch_init(const Item *items, int n_
     }
 }
 
-template <int D>
+template <int D, int KB>
 __device__ __forceinline__ void batch_write(const BatchItem &b, long long k, bool valid, int prim, const float *p,
                                             float contrib, const float *bas) {
     float v = 0.0f, t0 = 0.0f, t1 = 0.0f;
@@ -129,8 +129,10 @@ __device__ __forceinline__ void batch_write(const BatchItem &b, long long k, boo
         if (D == 2) b.rec_t[k] = t0;
         else { b.rec_t[2 * k] = t0; b.rec_t[2 * k + 1] = t1; }
     }
-    if (valid && b.basis)
-        add_basis<kMaxBatchBasis>(b.basis + point_of_walk(k, b.w.nwalks), b.n_points, b.K, b.basis_scale, bas);
+    if constexpr (KB > 0) {
+        if (valid && b.basis)
+            add_basis<KB>(b.basis + point_of_walk(k, b.w.nwalks), b.n_points, b.K, b.basis_scale, bas);
+    }
 }
 
 // Per item (blockIdx.y): zeroes the fixed-point basis accumulators before the
@@ -153,12 +155,19 @@ __global__ void k_batch_basis(const BatchItem *items, int finish) {
 // scene view from the item table (cached; items are few and shared).
 // Finished walks wait until kBatchFinish lanes of the wave are done or idle
 // and then write their outputs together.
-template <int D>
-__global__ __launch_bounds__(kBlock) void k_batch_walk(const BatchItem *items, int n_items, long long total,
+// KB: Fourier basis capacity (the group's largest K rounded up to 8 or 16; 0
+// in 3D, which has no source).  With K <= 8 the 2D kernel is held to 64 VGPRs
+// (8 waves per SIMD, as the single-scene walk: 8 % faster on the clover); the
+// 3D kernel measured 11 % slower at 72 VGPRs (7 waves) than unconstrained.
+template <int D, int KB>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(D == 2 && KB <= 8 ? 8 : 1))) void k_batch_walk(const BatchItem *items, int n_items, long long total,
                                                        unsigned long long *counter, int chunk, int window,
                                                        const int *iprim, const float *isd, int2 *spill,
                                                        long stride) {
-    __shared__ long long s_begin[kMaxBatchItems];
+    // the items' first walks, n_items entries of dynamic LDS (a static
+    // kMaxBatchItems table would hold the kernel to 6 waves per SIMD)
+    extern __shared__ long long s_walk_begin[];
+    long long *s_begin = s_walk_begin;
     for (int i = threadIdx.x; i < n_items; i += kBlock) s_begin[i] = items[i].walk_begin;
     __syncthreads();
     TravStack st{spill, stride};
@@ -184,9 +193,10 @@ __global__ __launch_bounds__(kBlock) void k_batch_walk(const BatchItem *items, i
     tv.leaves = nullptr;
     tv.n_top = 0;
     float tol = 0.0f;
-    float bas[kMaxBatchBasis];
+    constexpr int NB = KB > 0 ? KB : 1;
+    float bas[NB];
 #pragma unroll
-    for (int j = 0; j < kMaxBatchBasis; ++j) bas[j] = 0.0f;
+    for (int j = 0; j < NB; ++j) bas[j] = 0.0f;
     long long c_next = 0, c_end = 0;
     bool exhausted = false;
 
@@ -205,7 +215,7 @@ __global__ __launch_bounds__(kBlock) void k_batch_walk(const BatchItem *items, i
         if (__popcll(__ballot(mode == kDone || mode == kIdle)) >= kBatchFinish || exhausted ||
             __ballot(mode == kStart || mode == kQuery) == 0) {
             if (mode == kDone) {
-                batch_write<D>(items[item], k, done_valid, prim, p, contrib, bas);
+                batch_write<D, KB>(items[item], k, done_valid, prim, p, contrib, bas);
                 mode = kIdle;
                 k = -1;
             }
@@ -246,15 +256,15 @@ __global__ __launch_bounds__(kBlock) void k_batch_walk(const BatchItem *items, i
                     load_point<D>(b.points, i, p);
                     contrib = 0.0f;
 #pragma unroll
-                    for (int j = 0; j < kMaxBatchBasis; ++j) bas[j] = 0.0f;
+                    for (int j = 0; j < NB; ++j) bas[j] = 0.0f;
                     // include/wos.cuh:100-106, then the first loop test
                     bool outside = ip < 0 || (sd >= 0.0f && !b.w.double_sided);
                     float d0 = fabsf(sd);
                     if (b.w.prevent_fd_artifacts) d0 *= 0.5f;
                     if (outside || b.w.nsteps <= 0) {
-                        batch_write<D>(b, kk, false, ip, p, 0.0f, bas);
+                        batch_write<D, KB>(b, kk, false, ip, p, 0.0f, bas);
                     } else if (d0 < b.w.epsilon) {
-                        batch_write<D>(b, kk, true, ip, p, 0.0f, bas);
+                        batch_write<D, KB>(b, kk, true, ip, p, 0.0f, bas);
                     } else {
                         k = kk;
                         rng = walk_stream(b.seed, i, kk);
@@ -290,10 +300,10 @@ __global__ __launch_bounds__(kBlock) void k_batch_walk(const BatchItem *items, i
                 float dyx, dyy, G, w;
                 if (interior_sample(u0, u1, d, b.w.use_is != 0, g_green_cdf, dyx, dyy, G, w)) {
                     float yx = p[0] + dyx, yy = p[1] + dyy;
-                    if (K > 0) {
+                    if (KB > 0 && K > 0) {
                         float f = 0.0f;
 #pragma unroll
-                        for (int j = 0; j < kMaxBatchBasis; ++j) {
+                        for (int j = 0; j < NB; ++j) {
                             if (j < K) {
                                 const const_f32 *c = as_constant(s.src) + 4 * j;
                                 float bq = sin_cw(fmaf(c[0], yx, fmaf(c[1], yy, c[2])));
@@ -568,6 +578,7 @@ struct BatchGroup {
     AdjointTile *tiles = nullptr; // device copy (null: no item has a reverse pass)
     int n_tiles = 0;
     int hist = 0;                 // LDS histogram entries of the reverse pass
+    int max_k = 0;                // largest Fourier term count of the group's items
     // shape-gradient items (wos_batch_shape_gradient)
     GradItem *gitems = nullptr;   // device copy
     int n_gitems = 0;
@@ -758,6 +769,7 @@ extern "C" int wos_batch_create(const wos_batch_item *items, int n_items, wos_ba
         }
         BatchGroup &g = B->g[d];
         g.n_items = (int)v.size();
+        for (const BatchItem &b : v) g.max_k = std::max(g.max_k, b.K);
         g.walks = wb;
         g.points = pb;
         g.max_depth = depth;
@@ -866,15 +878,20 @@ extern "C" int wos_batch_run(wos_batch *B, void *stream) {
         int gwk = (int)std::min<long>(std::max<long>(gw, 1), cap);
         int chunk = g_tuning.chunk > 0 ? g_tuning.chunk : 256;
         int window = window_for(d == 0 ? 2 : 3);
+        const size_t hb = sizeof(long long) * (size_t)g.n_items;   // item table of k_batch_walk
         k_batch_basis<<<dim3(256, g.n_items), kBlock, 0, st>>>(g.items, 0);
         if (d == 0) {
             k_batch_init<2, BatchItem><<<gi, kBlock, 0, st>>>(g.items, g.n_items, g.points, g.iprim, g.isd, spill, stride);
-            k_batch_walk<2><<<gwk, kBlock, 0, st>>>(g.items, g.n_items, g.walks, B->counters + d, chunk, window,
-                                                    g.iprim, g.isd, spill, stride);
+            if (g.max_k <= 8)
+                k_batch_walk<2, 8><<<gwk, kBlock, hb, st>>>(g.items, g.n_items, g.walks, B->counters + d, chunk,
+                                                           window, g.iprim, g.isd, spill, stride);
+            else
+                k_batch_walk<2, kMaxBatchBasis><<<gwk, kBlock, hb, st>>>(g.items, g.n_items, g.walks, B->counters + d,
+                                                                        chunk, window, g.iprim, g.isd, spill, stride);
         } else {
             k_batch_init<3, BatchItem><<<gi, kBlock, 0, st>>>(g.items, g.n_items, g.points, g.iprim, g.isd, spill, stride);
-            k_batch_walk<3><<<gwk, kBlock, 0, st>>>(g.items, g.n_items, g.walks, B->counters + d, chunk, window,
-                                                    g.iprim, g.isd, spill, stride);
+            k_batch_walk<3, 0><<<gwk, kBlock, hb, st>>>(g.items, g.n_items, g.walks, B->counters + d, chunk, window,
+                                                       g.iprim, g.isd, spill, stride);
         }
         k_batch_basis<<<dim3(256, g.n_items), kBlock, 0, st>>>(g.items, 1);
         long gr = std::min<long>((g.points * 64 + kBlock - 1) / kBlock, 8192);
