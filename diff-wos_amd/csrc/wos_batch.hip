@@ -157,10 +157,11 @@ __global__ void k_batch_basis(const BatchItem *items, int finish) {
 // and then write their outputs together.
 // KB: Fourier basis capacity (the group's largest K rounded up to 8 or 16; 0
 // in 3D, which has no source).  With K <= 8 the 2D kernel is held to 64 VGPRs
-// (8 waves per SIMD, as the single-scene walk: 8 % faster on the clover); the
-// 3D kernel measured 11 % slower at 72 VGPRs (7 waves) than unconstrained.
+// (8 waves per SIMD, as the single-scene walk: 8 % faster on the clover), with
+// 16 terms to 80 (6 waves); the 3D kernel measured 11 % slower at 72 VGPRs (7
+// waves) than unconstrained.
 template <int D, int KB>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(D == 2 && KB <= 8 ? 8 : 1))) void k_batch_walk(const BatchItem *items, int n_items, long long total,
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(D == 2 ? (KB <= 8 ? 8 : 6) : 1))) void k_batch_walk(const BatchItem *__restrict__ items, int n_items, long long total,
                                                        unsigned long long *counter, int chunk, int window,
                                                        const int *iprim, const float *isd, int2 *spill,
                                                        long stride) {
@@ -290,52 +291,64 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(D == 2 &
         }
         if (exhausted && __ballot(mode != kIdle) == 0) break;
 
-        // 3. a new step: source sample, jump to the sphere, begin the query
+        // 3. a new step: source sample, jump to the sphere, begin the query.
+        //    In 2D, when every stepping lane of the wave walks the same item
+        //    (the common case: chunks lie inside items), the item is read at a
+        //    wave-uniform index, so its fields come in by scalar loads (the
+        //    64-scene batch's 2D kernel 7.5 % faster).
         if (mode == kStart) {
-            const BatchItem &b = items[item];
-            const SceneView &s = b.v;
-            if (D == 2 && s.source_type != 0) {
-                const int K = b.K;
-                const float u0 = rng.next_1d(), u1 = rng.next_1d();
-                float dyx, dyy, G, w;
-                if (interior_sample(u0, u1, d, b.w.use_is != 0, g_green_cdf, dyx, dyy, G, w)) {
-                    float yx = p[0] + dyx, yy = p[1] + dyy;
-                    if (KB > 0 && K > 0) {
-                        float f = 0.0f;
+            auto step_phase = [&](const BatchItem &b) {
+                const SceneView &s = b.v;
+                if (D == 2 && s.source_type != 0) {
+                    const int K = b.K;
+                    const float u0 = rng.next_1d(), u1 = rng.next_1d();
+                    float dyx, dyy, G, w;
+                    if (interior_sample(u0, u1, d, b.w.use_is != 0, g_green_cdf, dyx, dyy, G, w)) {
+                        float yx = p[0] + dyx, yy = p[1] + dyy;
+                        if (KB > 0 && K > 0) {
+                            float f = 0.0f;
 #pragma unroll
-                        for (int j = 0; j < NB; ++j) {
-                            if (j < K) {
-                                const const_f32 *c = as_constant(s.src) + 4 * j;
-                                float bq = sin_cw(fmaf(c[0], yx, fmaf(c[1], yy, c[2])));
-                                f = fmaf(c[3], bq, f);
-                                bas[j] = fmaf(bq * G, w, bas[j]);
+                            for (int j = 0; j < NB; ++j) {
+                                if (j < K) {
+                                    const const_f32 *c = as_constant(s.src) + 4 * j;
+                                    float bq = sin_cw(fmaf(c[0], yx, fmaf(c[1], yy, c[2])));
+                                    f = fmaf(c[3], bq, f);
+                                    bas[j] = fmaf(bq * G, w, bas[j]);
+                                }
                             }
+                            contrib = fmaf(f * G, w, contrib);
+                        } else {
+                            contrib = fmaf(source_at(s, yx, yy) * G, w, contrib);
                         }
-                        contrib = fmaf(f * G, w, contrib);
-                    } else {
-                        contrib = fmaf(source_at(s, yx, yy) * G, w, contrib);
                     }
                 }
-            }
-            move_point<D>(p, d, rng);
-            ++step;
-            if (!s.use_bvh) {
-                float d2;
-                int id = closest_brute<D>(s, p, d2);
-                q.best = d2;
-                q.best_id = id;
-                mode = kReady;
-            } else {
-                const float seed_d2 = prim_d2<D>(s, prim, p);
-                if (s.n_prims == 1) {
-                    q.best = seed_d2;
-                    q.best_id = prim;
+                move_point<D>(p, d, rng);
+                ++step;
+                if (!s.use_bvh) {
+                    float d2;
+                    int id = closest_brute<D>(s, p, d2);
+                    q.best = d2;
+                    q.best_id = id;
                     mode = kReady;
                 } else {
-                    query_begin_from<D>(q, start_nodes_of<D>(s, p), bound_d2(s, 2.0f * d), prim, seed_d2, tol, st,
-                                        threadIdx.x, lane_g);
-                    mode = kQuery;
+                    const float seed_d2 = prim_d2<D>(s, prim, p);
+                    if (s.n_prims == 1) {
+                        q.best = seed_d2;
+                        q.best_id = prim;
+                        mode = kReady;
+                    } else {
+                        query_begin_from<D>(q, start_nodes_of<D>(s, p), bound_d2(s, 2.0f * d), prim, seed_d2, tol, st,
+                                            threadIdx.x, lane_g);
+                        mode = kQuery;
+                    }
                 }
+            };
+            if constexpr (D == 2) {
+                const int iu = __builtin_amdgcn_readfirstlane(item);
+                if (__ballot(item != iu) == 0) step_phase(items[iu]);
+                else step_phase(items[item]);
+            } else {
+                step_phase(items[item]);   // 3D: the uniform copy measured 5 % slower
             }
         }
         // 4. advance every pending query by up to `window` node visits
