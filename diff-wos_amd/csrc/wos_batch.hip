@@ -568,12 +568,14 @@ struct GradBatch {
 };
 
 // Shape-gradient samples of every gradient item of a dimension group in one
-// persistent launch; 80 VGPRs as the single-scene kernel.
+// persistent launch; registers held as in the single-scene kernel (2D 64, 3D
+// 80), the item-start table in dynamic LDS (n_items entries).
 template <int D>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6))) void k_batch_grad(
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(D == 2 ? 8 : 6))) void k_batch_grad(
     const GradItem *items, int n_items, long long n_samples, unsigned long long *queue, int claim, int visits,
     const int *iprim, const float *isd, int2 *spill, long stride) {
-    __shared__ long long s_begin[kMaxBatchItems];
+    extern __shared__ long long s_grad_begin[];
+    long long *s_begin = s_grad_begin;
     for (int i = threadIdx.x; i < n_items; i += kBlock) s_begin[i] = items[i].walk_begin;
     __syncthreads();
     const TravStack st{spill, stride};
@@ -929,11 +931,12 @@ extern "C" int wos_batch_shape_gradient(wos_batch *B, void *stream) {
         const int gw = (int)std::min<long>(std::max<long>((g.gsamples / 64 + 4) / 4, 1), cap);
         const int chunk = g_tuning.chunk > 0 ? g_tuning.chunk : 128;
         const int window = window_for(d == 0 ? 2 : 3);
+        const size_t hg = sizeof(long long) * (size_t)g.n_gitems;   // item table of k_batch_grad
         if (d == 0) {
             k_batch_init<2, GradItem><<<gi, kBlock, 0, st>>>(g.gitems, g.n_gitems, g.gpoints, g.giprim, g.gisd,
                                                              B->spill, stride);
-            k_batch_grad<2><<<gw, kBlock, 0, st>>>(g.gitems, g.n_gitems, g.gsamples, B->counters + 2 + d, chunk,
-                                                   window, g.giprim, g.gisd, B->spill, stride);
+            k_batch_grad<2><<<gw, kBlock, hg, st>>>(g.gitems, g.n_gitems, g.gsamples, B->counters + 2 + d, chunk,
+                                                    window, g.giprim, g.gisd, B->spill, stride);
             k_vgrad_bound<2><<<dim3(kVgradSlices, g.n_gitems), kBlock, 0, st>>>(g.vstage, g.vranges, g.vpart);
             k_vgrad_scale<<<g.n_gitems, 64, 0, st>>>(g.vranges, g.vpart);
             k_vgrad_scatter<2><<<dim3(64, g.n_gitems, g.vwin), kBlock, sizeof(unsigned long long) * kVgradLds, st>>>(
@@ -942,8 +945,8 @@ extern "C" int wos_batch_shape_gradient(wos_batch *B, void *stream) {
         } else {
             k_batch_init<3, GradItem><<<gi, kBlock, 0, st>>>(g.gitems, g.n_gitems, g.gpoints, g.giprim, g.gisd,
                                                              B->spill, stride);
-            k_batch_grad<3><<<gw, kBlock, 0, st>>>(g.gitems, g.n_gitems, g.gsamples, B->counters + 2 + d, chunk,
-                                                   window, g.giprim, g.gisd, B->spill, stride);
+            k_batch_grad<3><<<gw, kBlock, hg, st>>>(g.gitems, g.n_gitems, g.gsamples, B->counters + 2 + d, chunk,
+                                                    window, g.giprim, g.gisd, B->spill, stride);
             k_vgrad_bound<3><<<dim3(kVgradSlices, g.n_gitems), kBlock, 0, st>>>(g.vstage, g.vranges, g.vpart);
             k_vgrad_scale<<<g.n_gitems, 64, 0, st>>>(g.vranges, g.vpart);
             k_vgrad_scatter<3><<<dim3(64, g.n_gitems, g.vwin), kBlock, sizeof(unsigned long long) * kVgradLds, st>>>(

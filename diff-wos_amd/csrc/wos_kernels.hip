@@ -1014,15 +1014,17 @@ struct GradOne {
 };
 
 // Shape-gradient samples (OursCUDA / Ours3D) of one scene: grad_samples
-// (wos_grad.h) over GradOne.  Held to 80 VGPRs (6 waves per SIMD): the 3D
+// (wos_grad.h) over GradOne.  3D is held to 80 VGPRs (6 waves per SIMD): the
 // sample state needs ~106 (4 waves); at 80, with a few registers spilled,
 // the 200k-triangle shape gradient measured 7 % faster on one MI355X and the
-// same on another.  The 2D state fits.  Finished samples write in batches as
+// same on another, and 3 % / 6 % faster than at 96 (5 waves) / 72 (7 waves).
+// 2D is held to 64 (8 waves, a few spills): 1.5 % faster than at 80 on the
+// clover.  Finished samples write in batches as
 // in the walk kernel (2D: 4 % faster).  STATS: count bound tests, primitive
 // tests, queries and walk steps into s.stats (the instrumented launch of
 // the measurements).
 template <int D, bool STATS>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(6))) void k_grad_persistent(
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(D == 2 ? 8 : 6))) void k_grad_persistent(
     SceneView s, GradArgs a) {
     DECLARE_STACK(a.spill, a.spill_stride);
     const GradOne src{s, a};
