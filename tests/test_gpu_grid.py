@@ -112,3 +112,23 @@ def test_walks_identical_with_and_without_grid():
     for k in ("mean", "walk_value", "rec_prim", "rec_t", "basis"):
         np.testing.assert_array_equal(a[k].cpu().numpy(), b[k].cpu().numpy(), err_msg=k)
         np.testing.assert_array_equal(a[k].cpu().numpy(), c0[k].cpu().numpy(), err_msg=k)
+
+
+@pytest.mark.parametrize("res", [-1, 700])
+def test_start_grid_wide_node_ids_bitexact(res):
+    """70,000 segments: internal node ids beyond 16 bits, so the start grid
+    keeps 16-byte cells instead of the packed 8-byte ones."""
+    import wos_ext
+    v, ix = C.star_polyline(70000)
+    vals = C.g_circle(v).astype(np.float32)
+    sc = wos_ext.Scene(torch.from_numpy(v).cuda(), torch.from_numpy(ix).cuda(), torch.from_numpy(vals).cuda())
+    sc.set_grid(0.0)
+    sc.set_start_grid(res)
+    assert sc.start_grid_info()["nx"] > 0
+    rng = np.random.default_rng(70000 + res)
+    p = np.concatenate([_near_boundary_points(v, ix, rng, 3000),
+                        rng.uniform(-1.6, 1.6, size=(500, 2)).astype(np.float32)])
+    idx, d2 = sc.closest_point_index_d2(torch.from_numpy(p).cuda())
+    i_ref, d_ref = O.Scene(v, ix, vals).closest_index(p)
+    np.testing.assert_array_equal(idx.cpu().numpy(), i_ref)
+    np.testing.assert_array_equal(d2.cpu().numpy(), d_ref)
