@@ -640,7 +640,9 @@ def debug_fail_workspace(n):
 
 def shape_gradient(scene, points, seed, grad_out, *, nwalks, nsteps, epsilon, epsilon2, clamping,
                    double_sided=False, control_variates=True, sample_grad_u=False):
-    """OursCUDA / Ours3D backward onto vertex positions."""
+    """OursCUDA / Ours3D backward onto vertex positions: dL/d vertices (summed
+    in 64-bit fixed point, so bit-reproducible) and, with sample_grad_u, the
+    per-sample spatial gradient estimates."""
     q = _dev(points, torch.float32, scene.dim)
     P = q.shape[0]
     gp = _GradParams(int(nwalks), int(nsteps), float(epsilon), float(epsilon2), float(clamping),
@@ -704,8 +706,8 @@ class Batch:
     settings: nwalks, nsteps, epsilon, epsilon2, clamping, double_sided,
     control_variates) gets a ``grad_vertices`` buffer that
     ``shape_gradient()`` adds dL/d vertex positions into from the same
-    ``grad_out``, equal to ``shape_gradient`` on that item alone up to the
-    order of the float additions.  ``run()``, ``zero_grad()``, ``adjoint()``
+    ``grad_out``, equal to ``shape_gradient`` on that item alone bit for bit
+    (both sum in the same 64-bit fixed point).  ``run()``, ``zero_grad()``, ``adjoint()``
     and ``shape_gradient()`` allocate nothing, so a whole primal + loss +
     adjoint step can be captured with ``torch.cuda.graph``.  The
     batch holds its scenes' device buffers: changing an item's source, grids
