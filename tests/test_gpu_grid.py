@@ -114,18 +114,19 @@ def test_walks_identical_with_and_without_grid():
         np.testing.assert_array_equal(a[k].cpu().numpy(), c0[k].cpu().numpy(), err_msg=k)
 
 
-@pytest.mark.parametrize("res", [-1, 700])
-def test_start_grid_wide_node_ids_bitexact(res):
-    """70,000 segments: internal node ids beyond 16 bits, so the start grid
-    keeps 16-byte cells instead of the packed 8-byte ones."""
+@pytest.mark.parametrize("n,res", [(70000, -1), (70000, 700), (65537, -1), (65538, -1)])
+def test_start_grid_wide_node_ids_bitexact(n, res):
+    """Trees at and beyond 16-bit internal node ids: 65,537 segments (ids up
+    to 0xffff) still use the packed 8-byte cells, 65,538 and more the 16-byte
+    ones."""
     import wos_ext
-    v, ix = C.star_polyline(70000)
+    v, ix = C.star_polyline(n)
     vals = C.g_circle(v).astype(np.float32)
     sc = wos_ext.Scene(torch.from_numpy(v).cuda(), torch.from_numpy(ix).cuda(), torch.from_numpy(vals).cuda())
     sc.set_grid(0.0)
     sc.set_start_grid(res)
     assert sc.start_grid_info()["nx"] > 0
-    rng = np.random.default_rng(70000 + res)
+    rng = np.random.default_rng(n + res)
     p = np.concatenate([_near_boundary_points(v, ix, rng, 3000),
                         rng.uniform(-1.6, 1.6, size=(500, 2)).astype(np.float32)])
     idx, d2 = sc.closest_point_index_d2(torch.from_numpy(p).cuda())
