@@ -401,6 +401,89 @@ __global__ __launch_bounds__(kBlock) void k_point_init(SceneView s, const float 
 }
 
 
+// Unbounded 3D closest-primitive queries of many points on the walk kernel's
+// persistent machinery: each lane advances its query `window` node visits
+// per round and takes the next point of its wave's chunk as soon as its
+// query ends, so a wave is not held by its slowest query (the start queries
+// of interior points vary severalfold in length).  INIT: the start records
+// of a walk launch (primitive, signed distance); else the closest index and
+// its squared distance.  Needs the BVH (use_bvh, two or more triangles).
+template <bool INIT>
+__global__ __launch_bounds__(kBlock) void k_points_persistent3(SceneView s, const float *p, int n,
+                                                               unsigned long long *counter, int chunk, int window,
+                                                               int *out_id, float *out_f, int2 *spill, long stride) {
+    DECLARE_STACK(spill, stride);
+    const int lane = __lane_id();
+    long long i = -1;
+    bool busy = false;
+    float q3[3] = {0.0f, 0.0f, 0.0f};
+    Query q;
+    q.node = q.sp = 0;
+    q.best = q.thr = 0.0f;
+    q.best_id = -1;
+    q.nbox = q.nprim = 0;
+    long long c_next = 0, c_end = 0;
+    bool exhausted = false;
+    for (;;) {
+        const uint64_t idle = __ballot(!busy);
+        if (idle != 0 && !exhausted) {
+            const int need = __popcll(idle);
+            const long long avail = c_end - c_next;
+            long long base2 = 0, end2 = 0;
+            bool fresh = false;
+            if (avail < need) {
+                unsigned long long b = 0;
+                if (lane == 0) b = atomicAdd(counter, (unsigned long long)chunk);
+                const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)b);
+                const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(b >> 32));
+                const long long bb = (long long)(((uint64_t)hi << 32) | lo);
+                if (bb >= n) {
+                    exhausted = true;
+                } else {
+                    fresh = true;
+                    base2 = bb;
+                    end2 = bb + chunk < n ? bb + chunk : n;
+                }
+            }
+            if (!busy) {
+                const int r = rank_in_mask(idle);
+                long long j = -1;
+                if (r < avail) j = c_next + r;
+                else if (fresh && (r - avail) < end2 - base2) j = base2 + (r - avail);
+                if (j >= 0) {
+                    i = j;
+                    load_point<3>(p, i, q3);
+                    query_begin_at<3>(q, s, q3, __builtin_inff(), 0x7fffffff, __builtin_inff(), st, slot, lane_g);
+                    busy = true;
+                }
+            }
+            if (avail >= need) {
+                c_next += need;
+            } else if (fresh) {
+                const long long used2 = need - avail;
+                c_next = base2 + used2 < end2 ? base2 + used2 : end2;
+                c_end = end2;
+            } else {
+                c_next = c_end;
+            }
+        }
+        if (exhausted && __ballot(busy) == 0) break;
+        for (int it = 0; it < window; ++it) {
+            if (busy && query_step<3, true>(s, q3, q, st, slot, lane_g, s.tol)) {
+                query_flush_stats<true>(s, q);
+                if (s.stats) atomicAdd(&s.stats[2], 1ull);
+                const int id = q.best_id == 0x7fffffff ? -1 : q.best_id;
+                out_id[i] = id;
+                if (INIT) out_f[i] = id < 0 ? __builtin_inff() : signed_dist<3>(s, id, q3);
+                else if (out_f) out_f[i] = q.best;
+                busy = false;
+            }
+            if (__ballot(busy) == 0) break;
+        }
+    }
+}
+
+
 // Fixed-point basis accumulators -> doubles in place (x / scale).
 __global__ void k_basis_finish(unsigned long long *acc, long n, double inv_scale) {
     for (long j = (long)blockIdx.x * blockDim.x + threadIdx.x; j < n; j += (long)gridDim.x * blockDim.x) {
@@ -1071,14 +1154,34 @@ static int spatial_order(const SceneView &s, const float *points, int n_points, 
     return sort_points_morton(points, n_points, s.dim, s.amin, s.amax, *order, st);
 }
 
+__global__ void k_zero_u64(unsigned long long *p) { *p = 0ull; }
+__global__ void k_zero_u64_n(unsigned long long *p, long n) {
+    for (long j = (long)blockIdx.x * blockDim.x + threadIdx.x; j < n; j += (long)gridDim.x * blockDim.x) p[j] = 0ull;
+}
+
+// Points per queue claim of k_points_persistent3.
+constexpr int kPointChunk = 64;
+
+// 3D scenes with a BVH take the persistent point-query kernel.
+static bool persistent_points(const SceneView &s) { return s.dim == 3 && s.use_bvh && s.n_prims > 1; }
+
 int launch_closest_index(const SceneView &s, const float *p, int n, int *out, float *out_d2, hipStream_t st) {
     if (n <= 0) return 0;
     int g = grid_of(n, occupancy_grid());
     long stride;
     int2 *spill = nullptr;
     if (spill_buffer(s, (long)g * kBlock, &stride, &spill, st) != 0) return -1;
-    if (s.dim == 2) k_closest_index<2><<<g, kBlock, top_bytes(s), st>>>(s, p, n, out, out_d2, spill, stride);
-    else k_closest_index<3><<<g, kBlock, top_bytes(s), st>>>(s, p, n, out, out_d2, spill, stride);
+    if (persistent_points(s)) {
+        unsigned long long *counter = launch_counter(st);
+        if (!counter) return -1;
+        k_zero_u64<<<1, 1, 0, st>>>(counter);
+        k_points_persistent3<false><<<g, kBlock, top_bytes(s), st>>>(s, p, n, counter, kPointChunk, window_for(3),
+                                                                      out, out_d2, spill, stride);
+    } else if (s.dim == 2) {
+        k_closest_index<2><<<g, kBlock, top_bytes(s), st>>>(s, p, n, out, out_d2, spill, stride);
+    } else {
+        k_closest_index<3><<<g, kBlock, top_bytes(s), st>>>(s, p, n, out, out_d2, spill, stride);
+    }
     KCHK();
     return 0;
 }
@@ -1149,11 +1252,6 @@ int launch_single_walk(const SceneView &s, const WalkSettings &w, const float *p
                                                stride);
     KCHK();
     return 0;
-}
-
-__global__ void k_zero_u64(unsigned long long *p) { *p = 0ull; }
-__global__ void k_zero_u64_n(unsigned long long *p, long n) {
-    for (long j = (long)blockIdx.x * blockDim.x + threadIdx.x; j < n; j += (long)gridDim.x * blockDim.x) p[j] = 0ull;
 }
 
 // Per-launch start records of the persistent kernels (closest primitive and
@@ -1233,6 +1331,24 @@ static void walk_reduce_grids(int n_points, int nwalks, int *g, int *gi) {
     *gi = grid_of(n_points, cap);
 }
 
+// Start records of a walk launch (closest primitive and signed distance per
+// point) on grid gi; counter: the launch's queue head (3D scenes with a BVH
+// use it here; the caller zeroes it again before the walks).
+static void launch_point_init(const SceneView &s, const float *points, int n_points, const int *order, int *prim,
+                              float *sd, int2 *spill, long stride, unsigned long long *counter, int gi,
+                              hipStream_t st) {
+    const size_t tb = top_bytes(s);
+    if (persistent_points(s)) {
+        k_zero_u64<<<1, 1, 0, st>>>(counter);
+        k_points_persistent3<true><<<gi, kBlock, tb, st>>>(s, points, n_points, counter, kPointChunk, window_for(3),
+                                                            prim, sd, spill, stride);
+    } else if (s.dim == 2) {
+        k_point_init<2><<<gi, kBlock, tb, st>>>(s, points, n_points, order, prim, sd, spill, stride);
+    } else {
+        k_point_init<3><<<gi, kBlock, tb, st>>>(s, points, n_points, nullptr, prim, sd, spill, stride);
+    }
+}
+
 static int walk_reduce_launch(const SceneView &s, const WalkSettings &w, const float *points, int n_points,
                               int32_t seed, int chunk, int K, const WalkOutputs &o, const WalkScratch &ws,
                               hipStream_t st) {
@@ -1265,8 +1381,7 @@ static int walk_reduce_launch(const SceneView &s, const WalkSettings &w, const f
     bool bas = src && o.basis != nullptr && K > 0;
     const long nbas = (long)K * n_points;
     if (!ws.stage && o.basis && K > 0) k_zero_u64_n<<<grid_of(nbas), kBlock, 0, st>>>(a.basis, nbas);
-    if (s.dim == 2) k_point_init<2><<<gi, kBlock, tb, st>>>(s, points, n_points, ws.order, ws.init_prim, ws.init_sd, a.spill, a.spill_stride);
-    else k_point_init<3><<<gi, kBlock, tb, st>>>(s, points, n_points, nullptr, ws.init_prim, ws.init_sd, a.spill, a.spill_stride);
+    launch_point_init(s, points, n_points, ws.order, ws.init_prim, ws.init_sd, a.spill, a.spill_stride, a.counter, gi, st);
     k_zero_u64<<<1, 1, 0, st>>>(a.counter);
     if (s.dim == 2) {
         if (s.stats) {
@@ -1561,8 +1676,7 @@ int launch_shape_gradient(const SceneView &s, const GradSettings &g, const float
         k_vgrad_setup<<<1, 1, 0, st>>>(range, 0, a.total, s.n_vertices, s.idx, vacc, grad_vertices);
     }
     size_t tb = top_bytes(s);
-    if (s.dim == 2) k_point_init<2><<<gi, kBlock, tb, st>>>(s, points, n_points, sr.order, sr.prim, sr.sd, a.spill, a.spill_stride);
-    else k_point_init<3><<<gi, kBlock, tb, st>>>(s, points, n_points, nullptr, sr.prim, sr.sd, a.spill, a.spill_stride);
+    launch_point_init(s, points, n_points, sr.order, sr.prim, sr.sd, a.spill, a.spill_stride, a.counter, gi, st);
     k_zero_u64<<<1, 1, 0, st>>>(a.counter);
     if (s.stats) {
         if (s.dim == 2) k_grad_persistent<2, true><<<gr, kBlock, tb, st>>>(s, a);
