@@ -401,17 +401,32 @@ __global__ __launch_bounds__(kBlock) void k_point_init(SceneView s, const float 
 }
 
 
+// What k_points_persistent3 writes per point.
+enum : int {
+    kPqIndex = 0,    // closest index (id) and its squared distance (f, nullable)
+    kPqInit = 1,     // start records of a walk launch: closest primitive (id), signed distance (f)
+    kPqSdf = 2,      // signed distance (f)
+    kPqRecord = 3    // closest-point record (store_record's outputs)
+};
+struct PointOut {
+    int *id;
+    float *f;
+    uint8_t *valid;
+    float *op, *on, *ot;
+    int *oprim;
+    float *ocontrib;
+};
+
 // Unbounded 3D closest-primitive queries of many points on the walk kernel's
 // persistent machinery: each lane advances its query `window` node visits
 // per round and takes the next point of its wave's chunk as soon as its
 // query ends, so a wave is not held by its slowest query (the start queries
-// of interior points vary severalfold in length).  INIT: the start records
-// of a walk launch (primitive, signed distance); else the closest index and
-// its squared distance.  Needs the BVH (use_bvh, two or more triangles).
-template <bool INIT>
+// of interior points vary severalfold in length).  MODE: what is written
+// (kPq*).  Needs the BVH (use_bvh, two or more triangles).
+template <int MODE>
 __global__ __launch_bounds__(kBlock) void k_points_persistent3(SceneView s, const float *p, int n,
                                                                unsigned long long *counter, int chunk, int window,
-                                                               int *out_id, float *out_f, int2 *spill, long stride) {
+                                                               PointOut o, int2 *spill, long stride) {
     DECLARE_STACK(spill, stride);
     const int lane = __lane_id();
     long long i = -1;
@@ -473,9 +488,18 @@ __global__ __launch_bounds__(kBlock) void k_points_persistent3(SceneView s, cons
                 query_flush_stats<true>(s, q);
                 if (s.stats) atomicAdd(&s.stats[2], 1ull);
                 const int id = q.best_id == 0x7fffffff ? -1 : q.best_id;
-                out_id[i] = id;
-                if (INIT) out_f[i] = id < 0 ? __builtin_inff() : signed_dist<3>(s, id, q3);
-                else if (out_f) out_f[i] = q.best;
+                if constexpr (MODE == kPqIndex) {
+                    o.id[i] = id;
+                    if (o.f) o.f[i] = q.best;
+                } else if constexpr (MODE == kPqInit || MODE == kPqSdf) {
+                    if (MODE == kPqInit) o.id[i] = id;
+                    o.f[i] = id < 0 ? __builtin_inff() : signed_dist<3>(s, id, q3);
+                } else {
+                    Rec r;
+                    r.valid = 0;
+                    if (id >= 0) closest_record<3>(s, id, q3, r);
+                    store_record<3>(r, i, o.valid, o.op, o.on, o.ot, o.oprim, o.ocontrib, 0.0f);
+                }
                 busy = false;
             }
             if (__ballot(busy) == 0) break;
@@ -1175,8 +1199,11 @@ int launch_closest_index(const SceneView &s, const float *p, int n, int *out, fl
         unsigned long long *counter = launch_counter(st);
         if (!counter) return -1;
         k_zero_u64<<<1, 1, 0, st>>>(counter);
-        k_points_persistent3<false><<<g, kBlock, top_bytes(s), st>>>(s, p, n, counter, kPointChunk, window_for(3),
-                                                                      out, out_d2, spill, stride);
+        PointOut o{};
+        o.id = out;
+        o.f = out_d2;
+        k_points_persistent3<kPqIndex><<<g, kBlock, top_bytes(s), st>>>(s, p, n, counter, kPointChunk, window_for(3),
+                                                                         o, spill, stride);
     } else if (s.dim == 2) {
         k_closest_index<2><<<g, kBlock, top_bytes(s), st>>>(s, p, n, out, out_d2, spill, stride);
     } else {
@@ -1192,8 +1219,19 @@ int launch_sdf(const SceneView &s, const float *p, int n, float *out, hipStream_
     long stride;
     int2 *spill = nullptr;
     if (spill_buffer(s, (long)g * kBlock, &stride, &spill, st) != 0) return -1;
-    if (s.dim == 2) k_sdf<2><<<g, kBlock, top_bytes(s), st>>>(s, p, n, out, spill, stride);
-    else k_sdf<3><<<g, kBlock, top_bytes(s), st>>>(s, p, n, out, spill, stride);
+    if (persistent_points(s)) {
+        unsigned long long *counter = launch_counter(st);
+        if (!counter) return -1;
+        PointOut o{};
+        o.f = out;
+        k_zero_u64<<<1, 1, 0, st>>>(counter);
+        k_points_persistent3<kPqSdf><<<g, kBlock, top_bytes(s), st>>>(s, p, n, counter, kPointChunk, window_for(3),
+                                                                       o, spill, stride);
+    } else if (s.dim == 2) {
+        k_sdf<2><<<g, kBlock, top_bytes(s), st>>>(s, p, n, out, spill, stride);
+    } else {
+        k_sdf<3><<<g, kBlock, top_bytes(s), st>>>(s, p, n, out, spill, stride);
+    }
     KCHK();
     return 0;
 }
@@ -1205,10 +1243,18 @@ int launch_closest_point(const SceneView &s, const float *p, int n, uint8_t *val
     long stride;
     int2 *spill = nullptr;
     if (spill_buffer(s, (long)g * kBlock, &stride, &spill, st) != 0) return -1;
-    if (s.dim == 2)
+    if (persistent_points(s)) {
+        unsigned long long *counter = launch_counter(st);
+        if (!counter) return -1;
+        const PointOut o{nullptr, nullptr, valid, op, on, ot, oprim, ocontrib};
+        k_zero_u64<<<1, 1, 0, st>>>(counter);
+        k_points_persistent3<kPqRecord><<<g, kBlock, top_bytes(s), st>>>(s, p, n, counter, kPointChunk,
+                                                                          window_for(3), o, spill, stride);
+    } else if (s.dim == 2) {
         k_closest_point<2><<<g, kBlock, top_bytes(s), st>>>(s, p, n, valid, op, on, ot, oprim, ocontrib, spill, stride);
-    else
+    } else {
         k_closest_point<3><<<g, kBlock, top_bytes(s), st>>>(s, p, n, valid, op, on, ot, oprim, ocontrib, spill, stride);
+    }
     KCHK();
     return 0;
 }
@@ -1340,8 +1386,11 @@ static void launch_point_init(const SceneView &s, const float *points, int n_poi
     const size_t tb = top_bytes(s);
     if (persistent_points(s)) {
         k_zero_u64<<<1, 1, 0, st>>>(counter);
-        k_points_persistent3<true><<<gi, kBlock, tb, st>>>(s, points, n_points, counter, kPointChunk, window_for(3),
-                                                            prim, sd, spill, stride);
+        PointOut o{};
+        o.id = prim;
+        o.f = sd;
+        k_points_persistent3<kPqInit><<<gi, kBlock, tb, st>>>(s, points, n_points, counter, kPointChunk,
+                                                               window_for(3), o, spill, stride);
     } else if (s.dim == 2) {
         k_point_init<2><<<gi, kBlock, tb, st>>>(s, points, n_points, order, prim, sd, spill, stride);
     } else {
