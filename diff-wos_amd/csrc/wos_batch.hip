@@ -28,6 +28,7 @@ constexpr int kMaxBatchBasis = 16;
 constexpr int kBatchHist = 6144;    // vertex-value histogram (64-bit entries) of the reverse pass kept in LDS up to this size
 constexpr long long kAdjointTile = 16384;   // walks of one item per adjoint tile
 constexpr int kBatchFinish = 8;             // finished walks written together (see k_walk_persistent)
+constexpr int kInitChunk = 64;              // points per queue claim of the 3D start queries
 
 // This file's device copy of the tabulated Green's-function CDF.
 static __device__ float g_green_cdf[kGreenCdfSize];
@@ -88,25 +89,106 @@ __device__ __forceinline__ int find_item(const long long *begins, int n, long lo
     return lo;
 }
 
-// Start query of every point of a group's items (BatchItem or GradItem).
+// Start queries of every point of a group's items with persistent lanes
+// (the scheme of k_points_persistent3): a lane advances its query `window`
+// node visits per round and takes the next point of its wave's chunk when the
+// query ends.  Points of items without a usable tree (no BVH, one primitive)
+// or, in 2D, with a near-boundary candidate grid are answered at once by the
+// one-shot query.  The item-start table is dynamic LDS (n_items entries).
 template <int D, class Item>
-__global__ __launch_bounds__(kBlock) void k_batch_init(const Item *items, int n_items, long long n_total_pts,
-                                                       int *iprim, float *isd, int2 *spill, long stride) {
-    __shared__ long long s_begin[kMaxBatchItems];
+__global__ __launch_bounds__(kBlock) void k_batch_init_persistent(const Item *items, int n_items,
+                                                                   long long n_total_pts, unsigned long long *counter,
+                                                                   int chunk, int window, int *iprim, float *isd,
+                                                                   int2 *spill, long stride) {
+    extern __shared__ long long s_init_begin[];
+    long long *s_begin = s_init_begin;
     for (int i = threadIdx.x; i < n_items; i += kBlock) s_begin[i] = items[i].point_begin;
     __syncthreads();
     TravStack st{spill, stride};
     const long lane_g = (long)blockIdx.x * kBlock + threadIdx.x;
-    for (long long g = lane_g; g < n_total_pts; g += (long long)gridDim.x * kBlock) {
-        int it = find_item(s_begin, n_items, g);
-        const SceneView s = items[it].v;
-        long long i = g - s_begin[it];
-        float q[3];
-        load_point<D>(items[it].points, i, q);
-        float d2;
-        int id = closest_prim<D>(s, q, __builtin_inff(), d2, st, threadIdx.x, lane_g);
-        iprim[g] = id;
-        isd[g] = id < 0 ? __builtin_inff() : signed_dist<D>(s, id, q);
+    const int lane = __lane_id();
+    long long g = -1;
+    int it = 0;
+    bool busy = false;
+    float q3[3] = {0.0f, 0.0f, 0.0f};
+    Query q;
+    q.node = q.sp = 0;
+    q.best = q.thr = 0.0f;
+    q.best_id = -1;
+    q.nbox = q.nprim = 0;
+    SceneView tv;            // the traversal's view: the item's nodes (no LDS copy)
+    tv.nodes = nullptr;
+    tv.leaves = nullptr;
+    tv.pleaves = nullptr;
+    tv.n_top = 0;
+    float tol = 0.0f;
+    long long c_next = 0, c_end = 0;
+    bool exhausted = false;
+    for (;;) {
+        const uint64_t idle = __ballot(!busy);
+        if (idle != 0 && !exhausted) {
+            const int need = __popcll(idle);
+            const long long avail = c_end - c_next;
+            long long base2 = 0, end2 = 0;
+            bool fresh = false;
+            if (avail < need) {
+                unsigned long long b = 0;
+                if (lane == 0) b = atomicAdd(counter, (unsigned long long)chunk);
+                const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)b);
+                const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(b >> 32));
+                const long long bb = (long long)(((uint64_t)hi << 32) | lo);
+                if (bb >= n_total_pts) {
+                    exhausted = true;
+                } else {
+                    fresh = true;
+                    base2 = bb;
+                    end2 = bb + chunk < n_total_pts ? bb + chunk : n_total_pts;
+                }
+            }
+            if (!busy) {
+                const int r = rank_in_mask(idle);
+                long long j = -1;
+                if (r < avail) j = c_next + r;
+                else if (fresh && (r - avail) < end2 - base2) j = base2 + (r - avail);
+                if (j >= 0) {
+                    g = j;
+                    it = find_item(s_begin, n_items, g);
+                    const SceneView &s = items[it].v;
+                    load_point<D>(items[it].points, g - s_begin[it], q3);
+                    if (!s.use_bvh || s.n_prims == 1 || (D == 2 && s.grid_start)) {
+                        float d2;
+                        const int id = closest_prim<D>(s, q3, __builtin_inff(), d2, st, threadIdx.x, lane_g);
+                        iprim[g] = id;
+                        isd[g] = id < 0 ? __builtin_inff() : signed_dist<D>(s, id, q3);
+                    } else {
+                        tv.nodes = s.nodes;
+                        tol = s.tol;
+                        query_begin_at<D>(q, s, q3, __builtin_inff(), 0x7fffffff, __builtin_inff(), st, threadIdx.x,
+                                          lane_g);
+                        busy = true;
+                    }
+                }
+            }
+            if (avail >= need) {
+                c_next += need;
+            } else if (fresh) {
+                const long long used2 = need - avail;
+                c_next = base2 + used2 < end2 ? base2 + used2 : end2;
+                c_end = end2;
+            } else {
+                c_next = c_end;
+            }
+        }
+        if (exhausted && __ballot(busy) == 0) break;
+        for (int k = 0; k < window; ++k) {
+            if (busy && query_step<D, false>(tv, q3, q, st, threadIdx.x, lane_g, tol)) {
+                const int id = q.best_id == 0x7fffffff ? -1 : q.best_id;
+                iprim[g] = id;
+                isd[g] = id < 0 ? __builtin_inff() : signed_dist<D>(items[it].v, id, q3);
+                busy = false;
+            }
+            if (__ballot(busy) == 0) break;
+        }
     }
 }
 
@@ -612,6 +694,8 @@ using namespace wos;
 
 struct wos_batch {
     BatchGroup g[2];          // [0] 2D items, [1] 3D items
+    // queue heads: walks (2D, 3D), shape-gradient samples (2D, 3D), start
+    // queries of the walks and of the shape-gradient samples
     unsigned long long *counters = nullptr;
     int2 *spill = nullptr;    // traversal spill owned by the batch
     long spill_lanes = 0;
@@ -851,7 +935,7 @@ extern "C" int wos_batch_create(const wos_batch_item *items, int n_items, wos_ba
         }
     }
     // queue heads: primal walks of each group, then shape-gradient samples
-    if (hipMalloc((void **)&B->counters, 4 * sizeof(unsigned long long)) != hipSuccess) {
+    if (hipMalloc((void **)&B->counters, 6 * sizeof(unsigned long long)) != hipSuccess) {
         (void)hipGetLastError();
         free_batch(B);
         set_error("cannot allocate the batch counters");
@@ -896,7 +980,10 @@ extern "C" int wos_batch_run(wos_batch *B, void *stream) {
         const size_t hb = sizeof(long long) * (size_t)g.n_items;   // item table of k_batch_walk
         k_batch_basis<<<dim3(256, g.n_items), kBlock, 0, st>>>(g.items, 0);
         if (d == 0) {
-            k_batch_init<2, BatchItem><<<gi, kBlock, 0, st>>>(g.items, g.n_items, g.points, g.iprim, g.isd, spill, stride);
+            k_batch_zero<<<1, 64, 0, st>>>(B->counters + 4, 1);
+            k_batch_init_persistent<2, BatchItem><<<gi, kBlock, hb, st>>>(g.items, g.n_items, g.points, B->counters + 4,
+                                                                          kInitChunk, window, g.iprim, g.isd, spill,
+                                                                          stride);
             if (g.max_k <= 8)
                 k_batch_walk<2, 8><<<gwk, kBlock, hb, st>>>(g.items, g.n_items, g.walks, B->counters + d, chunk,
                                                            window, g.iprim, g.isd, spill, stride);
@@ -904,7 +991,10 @@ extern "C" int wos_batch_run(wos_batch *B, void *stream) {
                 k_batch_walk<2, kMaxBatchBasis><<<gwk, kBlock, hb, st>>>(g.items, g.n_items, g.walks, B->counters + d,
                                                                         chunk, window, g.iprim, g.isd, spill, stride);
         } else {
-            k_batch_init<3, BatchItem><<<gi, kBlock, 0, st>>>(g.items, g.n_items, g.points, g.iprim, g.isd, spill, stride);
+            k_batch_zero<<<1, 64, 0, st>>>(B->counters + 4, 1);
+            k_batch_init_persistent<3, BatchItem><<<gi, kBlock, hb, st>>>(g.items, g.n_items, g.points, B->counters + 4,
+                                                                          kInitChunk, window, g.iprim, g.isd, spill,
+                                                                          stride);
             k_batch_walk<3, 0><<<gwk, kBlock, hb, st>>>(g.items, g.n_items, g.walks, B->counters + d, chunk, window,
                                                        g.iprim, g.isd, spill, stride);
         }
@@ -933,8 +1023,10 @@ extern "C" int wos_batch_shape_gradient(wos_batch *B, void *stream) {
         const int window = window_for(d == 0 ? 2 : 3);
         const size_t hg = sizeof(long long) * (size_t)g.n_gitems;   // item table of k_batch_grad
         if (d == 0) {
-            k_batch_init<2, GradItem><<<gi, kBlock, 0, st>>>(g.gitems, g.n_gitems, g.gpoints, g.giprim, g.gisd,
-                                                             B->spill, stride);
+            k_batch_zero<<<1, 64, 0, st>>>(B->counters + 5, 1);
+            k_batch_init_persistent<2, GradItem><<<gi, kBlock, hg, st>>>(g.gitems, g.n_gitems, g.gpoints,
+                                                                         B->counters + 5, kInitChunk, window, g.giprim,
+                                                                         g.gisd, B->spill, stride);
             k_batch_grad<2><<<gw, kBlock, hg, st>>>(g.gitems, g.n_gitems, g.gsamples, B->counters + 2 + d, chunk,
                                                     window, g.giprim, g.gisd, B->spill, stride);
             k_vgrad_bound<2><<<dim3(kVgradSlices, g.n_gitems), kBlock, 0, st>>>(g.vstage, g.vranges, g.vpart);
@@ -943,8 +1035,10 @@ extern "C" int wos_batch_shape_gradient(wos_batch *B, void *stream) {
                 g.vstage, g.vranges, kVgradLds, nullptr);
             k_vgrad_finish<2><<<dim3(64, g.n_gitems), kBlock, 0, st>>>(g.vranges);
         } else {
-            k_batch_init<3, GradItem><<<gi, kBlock, 0, st>>>(g.gitems, g.n_gitems, g.gpoints, g.giprim, g.gisd,
-                                                             B->spill, stride);
+            k_batch_zero<<<1, 64, 0, st>>>(B->counters + 5, 1);
+            k_batch_init_persistent<3, GradItem><<<gi, kBlock, hg, st>>>(g.gitems, g.n_gitems, g.gpoints,
+                                                                         B->counters + 5, kInitChunk, window, g.giprim,
+                                                                         g.gisd, B->spill, stride);
             k_batch_grad<3><<<gw, kBlock, hg, st>>>(g.gitems, g.n_gitems, g.gsamples, B->counters + 2 + d, chunk,
                                                     window, g.giprim, g.gisd, B->spill, stride);
             k_vgrad_bound<3><<<dim3(kVgradSlices, g.n_gitems), kBlock, 0, st>>>(g.vstage, g.vranges, g.vpart);
