@@ -305,47 +305,71 @@ __global__ __launch_bounds__(kBlock) void k_single_walk(SceneView s, WalkSetting
 // W + w is re-run from its key (TEA(seed, i), sequence k -- the streams of
 // the forward launch, so the same paths), accumulating d contrib / d theta
 // along the way; a walk that ends on the boundary adds grad_out[i] / W times
-// it (an invalid walk's value, and so its derivative, is 0).  Loops are
-// wave-uniform so the per-walk sums reduce across the wave by shuffles, then
-// in LDS per workgroup, then once per workgroup into grad_src.
+// it (an invalid walk's value, and so its derivative, is 0).  The walks are
+// cut into slices of `batches` consecutive 64-walk batches (fixed by the walk
+// count alone); a wave takes whole slices, reduces each batch across the wave
+// by shuffles (lane q keeps parameter q's running sum over the slice's
+// batches, in order) and writes the slice's sums to part[slice * np + q];
+// k_source_adjoint_sum adds the slices in a fixed order.  The result depends
+// neither on timing nor on the launch shape.
 __global__ __launch_bounds__(kBlock) void k_source_adjoint(SceneView s, WalkSettings w, const float *points,
                                                            int n_points, int32_t seed, const float *grad_out,
-                                                           float *grad_src, int2 *spill, long stride) {
+                                                           int batches, float *part, int2 *spill, long stride) {
     DECLARE_STACK(spill, stride);
-    __shared__ float sacc[kMaxSourceParams];
-    for (int q = threadIdx.x; q < kMaxSourceParams; q += kBlock) sacc[q] = 0.0f;
-    __syncthreads();
     const int np = s.n_src;
     const int W = w.nwalks;
+    const int lane = __lane_id();
     const long long total = (long long)n_points * W;
-    const long long wave_base = (long long)blockIdx.x * kBlock + (threadIdx.x & ~63);
-    for (long long kb = wave_base; kb < total; kb += (long long)gridDim.x * kBlock) {
-        const long long k = kb + __lane_id();
-        float acc[kMaxSourceParams];
+    const long long slice = 64LL * batches;
+    const long long n_slices = (total + slice - 1) / slice;
+    const long long wave = ((long long)blockIdx.x * kBlock + threadIdx.x) >> 6;
+    const long long n_waves = ((long long)gridDim.x * kBlock) >> 6;
+    for (long long sl = wave; sl < n_slices; sl += n_waves) {
+        float mine = 0.0f;   // lane q: parameter q's sum over the slice
+        for (int b = 0; b < batches; ++b) {
+            const long long k = sl * slice + b * 64LL + lane;
+            float acc[kMaxSourceParams];
 #pragma unroll
-        for (int q = 0; q < kMaxSourceParams; ++q) acc[q] = 0.0f;
-        float gw = 0.0f;
-        if (k < total) {
-            const uint32_t i = point_of_walk(k, W);
-            float p[3];
-            load_point<2>(points, i, p);
-            Pcg32 rng = walk_stream(seed, i, k);
-            Rec r;
-            float contrib;
-            walk_one<2, true>(s, w, p, rng, -1, 0.0f, st, slot, lane_g, r, contrib, acc);
-            if (r.valid) gw = grad_out[i] / (float)W;
-        }
+            for (int q = 0; q < kMaxSourceParams; ++q) acc[q] = 0.0f;
+            float gw = 0.0f;
+            if (k < total) {
+                const uint32_t i = point_of_walk(k, W);
+                float p[3];
+                load_point<2>(points, i, p);
+                Pcg32 rng = walk_stream(seed, i, k);
+                Rec r;
+                float contrib;
+                walk_one<2, true>(s, w, p, rng, -1, 0.0f, st, slot, lane_g, r, contrib, acc);
+                if (r.valid) gw = grad_out[i] / (float)W;
+            }
 #pragma unroll
-        for (int q = 0; q < kMaxSourceParams; ++q) {
-            if (q < np) {
-                float v = acc[q] * gw;
-                for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
-                if (__lane_id() == 0) atomicAdd(&sacc[q], v);
+            for (int q = 0; q < kMaxSourceParams; ++q) {
+                if (q < np) {
+                    float v = acc[q] * gw;
+                    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+                    if (lane == q) mine += v;
+                }
             }
         }
+        if (lane < np) part[sl * np + lane] = mine;
     }
+}
+
+// grad_src[q] += the slices' sums of parameter q (blockIdx.x), added in FP64
+// in a fixed order: thread t takes slices t, t + 256, ..., then a tree.
+__global__ __launch_bounds__(kBlock) void k_source_adjoint_sum(const float *part, long long n_slices, int np,
+                                                               float *grad_src) {
+    __shared__ double red[kBlock];
+    const int q = blockIdx.x;
+    double acc = 0.0;
+    for (long long sl = threadIdx.x; sl < n_slices; sl += kBlock) acc += (double)part[sl * np + q];
+    red[threadIdx.x] = acc;
     __syncthreads();
-    for (int q = threadIdx.x; q < np; q += kBlock) atomicAdd(&grad_src[q], sacc[q]);
+    for (int h = kBlock / 2; h > 0; h >>= 1) {
+        if ((int)threadIdx.x < h) red[threadIdx.x] += red[threadIdx.x + h];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) grad_src[q] += (float)red[0];
 }
 
 // ---------------------------------------------------------------------------
@@ -1650,11 +1674,26 @@ int launch_source_adjoint(const SceneView &s, const WalkSettings &w, const float
     const long long total = (long long)n_points * w.nwalks;
     if (!walk_count_fits(total)) return -1;
     if (w.use_is && ensure_green_cdf() != 0) return -1;
-    int g = grid_of((long)std::min<long long>(total, 1LL << 30), occupancy_grid());
+    // slices of 64 * batches walks, batches the least power of two that keeps
+    // the partial sums within 2^20 slices (a function of the walk count only)
+    int batches = 1;
+    while ((total + 64LL * batches - 1) / (64LL * batches) > (1LL << 20)) batches *= 2;
+    const long long n_slices = (total + 64LL * batches - 1) / (64LL * batches);
+    // one wave per slice at most
+    const int g = (int)std::max<long long>(1, std::min<long long>((n_slices + 3) / 4, occupancy_grid()));
     long stride;
     int2 *spill = nullptr;
     if (spill_buffer(s, (long)g * kBlock, &stride, &spill, st) != 0) return -1;
-    k_source_adjoint<<<g, kBlock, top_bytes(s), st>>>(s, w, points, n_points, seed, grad_out, grad_src, spill, stride);
+    float *part = nullptr;
+    if (hipMallocAsync((void **)&part, sizeof(float) * (size_t)n_slices * s.n_src, st) != hipSuccess) {
+        (void)hipGetLastError();
+        set_error("cannot allocate the source-gradient partial sums of %lld slices", n_slices);
+        return -1;
+    }
+    k_source_adjoint<<<g, kBlock, top_bytes(s), st>>>(s, w, points, n_points, seed, grad_out, batches, part, spill,
+                                                       stride);
+    k_source_adjoint_sum<<<s.n_src, kBlock, 0, st>>>(part, n_slices, s.n_src, grad_src);
+    (void)hipFreeAsync(part, st);
     KCHK();
     return 0;
 }

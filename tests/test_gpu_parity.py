@@ -223,6 +223,27 @@ def test_source_adjoint_matches_central_differences(stype, use_is):
     sc.set_source(stype, _t(SOURCES[stype]))
 
 
+
+def test_source_adjoint_is_bit_reproducible():
+    """The source-parameter gradient sums fixed slices of walks in a fixed
+    order: repeated calls, and calls under another launch shape, give the
+    same bits."""
+    import wos_ext
+    sc, _ = _source_scene(3, n=256)
+    p = _t(_disk_points(3000, seed=4))
+    w = wos_ext.WoS(nwalks=32, nsteps=256, epsilon=1e-3)
+    g = _t(np.random.default_rng(2).normal(size=3000).astype(np.float32))
+    a = wos_ext.adjoint_source(sc, w, p, 9, g)
+    b = wos_ext.adjoint_source(sc, w, p, 9, g)
+    old = wos_ext.get_tuning()
+    try:
+        wos_ext.set_tuning(wgs_per_cu=3)
+        c = wos_ext.adjoint_source(sc, w, p, 9, g)
+    finally:
+        wos_ext.set_tuning(chunk=old[0], wgs_per_cu=old[1], top_levels=old[2], window=old[3])
+    assert torch.equal(a, b)
+    assert torch.equal(a, c)
+
 def test_workspace_failure_is_an_error_not_a_launch():
     """A traversal-spill request that fails makes the entry point return an
     error (WoSError) before any launch; the next call works."""
