@@ -757,8 +757,12 @@ __global__ void k_face_normals_2d(const float *vtx, const int *idx, int n, float
     }
 }
 
-// 3D angle-weighted vertex normals (include/scene.cuh:381-408).
-__global__ void k_face_normals_3d(const float *vtx, const int *idx, int n, float *nrm) {
+// 3D angle-weighted vertex normals (include/scene.cuh:381-408), summed in
+// 64-bit fixed point (units of 2^-40; a vertex's sum of angle-weighted unit
+// normals stays below valence * pi, far inside the range) so that the result
+// does not depend on the order in which the faces add, then converted.
+constexpr double kNormalScale = 1099511627776.0;   // 2^40
+__global__ void k_face_normals_3d(const float *vtx, const int *idx, int n, unsigned long long *acc) {
     for (int f = blockIdx.x * blockDim.x + threadIdx.x; f < n; f += gridDim.x * blockDim.x) {
         int ids[3] = {idx[3 * f], idx[3 * f + 1], idx[3 * f + 2]};
         float p[3][3];
@@ -777,9 +781,16 @@ __global__ void k_face_normals_3d(const float *vtx, const int *idx, int n, float
             }
             cross3(a, b, cr);
             float ang = atan2f(sqrtf(dot3(cr, cr)), dot3(a, b));
-            for (int k = 0; k < 3; ++k) atomicAdd(&nrm[3 * ids[c] + k], ang * fn[k]);
+            for (int k = 0; k < 3; ++k)
+                atomicAdd(&acc[3 * ids[c] + k],
+                          (unsigned long long)__double2ll_rn((double)(ang * fn[k]) * kNormalScale));
         }
     }
+}
+
+__global__ void k_normals_from_fixed(const unsigned long long *acc, long m, float *nrm) {
+    for (long j = (long)blockIdx.x * blockDim.x + threadIdx.x; j < m; j += (long)gridDim.x * blockDim.x)
+        nrm[j] = (float)((double)(long long)acc[j] / kNormalScale);
 }
 
 __global__ void k_normalize(float *nrm, int nv, int D) {
@@ -917,8 +928,12 @@ int build_scene_gpu(BuildInputs &in, hipStream_t stream, BuildOutputs &out) {
         k_fill_u32<<<grid_for(2L * nv), kBlock, 0, stream>>>(reinterpret_cast<uint32_t *>(out.normals), 0u, 2L * nv);
         k_face_normals_2d<<<grid_for(n), kBlock, 0, stream>>>(in.vtx, in.idx, n, out.normals);
     } else {
-        k_fill_u32<<<grid_for(3L * nv), kBlock, 0, stream>>>(reinterpret_cast<uint32_t *>(out.normals), 0u, 3L * nv);
-        k_face_normals_3d<<<grid_for(n), kBlock, 0, stream>>>(in.vtx, in.idx, n, out.normals);
+        unsigned long long *nacc = nullptr;
+        BCHK(hipMallocAsync((void **)&nacc, sizeof(unsigned long long) * 3 * (size_t)nv, stream));
+        k_fill_u32<<<grid_for(6L * nv), kBlock, 0, stream>>>(reinterpret_cast<uint32_t *>(nacc), 0u, 6L * nv);
+        k_face_normals_3d<<<grid_for(n), kBlock, 0, stream>>>(in.vtx, in.idx, n, nacc);
+        k_normals_from_fixed<<<grid_for(3L * nv), kBlock, 0, stream>>>(nacc, 3L * nv, out.normals);
+        (void)hipFreeAsync(nacc, stream);
     }
     k_normalize<<<grid_for(nv), kBlock, 0, stream>>>(out.normals, nv, D);
     BCHK(hipGetLastError());

@@ -132,3 +132,27 @@ def test_scene_errors():
     ix2 = torch.tensor([[0, 1], [1, 5]], dtype=torch.int32).cuda()
     with pytest.raises(wos_ext.WoSError, match="out of range"):
         wos_ext.Scene(v, ix2, torch.zeros(3).cuda())
+
+
+def test_vertex_normals_3d_angle_weighted_and_reproducible():
+    """3D vertex normals: the angle-weighted sum of unit face normals
+    (include/scene.cuh:381-408) against an FP64 numpy restatement, and the
+    same bits from two builds (the sums are order-independent)."""
+    import wos_ext
+    v, f, _ = C.displaced_sphere(4)
+    vals = C.g_harmonic3(v).astype(np.float32)
+    t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).cuda()
+    n1 = wos_ext.Scene3D(t(v), t(f), t(vals)).normals
+    n2 = wos_ext.Scene3D(t(v), t(f), t(vals)).normals
+    assert torch.equal(n1, n2)
+    p = v.astype(np.float64)[f]                                   # (F, 3 corners, 3)
+    fn = np.cross(p[:, 1] - p[:, 0], p[:, 2] - p[:, 0])
+    fn /= np.linalg.norm(fn, axis=1, keepdims=True)
+    ref = np.zeros_like(v, dtype=np.float64)
+    for c in range(3):
+        a = p[:, (c + 1) % 3] - p[:, c]
+        b = p[:, (c + 2) % 3] - p[:, c]
+        ang = np.arctan2(np.linalg.norm(np.cross(a, b), axis=1), np.einsum("ij,ij->i", a, b))
+        np.add.at(ref, f[:, c], ang[:, None] * fn)
+    ref /= np.linalg.norm(ref, axis=1, keepdims=True)
+    np.testing.assert_allclose(n1.cpu().numpy(), ref, atol=2e-6)
