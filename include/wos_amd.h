@@ -219,7 +219,10 @@ int wos_adjoint_values(const wos_scene *scene, int n_points, int nwalks, const i
  * ax, ay, phase, coeff per term) sources.  Each walk is re-run from its key
  * (the streams of a walk_reduce call with the same seed and settings), so
  * nothing is stored.  grad_out: dL/d mean (n_points).  Adds into grad_source
- * (n_source_params floats); a scene without a source adds nothing. */
+ * (n_source_params floats); a scene without a source adds nothing.  The
+ * walks' terms are summed over fixed slices of walks in a fixed order, so the
+ * result is bit-reproducible; allocates 4 bytes per parameter per slice
+ * (at most 2^20 slices) on the stream. */
 int wos_adjoint_source(const wos_scene *scene, const wos_walk_params *params, const float *points, int n_points,
                        int seed, const float *grad_out, float *grad_source, void *stream);
 
