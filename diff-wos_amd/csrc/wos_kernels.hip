@@ -198,6 +198,8 @@ constexpr int kMaxSourceParams = 4 * kMaxBasis;   // Fourier: (ax, ay, phase, co
 // the scene's source (include/scene.cuh:278-301: Gaussian (mu_x, mu_y,
 // sigma_x, sigma_y, A), sinusoid (A, B, C, amplitude, offset); Fourier
 // sum_k c_k sin(ax_k x + ay_k y + ph_k) as (ax, ay, ph, c) per term).
+// KMAX: Fourier terms the unrolled loop covers (the scene's K <= KMAX).
+template <int KMAX = kMaxBasis>
 __device__ __forceinline__ void add_source_param_grad(const SceneView &s, float x, float y, float Gw, float *acc) {
     const float *q = s.src;
     if (s.source_type == 1) {
@@ -221,7 +223,7 @@ __device__ __forceinline__ void add_source_param_grad(const SceneView &s, float 
     } else if (s.source_type == 3) {
         const int K = s.n_src / 4;
 #pragma unroll
-        for (int k = 0; k < kMaxBasis; ++k) {
+        for (int k = 0; k < KMAX; ++k) {
             if (k < K) {
                 const float phi = fmaf(q[4 * k], x, fmaf(q[4 * k + 1], y, q[4 * k + 2]));
                 const float ac = q[4 * k + 3] * sin_cw(phi + 1.57079632679489661923f);
@@ -304,54 +306,177 @@ __global__ __launch_bounds__(kBlock) void k_single_walk(SceneView s, WalkSetting
 // Reverse mode of walk_reduce w.r.t. every source parameter: each walk k = i
 // W + w is re-run from its key (TEA(seed, i), sequence k -- the streams of
 // the forward launch, so the same paths), accumulating d contrib / d theta
-// along the way; a walk that ends on the boundary adds grad_out[i] / W times
-// it (an invalid walk's value, and so its derivative, is 0).  The walks are
-// cut into slices of `batches` consecutive 64-walk batches (fixed by the walk
-// count alone); a wave takes whole slices, reduces each batch across the wave
-// by shuffles (lane q keeps parameter q's running sum over the slice's
-// batches, in order) and writes the slice's sums to part[slice * np + q];
-// k_source_adjoint_sum adds the slices in a fixed order.  The result depends
-// neither on timing nor on the launch shape.
-__global__ __launch_bounds__(kBlock) void k_source_adjoint(SceneView s, WalkSettings w, const float *points,
-                                                           int n_points, int32_t seed, const float *grad_out,
-                                                           int batches, float *part, int2 *spill, long stride) {
+// along the way (walk_one's steps, one per round); a walk that ends on the
+// boundary contributes grad_out[i] / W times it (an invalid walk's value, and
+// so its derivative, is 0).  Lanes are persistent: a lane whose walk ends
+// writes the walk's terms to stage[(k - k0) * np + q] and takes the next walk
+// of [k0, k1) from its wave's chunk, so a wave is not held by its longest
+// walk.  k_source_slices then reduces fixed slices of 64 * batches walks.
+// NP: parameters carried per lane (8 for the Gaussian and sinusoid, 4 K_max
+// for the Fourier source), so the accumulators stay in registers.
+// init_prim / init_sd: each point's start record (closest primitive and
+// signed distance, as walk_one's start computes them).  Steps whose query is
+// pending advance it `window` node visits per round (the walk kernel's
+// resumable queries, seeded with the previous primitive).
+template <int NP>
+__global__ __launch_bounds__(kBlock) void k_source_walks(SceneView s, WalkSettings w, const float *points,
+                                                         int32_t seed, const float *grad_out, const int *init_prim,
+                                                         const float *init_sd, long long k0, long long k1,
+                                                         unsigned long long *counter, int chunk, int window,
+                                                         float *stage, int2 *spill, long stride) {
     DECLARE_STACK(spill, stride);
     const int np = s.n_src;
     const int W = w.nwalks;
     const int lane = __lane_id();
-    const long long total = (long long)n_points * W;
-    const long long slice = 64LL * batches;
-    const long long n_slices = (total + slice - 1) / slice;
-    const long long wave = ((long long)blockIdx.x * kBlock + threadIdx.x) >> 6;
-    const long long n_waves = ((long long)gridDim.x * kBlock) >> 6;
-    for (long long sl = wave; sl < n_slices; sl += n_waves) {
-        float mine = 0.0f;   // lane q: parameter q's sum over the slice
-        for (int b = 0; b < batches; ++b) {
-            const long long k = sl * slice + b * 64LL + lane;
-            float acc[kMaxSourceParams];
+    const long long n = k1 - k0;
+    long long j = -1;             // the lane's walk, relative to k0
+    uint32_t i = 0;               // its point
+    bool busy = false, querying = false;
+    float p[3] = {0.0f, 0.0f, 0.0f};
+    float d = 0.0f;
+    int prim = -1, step = 0;
+    Pcg32 rng;
+    rng.state = rng.inc = 0;
+    Query qq;
+    qq.node = qq.sp = 0;
+    qq.best = qq.thr = 0.0f;
+    qq.best_id = -1;
+    qq.nbox = qq.nprim = 0;
+    const bool tree = s.use_bvh && s.n_prims > 1;
+    float acc[NP];
 #pragma unroll
-            for (int q = 0; q < kMaxSourceParams; ++q) acc[q] = 0.0f;
-            float gw = 0.0f;
-            if (k < total) {
-                const uint32_t i = point_of_walk(k, W);
-                float p[3];
-                load_point<2>(points, i, p);
-                Pcg32 rng = walk_stream(seed, i, k);
-                Rec r;
-                float contrib;
-                walk_one<2, true>(s, w, p, rng, -1, 0.0f, st, slot, lane_g, r, contrib, acc);
-                if (r.valid) gw = grad_out[i] / (float)W;
+    for (int q = 0; q < NP; ++q) acc[q] = 0.0f;
+    long long c_next = 0, c_end = 0;
+    bool exhausted = false;
+    for (;;) {
+        const uint64_t idle = __ballot(!busy);
+        if (idle != 0 && !exhausted) {
+            const int need = __popcll(idle);
+            const long long avail = c_end - c_next;
+            long long base2 = 0, end2 = 0;
+            bool fresh = false;
+            if (avail < need) {
+                unsigned long long b = 0;
+                if (lane == 0) b = atomicAdd(counter, (unsigned long long)chunk);
+                const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)b);
+                const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(b >> 32));
+                const long long bb = (long long)(((uint64_t)hi << 32) | lo);
+                if (bb >= n) {
+                    exhausted = true;
+                } else {
+                    fresh = true;
+                    base2 = bb;
+                    end2 = bb + chunk < n ? bb + chunk : n;
+                }
             }
+            if (!busy) {
+                const int r = rank_in_mask(idle);
+                long long jj = -1;
+                if (r < avail) jj = c_next + r;
+                else if (fresh && (r - avail) < end2 - base2) jj = base2 + (r - avail);
+                if (jj >= 0) {
+                    // walk_one's start: the closest primitive and the side
+                    j = jj;
+                    const long long k = k0 + j;
+                    i = point_of_walk(k, W);
+                    load_point<2>(points, i, p);
+                    rng = walk_stream(seed, i, k);
 #pragma unroll
-            for (int q = 0; q < kMaxSourceParams; ++q) {
-                if (q < np) {
-                    float v = acc[q] * gw;
-                    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
-                    if (lane == q) mine += v;
+                    for (int q = 0; q < NP; ++q) acc[q] = 0.0f;
+                    step = 0;
+                    prim = init_prim[i];
+                    const float sd = init_sd[i];
+                    bool start = prim >= 0;
+                    if (start) {
+                        start = !(sd >= 0.0f && !w.double_sided);
+                        d = fabsf(sd);
+                        if (w.prevent_fd_artifacts) d *= 0.5f;
+                    }
+                    if (start) {
+                        busy = true;
+                        querying = false;
+                    } else {
+#pragma unroll
+                        for (int q = 0; q < NP; ++q)
+                            if (q < np) stage[j * np + q] = acc[q] * 0.0f;
+                    }
+                }
+            }
+            if (avail >= need) {
+                c_next += need;
+            } else if (fresh) {
+                const long long used2 = need - avail;
+                c_next = base2 + used2 < end2 ? base2 + used2 : end2;
+                c_end = end2;
+            } else {
+                c_next = c_end;
+            }
+        }
+        if (exhausted && __ballot(busy) == 0) break;
+        if (busy && !querying) {
+            // one iteration of walk_one's loop: stop on the boundary, give up
+            // after nsteps, or take a step
+            const bool valid = step < w.nsteps && d < w.epsilon;
+            if (valid || step >= w.nsteps) {
+                const float gw = valid ? grad_out[i] / (float)W : 0.0f;
+#pragma unroll
+                for (int q = 0; q < NP; ++q)
+                    if (q < np) stage[j * np + q] = acc[q] * gw;
+                busy = false;
+            } else {
+                if (s.source_type != 0) {
+                    const float u0 = rng.next_1d(), u1 = rng.next_1d();
+                    float dyx, dyy, G, wt;
+                    if (interior_sample(u0, u1, d, w.use_is != 0, g_green_cdf, dyx, dyy, G, wt))
+                        add_source_param_grad<NP / 4>(s, p[0] + dyx, p[1] + dyy, G * wt, acc);
+                }
+                move_point<2>(p, d, rng);
+                ++step;
+                if (tree) {
+                    // the walk kernel's query: from the start-grid nodes,
+                    // seeded with the previous primitive (within 2d of the
+                    // new point); the same exact closest primitive
+                    query_begin_from<2>(qq, start_nodes_of<2>(s, p), bound_d2(s, 2.0f * d), prim,
+                                        prim_d2<2>(s, prim, p), s.tol, st, slot, lane_g);
+                    querying = true;
+                } else {
+                    float d2;
+                    prim = closest_prim<2>(s, p, bound_d2(s, 2.0f * d), d2, st, slot, lane_g);
+                    d = sqrtf(d2);
                 }
             }
         }
-        if (lane < np) part[sl * np + lane] = mine;
+        for (int it = 0; it < window; ++it) {
+            if (busy && querying && query_step<2, false>(s, p, qq, st, slot, lane_g, s.tol)) {
+                prim = qq.best_id;
+                d = sqrtf(qq.best);
+                querying = false;
+            }
+            if (__ballot(busy && querying) == 0) break;
+        }
+    }
+}
+
+// Slices sl of 64 * batches staged walks (walk k0 + j at stage[j * np]):
+// each 64-walk batch is reduced across a wave by shuffles and lane q adds
+// parameter q's batch sums in order, into part[sl * np + q].  A function of
+// the staged values alone (not of which wave took a slice).
+__global__ __launch_bounds__(kBlock) void k_source_slices(const float *stage, long long n, int np, int batches,
+                                                          long long sl0, long long n_slices, float *part) {
+    const int lane = __lane_id();
+    const long long wave = ((long long)blockIdx.x * kBlock + threadIdx.x) >> 6;
+    const long long n_waves = ((long long)gridDim.x * kBlock) >> 6;
+    for (long long sl = wave; sl < n_slices; sl += n_waves) {
+        float mine = 0.0f;
+        for (int b = 0; b < batches; ++b) {
+            const long long jw = (sl * batches + b) * 64 + lane;
+            for (int q = 0; q < np; ++q) {
+                float v = jw < n ? stage[jw * np + q] : 0.0f;
+                for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+                if (lane == q) mine += v;
+            }
+        }
+        if (lane < np) part[(sl0 + sl) * np + lane] = mine;
     }
 }
 
@@ -1678,21 +1803,49 @@ int launch_source_adjoint(const SceneView &s, const WalkSettings &w, const float
     // the partial sums within 2^20 slices (a function of the walk count only)
     int batches = 1;
     while ((total + 64LL * batches - 1) / (64LL * batches) > (1LL << 20)) batches *= 2;
-    const long long n_slices = (total + 64LL * batches - 1) / (64LL * batches);
-    // one wave per slice at most
-    const int g = (int)std::max<long long>(1, std::min<long long>((n_slices + 3) / 4, occupancy_grid()));
+    const long long slice = 64LL * batches;
+    const long long n_slices = (total + slice - 1) / slice;
+    // walks staged per round: whole slices, about 2^22 walks
+    const long long round = std::max<long long>(1, (1LL << 22) / slice) * slice;
+    const long long per_round = std::min<long long>(round, total);
+    const int g = grid_of((long)per_round, occupancy_grid());
+    const int gi = grid_of(n_points, occupancy_grid());
     long stride;
     int2 *spill = nullptr;
-    if (spill_buffer(s, (long)g * kBlock, &stride, &spill, st) != 0) return -1;
-    float *part = nullptr;
-    if (hipMallocAsync((void **)&part, sizeof(float) * (size_t)n_slices * s.n_src, st) != hipSuccess) {
+    if (spill_buffer(s, (long)std::max(g, gi) * kBlock, &stride, &spill, st) != 0) return -1;
+    unsigned long long *counter = launch_counter(st);
+    if (!counter) return -1;
+    StartRecords sr(st);
+    if (sr.alloc(s, points, n_points) != 0) return -1;
+    float *part = nullptr, *stage = nullptr;
+    if (hipMallocAsync((void **)&part, sizeof(float) * (size_t)n_slices * s.n_src, st) != hipSuccess ||
+        hipMallocAsync((void **)&stage, sizeof(float) * (size_t)per_round * s.n_src, st) != hipSuccess) {
         (void)hipGetLastError();
-        set_error("cannot allocate the source-gradient partial sums of %lld slices", n_slices);
+        if (part) (void)hipFreeAsync(part, st);
+        set_error("cannot allocate the source-gradient terms of %lld walks", per_round);
         return -1;
     }
-    k_source_adjoint<<<g, kBlock, top_bytes(s), st>>>(s, w, points, n_points, seed, grad_out, batches, part, spill,
-                                                       stride);
+    const int gs = (int)std::max<long long>(1, std::min<long long>((per_round / slice + 3) / 4, occupancy_grid()));
+    launch_point_init(s, points, n_points, sr.order, sr.prim, sr.sd, spill, stride, counter, gi, st);
+    const int win = window_for(2);
+    for (long long k0 = 0; k0 < total; k0 += round) {
+        const long long k1 = std::min(total, k0 + round);
+        k_zero_u64<<<1, 1, 0, st>>>(counter);
+        if (s.n_src <= 8)
+            k_source_walks<8><<<g, kBlock, top_bytes(s), st>>>(s, w, points, seed, grad_out, sr.prim, sr.sd, k0, k1,
+                                                                counter, 128, win, stage, spill, stride);
+        else if (s.n_src <= 32)
+            k_source_walks<32><<<g, kBlock, top_bytes(s), st>>>(s, w, points, seed, grad_out, sr.prim, sr.sd, k0, k1,
+                                                                 counter, 128, win, stage, spill, stride);
+        else
+            k_source_walks<kMaxSourceParams><<<g, kBlock, top_bytes(s), st>>>(s, w, points, seed, grad_out, sr.prim,
+                                                                               sr.sd, k0, k1, counter, 128, win,
+                                                                               stage, spill, stride);
+        k_source_slices<<<gs, kBlock, 0, st>>>(stage, k1 - k0, s.n_src, batches, k0 / slice,
+                                               (k1 - k0 + slice - 1) / slice, part);
+    }
     k_source_adjoint_sum<<<s.n_src, kBlock, 0, st>>>(part, n_slices, s.n_src, grad_src);
+    (void)hipFreeAsync(stage, st);
     (void)hipFreeAsync(part, st);
     KCHK();
     return 0;
