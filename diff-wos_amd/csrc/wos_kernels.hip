@@ -178,6 +178,119 @@ __global__ __launch_bounds__(kBlock) void k_lib(SceneView s, const uint8_t *vali
     }
 }
 
+// lib_one for many 3D boundary points on persistent lanes: each lane runs
+// its point's ten bisection queries as resumable queries, advanced `window`
+// node visits per round, each seeded with the previous iteration's closest
+// primitive (any seed leaves the answer exact: a bounded search that ends
+// beyond its bound continues unbounded, as closest_prim falls back), and
+// takes the next point of its wave's chunk when the tenth ends.  Needs the
+// BVH (use_bvh, two or more triangles).
+__global__ __launch_bounds__(kBlock) void k_lib_persistent3(SceneView s, const uint8_t *valid, const float *p,
+                                                            const float *nrm, int n, float eps,
+                                                            unsigned long long *counter, int chunk, int window,
+                                                            float *out, int2 *spill, long stride) {
+    DECLARE_STACK(spill, stride);
+    const int lane = __lane_id();
+    float upper = fminf(fabsf(s.amax[0] - s.amin[0]), fabsf(s.amax[1] - s.amin[1]));
+    upper = fminf(upper, fabsf(s.amax[2] - s.amin[2])) / 2.0f;
+    long long i = -1;
+    bool busy = false, widened = false;
+    int it = 0, prim = -1;
+    float a = 0.0f, b = 0.0f, m = 0.0f, bound = 0.0f;
+    float pp[3] = {0.0f, 0.0f, 0.0f}, nn[3] = {0.0f, 0.0f, 0.0f}, q3[3] = {0.0f, 0.0f, 0.0f};
+    Query qq;
+    qq.node = qq.sp = 0;
+    qq.best = qq.thr = 0.0f;
+    qq.best_id = -1;
+    qq.nbox = qq.nprim = 0;
+    // the query of iteration it (m from the current a, b), seeded with prim
+    auto begin = [&]() {
+        m = (a + b) / 2.0f;
+        for (int k = 0; k < 3; ++k) q3[k] = fmaf(nn[k], m, pp[k]);
+        bound = bound_d2(s, m);
+        if (prim >= 0) query_begin_at<3>(qq, s, q3, bound, prim, prim_d2<3>(s, prim, q3), st, slot, lane_g);
+        else query_begin_at<3>(qq, s, q3, bound, 0x7fffffff, __builtin_inff(), st, slot, lane_g);
+        widened = false;
+    };
+    long long c_next = 0, c_end = 0;
+    bool exhausted = false;
+    for (;;) {
+        const uint64_t idle = __ballot(!busy);
+        if (idle != 0 && !exhausted) {
+            const int need = __popcll(idle);
+            const long long avail = c_end - c_next;
+            long long base2 = 0, end2 = 0;
+            bool fresh = false;
+            if (avail < need) {
+                unsigned long long bq = 0;
+                if (lane == 0) bq = atomicAdd(counter, (unsigned long long)chunk);
+                const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)bq);
+                const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(bq >> 32));
+                const long long bb = (long long)(((uint64_t)hi << 32) | lo);
+                if (bb >= n) {
+                    exhausted = true;
+                } else {
+                    fresh = true;
+                    base2 = bb;
+                    end2 = bb + chunk < n ? bb + chunk : n;
+                }
+            }
+            if (!busy) {
+                const int r = rank_in_mask(idle);
+                long long j = -1;
+                if (r < avail) j = c_next + r;
+                else if (fresh && (r - avail) < end2 - base2) j = base2 + (r - avail);
+                if (j >= 0) {
+                    i = j;
+                    if (valid && !valid[i]) {
+                        out[i] = 0.0f;
+                    } else {
+                        load_point<3>(p, i, pp);
+                        load_point<3>(nrm, i, nn);
+                        a = eps;
+                        b = upper;
+                        it = 0;
+                        prim = -1;
+                        begin();
+                        busy = true;
+                    }
+                }
+            }
+            if (avail >= need) {
+                c_next += need;
+            } else if (fresh) {
+                const long long used2 = need - avail;
+                c_next = base2 + used2 < end2 ? base2 + used2 : end2;
+                c_end = end2;
+            } else {
+                c_next = c_end;
+            }
+        }
+        if (exhausted && __ballot(busy) == 0) break;
+        for (int k = 0; k < window; ++k) {
+            if (busy && query_step<3, false>(s, q3, qq, st, slot, lane_g, s.tol)) {
+                if (!widened && (qq.best_id == 0x7fffffff || qq.best > bound)) {
+                    // nothing within the bound: the unbounded search
+                    query_begin_at<3>(qq, s, q3, __builtin_inff(), qq.best_id, qq.best, st, slot, lane_g);
+                    widened = true;
+                } else {
+                    prim = qq.best_id;
+                    const float d = fabsf(signed_dist<3>(s, prim, q3));
+                    if (fabsf(d - m) < 1e-5f) a = m;
+                    else b = m;
+                    if (++it == 10) {
+                        out[i] = a;
+                        busy = false;
+                    } else {
+                        begin();
+                    }
+                }
+            }
+            if (__ballot(busy) == 0) break;
+        }
+    }
+}
+
 template <int D>
 __global__ void k_dirichlet(SceneView s, const uint8_t *valid, const int *prim, const float *t, int n, float *out) {
     for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
@@ -1415,8 +1528,17 @@ int launch_lib(const SceneView &s, const uint8_t *valid, const float *p, const f
     long stride;
     int2 *spill = nullptr;
     if (spill_buffer(s, (long)g * kBlock, &stride, &spill, st) != 0) return -1;
-    if (s.dim == 2) k_lib<2><<<g, kBlock, top_bytes(s), st>>>(s, valid, p, nrm, n, eps, out, spill, stride);
-    else k_lib<3><<<g, kBlock, top_bytes(s), st>>>(s, valid, p, nrm, n, eps, out, spill, stride);
+    if (persistent_points(s)) {
+        unsigned long long *counter = launch_counter(st);
+        if (!counter) return -1;
+        k_zero_u64<<<1, 1, 0, st>>>(counter);
+        k_lib_persistent3<<<g, kBlock, top_bytes(s), st>>>(s, valid, p, nrm, n, eps, counter, kPointChunk,
+                                                            window_for(3), out, spill, stride);
+    } else if (s.dim == 2) {
+        k_lib<2><<<g, kBlock, top_bytes(s), st>>>(s, valid, p, nrm, n, eps, out, spill, stride);
+    } else {
+        k_lib<3><<<g, kBlock, top_bytes(s), st>>>(s, valid, p, nrm, n, eps, out, spill, stride);
+    }
     KCHK();
     return 0;
 }
