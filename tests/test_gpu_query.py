@@ -123,6 +123,32 @@ def test_records_sdf_lib_match_oracle(dim):
     assert np.isfinite(val).all()
 
 
+
+@pytest.mark.parametrize("n", [1, 3, 65])
+def test_persistent_3d_queries_small_sizes(n):
+    """The 3D point queries run on persistent lanes that claim points in
+    chunks: point counts below, at and just above a wave, against the oracle
+    (closest index and squared distance, sdf, records, and LIB with one record
+    marked invalid)."""
+    rng = np.random.default_rng(n)
+    sc, ref, _, noise = _scene3(4)
+    p = C.points_in_star(n, noise, seed=int(rng.integers(1 << 30)))
+    pt = torch.from_numpy(p).cuda()
+    idx, d2 = sc.closest_point_index_d2(pt)
+    i_ref, d_ref = ref.closest_index(p)
+    np.testing.assert_array_equal(idx.cpu().numpy(), i_ref)
+    np.testing.assert_array_equal(d2.cpu().numpy(), d_ref)
+    np.testing.assert_array_equal(sc.sdf_native(pt).cpu().numpy(), ref.sdf(p))
+    rec = sc.closest_point(pt)
+    r = ref.closest_point(p)
+    np.testing.assert_array_equal(rec.prim_id.cpu().numpy(), r["prim"])
+    rec.valid[0] = False
+    r["valid"][0] = 0
+    R = sc.largest_inscribed_ball(rec).cpu().numpy()
+    Rr = ref.largest_inscribed_ball(r["valid"], r["p"], r["n"], 1e-3)
+    np.testing.assert_array_equal(R, Rr)
+    assert R[0] == 0.0
+
 def test_scene_errors():
     import wos_ext
     v = torch.tensor([[0.0, 0.0], [1.0, 0.0], [1.0, 0.0]]).cuda()
