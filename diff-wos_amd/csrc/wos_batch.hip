@@ -297,7 +297,12 @@ k_batch_walk(const BatchItem *__restrict__ items, int n_items, long long total, 
         if (__popcll(__ballot(mode == kDone || mode == kIdle)) >= kBatchFinish || exhausted ||
             __ballot(mode == kStart || mode == kQuery) == 0) {
             if (mode == kDone) {
-                batch_write<D, KB>(items[item], k, done_valid, prim, p, contrib, bas);
+                // (the item at a wave-uniform index as in the refill below: 1 % faster)
+                const int iu = __builtin_amdgcn_readfirstlane(item);
+                if (D == 2 && __ballot(item != iu) == 0)
+                    batch_write<D, KB>(items[iu], k, done_valid, prim, p, contrib, bas);
+                else
+                    batch_write<D, KB>(items[item], k, done_valid, prim, p, contrib, bas);
                 mode = kIdle;
                 k = -1;
             }
